@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""TreePM gravity-step benchmark on one MI355X.
+
+Workload (the single-box clustered configuration): n^3 particles in a periodic box of n cells
+(default n = 256), 70% on seeded-shuffled lattice sites displaced by a Zel'dovich field with
+P(k) ~ k^-1 (rms 0.3 cell), 30% in 2000 Plummer spheres (a = 0.5 cell, truncated at 10 a,
+counts from dN/dM ~ M^-1.9 on [32, 1e5]); n^3 PM mesh; Gaussian split r_s = 1.25 cells
+(efold 2.5) with TSC^2 deconvolution; erfc short-range law, cutoff 4.5 r_s = 5.625 cells;
+theta = 0.5; flat LCDM Om = 0.26, AMAX = 48, a = 1, da = 0.25.
+
+One step = one GOTPM step through the C ABI: Hubble drag, PM kick (pmmain), tree kick
+(treecorrection), drift. The sort, tree build and every kernel run inside every step.
+value = particles / step time = TreePM forces/s (one force evaluation per step).
+
+Printed: one JSON line (rank 0) with roofline (dominant kernel, live HIP-event durations),
+cpu_baseline (the CPU oracle on a bounded sample), accuracy vs the plain FP64 kernels.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TreePM forces/s @256^3, rms err<=1e-3 vs FP64; batched p50/p99 step latency"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def rel_errors(a, ref):
+    d = np.linalg.norm(a - ref, axis=1)
+    r = np.linalg.norm(ref, axis=1)
+    rms = float(np.sqrt(np.sum(d ** 2) / np.sum(r ** 2)))
+    per = d / np.maximum(r, 1e-300)
+    return rms, float(np.percentile(per, 99)), float(np.median(per))
+
+
+def force_at(ctx, lagpm, params, pos):
+    """PM + tree acceleration at fixed positions (fact2 = 1, zero velocities)."""
+    with lagpm.Box(ctx, params, len(pos)) as b:
+        b.upload_pos(pos)
+        t0 = time.perf_counter()
+        b.pmmain(lagpm.evolfact(1.0), lagpm.KICK)
+        b.treecorrection(lagpm.evolfact(1.0))
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        v = b.velocities()
+    return v, dt
+
+
+def cpu_baseline(args):
+    """The CPU oracle (reference algorithm: TSC, FFT, psolver, FDA4 + per-cell reference tree
+    walk) on the same recipe at a bounded size, timed on this host's cores."""
+    from lageunha_amd import ics
+    from oracle import pyoracle as orc
+    if args.cpu_threads > 0:
+        orc.set_threads(args.cpu_threads)
+    threads = orc.num_threads()
+    n = args.cpu_n
+    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=max(4, int(round(2000 * (n / 256) ** 3))), seed=args.seed)
+    diff, slope = orc.force_spline(1, n, 0.1, 1.25)
+    t0 = time.perf_counter()
+    orc.pm_accel(n, pos, mass=1.0, efold=2.5, deconvolve=1, prec=0)
+    t1 = time.perf_counter()
+    orc.treecorrection_ref(n, pos, diff, slope, 5.625, args.theta, cellw=5.625)
+    t2 = time.perf_counter()
+    N = len(pos)
+    return {
+        "value": N / (t2 - t0), "unit": "forces/s", "cores": threads, "kind": "port",
+        "sample": f"oracle restatement of the reference path (TSC, FFT, psolver, FDA4, per-cell reference "
+                  f"tree walk theta={args.theta}) on the same clustered recipe at {n}^3 ({N} particles, "
+                  f"{max(4, int(round(2000 * (n / 256) ** 3)))} halos), one force evaluation; "
+                  f"PM {t1 - t0:.2f} s, tree {t2 - t1:.2f} s, {threads} OpenMP threads",
+        "seconds": t2 - t0,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print(json.dumps({"error": "this build runs on one GPU: the TreePM step of one box is not sharded; "
+                                   "run with --gpus 1 (multi-GPU sharding is described in DESIGN.md)"}))
+        sys.exit(2)
+
+    from lageunha_amd import ics, lagpm
+
+    n = args.n
+    t_ic = time.perf_counter()
+    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=2000, seed=args.seed)
+    N = len(pos)
+    log(f"[bench] workload: {n}^3 box, {N} particles (IC {time.perf_counter() - t_ic:.1f} s)")
+    params = lagpm.make_params(n, law=lagpm.LAW_ERFC, precision=lagpm.PREC_FP32, deconvolve=1, efold=2.5,
+                               epsilon=0.1, rsphere=5.625, theta=args.theta, amax=48.0, omep=0.26, omeplam=0.74)
+    ef = lagpm.get_evol_factor(params, 1.0, 0.25)
+
+    ctx = lagpm.Context(0)
+    box = lagpm.Box(ctx, params, N)
+    box.upload_pos(pos)
+
+    phases, stats = [], []
+
+    def step():
+        box.hubble_drag(ef.fact1)
+        box.pmmain(ef, lagpm.KICK)
+        box.treecorrection(ef)
+        t = box.timings().as_dict()
+        s = box.tree_stats()
+        box.drift(ef.pfact)
+        return t, s
+
+    for i in range(args.warmup):
+        step()
+        log(f"[bench] warmup step {i + 1}/{args.warmup} done")
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        t, s = step()
+        phases.append(t)
+        stats.append((s.n_pp, s.n_pc, s.n_nodes, s.n_groups))
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    log(f"[bench] timed {args.steps} steps: {ms:.2f} ms/step")
+
+    ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    npp = float(np.mean([s[0] for s in stats]))
+    npc = float(np.mean([s[1] for s in stats]))
+    M = n ** 3
+    K = (n // 2 + 1) * n * n
+    # algorithmic bytes / flops per launch
+    kern = {
+        "walk": {"bound": "fp32", "work": 21 * npp + 69 * npc, "ms": ph["ms_walk"]},
+        "tsc": {"bound": "hbm", "work": 24 * N + 20 * M, "ms": ph["ms_tsc"]},
+        "green": {"bound": "hbm", "work": 16 * K, "ms": ph["ms_green"]},
+        "grad": {"bound": "hbm", "work": 16 * M, "ms": ph["ms_grad"]},
+        "interp": {"bound": "hbm", "work": 72 * N + 12 * M, "ms": ph["ms_interp"]},
+    }
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    d = kern[dom]
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        try:
+            traffic = json.load(open(PROFILE_TRAFFIC)).get(dom)
+        except Exception:
+            traffic = None
+    if d["bound"] == "hbm":
+        achieved = d["work"] / (d["ms"] * 1e-3) / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}
+    else:
+        achieved = d["work"] / (d["ms"] * 1e-3) / 1e12
+        roof = {"kernel": "walk_f32_kernel", "bound": "fp32-vector", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+                "work": "21 flops per pair-particle + 69 per particle-node interaction evaluated",
+                "n_pp": npp, "n_pc": npc}
+    phase_report = {}
+    for k, v in kern.items():
+        if v["bound"] == "hbm" and v["ms"] > 0:
+            phase_report[k] = {"ms": v["ms"], "GB/s": v["work"] / (v["ms"] * 1e-3) / 1e9}
+        else:
+            phase_report[k] = {"ms": v["ms"], "TFLOP/s": v["work"] / max(v["ms"], 1e-9) / 1e9}
+    for k in ("ms_sort", "ms_fft_r2c", "ms_fft_c2r", "ms_tree_sort", "ms_build"):
+        phase_report[k[3:]] = {"ms": ph[k]}
+
+    out = {
+        "metric": METRIC, "value": N / (ms * 1e-3), "unit": "forces/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32",
+        "dtype_detail": "FP32 mesh/FFT/pair arithmetic with FP32 accumulation, int64 fixed-point deposit, "
+                        "FP64 positions/velocities and kick-drift",
+        "data": "synthetic",
+        "config": {"workload": f"single box {n}^3 clustered (70% perturbed lattice + 30% in 2000 Plummer halos), "
+                               f"{n}^3 mesh, Rsplit 1.25, rcut 5.625, theta {args.theta}, one GOTPM step "
+                               f"(drag + PM kick + tree kick + drift) per step",
+                   "n": n, "particles": N, "parallelism": "single GPU"},
+        "roofline": roof,
+        "phases": phase_report,
+    }
+    box.close()
+
+    if not args.no_accuracy:
+        # forces at the initial positions: FP32 fast path vs the plain FP64 kernels
+        a32, t32 = force_at(ctx, lagpm, params, pos)
+        acc = {"vs": "plain FP64 kernels, same tree/groups/criteria", "fp32_seconds": t32}
+        if not args.no_fp64:
+            p64 = lagpm.make_params(n, law=lagpm.LAW_ERFC, precision=lagpm.PREC_FP64, deconvolve=1, efold=2.5,
+                                    epsilon=0.1, rsphere=5.625, theta=args.theta)
+            a64, t64 = force_at(ctx, lagpm, p64, pos)
+            rms, p99, med = rel_errors(a32, a64)
+            acc.update({"rms_rel": rms, "p99_rel": p99, "median_rel": med, "fp64_plain_seconds": t64,
+                        "speedup_vs_fp64_plain": t64 / max(t32, 1e-12)})
+            log(f"[bench] accuracy vs FP64 plain: rms {rms:.2e} p99 {p99:.2e}; FP64 plain force eval {t64:.3f} s")
+        out["accuracy"] = acc
+
+    if not args.no_cpu_baseline:
+        log(f"[bench] cpu baseline at {args.cpu_n}^3 ...")
+        out["cpu_baseline"] = cpu_baseline(args)
+        log(f"[bench] cpu baseline: {out['cpu_baseline']['value']:.3e} forces/s")
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

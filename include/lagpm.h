@@ -1,0 +1,191 @@
+/*
+ * lagpm.h — C ABI of the MI355X-native TreePM gravity step.
+ *
+ * Drop-in boundary for the GOTPM TreePM force path of EUNHA (kjhan0606/LagEunha).
+ * Every entry point is plain C: opaque handles, plain pointers and sizes, int status.
+ * No HIP, torch or STL types appear here; device streams are passed as void*.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   pmmain        Cosmos/pmmain.c:390      void pmmain(SimParameters*, EvolFact)
+ *   TSC           Cosmos/pmmain.c:112,188  void TSC(SimParameters*, GridInfo*)
+ *   psolver       Cosmos/psolver.c:13      void psolver(SimParameters*, GridInfo*)
+ *   fda4          Cosmos/fda4.c:31         void fda4(SimParameters*, GridInfo*, EvolFact)
+ *   treecorrection Cosmos/treecorrection.c:225 void treecorrection(SimParameters*, DeterminedEvolFact*)
+ *   i_force_spline OST/Model4TreeForce.c:48 void i_force_spline(SimParameters*)
+ *   ForceLaw      ForceLaw/treeforce.c:150  void i_force_spline(int nx, float Epsilon)
+ *   HubbleDragDM  Cosmos/globalT.c:102     void HubbleDragDM(SimParameters*, float)
+ *   drift         Cosmos/stepevolv.c:38    void onestepforwardposition(SimParameters*)
+ *   XofP decode   eunha.h:1228-1232        (GOTPM + XYZDBL 32-byte particle layout)
+ *
+ * Error behaviour: the reference returns void and calls exit()/MPI_Finalize() on error
+ * (Cosmos/treecorrection.c:240-247). Here every call returns 0 on success or a negative
+ * LAGPM_E_* code, and lagpm_last_error() gives the message. Nothing here exits the process.
+ */
+#ifndef LAGPM_H
+#define LAGPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LAGPM_ABI_VERSION 1
+
+/* status codes */
+#define LAGPM_OK          0
+#define LAGPM_E_INVAL    -1   /* bad argument (sizes, pointers, parameter ranges) */
+#define LAGPM_E_NOMEM    -2   /* device allocation failed / HBM budget exceeded */
+#define LAGPM_E_HIP      -3   /* HIP runtime error */
+#define LAGPM_E_FFT      -4   /* rocFFT error */
+#define LAGPM_E_STATE    -5   /* call out of order (e.g. no particles uploaded) */
+#define LAGPM_E_OVERFLOW -6   /* a bounded device structure overflowed (walk stack, node pool) */
+
+/* PM status of the kick, Cosmos/globalT.c:113-131 and Cosmos/fda4.c:49-63 */
+#define LAGPM_KICK 0
+#define LAGPM_PUSH 1
+#define LAGPM_PULL 2
+
+/* short-range force law of the tree correction */
+#define LAGPM_LAW_PMFIT 0  /* reference: Plummer Newton minus fitted PM force, OST/Model4TreeForce.c:92-116 */
+#define LAGPM_LAW_ERFC  1  /* Plummer Newton times the erfc complement of a Gaussian filter exp(-k^2 rs^2) */
+
+/* arithmetic of the force evaluation */
+#define LAGPM_PREC_FP32 0  /* FP32 mesh/FFT/pairs, FP64 positions and velocities (fast path) */
+#define LAGPM_PREC_FP64 1  /* plain FP64 kernels everywhere (baseline and accuracy reference) */
+
+/* Kick factors; same fields and meaning as EvolFact, Cosmos/CosmosEvolFactor.h:1-6. */
+typedef struct lagpm_evolfact {
+    float afact, bfact, pfact;
+    float fact1, fact2;
+    float fact1_push, fact2_push;
+    float fact1_pull, fact2_pull;
+} lagpm_evolfact;
+
+/* Per-box gravity parameters (the gravity keys of SimParameters: NX, GRV_EFOLD,
+ * GRV_EPSILON, RSPHERE, THETA, eunha.h:1033-1090, plus this build's switches). */
+typedef struct lagpm_params {
+    int32_t n;          /* mesh cells per side; the box side is n cells (NX) */
+    int32_t law;        /* LAGPM_LAW_* */
+    int32_t precision;  /* LAGPM_PREC_* */
+    int32_t deconvolve; /* 1: divide the Green function by the TSC window squared */
+    double  mass;       /* particle mass (DM_MASS); mean density n^-3 * np * mass */
+    double  efold;      /* Gaussian filter exp(-w (pi efold / n)^2) (GRV_EFOLD); r_s = efold/2 cells */
+    double  epsilon;    /* Plummer softening in cells (GRV_EPSILON, default 0.1) */
+    double  rsphere;    /* short-range cutoff in cells (RSPHERE) */
+    double  theta;      /* opening angle (THETA); a node opens when d^2 < rmax^2/theta^2 */
+    double  amax;       /* AMAX: scale factor today (1+z = amax/a) */
+    double  omep;       /* Omega_matter (OMEP) */
+    double  omeplam;    /* Omega_Lambda (OMEPLAM) */
+} lagpm_params;
+
+typedef struct lagpm_ctx lagpm_ctx;  /* one per device; owns streams, FFT plan cache */
+typedef struct lagpm_box lagpm_box;  /* one periodic box; owns resident particle/mesh/tree state */
+
+/* Interaction and timing counters of the last tree correction of a box. */
+typedef struct lagpm_tree_stats {
+    uint64_t n_pp;      /* particle-particle pairs evaluated */
+    uint64_t n_pc;      /* particle-node (quadrupole) pairs evaluated */
+    uint64_t n_nodes;   /* octree nodes built */
+    uint64_t n_groups;  /* target groups walked */
+    double   ms_build;  /* sort + build + moments, device time */
+    double   ms_walk;   /* walk kernel, device time */
+} lagpm_tree_stats;
+
+/* Device time (HIP events on the context stream) of the phases of the last pmmain and
+ * treecorrection of a box; phase names follow the reference's timers (Cosmos/pmmain.c:447-522). */
+typedef struct lagpm_timings {
+    double ms_sort;     /* Morton keys + radix sort + permutation (0 when already sorted) */
+    double ms_tsc;      /* TSC deposit + fixed-point convert (CPU(TSC)) */
+    double ms_fft_r2c;  /* CPU(F-FFT) */
+    double ms_green;    /* CPU(psolv) */
+    double ms_fft_c2r;  /* CPU(B-FFT) */
+    double ms_grad;     /* FDA4 gradient mesh */
+    double ms_interp;   /* TSC interpolation + kick (with ms_grad: CPU(4FDA)) */
+    double ms_tree_sort;/* sort inside treecorrection (0 when pmmain already sorted) */
+    double ms_build;    /* octree build + moments + groups */
+    double ms_walk;     /* walk + kick (CPU(TREE) with ms_build) */
+} lagpm_timings;
+
+/* ---- context --------------------------------------------------------------------- */
+int         lagpm_create(lagpm_ctx** out, int device);
+int         lagpm_destroy(lagpm_ctx* ctx);
+const char* lagpm_last_error(const lagpm_ctx* ctx);
+int         lagpm_abi_version(void);
+/* Stream all work of this context is queued on (hipStream_t as void*). */
+void*       lagpm_stream(lagpm_ctx* ctx);
+int         lagpm_synchronize(lagpm_ctx* ctx);
+
+/* ---- boxes ------------------------------------------------------------------------ */
+/* Allocates device state for np particles. Fails with LAGPM_E_NOMEM beyond device memory. */
+int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box** out);
+int lagpm_box_destroy(lagpm_ctx* ctx, lagpm_box* box);
+int lagpm_box_params(const lagpm_box* box, lagpm_params* out);
+size_t lagpm_box_np(const lagpm_box* box);
+/* Bytes of device memory the box holds (used for HBM-budget admission). */
+size_t lagpm_box_device_bytes(const lagpm_box* box);
+size_t lagpm_estimate_box_bytes(const lagpm_params* p, size_t np);
+
+/* Host SoA upload/download in the caller's particle order. Positions are wrapped into
+ * [0,n). vx..vz may be NULL on upload (zero velocities). */
+int lagpm_box_upload(lagpm_ctx* ctx, lagpm_box* box,
+                     const double* x, const double* y, const double* z,
+                     const double* vx, const double* vy, const double* vz);
+int lagpm_box_download(lagpm_ctx* ctx, lagpm_box* box,
+                       double* x, double* y, double* z,
+                       double* vx, double* vy, double* vz);
+/* Upload from the GOTPM+XYZDBL particle layout, eunha.h:372-400 (32 bytes: u64 index|flag,
+ * f32 dx,dy,dz displacement from lattice site, f32 vx,vy,vz). The lattice index uses the
+ * low 56 bits, x fastest: idx = i + nx*(j + nx*k); XofP = fmod(dx + i + nx, nx). */
+int lagpm_box_upload_eunha(lagpm_ctx* ctx, lagpm_box* box, const void* dm_bp, size_t np, long nx);
+/* Download velocities back into the same 32-byte layout (positions untouched). */
+int lagpm_box_download_eunha_vel(lagpm_ctx* ctx, lagpm_box* box, void* dm_bp, size_t np);
+
+/* ---- PM long-range force (pmmain / TSC / psolver / fda4) -------------------------- */
+/* Full PM kick: deposit -> r2c -> Green -> c2r -> FDA4 gradient -> TSC interpolation,
+ * v += f * F_PM with f = fact2 (KICK), fact2_pull (PULL), fact2_push/GetFact1(a,a+da/2)
+ * (PUSH; the caller passes that quotient in fact2_push). Cosmos/pmmain.c:390-527. */
+int lagpm_pmmain(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef, int pmstatus);
+/* TSC mass deposit with periodic fold, minus 1 per cell (pmmain.c:304). Result stays
+ * resident; den (n^3 floats, x fastest) is filled when not NULL. Cosmos/pmmain.h:3-78. */
+int lagpm_tsc(lagpm_ctx* ctx, lagpm_box* box, float* den);
+/* Raw int64 fixed-point TSC accumulator (n^3, x fastest) and its scale exponent:
+ * density = acc * 2^-scale_bits (no -1 shift). For bit-exact parity tests. */
+int lagpm_tsc_fixed(lagpm_ctx* ctx, lagpm_box* box, int64_t* acc, int* scale_bits);
+/* r2c of the resident density, Green multiply (psolver.c:13-86), c2r. phi (n^3) optional. */
+int lagpm_psolver(lagpm_ctx* ctx, lagpm_box* box, float* phi);
+/* Green multiply alone on a caller spectrum (n*n*(n/2+1) complex float, x fastest),
+ * in place, exactly as Cosmos/psolver.c:49-76 (FP64 factor, FP32 storage). */
+int lagpm_green_apply(lagpm_ctx* ctx, int n, double efold, int deconvolve, float* spec_host);
+/* FDA4 gradient + TSC interpolation + kick from the resident potential, fda4.c:31-153. */
+int lagpm_fda4(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef, int pmstatus);
+/* Replace the resident potential with a caller mesh (n^3 floats), for stage tests. */
+int lagpm_set_potential(lagpm_ctx* ctx, lagpm_box* box, const float* phi);
+/* Download the gradient mesh (n^3 x 4 floats: fx, fy, fz, phi). */
+int lagpm_get_gradient(lagpm_ctx* ctx, lagpm_box* box, float* grad4);
+
+/* ---- short-range tree correction --------------------------------------------------- */
+/* Builds FP64 force tables (NSPLINE=4096 entries over r in [0,6)): diff[i][3], slope[i][3]
+ * exactly as OST/Model4TreeForce.c:48-91 (law PMFIT) or for the erfc law. Host only. */
+#define LAGPM_NSPLINE 4096
+int lagpm_force_spline(const lagpm_params* p, double* diff /*[4096*3]*/, double* slope /*[4096*3]*/);
+/* Tree correction kick v += fact2 * a_short (GOTPM: the tree uses EvolFactArr[0].fact2,
+ * Cosmos/treecorrection.c:227). Sorts, builds the octree, walks. */
+int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef);
+int lagpm_tree_stats_get(const lagpm_box* box, lagpm_tree_stats* out);
+/* Waits for the box's last recorded phase and fills the timings. */
+int lagpm_box_timings(lagpm_ctx* ctx, lagpm_box* box, lagpm_timings* out);
+
+/* ---- integrator pieces ----------------------------------------------------------- */
+/* Kick factors in FP64 quadrature (Cosmos/CosmosEvolFactor.c:230-256). */
+int lagpm_evolfactor(const lagpm_params* p, double anow, double astep, lagpm_evolfact* out);
+/* v *= fact1 (Cosmos/globalT.c:102-110). */
+int lagpm_hubble_drag(lagpm_ctx* ctx, lagpm_box* box, double fact1);
+/* x += pfact * v, wrapped into [0,n) (Cosmos/stepevolv.c:19-113). */
+int lagpm_drift(lagpm_ctx* ctx, lagpm_box* box, double pfact);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LAGPM_H */

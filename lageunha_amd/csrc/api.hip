@@ -1,0 +1,695 @@
+// api.hip — the C ABI of include/lagpm.h: contexts, boxes, uploads, the drop-in entry points,
+// force tables, kick factors, drift and drag.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "lagpm_internal.h"
+
+size_t lagpm_walk_stack_entries(int num_cu);
+
+int lagpm_fail(lagpm_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+namespace {
+
+std::once_flag g_rocfft_once;
+
+__global__ void wrap_kernel(double* __restrict__ a, size_t np, double L) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double v = fmod(a[i], L);
+    if (v < 0) v += L;
+    if (v >= L) v = 0.0;
+    a[i] = v;
+}
+__global__ void iota_kernel(uint32_t* __restrict__ a, size_t np) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) a[i] = (uint32_t)i;
+}
+__global__ void scatter_kernel(const double* __restrict__ src, const uint32_t* __restrict__ id, size_t np,
+                               double* __restrict__ dst) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) dst[id[i]] = src[i];
+}
+__global__ void scale3_kernel(double* __restrict__ a, double* __restrict__ b, double* __restrict__ c, size_t np, double f) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    a[i] *= f;
+    b[i] *= f;
+    c[i] *= f;
+}
+__global__ void drift_kernel(double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+                             const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
+                             size_t np, double pfact, double L) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    double p[3] = {x[i] + pfact * vx[i], y[i] + pfact * vy[i], z[i] + pfact * vz[i]};
+    for (int a = 0; a < 3; a++) {
+        double v = fmod(p[a], L);
+        if (v < 0) v += L;
+        if (v >= L) v = 0.0;
+        p[a] = v;
+    }
+    x[i] = p[0];
+    y[i] = p[1];
+    z[i] = p[2];
+}
+
+unsigned nblocks(size_t n, int t) { return (unsigned)std::max<size_t>(1, (n + t - 1) / t); }
+
+template <typename T>
+int dalloc(lagpm_ctx* ctx, lagpm_box* box, T** p, size_t count) {
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return lagpm_fail(ctx, LAGPM_E_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    box->bytes += bytes;
+    return LAGPM_OK;
+}
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+int fixed_scale_bits(double total_mass) {
+    int e = 0;
+    if (total_mass > 0) e = (int)std::ceil(std::log2(total_mass));
+    int s = 61 - e;
+    return std::min(40, std::max(8, s));
+}
+
+void free_box(lagpm_box* b) {
+    dfree(b->x); dfree(b->y); dfree(b->z); dfree(b->vx); dfree(b->vy); dfree(b->vz); dfree(b->id);
+    dfree(b->key); dfree(b->key2); dfree(b->perm); dfree(b->perm2); dfree(b->dtmp); dfree(b->utmp);
+    dfree(b->acc); dfree(b->rho); dfree(b->spec); dfree(b->phi); dfree(b->grad);
+    dfree(b->pp); dfree(b->ncount); dfree(b->noff); dfree(b->nd_info); dfree(b->nd_parent); dfree(b->nd_child);
+    dfree(b->nd_com); dfree(b->nd_q0); dfree(b->nd_q1); dfree(b->nd_d); dfree(b->cellroot); dfree(b->gflag);
+    dfree(b->goff); dfree(b->groups); dfree(b->gbox); dfree(b->pgroup); dfree(b->walk_stack); dfree(b->counters);
+    dfree(b->tab0); dfree(b->tab12); dfree(b->tabd);
+}
+
+int check_params(lagpm_ctx* ctx, const lagpm_params* p) {
+    if (!p) return lagpm_fail(ctx, LAGPM_E_INVAL, "params is NULL");
+    if (p->n < 8 || p->n > 4096) return lagpm_fail(ctx, LAGPM_E_INVAL, "n must be in [8, 4096]");
+    if (p->law != LAGPM_LAW_PMFIT && p->law != LAGPM_LAW_ERFC) return lagpm_fail(ctx, LAGPM_E_INVAL, "unknown force law");
+    if (p->precision != LAGPM_PREC_FP32 && p->precision != LAGPM_PREC_FP64)
+        return lagpm_fail(ctx, LAGPM_E_INVAL, "unknown precision");
+    if (!(p->mass > 0)) return lagpm_fail(ctx, LAGPM_E_INVAL, "mass must be positive");
+    if (!(p->efold >= 0)) return lagpm_fail(ctx, LAGPM_E_INVAL, "efold must be >= 0");
+    if (!(p->rsphere > 0 && p->rsphere < 6.0)) return lagpm_fail(ctx, LAGPM_E_INVAL, "rsphere must be in (0, 6) (force table range)");
+    if (!(p->theta >= 0)) return lagpm_fail(ctx, LAGPM_E_INVAL, "theta must be >= 0");
+    if (!(p->epsilon >= 0)) return lagpm_fail(ctx, LAGPM_E_INVAL, "epsilon must be >= 0");
+    return LAGPM_OK;
+}
+
+// ---- FP64 quadrature for the kick factors: composite 16-point Gauss-Legendre ----
+const double GL_X[8] = {0.0950125098376374, 0.2816035507792589, 0.4580167776572274, 0.6178762444026438,
+                        0.7554044083550030, 0.8656312023878318, 0.9445750230732326, 0.9894009349916499};
+const double GL_W[8] = {0.1894506104550685, 0.1826034150449236, 0.1691565193950025, 0.1495959888165767,
+                        0.1246289712555339, 0.0951585116824928, 0.0622535239386479, 0.0271524594117541};
+struct cosmo { double om, ol, amax; };
+double hofez(const cosmo& c, double a) {
+    double zp1 = c.amax / a;
+    return std::sqrt(zp1 * zp1 * zp1 * c.om + zp1 * zp1 * (1 - c.om - c.ol) + c.ol);
+}
+double apf(const cosmo& c, double a) {
+    double zp1 = c.amax / a, h = hofez(c, a);
+    return std::sqrt(8.0 * M_PI / 3.0 / (c.amax * c.om) / (zp1 * zp1) * h * h);
+}
+double integ(const cosmo& c, double a1, double a2) {
+    const int nsub = 32;
+    double s = 0, hstep = (a2 - a1) / nsub;
+    for (int k = 0; k < nsub; k++) {
+        double lo = a1 + k * hstep, mid = lo + 0.5 * hstep, half = 0.5 * hstep;
+        for (int q = 0; q < 8; q++) {
+            double u1 = mid - half * GL_X[q], u2 = mid + half * GL_X[q];
+            s += GL_W[q] * half * (1.0 / (u1 * apf(c, u1)) + 1.0 / (u2 * apf(c, u2)));
+        }
+    }
+    return s;
+}
+double fact1_of(const cosmo& c, double a1, double a2) {
+    return (a1 * a1 * a1 * hofez(c, a1)) / (a2 * a2 * a2 * hofez(c, a2));
+}
+double fact2_of(const cosmo& c, int n, double a1, double a2) {
+    return 1.0 / (a2 * a2 * apf(c, a2)) * integ(c, a1, a2) * n;
+}
+
+// ---- force-law tables (FP64), OST/Model4TreeForce.c:48-116 ----
+const double PMFITPAR[8] = {1.0 / 256.0 / 256.0, 0.78224e+00, 0.37971e-06, -0.60338e+00,
+                            -0.34419e-07, -0.39741e+01, -0.10607e+01, -0.38145e+00};
+double law_pmfit(int n, double eps, double r) {
+    const double* a = PMFITPAR;
+    double ch = std::cosh(a[1] * r);
+    double newton = r / std::pow(r * r + eps * eps, 1.5);
+    double fit = 1.0 / (r * r) * std::tanh(a[1] * r) - a[1] / (ch * ch * r) - 2.0 * a[2] / a[0] * r * std::exp(a[3] * r * r) -
+                 2.0 * a[2] * a[3] / a[0] * r * r * r * std::exp(a[3] * r * r) +
+                 a[4] / a[0] * (1.0 + a[5] * r * r + a[6] * r * r * r * r) * std::exp(a[7] * r * r);
+    return (newton - fit) / ((double)n * n);
+}
+double law_erfc(int n, double eps, double rs, double r) {
+    double newton = r / std::pow(r * r + eps * eps, 1.5);
+    double u = r / (2.0 * rs);
+    double flong = (std::erf(u) - (r / (rs * std::sqrt(M_PI))) * std::exp(-u * u)) / (r * r);
+    return (newton - flong) / ((double)n * n);
+}
+
+}  // namespace
+
+void lagpm_build_tables(const lagpm_params* p, double* diff, double* slope) {
+    const int NS = LAGPM_NSPLINE;
+    const double ran = 6.0 / (double)NS, dr = 0.1 * ran;
+    const double rs = 0.5 * p->efold;
+    auto law = [&](double r) { return p->law == LAGPM_LAW_PMFIT ? law_pmfit(p->n, p->epsilon, r) : law_erfc(p->n, p->epsilon, rs, r); };
+    std::memset(diff, 0, 3 * NS * sizeof(double));
+    std::memset(slope, 0, 3 * NS * sizeof(double));
+    for (int i = 1; i < NS; i++) {
+        double r = i * ran;
+        double fp = (law(r + dr) - law(r - dr)) / (dr + dr);
+        double fpp = (law(r + 2 * dr) - 2 * law(r) + law(r - 2 * dr)) / (4 * dr * dr);
+        double fr = law(r);
+        diff[3 * i + 0] = -(fr / r);
+        diff[3 * i + 1] = -(0.5 * (fp * r - fr) / r / r / r);
+        diff[3 * i + 2] = -(0.5 * (fpp * r * r - 3 * fp * r + 3 * fr) / r / r / r / r / r);
+    }
+    for (int i = 1; i < NS - 1; i++)
+        for (int c = 0; c < 3; c++) slope[3 * i + c] = (diff[3 * (i + 1) + c] - diff[3 * i + c]) / ran;
+    for (int c = 0; c < 3; c++) {
+        slope[c] = slope[3 + c];
+        diff[c] = diff[3 + c];
+        diff[3 * (NS - 1) + c] = 0;
+        slope[3 * (NS - 1) + c] = 0;
+    }
+}
+
+int lagpm_tables_upload(lagpm_ctx* ctx, lagpm_box* box) {
+    const int NS = LAGPM_NSPLINE;
+    std::vector<double> diff(3 * NS), slope(3 * NS);
+    lagpm_build_tables(&box->p, diff.data(), slope.data());
+    if (box->p.precision == LAGPM_PREC_FP64) {
+        std::vector<double> t(6 * NS);
+        for (int i = 0; i < NS; i++)
+            for (int c = 0; c < 3; c++) {
+                t[6 * i + 2 * c] = diff[3 * i + c];
+                t[6 * i + 2 * c + 1] = slope[3 * i + c];
+            }
+        HIPCHK(ctx, hipMemcpyAsync(box->tabd, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        std::vector<float2> t0(NS);
+        std::vector<float4> t12(NS);
+        for (int i = 0; i < NS; i++) {
+            t0[i] = make_float2((float)diff[3 * i], (float)slope[3 * i]);
+            t12[i] = make_float4((float)diff[3 * i + 1], (float)slope[3 * i + 1], (float)diff[3 * i + 2], (float)slope[3 * i + 2]);
+        }
+        HIPCHK(ctx, hipMemcpyAsync(box->tab0, t0.data(), NS * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(box->tab12, t12.data(), NS * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    box->tables_ready = true;
+    return LAGPM_OK;
+}
+
+extern "C" {
+
+int lagpm_abi_version(void) { return LAGPM_ABI_VERSION; }
+
+int lagpm_create(lagpm_ctx** out, int device) {
+    if (!out) return LAGPM_E_INVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return LAGPM_E_HIP;
+    if (device < 0 || device >= ndev) return LAGPM_E_INVAL;
+    if (hipSetDevice(device) != hipSuccess) return LAGPM_E_HIP;
+    lagpm_ctx* ctx = new lagpm_ctx();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete ctx;
+        return LAGPM_E_HIP;
+    }
+    ctx->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return LAGPM_E_HIP;
+    }
+    for (int i = 0; i < 8; i++) (void)hipEventCreate(&ctx->ev[i]);
+    std::call_once(g_rocfft_once, []() { rocfft_setup(); });
+    *out = ctx;
+    return LAGPM_OK;
+}
+
+int lagpm_destroy(lagpm_ctx* ctx) {
+    if (!ctx) return LAGPM_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    lagpm_fft_release(ctx);
+    if (ctx->scan_tmp) (void)hipFree(ctx->scan_tmp);
+    for (int i = 0; i < 8; i++)
+        if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LAGPM_OK;
+}
+
+const char* lagpm_last_error(const lagpm_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+void* lagpm_stream(lagpm_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+int lagpm_synchronize(lagpm_ctx* ctx) {
+    if (!ctx) return LAGPM_E_INVAL;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return LAGPM_OK;
+}
+
+size_t lagpm_estimate_box_bytes(const lagpm_params* p, size_t np) {
+    if (!p) return 0;
+    const size_t n = (size_t)p->n, m = n * n * n, K = (n / 2 + 1) * n * n;
+    const size_t real = p->precision == LAGPM_PREC_FP64 ? 8 : 4;
+    size_t b = np * (7 * 8 + 4 + 2 * 8 + 2 * 4 + 4 + 2 * 4);  // particles, keys, perm, tmp, node counts
+    b += m * 8 + m * real * 2 + K * 2 * real + m * 4 * real;   // acc, rho, phi, spec, grad
+    if (p->precision == LAGPM_PREC_FP32) b += np * 16;
+    else b += np * 4;
+    b += (np / 4 + 1024) * (16 + 4 + 32 + 8 + (p->precision == LAGPM_PREC_FP64 ? 96 : 48));  // nodes (estimate)
+    b += (np / 8 + 1024) * 8;                                                                   // groups
+    return b;
+}
+
+int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box** out) {
+    if (!ctx || !out) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    *out = nullptr;
+    RCHK(check_params(ctx, p));
+    if (np >= (size_t)LAGPM_MAX_INDEX) return lagpm_fail(ctx, LAGPM_E_INVAL, "np must be below 2^27");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    lagpm_box* box = new lagpm_box();
+    box->p = *p;
+    box->np = np;
+    const size_t n = (size_t)p->n, m = n * n * n, K = (n / 2 + 1) * n * n;
+    const bool f64 = p->precision == LAGPM_PREC_FP64;
+    const size_t real = f64 ? 8 : 4;
+    box->level_cell = lagpm_choose_cell_level(p->n, p->rsphere);
+    box->mcell = 1 << box->level_cell;
+    box->scale_bits = fixed_scale_bits((double)np * p->mass);
+    int rc = LAGPM_OK;
+#define ALLOC(ptr, cnt)                                                 \
+    do {                                                                \
+        if (rc == LAGPM_OK) rc = dalloc(ctx, box, &(ptr), (cnt));       \
+    } while (0)
+    ALLOC(box->x, np); ALLOC(box->y, np); ALLOC(box->z, np);
+    ALLOC(box->vx, np); ALLOC(box->vy, np); ALLOC(box->vz, np);
+    ALLOC(box->id, np); ALLOC(box->key, np); ALLOC(box->key2, np); ALLOC(box->perm, np); ALLOC(box->perm2, np);
+    ALLOC(box->dtmp, np); ALLOC(box->utmp, np);
+    ALLOC(box->acc, m);
+    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->rho, m * real);
+    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->spec, K * 2 * real);
+    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->phi, m * real);
+    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->grad, m * 4 * real);
+    ALLOC(box->ncount, np); ALLOC(box->noff, np);
+    ALLOC(box->cellroot, (size_t)box->mcell * box->mcell * box->mcell);
+    ALLOC(box->counters, 4);
+    if (f64) {
+        ALLOC(box->tabd, 6 * LAGPM_NSPLINE);
+        ALLOC(box->pgroup, np);
+    } else {
+        ALLOC(box->pp, np);
+        ALLOC(box->tab0, LAGPM_NSPLINE);
+        ALLOC(box->tab12, LAGPM_NSPLINE);
+        box->walk_stack_entries = lagpm_walk_stack_entries(ctx->num_cu);
+        ALLOC(box->walk_stack, box->walk_stack_entries);
+    }
+#undef ALLOC
+    if (rc != LAGPM_OK) {
+        free_box(box);
+        delete box;
+        return rc;
+    }
+    for (int i = 0; i < 12; i++) (void)hipEventCreate(&box->ev[i]);
+    (void)hipMemsetAsync(box->vx, 0, np * sizeof(double), ctx->stream);
+    (void)hipMemsetAsync(box->vy, 0, np * sizeof(double), ctx->stream);
+    (void)hipMemsetAsync(box->vz, 0, np * sizeof(double), ctx->stream);
+    *out = box;
+    return LAGPM_OK;
+}
+
+int lagpm_box_destroy(lagpm_ctx* ctx, lagpm_box* box) {
+    if (!box) return LAGPM_OK;
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 12; i++)
+        if (box->ev[i]) (void)hipEventDestroy(box->ev[i]);
+    free_box(box);
+    delete box;
+    return LAGPM_OK;
+}
+
+int lagpm_box_params(const lagpm_box* box, lagpm_params* out) {
+    if (!box || !out) return LAGPM_E_INVAL;
+    *out = box->p;
+    return LAGPM_OK;
+}
+size_t lagpm_box_np(const lagpm_box* box) { return box ? box->np : 0; }
+size_t lagpm_box_device_bytes(const lagpm_box* box) { return box ? box->bytes : 0; }
+
+int lagpm_box_upload(lagpm_ctx* ctx, lagpm_box* box, const double* x, const double* y, const double* z, const double* vx,
+                     const double* vy, const double* vz) {
+    if (!ctx || !box || (box->np && (!x || !y || !z))) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    const size_t np = box->np, b = np * sizeof(double);
+    if (np == 0) {
+        box->uploaded = true;
+        return LAGPM_OK;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(box->x, x, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(box->y, y, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(box->z, z, b, hipMemcpyHostToDevice, ctx->stream));
+    double* vs[3] = {box->vx, box->vy, box->vz};
+    const double* hs[3] = {vx, vy, vz};
+    for (int a = 0; a < 3; a++) {
+        if (hs[a]) HIPCHK(ctx, hipMemcpyAsync(vs[a], hs[a], b, hipMemcpyHostToDevice, ctx->stream));
+        else HIPCHK(ctx, hipMemsetAsync(vs[a], 0, b, ctx->stream));
+    }
+    double* ps[3] = {box->x, box->y, box->z};
+    for (int a = 0; a < 3; a++)
+        hipLaunchKernelGGL(wrap_kernel, dim3(nblocks(np, 256)), dim3(256), 0, ctx->stream, ps[a], np, (double)box->p.n);
+    hipLaunchKernelGGL(iota_kernel, dim3(nblocks(np, 256)), dim3(256), 0, ctx->stream, box->id, np);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    box->uploaded = true;
+    box->sorted = false;
+    return LAGPM_OK;
+}
+
+int lagpm_box_download(lagpm_ctx* ctx, lagpm_box* box, double* x, double* y, double* z, double* vx, double* vy, double* vz) {
+    if (!ctx || !box) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    const size_t np = box->np;
+    double* src[6] = {box->x, box->y, box->z, box->vx, box->vy, box->vz};
+    double* dst[6] = {x, y, z, vx, vy, vz};
+    for (int a = 0; a < 6; a++) {
+        if (!dst[a] || np == 0) continue;
+        hipLaunchKernelGGL(scatter_kernel, dim3(nblocks(np, 256)), dim3(256), 0, ctx->stream, src[a], box->id, np, box->dtmp);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(dst[a], box->dtmp, np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return LAGPM_OK;
+}
+
+// GOTPM + XYZDBL particle: u64 index|flag (flag in the top byte), f32 dx,dy,dz, f32 vx,vy,vz.
+struct eunha_dm {
+    uint64_t u4if;
+    float x, y, z, vx, vy, vz;
+};
+static_assert(sizeof(eunha_dm) == 32, "GOTPM particle is 32 bytes");
+
+int lagpm_box_upload_eunha(lagpm_ctx* ctx, lagpm_box* box, const void* dm_bp, size_t np, long nx) {
+    if (!ctx || !box || (np && !dm_bp)) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    if (np != box->np) return lagpm_fail(ctx, LAGPM_E_INVAL, "particle count differs from the box");
+    if (nx != box->p.n) return lagpm_fail(ctx, LAGPM_E_INVAL, "nx differs from the box mesh size");
+    const eunha_dm* p = (const eunha_dm*)dm_bp;
+    std::vector<double> a(6 * np);
+    const double L = (double)nx;
+    const uint64_t nxny = (uint64_t)nx * (uint64_t)nx;
+    for (size_t i = 0; i < np; i++) {
+        uint64_t idx = (p[i].u4if << 8) >> 8;  // PINDX, eunha.h:1212
+        double ii = (double)(idx % (uint64_t)nx), jj = (double)((idx % nxny) / (uint64_t)nx), kk = (double)(idx / nxny);
+        a[i] = std::fmod((double)p[i].x + ii + L, L);
+        a[np + i] = std::fmod((double)p[i].y + jj + L, L);
+        a[2 * np + i] = std::fmod((double)p[i].z + kk + L, L);
+        a[3 * np + i] = p[i].vx;
+        a[4 * np + i] = p[i].vy;
+        a[5 * np + i] = p[i].vz;
+    }
+    return lagpm_box_upload(ctx, box, a.data(), a.data() + np, a.data() + 2 * np, a.data() + 3 * np, a.data() + 4 * np,
+                            a.data() + 5 * np);
+}
+
+int lagpm_box_download_eunha_vel(lagpm_ctx* ctx, lagpm_box* box, void* dm_bp, size_t np) {
+    if (!ctx || !box || (np && !dm_bp)) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    if (np != box->np) return lagpm_fail(ctx, LAGPM_E_INVAL, "particle count differs from the box");
+    std::vector<double> v(3 * np);
+    RCHK(lagpm_box_download(ctx, box, nullptr, nullptr, nullptr, v.data(), v.data() + np, v.data() + 2 * np));
+    eunha_dm* p = (eunha_dm*)dm_bp;
+    for (size_t i = 0; i < np; i++) {
+        p[i].vx = (float)v[i];
+        p[i].vy = (float)v[np + i];
+        p[i].vz = (float)v[2 * np + i];
+    }
+    return LAGPM_OK;
+}
+
+static int pm_factor(lagpm_ctx* ctx, const lagpm_evolfact* ef, int pmstatus, double* f) {
+    if (!ef) return lagpm_fail(ctx, LAGPM_E_INVAL, "evolfact is NULL");
+    switch (pmstatus) {
+        case LAGPM_KICK: *f = ef->fact2; break;
+        case LAGPM_PULL: *f = ef->fact2_pull; break;
+        case LAGPM_PUSH:
+            if (ef->fact1_push == 0.0f) return lagpm_fail(ctx, LAGPM_E_INVAL, "fact1_push is zero");
+            *f = (double)(ef->fact2_push / ef->fact1_push);  // fda4.c:135-141, GetFact1(a, a+da/2) == fact1_push
+            break;
+        default: return lagpm_fail(ctx, LAGPM_E_INVAL, "unknown pmstatus");
+    }
+    return LAGPM_OK;
+}
+
+static int need_particles(lagpm_ctx* ctx, lagpm_box* box) {
+    if (!ctx || !box) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    if (!box->uploaded) return lagpm_fail(ctx, LAGPM_E_STATE, "no particles uploaded");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    return LAGPM_OK;
+}
+
+int lagpm_tsc(lagpm_ctx* ctx, lagpm_box* box, float* den) {
+    RCHK(need_particles(ctx, box));
+    if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
+    RCHK(lagpm_pm_deposit(ctx, box));
+    if (den) {
+        const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
+        if (box->p.precision == LAGPM_PREC_FP64) {
+            std::vector<double> t(m);
+            HIPCHK(ctx, hipMemcpyAsync(t.data(), box->rho, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            for (size_t i = 0; i < m; i++) den[i] = (float)t[i];
+        } else {
+            HIPCHK(ctx, hipMemcpyAsync(den, box->rho, m * 4, hipMemcpyDeviceToHost, ctx->stream));
+        }
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return LAGPM_OK;
+}
+
+int lagpm_tsc_fixed(lagpm_ctx* ctx, lagpm_box* box, int64_t* acc, int* scale_bits) {
+    RCHK(need_particles(ctx, box));
+    if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
+    RCHK(lagpm_pm_deposit(ctx, box));
+    const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
+    if (acc) HIPCHK(ctx, hipMemcpyAsync(acc, box->acc, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (scale_bits) *scale_bits = box->scale_bits;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return LAGPM_OK;
+}
+
+int lagpm_psolver(lagpm_ctx* ctx, lagpm_box* box, float* phi) {
+    RCHK(need_particles(ctx, box));
+    RCHK(lagpm_pm_solve(ctx, box));
+    if (phi) {
+        const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
+        if (box->p.precision == LAGPM_PREC_FP64) {
+            std::vector<double> t(m);
+            HIPCHK(ctx, hipMemcpyAsync(t.data(), box->phi, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            for (size_t i = 0; i < m; i++) phi[i] = (float)t[i];
+        } else {
+            HIPCHK(ctx, hipMemcpyAsync(phi, box->phi, m * 4, hipMemcpyDeviceToHost, ctx->stream));
+        }
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return LAGPM_OK;
+}
+
+int lagpm_green_apply(lagpm_ctx* ctx, int n, double efold, int deconvolve, float* spec_host) {
+    if (!ctx || !spec_host || n < 2) return lagpm_fail(ctx, LAGPM_E_INVAL, "bad argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t K = (size_t)(n / 2 + 1) * n * n;
+    void* d = nullptr;
+    HIPCHK(ctx, hipMalloc(&d, K * 2 * sizeof(float)));
+    int rc = LAGPM_OK;
+    if (hipMemcpyAsync(d, spec_host, K * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
+    if (rc == LAGPM_OK) rc = lagpm_green_device(ctx, n, LAGPM_PREC_FP32, efold, deconvolve, d);
+    if (rc == LAGPM_OK && hipMemcpyAsync(spec_host, d, K * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (rc == LAGPM_E_HIP) return lagpm_fail(ctx, rc, "green_apply copy failed");
+    return rc;
+}
+
+int lagpm_fda4(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef, int pmstatus) {
+    RCHK(need_particles(ctx, box));
+    double f;
+    RCHK(pm_factor(ctx, ef, pmstatus, &f));
+    RCHK(lagpm_pm_gradient(ctx, box));
+    RCHK(lagpm_pm_interp(ctx, box, f));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return LAGPM_OK;
+}
+
+int lagpm_set_potential(lagpm_ctx* ctx, lagpm_box* box, const float* phi) {
+    if (!ctx || !box || !phi) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
+    if (box->p.precision == LAGPM_PREC_FP64) {
+        std::vector<double> t(phi, phi + m);
+        HIPCHK(ctx, hipMemcpyAsync(box->phi, t.data(), m * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+        HIPCHK(ctx, hipMemcpyAsync(box->phi, phi, m * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return LAGPM_OK;
+}
+
+int lagpm_get_gradient(lagpm_ctx* ctx, lagpm_box* box, float* grad4) {
+    if (!ctx || !box || !grad4) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
+    if (box->p.precision == LAGPM_PREC_FP64) {
+        std::vector<double> t(4 * m);
+        HIPCHK(ctx, hipMemcpyAsync(t.data(), box->grad, 4 * m * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < 4 * m; i++) grad4[i] = (float)t[i];
+    } else {
+        HIPCHK(ctx, hipMemcpyAsync(grad4, box->grad, 4 * m * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return LAGPM_OK;
+}
+
+int lagpm_pmmain(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef, int pmstatus) {
+    RCHK(need_particles(ctx, box));
+    double f;
+    RCHK(pm_factor(ctx, ef, pmstatus, &f));
+    hipEvent_t* ev = box->ev;
+    HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
+    if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
+    HIPCHK(ctx, hipEventRecord(ev[1], ctx->stream));
+    RCHK(lagpm_pm_deposit(ctx, box));
+    HIPCHK(ctx, hipEventRecord(ev[2], ctx->stream));
+    RCHK(lagpm_pm_solve(ctx, box, ev + 3));
+    RCHK(lagpm_pm_gradient(ctx, box));
+    HIPCHK(ctx, hipEventRecord(ev[6], ctx->stream));
+    RCHK(lagpm_pm_interp(ctx, box, f));
+    HIPCHK(ctx, hipEventRecord(ev[7], ctx->stream));
+    box->pm_timed = true;
+    return LAGPM_OK;
+}
+
+int lagpm_force_spline(const lagpm_params* p, double* diff, double* slope) {
+    if (!p || !diff || !slope) return LAGPM_E_INVAL;
+    if (check_params(nullptr, p) != LAGPM_OK) return LAGPM_E_INVAL;
+    lagpm_build_tables(p, diff, slope);
+    return LAGPM_OK;
+}
+
+int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef) {
+    RCHK(need_particles(ctx, box));
+    if (!ef) return lagpm_fail(ctx, LAGPM_E_INVAL, "evolfact is NULL");
+    if (box->level_cell < 2)
+        return lagpm_fail(ctx, LAGPM_E_INVAL, "box too small for the cutoff: need n >= 4 * rsphere");
+    if (box->np == 0) return LAGPM_OK;
+    if (!box->tables_ready) RCHK(lagpm_tables_upload(ctx, box));
+    hipEvent_t* ev = box->ev + 8;
+    HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
+    if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
+    HIPCHK(ctx, hipEventRecord(ev[1], ctx->stream));
+    RCHK(lagpm_tree_build(ctx, box));
+    if (box->nnodes >= (size_t)(1u << 26)) return lagpm_fail(ctx, LAGPM_E_OVERFLOW, "more than 2^26 tree nodes");
+    HIPCHK(ctx, hipEventRecord(ev[2], ctx->stream));
+    RCHK(lagpm_tree_walk(ctx, box, (double)ef->fact2));
+    HIPCHK(ctx, hipEventRecord(ev[3], ctx->stream));
+    unsigned long long c[4];
+    HIPCHK(ctx, hipMemcpyAsync(c, box->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float t01 = 0, t12 = 0, t23 = 0;
+    (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
+    (void)hipEventElapsedTime(&t23, ev[2], ev[3]);
+    box->tree_timed = true;
+    box->stats.n_pp = c[0];
+    box->stats.n_pc = c[1];
+    box->stats.ms_build = t01 + t12;
+    box->stats.ms_walk = t23;
+    if (c[3]) return lagpm_fail(ctx, LAGPM_E_OVERFLOW, "tree walk stack overflow in " + std::to_string(c[3]) + " groups");
+    return LAGPM_OK;
+}
+
+int lagpm_tree_stats_get(const lagpm_box* box, lagpm_tree_stats* out) {
+    if (!box || !out) return LAGPM_E_INVAL;
+    *out = box->stats;
+    return LAGPM_OK;
+}
+
+int lagpm_box_timings(lagpm_ctx* ctx, lagpm_box* box, lagpm_timings* out) {
+    if (!ctx || !box || !out) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    std::memset(out, 0, sizeof(*out));
+    auto el = [&](int a, int b) {
+        float t = 0;
+        (void)hipEventSynchronize(box->ev[b]);
+        (void)hipEventElapsedTime(&t, box->ev[a], box->ev[b]);
+        return (double)t;
+    };
+    if (box->pm_timed) {
+        out->ms_sort = el(0, 1);
+        out->ms_tsc = el(1, 2);
+        out->ms_fft_r2c = el(2, 3);
+        out->ms_green = el(3, 4);
+        out->ms_fft_c2r = el(4, 5);
+        out->ms_grad = el(5, 6);
+        out->ms_interp = el(6, 7);
+    }
+    if (box->tree_timed) {
+        out->ms_tree_sort = el(8, 9);
+        out->ms_build = el(9, 10);
+        out->ms_walk = el(10, 11);
+    }
+    return LAGPM_OK;
+}
+
+int lagpm_evolfactor(const lagpm_params* p, double anow, double astep, lagpm_evolfact* out) {
+    if (!p || !out || !(anow > 0) || !(p->omep > 0) || !(p->amax > 0)) return LAGPM_E_INVAL;
+    if (!(anow - 0.5 * astep > 0)) return LAGPM_E_INVAL;
+    cosmo c{p->omep, p->omeplam, p->amax};
+    const int n = p->n;
+    out->afact = out->bfact = 0;
+    out->fact1 = (float)fact1_of(c, anow - 0.5 * astep, anow + 0.5 * astep);
+    out->fact2 = (float)fact2_of(c, n, anow - 0.5 * astep, anow + 0.5 * astep);
+    out->pfact = (float)(astep * n);
+    out->fact1_push = (float)fact1_of(c, anow, anow + 0.5 * astep);
+    out->fact1_pull = (float)fact1_of(c, anow - 0.5 * astep, anow);
+    out->fact2_push = (float)fact2_of(c, n, anow, anow + 0.5 * astep);
+    out->fact2_pull = (float)fact2_of(c, n, anow - 0.5 * astep, anow);
+    return LAGPM_OK;
+}
+
+int lagpm_hubble_drag(lagpm_ctx* ctx, lagpm_box* box, double fact1) {
+    RCHK(need_particles(ctx, box));
+    if (box->np == 0) return LAGPM_OK;
+    hipLaunchKernelGGL(scale3_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, ctx->stream, box->vx, box->vy, box->vz,
+                       box->np, fact1);
+    HIPCHK(ctx, hipGetLastError());
+    return LAGPM_OK;
+}
+
+int lagpm_drift(lagpm_ctx* ctx, lagpm_box* box, double pfact) {
+    RCHK(need_particles(ctx, box));
+    if (box->np == 0) return LAGPM_OK;
+    hipLaunchKernelGGL(drift_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, ctx->stream, box->x, box->y, box->z, box->vx,
+                       box->vy, box->vz, box->np, pfact, (double)box->p.n);
+    HIPCHK(ctx, hipGetLastError());
+    box->sorted = false;
+    return LAGPM_OK;
+}
+
+}  // extern "C"

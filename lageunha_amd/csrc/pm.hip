@@ -1,0 +1,484 @@
+// pm.hip — PM long-range force: TSC deposit, rocFFT Poisson solve with the Gaussian-filtered
+// Green function, FDA4 gradient mesh, TSC interpolation and kick.
+//
+// Reference semantics (paths relative to the reference tree):
+//   deposit       Cosmos/pmmain.h:3-78 (Get_TSC_Den) + periodic fold RkS/gmigrate.c:212-275
+//   mean shift    Cosmos/pmmain.c:304 (grid initialised to -1 before the fold)
+//   Green         Cosmos/psolver.c:13-86
+//   gradient      Cosmos/fda4.c:46-47, Cosmos/fda4.h:302-384 (FDA4_SLOW)
+//   interpolation Cosmos/fda4.h:11-102 (FDA4_HEAD weights), :276-300 (FDA4_TAIL sum + kick)
+//
+// This file is compiled with -ffp-contract=off: every float statement rounds once, exactly
+// as the reference's C statements do, so the FP32 path is bit-comparable with the oracle.
+//
+// Deposit design: particles arrive in Morton order; a block takes 1024 consecutive particles,
+// reduces their TSC footprint to a bounding box and, when it fits 4096 cells, accumulates the
+// 27 contributions per particle into LDS with 64-bit integer atomics before flushing non-zero
+// cells with one global 64-bit atomic each. Contributions are fixed point (x 2^S, round to
+// nearest even), so the sum is exact and independent of the order: deterministic.
+#include "lagpm_internal.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int DEP_THREADS = 256;
+constexpr int DEP_PPT = 4;
+constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;
+constexpr int DEP_LDS = 4096;
+
+__device__ inline int wrapi(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+// Deposit weights, x axis carries the mass (pmmain.h:11-40).
+__device__ inline void tsc_axis_mass(double xp, float pmas0, int& near, int& s, float w[3]) {
+    double r = rint(xp);
+    near = (int)r;
+    double xmin = xp - r;
+    s = signbit(xmin) ? -1 : 1;
+    float xd1 = (float)fabs(xmin);
+    float p05 = (float)((double)pmas0 * 0.5);
+    float w1 = (float)((0.75 - (double)(xd1 * xd1)) * (double)pmas0);
+    float w3 = (float)((double)p05 * (0.25 + (double)xd1 * ((double)xd1 - 1.0)));
+    float w2 = w3 + pmas0 * xd1;
+    w[0] = w1; w[1] = w2; w[2] = w3;
+}
+__device__ inline void tsc_axis_unit(double xp, int& near, int& s, float w[3]) {
+    double r = rint(xp);
+    near = (int)r;
+    double xmin = xp - r;
+    s = signbit(xmin) ? -1 : 1;
+    float xd1 = (float)fabs(xmin);
+    float w1 = (float)(0.75 - (double)(xd1 * xd1));
+    float w3 = (float)(0.5 * (0.25 + (double)xd1 * ((double)xd1 - 1.0)));
+    float w2 = w3 + xd1;
+    w[0] = w1; w[1] = w2; w[2] = w3;
+}
+// FP64 weights for the plain FP64 path
+__device__ inline void tsc_axis_d(double xp, int& near, int& s, double w[3]) {
+    double r = rint(xp);
+    near = (int)r;
+    double xmin = xp - r;
+    s = signbit(xmin) ? -1 : 1;
+    double d = fabs(xmin);
+    w[0] = 0.75 - d * d;
+    w[2] = 0.5 * (0.25 + d * (d - 1.0));
+    w[1] = w[2] + d;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* __restrict__ a, size_t m) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < m; i += stride) a[i] = 0ull;
+}
+
+template <bool F64>
+__global__ void __launch_bounds__(DEP_THREADS) deposit_kernel(const double* __restrict__ px, const double* __restrict__ py,
+                                                              const double* __restrict__ pz, size_t np, int n, float mass,
+                                                              double scale, unsigned long long* __restrict__ acc) {
+    __shared__ unsigned long long box[DEP_LDS];
+    __shared__ int red[6][DEP_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const size_t base = (size_t)blockIdx.x * DEP_CHUNK;
+
+    int ix[DEP_PPT], iy[DEP_PPT], iz[DEP_PPT], sx[DEP_PPT], sy[DEP_PPT], sz[DEP_PPT];
+    float wx[DEP_PPT][3], wy[DEP_PPT][3], wz[DEP_PPT][3];
+    double dwx[F64 ? DEP_PPT : 1][3], dwy[F64 ? DEP_PPT : 1][3], dwz[F64 ? DEP_PPT : 1][3];
+    bool ok[DEP_PPT];
+    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+    for (int k = 0; k < DEP_PPT; k++) {
+        size_t i = base + (size_t)k * DEP_THREADS + tid;
+        ok[k] = i < np;
+        if (!ok[k]) continue;
+        if constexpr (F64) {
+            tsc_axis_d(px[i], ix[k], sx[k], dwx[k]);
+            tsc_axis_d(py[i], iy[k], sy[k], dwy[k]);
+            tsc_axis_d(pz[i], iz[k], sz[k], dwz[k]);
+        } else {
+            tsc_axis_mass(px[i], mass, ix[k], sx[k], wx[k]);
+            tsc_axis_unit(py[i], iy[k], sy[k], wy[k]);
+            tsc_axis_unit(pz[i], iz[k], sz[k], wz[k]);
+        }
+        lo[0] = min(lo[0], ix[k]); hi[0] = max(hi[0], ix[k]);
+        lo[1] = min(lo[1], iy[k]); hi[1] = max(hi[1], iy[k]);
+        lo[2] = min(lo[2], iz[k]); hi[2] = max(hi[2], iz[k]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[a] = min(lo[a], __shfl_xor(lo[a], o, 64));
+            hi[a] = max(hi[a], __shfl_xor(hi[a], o, 64));
+        }
+        if (lane == 0) { red[a][wid] = lo[a]; red[3 + a][wid] = hi[a]; }
+    }
+    __syncthreads();
+    int blo[3], bdim[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int l = INT_MAX, h = INT_MIN;
+        for (int w = 0; w < DEP_THREADS / 64; w++) { l = min(l, red[a][w]); h = max(h, red[3 + a][w]); }
+        blo[a] = l - 1;
+        bdim[a] = (h + 1) - (l - 1) + 1;
+    }
+    const long vol = (long)bdim[0] * bdim[1] * bdim[2];
+    const bool use_lds = vol > 0 && vol <= DEP_LDS;
+
+    if (use_lds) {
+        for (int c = tid; c < vol; c += DEP_THREADS) box[c] = 0ull;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < DEP_PPT; k++) {
+        if (!ok[k]) continue;
+        int xi[3] = {ix[k], ix[k] + sx[k], ix[k] - sx[k]};
+        int yi[3] = {iy[k], iy[k] + sy[k], iy[k] - sy[k]};
+        int zi[3] = {iz[k], iz[k] + sz[k], iz[k] - sz[k]};
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    double cv;
+                    if constexpr (F64) {
+                        cv = (double)mass * dwx[k][a] * dwy[k][b] * dwz[k][c];
+                    } else {
+                        float wxy = wx[k][a] * wy[k][b];
+                        float cf = wxy * wz[k][c];
+                        cv = (double)cf;
+                    }
+                    unsigned long long q = (unsigned long long)__double2ll_rn(cv * scale);
+                    if (use_lds) {
+                        int li = (xi[a] - blo[0]) + bdim[0] * ((yi[b] - blo[1]) + bdim[1] * (zi[c] - blo[2]));
+                        atomicAdd(&box[li], q);
+                    } else {
+                        size_t g = (size_t)wrapi(xi[a], n) + (size_t)n * ((size_t)wrapi(yi[b], n) + (size_t)n * wrapi(zi[c], n));
+                        atomicAdd(&acc[g], q);
+                    }
+                }
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int c = tid; c < vol; c += DEP_THREADS) {
+            unsigned long long v = box[c];
+            if (v == 0ull) continue;
+            int a = c % bdim[0];
+            int r = c / bdim[0];
+            int b = r % bdim[1];
+            int d = r / bdim[1];
+            size_t g = (size_t)wrapi(blo[0] + a, n) + (size_t)n * ((size_t)wrapi(blo[1] + b, n) + (size_t)n * wrapi(blo[2] + d, n));
+            atomicAdd(&acc[g], v);
+        }
+    }
+}
+
+template <typename T>
+__global__ void convert_kernel(const unsigned long long* __restrict__ acc, size_t m, double inv_scale, double mean,
+                               T* __restrict__ rho) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < m; i += stride) rho[i] = (T)((double)(long long)acc[i] * inv_scale - mean);
+}
+
+__device__ inline double tsc_w2(int n, double m) {
+    if (m == 0.0) return 1.0;
+    double h = M_PI * m / (double)n;
+    double s = sin(h) / h;
+    double s3 = s * s * s;
+    return s3 * s3;
+}
+
+// Green multiply over the half-complex spectrum, layout (i + h*(j + n*k)), psolver.c:49-76.
+template <typename T>
+__global__ void green_kernel(T* __restrict__ spec, int n, double sfact, double irngc, int deconvolve) {
+    const int h = n / 2 + 1;
+    const size_t K = (size_t)h * n * n;
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    const double rn = n;
+    for (; q < K; q += stride) {
+        int i = (int)(q % h);
+        size_t r = q / h;
+        int j = (int)(r % n);
+        int k = (int)(r / n);
+        float wy = (float)j;
+        if (wy > rn / 2) wy = wy - (float)rn;
+        float wz = (float)k;
+        if (wz > rn / 2) wz = wz - (float)rn;
+        double wave = (double)((float)(i * i) + wy * wy + wz * wz);
+        if (wave == 0.0) {
+            spec[2 * q] = (T)0;
+            spec[2 * q + 1] = (T)0;
+            continue;
+        }
+        double factr = (-1.0 / M_PI) / wave;
+        double factor = factr * exp(wave * sfact);
+        if (deconvolve) factor /= tsc_w2(n, (double)i) * tsc_w2(n, (double)wy) * tsc_w2(n, (double)wz);
+        spec[2 * q] = (T)(factor * (double)spec[2 * q] * irngc);
+        spec[2 * q + 1] = (T)(factor * (double)spec[2 * q + 1] * irngc);
+    }
+}
+
+__device__ inline float fda4_f(float m1, float p1, float m2, float p2) {
+    const float ffact1 = (float)((4. / 3.) / 2.);
+    const float ffact2 = (float)((-1. / 3.) / 4.);
+    float a = ffact1 * (m1 - p1);
+    float b = ffact2 * (m2 - p2);
+    return a + b;
+}
+__device__ inline double fda4_d(double m1, double p1, double m2, double p2) {
+    return (2.0 / 3.0) * (m1 - p1) + (-1.0 / 12.0) * (m2 - p2);
+}
+
+// FDA4 gradient mesh: one thread per cell, 12 neighbour reads (cached), one 16/32-byte store.
+template <typename T, typename T4>
+__global__ void gradient_kernel(const T* __restrict__ phi, int n, T4* __restrict__ grad) {
+    const size_t m = (size_t)n * n * n;
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t nn = (size_t)n * n;
+    for (; q < m; q += stride) {
+        int i = (int)(q % n);
+        int j = (int)((q / n) % n);
+        int k = (int)(q / nn);
+        size_t rowj = (size_t)n * j + nn * k;
+        auto X = [&](int a) { return phi[(size_t)wrapi(a, n) + rowj]; };
+        auto Y = [&](int b) { return phi[(size_t)i + (size_t)n * wrapi(b, n) + nn * k]; };
+        auto Z = [&](int c) { return phi[(size_t)i + (size_t)n * j + nn * wrapi(c, n)]; };
+        T4 g;
+        if constexpr (sizeof(T) == 4) {
+            g.x = fda4_f(X(i - 1), X(i + 1), X(i - 2), X(i + 2));
+            g.y = fda4_f(Y(j - 1), Y(j + 1), Y(j - 2), Y(j + 2));
+            g.z = fda4_f(Z(k - 1), Z(k + 1), Z(k - 2), Z(k + 2));
+        } else {
+            g.x = fda4_d(X(i - 1), X(i + 1), X(i - 2), X(i + 2));
+            g.y = fda4_d(Y(j - 1), Y(j + 1), Y(j - 2), Y(j + 2));
+            g.z = fda4_d(Z(k - 1), Z(k + 1), Z(k - 2), Z(k + 2));
+        }
+        g.w = phi[q];
+        grad[q] = g;
+    }
+}
+
+// Interpolation weights of FDA4_HEAD: xmin is a float there (fda4.h:40-66).
+__device__ inline void fda4_axis(double xp, int& near, int& s, float w[3]) {
+    double r = rint(xp);
+    near = (int)r;
+    float xmin = (float)(xp - r);
+    s = signbit(xmin) ? -1 : 1;
+    float xd1 = fabsf(xmin);
+    float w1 = (float)(0.75 - (double)(xd1 * xd1));
+    float w3 = (float)(0.5 * (0.25 + (double)xd1 * ((double)xd1 - 1.0)));
+    float w2 = w3 + xd1;
+    w[0] = w1; w[1] = w2; w[2] = w3;
+}
+
+__global__ void interp_kick_f32(const double* __restrict__ px, const double* __restrict__ py,
+                                const double* __restrict__ pz, double* __restrict__ vx, double* __restrict__ vy,
+                                double* __restrict__ vz, size_t np, int n, const float4* __restrict__ grad,
+                                float fact2) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int ix, iy, iz, sx, sy, sz;
+    float wx[3], wy[3], wz[3];
+    fda4_axis(px[p], ix, sx, wx);
+    fda4_axis(py[p], iy, sy, wy);
+    fda4_axis(pz[p], iz, sz, wz);
+    int xi[3] = {wrapi(ix, n), wrapi(ix + sx, n), wrapi(ix - sx, n)};
+    int yi[3] = {wrapi(iy, n), wrapi(iy + sy, n), wrapi(iy - sy, n)};
+    int zi[3] = {wrapi(iz, n), wrapi(iz + sz, n), wrapi(iz - sz, n)};
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            size_t row = (size_t)n * ((size_t)yi[b] + (size_t)n * zi[c]);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                float wxy = wx[a] * wy[b];
+                float wt = wxy * wz[c];
+                float4 g = grad[row + xi[a]];
+                if (a == 0 && b == 0 && c == 0) {
+                    fx = wt * g.x; fy = wt * g.y; fz = wt * g.z;
+                } else {
+                    fx = fx + wt * g.x; fy = fy + wt * g.y; fz = fz + wt * g.z;
+                }
+            }
+        }
+    vx[p] += (double)(fact2 * fx);
+    vy[p] += (double)(fact2 * fy);
+    vz[p] += (double)(fact2 * fz);
+}
+
+__global__ void interp_kick_f64(const double* __restrict__ px, const double* __restrict__ py,
+                                const double* __restrict__ pz, double* __restrict__ vx, double* __restrict__ vy,
+                                double* __restrict__ vz, size_t np, int n, const double4* __restrict__ grad,
+                                double fact2) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int ix, iy, iz, sx, sy, sz;
+    double wx[3], wy[3], wz[3];
+    tsc_axis_d(px[p], ix, sx, wx);
+    tsc_axis_d(py[p], iy, sy, wy);
+    tsc_axis_d(pz[p], iz, sz, wz);
+    int xi[3] = {wrapi(ix, n), wrapi(ix + sx, n), wrapi(ix - sx, n)};
+    int yi[3] = {wrapi(iy, n), wrapi(iy + sy, n), wrapi(iy - sy, n)};
+    int zi[3] = {wrapi(iz, n), wrapi(iz + sz, n), wrapi(iz - sz, n)};
+    double fx = 0, fy = 0, fz = 0;
+    for (int c = 0; c < 3; c++)
+        for (int b = 0; b < 3; b++) {
+            size_t row = (size_t)n * ((size_t)yi[b] + (size_t)n * zi[c]);
+            for (int a = 0; a < 3; a++) {
+                double wt = wx[a] * wy[b] * wz[c];
+                double4 g = grad[row + xi[a]];
+                fx += wt * g.x; fy += wt * g.y; fz += wt * g.z;
+            }
+        }
+    vx[p] += fact2 * fx;
+    vy[p] += fact2 * fy;
+    vz[p] += fact2 * fz;
+}
+
+unsigned grid_for(size_t work, int threads, unsigned cap = 65535u * 4) {
+    size_t g = (work + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+int fft_status(lagpm_ctx* ctx, rocfft_status s, const char* what) {
+    if (s == rocfft_status_success) return LAGPM_OK;
+    return lagpm_fail(ctx, LAGPM_E_FFT, std::string("rocFFT ") + what + " failed, status " + std::to_string((int)s));
+}
+
+int get_plans(lagpm_ctx* ctx, int n, int prec, fft_plans** out) {
+    auto key = std::make_pair(n, prec);
+    auto it = ctx->plans.find(key);
+    if (it != ctx->plans.end()) {
+        *out = &it->second;
+        return LAGPM_OK;
+    }
+    fft_plans fp;
+    size_t lengths[3] = {(size_t)n, (size_t)n, (size_t)n};
+    rocfft_precision rp = prec == LAGPM_PREC_FP64 ? rocfft_precision_double : rocfft_precision_single;
+    RCHK(fft_status(ctx, rocfft_plan_create(&fp.fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, rp, 3,
+                                            lengths, 1, nullptr), "plan_create(r2c)"));
+    RCHK(fft_status(ctx, rocfft_plan_create(&fp.inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, rp, 3,
+                                            lengths, 1, nullptr), "plan_create(c2r)"));
+    size_t w1 = 0, w2 = 0;
+    RCHK(fft_status(ctx, rocfft_plan_get_work_buffer_size(fp.fwd, &w1), "work_size"));
+    RCHK(fft_status(ctx, rocfft_plan_get_work_buffer_size(fp.inv, &w2), "work_size"));
+    fp.work_bytes = w1 > w2 ? w1 : w2;
+    RCHK(fft_status(ctx, rocfft_execution_info_create(&fp.info), "execution_info_create"));
+    if (fp.work_bytes) {
+        HIPCHK(ctx, hipMalloc(&fp.work, fp.work_bytes));
+        RCHK(fft_status(ctx, rocfft_execution_info_set_work_buffer(fp.info, fp.work, fp.work_bytes), "set_work_buffer"));
+    }
+    RCHK(fft_status(ctx, rocfft_execution_info_set_stream(fp.info, ctx->stream), "set_stream"));
+    auto res = ctx->plans.emplace(key, fp);
+    *out = &res.first->second;
+    return LAGPM_OK;
+}
+
+}  // namespace
+
+int lagpm_fft_release(lagpm_ctx* ctx) {
+    for (auto& kv : ctx->plans) {
+        fft_plans& fp = kv.second;
+        if (fp.fwd) rocfft_plan_destroy(fp.fwd);
+        if (fp.inv) rocfft_plan_destroy(fp.inv);
+        if (fp.info) rocfft_execution_info_destroy(fp.info);
+        if (fp.work) (void)hipFree(fp.work);
+    }
+    ctx->plans.clear();
+    return LAGPM_OK;
+}
+
+int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
+    const int n = box->p.n;
+    const size_t m = (size_t)n * n * n;
+    const bool f64 = box->p.precision == LAGPM_PREC_FP64;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, box->acc, m);
+    HIPCHK(ctx, hipGetLastError());
+    const double scale = std::ldexp(1.0, box->scale_bits);
+    const unsigned nblk = (unsigned)((box->np + DEP_CHUNK - 1) / DEP_CHUNK);
+    if (nblk) {
+        if (f64)
+            hipLaunchKernelGGL(deposit_kernel<true>, dim3(nblk), dim3(DEP_THREADS), 0, ctx->stream, box->x, box->y, box->z,
+                               box->np, n, (float)box->p.mass, scale, box->acc);
+        else
+            hipLaunchKernelGGL(deposit_kernel<false>, dim3(nblk), dim3(DEP_THREADS), 0, ctx->stream, box->x, box->y, box->z,
+                               box->np, n, (float)box->p.mass, scale, box->acc);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    const double inv_scale = std::ldexp(1.0, -box->scale_bits);
+    const double mean = (double)box->np * box->p.mass / (double)m;
+    if (f64)
+        hipLaunchKernelGGL(convert_kernel<double>, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, box->acc, m, inv_scale,
+                           mean, (double*)box->rho);
+    else
+        hipLaunchKernelGGL(convert_kernel<float>, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, box->acc, m, inv_scale,
+                           mean, (float*)box->rho);
+    HIPCHK(ctx, hipGetLastError());
+    return LAGPM_OK;
+}
+
+int lagpm_green_device(lagpm_ctx* ctx, int n, int prec, double efold, int deconvolve, void* spec) {
+    const double rn = n;
+    const double sfact = -std::pow(M_PI * efold / rn, 2);
+    const double irngc = 1.0 / (rn * rn * rn);
+    const size_t K = (size_t)(n / 2 + 1) * n * n;
+    if (prec == LAGPM_PREC_FP64)
+        hipLaunchKernelGGL(green_kernel<double>, dim3(grid_for(K, 256)), dim3(256), 0, ctx->stream, (double*)spec, n, sfact,
+                           irngc, deconvolve);
+    else
+        hipLaunchKernelGGL(green_kernel<float>, dim3(grid_for(K, 256)), dim3(256), 0, ctx->stream, (float*)spec, n, sfact,
+                           irngc, deconvolve);
+    HIPCHK(ctx, hipGetLastError());
+    return LAGPM_OK;
+}
+
+int lagpm_pm_solve(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
+    fft_plans* fp = nullptr;
+    RCHK(get_plans(ctx, box->p.n, box->p.precision, &fp));
+    void* in[1] = {box->rho};
+    void* out[1] = {box->spec};
+    RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, out, fp->info), "execute(r2c)"));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
+    RCHK(lagpm_green_device(ctx, box->p.n, box->p.precision, box->p.efold, box->p.deconvolve, box->spec));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[1], ctx->stream));
+    void* in2[1] = {box->spec};
+    void* out2[1] = {box->phi};
+    RCHK(fft_status(ctx, rocfft_execute(fp->inv, in2, out2, fp->info), "execute(c2r)"));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[2], ctx->stream));
+    return LAGPM_OK;
+}
+
+int lagpm_pm_gradient(lagpm_ctx* ctx, lagpm_box* box) {
+    const int n = box->p.n;
+    const size_t m = (size_t)n * n * n;
+    if (box->p.precision == LAGPM_PREC_FP64)
+        hipLaunchKernelGGL((gradient_kernel<double, double4>), dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream,
+                           (const double*)box->phi, n, (double4*)box->grad);
+    else
+        hipLaunchKernelGGL((gradient_kernel<float, float4>), dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream,
+                           (const float*)box->phi, n, (float4*)box->grad);
+    HIPCHK(ctx, hipGetLastError());
+    return LAGPM_OK;
+}
+
+int lagpm_pm_interp(lagpm_ctx* ctx, lagpm_box* box, double fact) {
+    if (box->np == 0) return LAGPM_OK;
+    const unsigned nblk = (unsigned)((box->np + 255) / 256);
+    if (box->p.precision == LAGPM_PREC_FP64)
+        hipLaunchKernelGGL(interp_kick_f64, dim3(nblk), dim3(256), 0, ctx->stream, box->x, box->y, box->z, box->vx, box->vy,
+                           box->vz, box->np, box->p.n, (const double4*)box->grad, fact);
+    else
+        hipLaunchKernelGGL(interp_kick_f32, dim3(nblk), dim3(256), 0, ctx->stream, box->x, box->y, box->z, box->vx, box->vy,
+                           box->vz, box->np, box->p.n, (const float4*)box->grad, (float)fact);
+    HIPCHK(ctx, hipGetLastError());
+    return LAGPM_OK;
+}

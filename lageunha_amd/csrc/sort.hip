@@ -1,0 +1,239 @@
+// sort.hip — device-wide exclusive scan and stable LSD radix sort (64-bit keys, 32-bit values).
+//
+// Radix sort: 8-bit digits. Per pass: a histogram kernel (LDS counters per 4096-key tile),
+// an exclusive scan of the digit-major histogram, and a scatter kernel that ranks keys
+// stably inside each wavefront with ballot-based digit matching (64-lane masks), then
+// orders wavefronts of a tile by their position. Deterministic and stable.
+#include "lagpm_internal.h"
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 4096 keys per tile
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Block-level exclusive scan of SCAN_TILE elements; writes per-block totals.
+__global__ void __launch_bounds__(SCAN_THREADS) scan_tile_kernel(const uint32_t* __restrict__ in,
+                                                                 uint32_t* __restrict__ out, size_t n,
+                                                                 uint32_t* __restrict__ block_sums) {
+    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)tid * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        size_t i = base + k;
+        v[k] = i < n ? in[i] : 0u;
+        s += v[k];
+    }
+    uint32_t incl = wave_incl_scan(s, lane);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t woff = 0, btot = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_THREADS / 64; w++) {
+        if (w < wid) woff += wsum[w];
+        btot += wsum[w];
+    }
+    uint32_t run = woff + incl - s;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        size_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+    if (tid == 0 && block_sums) block_sums[blockIdx.x] = btot;
+}
+
+__global__ void scan_add_kernel(uint32_t* __restrict__ out, size_t n, const uint32_t* __restrict__ block_off) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += block_off[i / SCAN_TILE];
+}
+
+__global__ void scan_total_kernel(const uint32_t* __restrict__ in, const uint32_t* __restrict__ out, size_t n,
+                                  uint32_t* __restrict__ total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = (n ? out[n - 1] + in[n - 1] : 0u);
+}
+
+// ---- radix sort ----
+__global__ void __launch_bounds__(RS_THREADS) rs_hist_kernel(const uint64_t* __restrict__ key, size_t n, int shift,
+                                                             uint32_t* __restrict__ ghist, uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    const int tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int k = 0; k < RS_ITEMS; k++) {
+        size_t i = base + (size_t)k * RS_THREADS + tid;
+        if (i < n) atomicAdd(&h[(uint32_t)(key[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    ghist[(size_t)tid * nblocks + blockIdx.x] = h[tid];
+}
+
+__global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* __restrict__ key,
+                                                                const uint32_t* __restrict__ val, size_t n, int shift,
+                                                                const uint32_t* __restrict__ goff, uint32_t nblocks,
+                                                                uint64_t* __restrict__ key_out,
+                                                                uint32_t* __restrict__ val_out) {
+    __shared__ uint32_t wcount[RS_WAVES][256];
+    __shared__ uint32_t boff[256];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int d = tid; d < 256; d += RS_THREADS) {
+        boff[d] = goff[(size_t)d * nblocks + blockIdx.x];
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) wcount[w][d] = 0;
+    }
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const size_t seg = (size_t)blockIdx.x * RS_TILE + (size_t)wid * (RS_ITEMS * 64);
+    uint64_t k[RS_ITEMS];
+    uint32_t v[RS_ITEMS];
+    uint32_t rank[RS_ITEMS];
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; it++) {
+        size_t i = seg + (size_t)it * 64 + lane;
+        bool act = i < n;
+        k[it] = act ? key[i] : 0ull;
+        v[it] = act ? val[i] : 0u;
+        uint32_t d = (uint32_t)(k[it] >> shift) & 255u;
+        uint64_t m = __ballot(act);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t bb = __ballot((d >> b) & 1u);
+            m &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t r = (uint32_t)__popcll(m & lt);
+        uint32_t c = (uint32_t)__popcll(m);
+        uint32_t before = act ? wcount[wid][d] : 0u;
+        rank[it] = act ? before + r : 0xffffffffu;
+        __builtin_amdgcn_wave_barrier();
+        if (act && r == 0) wcount[wid][d] = before + c;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) {
+            uint32_t t = wcount[w][tid];
+            wcount[w][tid] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; it++) {
+        if (rank[it] == 0xffffffffu) continue;
+        uint32_t d = (uint32_t)(k[it] >> shift) & 255u;
+        size_t dst = (size_t)boff[d] + wcount[wid][d] + rank[it];
+        key_out[dst] = k[it];
+        val_out[dst] = v[it];
+    }
+}
+
+int ensure_scan_tmp(lagpm_ctx* ctx, size_t bytes) {
+    if (ctx->scan_tmp_bytes >= bytes) return LAGPM_OK;
+    if (ctx->scan_tmp) (void)hipFree(ctx->scan_tmp);
+    ctx->scan_tmp = nullptr;
+    ctx->scan_tmp_bytes = 0;
+    HIPCHK(ctx, hipMalloc(&ctx->scan_tmp, bytes));
+    ctx->scan_tmp_bytes = bytes;
+    return LAGPM_OK;
+}
+
+// recursive exclusive scan using tmp (enough room for all levels)
+int scan_rec(lagpm_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp) {
+    size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb == 0) return LAGPM_OK;
+    uint32_t* sums = tmp;
+    uint32_t* sums_scan = tmp + nb;
+    hipLaunchKernelGGL(scan_tile_kernel, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n,
+                       nb > 1 ? sums : (uint32_t*)nullptr);
+    HIPCHK(ctx, hipGetLastError());
+    if (nb > 1) {
+        RCHK(scan_rec(ctx, sums, sums_scan, nb, tmp + 2 * nb));
+        hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, out, n,
+                           sums_scan);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    return LAGPM_OK;
+}
+
+size_t scan_tmp_elems(size_t n) {
+    size_t t = 1;
+    while (n > (size_t)SCAN_TILE) {
+        n = (n + SCAN_TILE - 1) / SCAN_TILE;
+        t += 2 * n;
+    }
+    return t + 2 * ((n + SCAN_TILE - 1) / SCAN_TILE) + 64;
+}
+
+}  // namespace
+
+// Exclusive scan of n u32 values (in and out must not alias). total_host (optional)
+// receives the sum; that copy synchronises the stream.
+int lagpm_exclusive_scan_u32(lagpm_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, uint32_t* total_host) {
+    if (n == 0) {
+        if (total_host) *total_host = 0;
+        return LAGPM_OK;
+    }
+    size_t elems = scan_tmp_elems(n) + 4;
+    RCHK(ensure_scan_tmp(ctx, elems * sizeof(uint32_t)));
+    uint32_t* tmp = (uint32_t*)ctx->scan_tmp;
+    RCHK(scan_rec(ctx, in, out, n, tmp + 4));
+    if (total_host) {
+        hipLaunchKernelGGL(scan_total_kernel, dim3(1), dim3(64), 0, ctx->stream, in, out, n, tmp);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(total_host, tmp, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return LAGPM_OK;
+}
+
+// Stable sort of (key, val) pairs on the low `bits` bits of key (rounded up to 8).
+int lagpm_radix_sort_pairs(lagpm_ctx* ctx, uint64_t* key, uint64_t* key_alt, uint32_t* val, uint32_t* val_alt,
+                           size_t n, int bits, bool* result_in_alt) {
+    *result_in_alt = false;
+    if (n <= 1) return LAGPM_OK;
+    uint32_t nblocks = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
+    size_t hist_elems = (size_t)256 * nblocks;
+    // histogram + its scan live after the scan temp area
+    size_t scan_elems = scan_tmp_elems(hist_elems) + 4;
+    RCHK(ensure_scan_tmp(ctx, (scan_elems + 2 * hist_elems) * sizeof(uint32_t)));
+    uint32_t* base = (uint32_t*)ctx->scan_tmp;
+    uint32_t* ghist = base + scan_elems;
+    uint32_t* goff = ghist + hist_elems;
+    uint64_t *ki = key, *ko = key_alt;
+    uint32_t *vi = val, *vo = val_alt;
+    int passes = (bits + 7) / 8;
+    for (int p = 0; p < passes; p++) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(rs_hist_kernel, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, ki, n, shift, ghist, nblocks);
+        HIPCHK(ctx, hipGetLastError());
+        RCHK(scan_rec(ctx, ghist, goff, hist_elems, base + 4));
+        hipLaunchKernelGGL(rs_scatter_kernel, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, ki, vi, n, shift, goff,
+                           nblocks, ko, vo);
+        HIPCHK(ctx, hipGetLastError());
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    *result_in_alt = (ki != key);
+    return LAGPM_OK;
+}

@@ -1,0 +1,902 @@
+// tree.hip — short-range tree correction: Morton sort, linear octree, node moments, target
+// groups and the wavefront-cooperative walk with the tabulated split force law.
+//
+// Reference semantics (paths relative to the reference tree):
+//   driver       Cosmos/treecorrection.c:225-475 (chaining mesh of width >= cutoff, 27 neighbours)
+//   tree         OST/ost.ver2.c:36-196 (monopole + quadrupole about the COM, rmax = max |x - COM|)
+//   opening      OST/ost.ver2.c:296-306 (open when d^2 < rmax^2 / theta^2)
+//   node force   OST/ost.ver2.c:247-277 (AddNode, with the y/z components of defect D1 restored)
+//   pair force   OST/ost.ver2.c:279-293 (AddPar); both use the cutoff test d^2 <= rsphere^2
+//   force law    OST/Model4TreeForce.c:34-46 (GetPointGrav / GetCellGrav, 4096-entry linear tables)
+//
+// MI355X design (not the reference's): one global octree per box instead of one tree per
+// 4x4x4 cell. Particles are sorted by a 48-bit Morton key; the chaining cells are the
+// octree nodes of level L (2^L per axis, width n/2^L >= cutoff). A node is the run of
+// particles sharing a key prefix; nodes are numbered in pre-order by (first particle, level),
+// which a per-particle count + exclusive scan produces without any level-by-level pass.
+// Target groups are subtrees of <= 64 particles; one wavefront walks the 27 neighbouring
+// cell trees for its group with a group-conservative opening test, stages accepted nodes
+// and opened leaves in LDS, and evaluates the interactions lane-per-target.
+#include "lagpm_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr int WALK_THREADS = 256;
+constexpr int WALK_WAVES = WALK_THREADS / 64;
+constexpr int PP_CAP = 1152;     // 63 left over + 64 lanes x 16 particles per append round
+constexpr int PC_CAP = 128;
+constexpr int PP_TAKE = 16;
+constexpr int STACK_MAX = 8192;  // per-wave walk stack entries (global memory)
+constexpr float RAN_F = (float)(6.0 / 4096.0);
+
+__device__ inline uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0xffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+__device__ inline uint32_t compact3(uint64_t x) {
+    x &= 0x1249249249249249ull;
+    x = (x | (x >> 2)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x >> 4)) & 0x100f00f00f00f00full;
+    x = (x | (x >> 8)) & 0x1f0000ff0000ffull;
+    x = (x | (x >> 16)) & 0x1f00000000ffffull;
+    x = (x | (x >> 32)) & 0x1fffffull;
+    return (uint32_t)x;
+}
+__device__ inline uint32_t quant(double x, double s) {
+    double q = floor(x * s);
+    if (q < 0) q = 0;
+    if (q > 65535.0) q = 65535.0;
+    return (uint32_t)q;
+}
+__device__ inline int key_shift(int level) { return 3 * (LAGPM_KEY_BITS - level); }
+// number of leading octal digits two keys share (16 when equal)
+__device__ inline int common_digits(uint64_t a, uint64_t b) {
+    uint64_t d = a ^ b;
+    if (d == 0) return LAGPM_KEY_BITS;
+    int p = 63 - __clzll(d);
+    return (47 - p) / 3;
+}
+// first index in [lo, hi) whose prefix (key >> sh) exceeds pfx
+__device__ inline size_t upper_bound_pfx(const uint64_t* __restrict__ K, size_t lo, size_t hi, uint64_t pfx, int sh) {
+    while (lo < hi) {
+        size_t mid = lo + ((hi - lo) >> 1);
+        if ((K[mid] >> sh) > pfx) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+// first index in [lo, hi) whose prefix is >= pfx
+__device__ inline size_t lower_bound_pfx(const uint64_t* __restrict__ K, size_t lo, size_t hi, uint64_t pfx, int sh) {
+    while (lo < hi) {
+        size_t mid = lo + ((hi - lo) >> 1);
+        if ((K[mid] >> sh) >= pfx) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+__global__ void keys_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                            size_t np, double s, uint64_t* __restrict__ key, uint32_t* __restrict__ perm) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    key[i] = spread3(quant(x[i], s)) | (spread3(quant(y[i], s)) << 1) | (spread3(quant(z[i], s)) << 2);
+    perm[i] = (uint32_t)i;
+}
+
+template <typename T>
+__global__ void gather_kernel(const T* __restrict__ src, const uint32_t* __restrict__ perm, size_t np, T* __restrict__ dst) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) dst[i] = src[perm[i]];
+}
+
+__global__ void pack_pp_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                               size_t np, float mass, float4* __restrict__ pp) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) pp[i] = make_float4((float)x[i], (float)y[i], (float)z[i], mass);
+}
+
+// Nodes beginning at particle i: levels l0(i) .. l0(i)+cnt-1 of the runs that start at i.
+__device__ inline int node_levels(const uint64_t* __restrict__ K, size_t N, size_t i, int L, int nleaf, int* l0_out,
+                                  uint32_t* ends /* may be null; up to 17 entries */) {
+    const uint64_t k = K[i];
+    int cpl = (i == 0) ? -1 : common_digits(K[i - 1], k);
+    int l0 = max(L, cpl + 1);
+    *l0_out = l0;
+    if (l0 > LAGPM_KEY_BITS) return 0;
+    size_t epar = N;
+    if (l0 > L) {
+        int sh = key_shift(l0 - 1);
+        uint64_t pfx = k >> sh;
+        size_t s = lower_bound_pfx(K, 0, i, pfx, sh);
+        size_t e = upper_bound_pfx(K, i, N, pfx, sh);
+        if (e - s <= (size_t)nleaf) return 0;  // parent is a leaf
+        epar = e;
+    }
+    int c = 0;
+    for (int l = l0; l <= LAGPM_KEY_BITS; l++) {
+        int sh = key_shift(l);
+        size_t e = upper_bound_pfx(K, i, epar, k >> sh, sh);
+        if (ends) ends[c] = (uint32_t)e;
+        c++;
+        if (e - i <= (size_t)nleaf) break;
+        epar = e;
+    }
+    return c;
+}
+
+__global__ void node_count_kernel(const uint64_t* __restrict__ K, size_t N, int L, int nleaf, uint32_t* __restrict__ cnt) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int l0;
+    cnt[i] = (uint32_t)node_levels(K, N, i, L, nleaf, &l0, nullptr);
+}
+
+__global__ void node_write_kernel(const uint64_t* __restrict__ K, size_t N, int L, int nleaf, const uint32_t* __restrict__ off,
+                                  int4* __restrict__ info, int* __restrict__ cellroot, int mcell) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    uint32_t ends[LAGPM_KEY_BITS + 1];
+    int l0;
+    int c = node_levels(K, N, i, L, nleaf, &l0, ends);
+    uint32_t v0 = off[i];
+    for (int t = 0; t < c; t++) {
+        int l = l0 + t;
+        info[v0 + t] = make_int4((int)i, (int)(ends[t] - i), 0, l);
+        if (l == L) {
+            uint64_t cell = K[i] >> key_shift(L);
+            uint32_t cx = compact3(cell), cy = compact3(cell >> 1), cz = compact3(cell >> 2);
+            cellroot[cx + (size_t)mcell * (cy + (size_t)mcell * cz)] = (int)(v0 + t);
+        }
+    }
+}
+
+// Child lists: children of internal node (l, b) are the level-(l+1) runs inside [b, b+count).
+__global__ void children_kernel(const uint64_t* __restrict__ K, size_t nnodes, int L, int nleaf, const uint32_t* __restrict__ off,
+                                int4* __restrict__ info, int* __restrict__ child, int* __restrict__ parent) {
+    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nnodes) return;
+    int4 nd = info[v];
+    if (nd.w == L) parent[v] = -1;
+    if (nd.y <= nleaf || nd.w >= LAGPM_KEY_BITS) return;  // leaf
+    const int lc = nd.w + 1;
+    const uint32_t e = (uint32_t)(nd.x + nd.y);
+    uint32_t b = (uint32_t)nd.x;
+    int nc = 0;
+    while (b < e && nc < 8) {
+        int cpl = (b == 0) ? -1 : common_digits(K[b - 1], K[b]);
+        int l0 = max(L, cpl + 1);
+        if (lc < l0) break;  // cannot happen for a well-formed key order
+        size_t cv = (size_t)off[b] + (size_t)(lc - l0);
+        if (cv >= nnodes) break;
+        int cnt = info[cv].y;
+        if (cnt <= 0) break;
+        child[8 * v + nc] = (int)cv;
+        parent[cv] = (int)v;
+        nc++;
+        b += (uint32_t)cnt;
+    }
+    info[v].z = nc;
+}
+
+__device__ inline void kahan(float& s, float& c, float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// FP32 moments with Kahan-compensated sums, one level per launch (deepest first).
+__global__ void moments_f32_kernel(const float4* __restrict__ pp, size_t nnodes, int level, const int4* __restrict__ info,
+                                   const int* __restrict__ child, float4* __restrict__ com, float4* __restrict__ q0,
+                                   float4* __restrict__ q1) {
+    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nnodes) return;
+    int4 nd = info[v];
+    if (nd.w != level) return;
+    float M = 0, cM = 0, Sx = 0, cx = 0, Sy = 0, cy = 0, Sz = 0, cz = 0;
+    float Q[6] = {0, 0, 0, 0, 0, 0}, cQ[6] = {0, 0, 0, 0, 0, 0};
+    float rmax = 0;
+    float3 cm;
+    if (nd.z == 0) {
+        const float4 r = pp[nd.x];
+        for (int p = nd.x; p < nd.x + nd.y; p++) {
+            float4 a = pp[p];
+            kahan(M, cM, a.w);
+            kahan(Sx, cx, a.w * (a.x - r.x));
+            kahan(Sy, cy, a.w * (a.y - r.y));
+            kahan(Sz, cz, a.w * (a.z - r.z));
+        }
+        cm = make_float3(r.x + Sx / M, r.y + Sy / M, r.z + Sz / M);
+        for (int p = nd.x; p < nd.x + nd.y; p++) {
+            float4 a = pp[p];
+            float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
+            kahan(Q[0], cQ[0], a.w * dx * dx);
+            kahan(Q[1], cQ[1], a.w * dy * dy);
+            kahan(Q[2], cQ[2], a.w * dz * dz);
+            kahan(Q[3], cQ[3], a.w * dx * dy);
+            kahan(Q[4], cQ[4], a.w * dx * dz);
+            kahan(Q[5], cQ[5], a.w * dy * dz);
+            rmax = fmaxf(rmax, sqrtf(dx * dx + dy * dy + dz * dz));
+        }
+    } else {
+        const float4 r = com[child[8 * v]];
+        for (int c = 0; c < nd.z; c++) {
+            float4 a = com[child[8 * v + c]];
+            kahan(M, cM, a.w);
+            kahan(Sx, cx, a.w * (a.x - r.x));
+            kahan(Sy, cy, a.w * (a.y - r.y));
+            kahan(Sz, cz, a.w * (a.z - r.z));
+        }
+        cm = make_float3(r.x + Sx / M, r.y + Sy / M, r.z + Sz / M);
+        for (int c = 0; c < nd.z; c++) {
+            int cv = child[8 * v + c];
+            float4 a = com[cv], b0 = q0[cv], b1 = q1[cv];
+            float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
+            kahan(Q[0], cQ[0], b0.x + a.w * dx * dx);
+            kahan(Q[1], cQ[1], b0.y + a.w * dy * dy);
+            kahan(Q[2], cQ[2], b0.z + a.w * dz * dz);
+            kahan(Q[3], cQ[3], b0.w + a.w * dx * dy);
+            kahan(Q[4], cQ[4], b1.x + a.w * dx * dz);
+            kahan(Q[5], cQ[5], b1.y + a.w * dy * dz);
+            rmax = fmaxf(rmax, sqrtf(dx * dx + dy * dy + dz * dz) + b1.w);
+        }
+    }
+    com[v] = make_float4(cm.x, cm.y, cm.z, M);
+    q0[v] = make_float4(Q[0], Q[1], Q[2], Q[3]);
+    q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], rmax);
+}
+
+// FP64 moments for the plain FP64 path (12 doubles per node).
+__global__ void moments_f64_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                                   double mass, size_t nnodes, int level, const int4* __restrict__ info,
+                                   const int* __restrict__ child, double* __restrict__ nd_d) {
+    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nnodes) return;
+    int4 nd = info[v];
+    if (nd.w != level) return;
+    double M = 0, Sx = 0, Sy = 0, Sz = 0, Q[6] = {0, 0, 0, 0, 0, 0}, rmax = 0;
+    double cxx, cyy, czz;
+    if (nd.z == 0) {
+        for (int p = nd.x; p < nd.x + nd.y; p++) { M += mass; Sx += mass * x[p]; Sy += mass * y[p]; Sz += mass * z[p]; }
+        cxx = Sx / M; cyy = Sy / M; czz = Sz / M;
+        for (int p = nd.x; p < nd.x + nd.y; p++) {
+            double dx = x[p] - cxx, dy = y[p] - cyy, dz = z[p] - czz;
+            Q[0] += mass * dx * dx; Q[1] += mass * dy * dy; Q[2] += mass * dz * dz;
+            Q[3] += mass * dx * dy; Q[4] += mass * dx * dz; Q[5] += mass * dy * dz;
+            rmax = fmax(rmax, sqrt(dx * dx + dy * dy + dz * dz));
+        }
+    } else {
+        for (int c = 0; c < nd.z; c++) {
+            const double* a = nd_d + 12 * (size_t)child[8 * v + c];
+            M += a[3]; Sx += a[3] * a[0]; Sy += a[3] * a[1]; Sz += a[3] * a[2];
+        }
+        cxx = Sx / M; cyy = Sy / M; czz = Sz / M;
+        for (int c = 0; c < nd.z; c++) {
+            const double* a = nd_d + 12 * (size_t)child[8 * v + c];
+            double dx = a[0] - cxx, dy = a[1] - cyy, dz = a[2] - czz, m = a[3];
+            Q[0] += a[4] + m * dx * dx; Q[1] += a[5] + m * dy * dy; Q[2] += a[6] + m * dz * dz;
+            Q[3] += a[7] + m * dx * dy; Q[4] += a[8] + m * dx * dz; Q[5] += a[9] + m * dy * dz;
+            rmax = fmax(rmax, sqrt(dx * dx + dy * dy + dz * dz) + a[11]);
+        }
+    }
+    double* o = nd_d + 12 * v;
+    o[0] = cxx; o[1] = cyy; o[2] = czz; o[3] = M;
+    for (int k = 0; k < 6; k++) o[4 + k] = Q[k];
+    o[10] = Q[0] + Q[1] + Q[2];
+    o[11] = rmax;
+}
+
+// Groups: the first node of size <= G on each root-to-leaf path; max-depth leaves larger
+// than G (coincident keys) are cut into chunks of G.
+__global__ void group_count_kernel(size_t nnodes, int L, int G, const int4* __restrict__ info, const int* __restrict__ parent,
+                                   uint32_t* __restrict__ gcnt) {
+    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nnodes) return;
+    int4 nd = info[v];
+    int pv = parent[v];
+    bool top = (nd.w == L) || pv < 0 || (info[pv].y > G);
+    uint32_t c = 0;
+    if (top) {
+        if (nd.y <= G) c = 1;
+        else if (nd.z == 0) c = (uint32_t)((nd.y + G - 1) / G);
+    }
+    gcnt[v] = c;
+}
+__global__ void group_write_kernel(size_t nnodes, int G, const int4* __restrict__ info, const uint32_t* __restrict__ gcnt,
+                                   const uint32_t* __restrict__ goff, int2* __restrict__ groups) {
+    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nnodes) return;
+    uint32_t c = gcnt[v];
+    if (!c) return;
+    int4 nd = info[v];
+    for (uint32_t t = 0; t < c; t++) {
+        int b = nd.x + (int)t * G;
+        int e = min(nd.x + nd.y, b + G);
+        groups[goff[v] + t] = make_int2(b, e - b);
+    }
+}
+__global__ void group_box_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                                 size_t ngroups, const int2* __restrict__ groups, double* __restrict__ gbox,
+                                 int* __restrict__ pgroup) {
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    int2 gr = groups[g];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int p = gr.x; p < gr.x + gr.y; p++) {
+        lo[0] = fmin(lo[0], x[p]); hi[0] = fmax(hi[0], x[p]);
+        lo[1] = fmin(lo[1], y[p]); hi[1] = fmax(hi[1], y[p]);
+        lo[2] = fmin(lo[2], z[p]); hi[2] = fmax(hi[2], z[p]);
+        pgroup[p] = (int)g;
+    }
+    for (int a = 0; a < 3; a++) { gbox[6 * g + a] = lo[a]; gbox[6 * g + 3 + a] = hi[a]; }
+}
+
+__global__ void fill_int_kernel(int* __restrict__ a, size_t m, int v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) a[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Walk helpers
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ inline int wave_excl_scan(int v, int lane, int& total) {
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    total = __shfl(incl, 63, 64);
+    return incl - v;
+}
+__device__ inline float wave_minf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline float wave_maxf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+// periodic image offset of neighbour nb (0..26) of cell (cx,cy,cz)
+__device__ inline float image_shift(int c, int d, int mcell, float box) {
+    int t = c + d;
+    return t < 0 ? -box : (t >= mcell ? box : 0.0f);
+}
+
+struct walk_args {
+    const float4* pp;
+    const int4* info;
+    const int* child;
+    const float4* com;
+    const float4* q0;
+    const float4* q1;
+    const int* cellroot;
+    const int2* groups;
+    const uint64_t* key;
+    int cell_shift;
+    const float2* tab0;
+    const float4* tab12;
+    double *vx, *vy, *vz;
+    int* stack;
+    unsigned long long* counters;
+    size_t ngroups;
+    int mcell;
+    float box;       // n
+    float inv_cellw;
+    float rc2;
+    float rcut;
+    float inv_theta2;
+    float fact;
+    int naccept;
+};
+
+__global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
+    __shared__ uint32_t s_ppidx[WALK_WAVES][PP_CAP];
+    __shared__ int s_pcidx[WALK_WAVES][PC_CAP];
+    __shared__ float4 s_pp[WALK_WAVES][64];
+    __shared__ float4 s_pc[WALK_WAVES][3][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t* ppidx = s_ppidx[wid];
+    int* pcidx = s_pcidx[wid];
+    float4* cpp = s_pp[wid];
+    float4* cpc0 = s_pc[wid][0];
+    float4* cpc1 = s_pc[wid][1];
+    float4* cpc2 = s_pc[wid][2];
+    int* stack = A.stack + (size_t)(blockIdx.x * WALK_WAVES + wid) * STACK_MAX;
+
+    for (;;) {
+        int g = 0;
+        if (lane == 0) g = (int)atomicAdd(&A.counters[2], 1ull);
+        g = __shfl(g, 0, 64);
+        if ((size_t)g >= A.ngroups) break;
+        const int2 gr = A.groups[g];
+        const bool active = lane < gr.y;
+        const int me = active ? gr.x + lane : gr.x;
+        const float4 xt = A.pp[me];
+        const float lox = wave_minf(xt.x), loy = wave_minf(xt.y), loz = wave_minf(xt.z);
+        const float hix = wave_maxf(xt.x), hiy = wave_maxf(xt.y), hiz = wave_maxf(xt.z);
+        const uint64_t cellkey = A.key[gr.x] >> A.cell_shift;
+        const int cx = (int)compact3(cellkey), cy = (int)compact3(cellkey >> 1), cz = (int)compact3(cellkey >> 2);
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        unsigned long long npp = 0, npc = 0;
+        int sp = 0;
+        bool overflow = false;
+        {
+            int e = -1;
+            if (lane < 27) {
+                int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
+                int nx = (cx + dx + A.mcell) % A.mcell, ny = (cy + dy + A.mcell) % A.mcell, nz = (cz + dz + A.mcell) % A.mcell;
+                int r = A.cellroot[nx + A.mcell * (ny + A.mcell * nz)];
+                if (r >= 0) e = (r << 5) | lane;
+            }
+            int tot;
+            int off = wave_excl_scan(e >= 0 ? 1 : 0, lane, tot);
+            if (e >= 0) stack[off] = e;
+            sp = tot;
+        }
+        int pp_n = 0, pc_n = 0;
+
+        auto shift_of = [&](int nb, float& sx, float& sy, float& sz) {
+            sx = image_shift(cx, nb % 3 - 1, A.mcell, A.box);
+            sy = image_shift(cy, (nb / 3) % 3 - 1, A.mcell, A.box);
+            sz = image_shift(cz, nb / 9 - 1, A.mcell, A.box);
+        };
+        auto flush_pp = [&](int start, int m) {
+            if (lane < m) {
+                uint32_t e = ppidx[start + lane];
+                float sx, sy, sz;
+                shift_of((int)(e & 31u), sx, sy, sz);
+                float4 s = A.pp[e >> 5];
+                cpp[lane] = make_float4(s.x + sx, s.y + sy, s.z + sz, s.w);
+            }
+            wave_sync();
+            for (int j = 0; j < m; j++) {
+                float4 s = cpp[j];
+                float dx = xt.x - s.x, dy = xt.y - s.y, dz = xt.z - s.z;
+                float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= A.rc2) {
+                    float r = __fsqrt_rn(r2);
+                    int nt = min((int)(r * (1.0f / RAN_F)), 4095);
+                    float t = r - (float)nt * RAN_F;
+                    float2 tb = A.tab0[nt];
+                    float f = (tb.x + tb.y * t) * s.w;
+                    ax += dx * f; ay += dy * f; az += dz * f;
+                }
+            }
+            npp += (unsigned long long)m;
+            wave_sync();
+        };
+        auto flush_pc = [&](int start, int m) {
+            if (lane < m) {
+                int e = pcidx[start + lane];
+                float sx, sy, sz;
+                shift_of(e & 31, sx, sy, sz);
+                int v = e >> 5;
+                float4 c = A.com[v];
+                cpc0[lane] = make_float4(c.x + sx, c.y + sy, c.z + sz, c.w);
+                cpc1[lane] = A.q0[v];
+                cpc2[lane] = A.q1[v];
+            }
+            wave_sync();
+            for (int j = 0; j < m; j++) {
+                float4 c = cpc0[j], qa = cpc1[j], qb = cpc2[j];
+                float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
+                float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= A.rc2) {
+                    float r = __fsqrt_rn(r2);
+                    int nt = min((int)(r * (1.0f / RAN_F)), 4095);
+                    float t = r - (float)nt * RAN_F;
+                    float2 t0 = A.tab0[nt];
+                    float4 t12 = A.tab12[nt];
+                    float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
+                    float qx1 = qa.x * dx + qa.w * dy + qb.x * dz;
+                    float qx2 = qa.w * dx + qa.y * dy + qb.y * dz;
+                    float qx3 = qb.x * dx + qb.y * dy + qa.z * dz;
+                    float qxy = qa.x * dx * dx + qa.y * dy * dy + qa.z * dz * dz +
+                                2.f * (qa.w * dx * dy + qb.x * dx * dz + qb.y * dy * dz);
+                    float tmp = c.w * f1 + qb.z * f2 + qxy * f3;
+                    float two = 2.f * f2;
+                    ax += dx * tmp + qx1 * two;
+                    ay += dy * tmp + qx2 * two;
+                    az += dz * tmp + qx3 * two;
+                }
+            }
+            npc += (unsigned long long)m;
+            wave_sync();
+        };
+
+        while (sp > 0) {
+            const int take = min(sp, 64);
+            sp -= take;
+            int e = lane < take ? stack[sp + lane] : -1;
+            int kind = 0;  // 0 none, 1 accept, 2 leaf, 3 open
+            int4 nd = make_int4(0, 0, 0, 0);
+            int v = 0, nb = 0;
+            if (e >= 0) {
+                v = e >> 5;
+                nb = e & 31;
+                float sx, sy, sz;
+                shift_of(nb, sx, sy, sz);
+                float4 c = A.com[v];
+                float rmax = A.q1[v].w;
+                nd = A.info[v];
+                float px = c.x + sx, py = c.y + sy, pz = c.z + sz;
+                float ddx = fmaxf(fmaxf(lox - px, px - hix), 0.f);
+                float ddy = fmaxf(fmaxf(loy - py, py - hiy), 0.f);
+                float ddz = fmaxf(fmaxf(loz - pz, pz - hiz), 0.f);
+                float dmin2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                float rr = A.rcut + rmax;
+                if (dmin2 > rr * rr) kind = 0;
+                else if (nd.y >= A.naccept && dmin2 >= rmax * rmax * A.inv_theta2) kind = 1;
+                else if (nd.z == 0) kind = 2;
+                else kind = 3;
+            }
+            // children onto the stack
+            {
+                int nch = kind == 3 ? nd.z : 0;
+                int tot;
+                int off = wave_excl_scan(nch, lane, tot);
+                if (sp + tot > STACK_MAX) {
+                    overflow = true;
+                    break;
+                }
+                for (int c = 0; c < nch; c++) stack[sp + off + c] = (A.child[8 * v + c] << 5) | nb;
+                sp += tot;
+                wave_sync();
+            }
+            // accepted nodes
+            {
+                int tot;
+                int off = wave_excl_scan(kind == 1 ? 1 : 0, lane, tot);
+                if (kind == 1) pcidx[pc_n + off] = e;
+                pc_n += tot;
+                wave_sync();
+                if (pc_n >= 64) {
+                    flush_pc(0, 64);
+                    pc_n -= 64;
+                    int t = (lane < pc_n) ? pcidx[64 + lane] : 0;
+                    wave_sync();
+                    if (lane < pc_n) pcidx[lane] = t;
+                    wave_sync();
+                }
+            }
+            // opened leaves: their particles
+            {
+                int rem = kind == 2 ? nd.y : 0;
+                int pb = nd.x;
+                for (;;) {
+                    int tk = min(rem, PP_TAKE);
+                    int tot;
+                    int off = wave_excl_scan(tk, lane, tot);
+                    if (tot == 0) break;
+                    if (pp_n + tot > PP_CAP) {
+                        int full = pp_n & ~63;
+                        for (int s = 0; s < full; s += 64) flush_pp(s, 64);
+                        int left = pp_n - full;
+                        uint32_t t = (lane < left) ? ppidx[full + lane] : 0u;
+                        wave_sync();
+                        if (lane < left) ppidx[lane] = t;
+                        wave_sync();
+                        pp_n = left;
+                    }
+                    for (int t = 0; t < tk; t++) ppidx[pp_n + off + t] = ((uint32_t)(pb + t) << 5) | (uint32_t)nb;
+                    pp_n += tot;
+                    pb += tk;
+                    rem -= tk;
+                    wave_sync();
+                }
+                if (pp_n >= 64) {
+                    int full = pp_n & ~63;
+                    for (int s = 0; s < full; s += 64) flush_pp(s, 64);
+                    int left = pp_n - full;
+                    uint32_t t = (lane < left) ? ppidx[full + lane] : 0u;
+                    wave_sync();
+                    if (lane < left) ppidx[lane] = t;
+                    wave_sync();
+                    pp_n = left;
+                }
+            }
+        }
+        if (pc_n > 0) flush_pc(0, pc_n);
+        if (pp_n > 0) flush_pp(0, pp_n);
+        if (overflow) {
+            if (lane == 0) atomicAdd(&A.counters[3], 1ull);
+        } else if (active) {
+            A.vx[me] += (double)(A.fact * ax);
+            A.vy[me] += (double)(A.fact * ay);
+            A.vz[me] += (double)(A.fact * az);
+        }
+        if (lane == 0) {
+            atomicAdd(&A.counters[0], npp * (unsigned long long)gr.y);
+            atomicAdd(&A.counters[1], npc * (unsigned long long)gr.y);
+        }
+    }
+}
+
+// Plain FP64 walk: one thread per target, private stack, same tree, groups and criteria.
+struct walk64_args {
+    const double *x, *y, *z;
+    double mass;
+    const int4* info;
+    const int* child;
+    const double* nd;
+    const int* cellroot;
+    const uint64_t* key;
+    int cell_shift;
+    const double* gbox;
+    const int* pgroup;
+    const double* tab;
+    double *vx, *vy, *vz;
+    unsigned long long* counters;
+    size_t np;
+    int mcell;
+    double box, cellw, rc2, rcut, inv_theta2, fact;
+    int naccept;
+};
+
+__global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.np) return;
+    const double px0 = A.x[i], py0 = A.y[i], pz0 = A.z[i];
+    const int g = A.pgroup[i];
+    const double lo[3] = {A.gbox[6 * g], A.gbox[6 * g + 1], A.gbox[6 * g + 2]};
+    const double hi[3] = {A.gbox[6 * g + 3], A.gbox[6 * g + 4], A.gbox[6 * g + 5]};
+    // all group members share a chaining cell: the level-L prefix of their Morton key
+    const uint64_t cellkey = A.key[i] >> A.cell_shift;
+    const int cx = (int)compact3(cellkey), cy = (int)compact3(cellkey >> 1), cz = (int)compact3(cellkey >> 2);
+    int stack[256];
+    int sp = 0;
+    for (int nb = 26; nb >= 0; nb--) {
+        int dx = nb % 3 - 1, dy = (nb / 3) % 3 - 1, dz = nb / 9 - 1;
+        int nx = (cx + dx + A.mcell) % A.mcell, ny = (cy + dy + A.mcell) % A.mcell, nz = (cz + dz + A.mcell) % A.mcell;
+        int r = A.cellroot[nx + A.mcell * (ny + A.mcell * nz)];
+        if (r >= 0) stack[sp++] = (r << 5) | nb;
+    }
+    double ax = 0, ay = 0, az = 0;
+    unsigned long long npp = 0, npc = 0;
+    const double ran = 6.0 / 4096.0;
+    bool overflow = false;
+    while (sp > 0) {
+        int e = stack[--sp];
+        int v = e >> 5, nb = e & 31;
+        int tx = cx + nb % 3 - 1, ty = cy + (nb / 3) % 3 - 1, tz = cz + nb / 9 - 1;
+        double sx = tx < 0 ? -A.box : (tx >= A.mcell ? A.box : 0.0);
+        double sy = ty < 0 ? -A.box : (ty >= A.mcell ? A.box : 0.0);
+        double sz = tz < 0 ? -A.box : (tz >= A.mcell ? A.box : 0.0);
+        const double* m = A.nd + 12 * (size_t)v;
+        int4 nd = A.info[v];
+        double qx = m[0] + sx, qy = m[1] + sy, qz = m[2] + sz, rmax = m[11];
+        double ddx = fmax(fmax(lo[0] - qx, qx - hi[0]), 0.0);
+        double ddy = fmax(fmax(lo[1] - qy, qy - hi[1]), 0.0);
+        double ddz = fmax(fmax(lo[2] - qz, qz - hi[2]), 0.0);
+        double dmin2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        double rr = A.rcut + rmax;
+        if (dmin2 > rr * rr) continue;
+        if (nd.y >= A.naccept && dmin2 >= rmax * rmax * A.inv_theta2) {
+            npc++;
+            double dx = px0 - qx, dy = py0 - qy, dz = pz0 - qz;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= A.rc2) {
+                double r = sqrt(r2);
+                int nt = min((int)(r / ran), 4095);
+                double t = r - nt * ran;
+                const double* tb = A.tab + 6 * nt;
+                double f1 = tb[0] + tb[1] * t, f2 = tb[2] + tb[3] * t, f3 = tb[4] + tb[5] * t;
+                double qx1 = m[4] * dx + m[7] * dy + m[8] * dz;
+                double qx2 = m[7] * dx + m[5] * dy + m[9] * dz;
+                double qx3 = m[8] * dx + m[9] * dy + m[6] * dz;
+                double qxy = m[4] * dx * dx + m[5] * dy * dy + m[6] * dz * dz + 2 * (m[7] * dx * dy + m[8] * dx * dz + m[9] * dy * dz);
+                double tmp = m[3] * f1 + m[10] * f2 + qxy * f3;
+                ax += dx * tmp + qx1 * 2 * f2;
+                ay += dy * tmp + qx2 * 2 * f2;
+                az += dz * tmp + qx3 * 2 * f2;
+            }
+        } else if (nd.z == 0) {
+            for (int p = nd.x; p < nd.x + nd.y; p++) {
+                npp++;
+                double dx = px0 - (A.x[p] + sx), dy = py0 - (A.y[p] + sy), dz = pz0 - (A.z[p] + sz);
+                double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= A.rc2) {
+                    double r = sqrt(r2);
+                    int nt = min((int)(r / ran), 4095);
+                    double t = r - nt * ran;
+                    double f = (A.tab[6 * nt] + A.tab[6 * nt + 1] * t) * A.mass;
+                    ax += dx * f; ay += dy * f; az += dz * f;
+                }
+            }
+        } else {
+            if (sp + nd.z > 256) {
+                overflow = true;
+                break;
+            }
+            for (int c = nd.z - 1; c >= 0; c--) stack[sp++] = (A.child[8 * v + c] << 5) | nb;
+        }
+    }
+    if (overflow) {
+        atomicAdd(&A.counters[3], 1ull);
+        return;
+    }
+    A.vx[i] += A.fact * ax;
+    A.vy[i] += A.fact * ay;
+    A.vz[i] += A.fact * az;
+    atomicAdd(&A.counters[0], npp);
+    atomicAdd(&A.counters[1], npc);
+}
+
+unsigned blocks_for(size_t n, int t) { return (unsigned)std::max<size_t>(1, (n + t - 1) / t); }
+
+template <typename T>
+int regrow(lagpm_ctx* ctx, T** p, size_t count) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    HIPCHK(ctx, hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+    return LAGPM_OK;
+}
+
+}  // namespace
+
+int lagpm_choose_cell_level(int n, double rsphere) {
+    int L = 0;
+    while (L < LAGPM_KEY_BITS && (double)n / (double)(1 << (L + 1)) >= rsphere) L++;
+    return L;
+}
+
+int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box) {
+    const size_t np = box->np;
+    if (np == 0) return LAGPM_OK;
+    const double s = 65536.0 / (double)box->p.n;
+    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, box->x, box->y, box->z, np, s,
+                       box->key, box->perm);
+    HIPCHK(ctx, hipGetLastError());
+    bool alt = false;
+    RCHK(lagpm_radix_sort_pairs(ctx, box->key, box->key2, box->perm, box->perm2, np, 3 * LAGPM_KEY_BITS, &alt));
+    if (alt) {
+        std::swap(box->key, box->key2);
+        std::swap(box->perm, box->perm2);
+    }
+    double** arrs[6] = {&box->x, &box->y, &box->z, &box->vx, &box->vy, &box->vz};
+    for (int a = 0; a < 6; a++) {
+        hipLaunchKernelGGL(gather_kernel<double>, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, *arrs[a], box->perm,
+                           np, box->dtmp);
+        HIPCHK(ctx, hipGetLastError());
+        std::swap(*arrs[a], box->dtmp);
+    }
+    hipLaunchKernelGGL(gather_kernel<uint32_t>, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, box->id, box->perm, np,
+                       box->utmp);
+    HIPCHK(ctx, hipGetLastError());
+    std::swap(box->id, box->utmp);
+    box->sorted = true;
+    return LAGPM_OK;
+}
+
+int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
+    const size_t np = box->np;
+    const int L = box->level_cell;
+    const int nleaf = box->tune.nleaf;
+    const int G = box->tune.group;
+    const bool f64 = box->p.precision == LAGPM_PREC_FP64;
+    if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
+    if (!f64) {
+        hipLaunchKernelGGL(pack_pp_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, box->x, box->y, box->z, np,
+                           (float)box->p.mass, box->pp);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    const size_t ncell = (size_t)box->mcell * box->mcell * box->mcell;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(ncell, 256)), dim3(256), 0, ctx->stream, box->cellroot, ncell, -1);
+    HIPCHK(ctx, hipGetLastError());
+    hipLaunchKernelGGL(node_count_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, box->key, np, L, nleaf,
+                       box->ncount);
+    HIPCHK(ctx, hipGetLastError());
+    uint32_t total = 0;
+    RCHK(lagpm_exclusive_scan_u32(ctx, box->ncount, box->noff, np, &total));
+    box->nnodes = total;
+    if (total > box->node_cap) {
+        size_t cap = (size_t)total + total / 4 + 1024;
+        RCHK(regrow(ctx, &box->nd_info, cap));
+        RCHK(regrow(ctx, &box->nd_parent, cap));
+        RCHK(regrow(ctx, &box->nd_child, 8 * cap));
+        RCHK(regrow(ctx, &box->gflag, cap));
+        RCHK(regrow(ctx, &box->goff, cap));
+        if (f64) {
+            RCHK(regrow(ctx, &box->nd_d, 12 * cap));
+        } else {
+            RCHK(regrow(ctx, &box->nd_com, cap));
+            RCHK(regrow(ctx, &box->nd_q0, cap));
+            RCHK(regrow(ctx, &box->nd_q1, cap));
+        }
+        box->node_cap = cap;
+    }
+    const size_t nn = total;
+    hipLaunchKernelGGL(node_write_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, box->key, np, L, nleaf,
+                       box->noff, box->nd_info, box->cellroot, box->mcell);
+    HIPCHK(ctx, hipGetLastError());
+    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, box->nd_parent, nn, -1);
+    HIPCHK(ctx, hipGetLastError());
+    hipLaunchKernelGGL(children_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, box->key, nn, L, nleaf,
+                       box->noff, box->nd_info, box->nd_child, box->nd_parent);
+    HIPCHK(ctx, hipGetLastError());
+    for (int l = LAGPM_KEY_BITS; l >= L; l--) {
+        if (f64)
+            hipLaunchKernelGGL(moments_f64_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, box->x, box->y,
+                               box->z, box->p.mass, nn, l, box->nd_info, box->nd_child, box->nd_d);
+        else
+            hipLaunchKernelGGL(moments_f32_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, box->pp, nn, l,
+                               box->nd_info, box->nd_child, box->nd_com, box->nd_q0, box->nd_q1);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    hipLaunchKernelGGL(group_count_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, nn, L, G, box->nd_info,
+                       box->nd_parent, box->gflag);
+    HIPCHK(ctx, hipGetLastError());
+    uint32_t ng = 0;
+    RCHK(lagpm_exclusive_scan_u32(ctx, box->gflag, box->goff, nn, &ng));
+    if (ng > box->group_cap) {
+        size_t cap = (size_t)ng + ng / 4 + 1024;
+        RCHK(regrow(ctx, &box->groups, cap));
+        if (f64) RCHK(regrow(ctx, &box->gbox, 6 * cap));
+        box->group_cap = cap;
+    }
+    box->ngroups = ng;
+    hipLaunchKernelGGL(group_write_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, nn, G, box->nd_info,
+                       box->gflag, box->goff, box->groups);
+    HIPCHK(ctx, hipGetLastError());
+    if (f64) {
+        hipLaunchKernelGGL(group_box_kernel, dim3(blocks_for(ng, 256)), dim3(256), 0, ctx->stream, box->x, box->y, box->z,
+                           (size_t)ng, box->groups, box->gbox, box->pgroup);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    box->stats.n_nodes = nn;
+    box->stats.n_groups = ng;
+    return LAGPM_OK;
+}
+
+int lagpm_tree_walk(lagpm_ctx* ctx, lagpm_box* box, double fact) {
+    const bool f64 = box->p.precision == LAGPM_PREC_FP64;
+    HIPCHK(ctx, hipMemsetAsync(box->counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    const double th = box->p.theta;
+    const double inv_theta2 = th > 0 ? 1.0 / (th * th) : 1e30;
+    const double cellw = (double)box->p.n / (double)box->mcell;
+    if (!f64) {
+        walk_args A;
+        A.pp = box->pp; A.info = box->nd_info; A.child = box->nd_child; A.com = box->nd_com; A.q0 = box->nd_q0; A.q1 = box->nd_q1;
+        A.cellroot = box->cellroot; A.groups = box->groups; A.tab0 = box->tab0; A.tab12 = box->tab12;
+        A.key = box->key; A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell);
+        A.vx = box->vx; A.vy = box->vy; A.vz = box->vz; A.stack = box->walk_stack; A.counters = box->counters;
+        A.ngroups = box->ngroups; A.mcell = box->mcell; A.box = (float)box->p.n; A.inv_cellw = (float)(1.0 / cellw);
+        A.rc2 = (float)(box->p.rsphere * box->p.rsphere); A.rcut = (float)box->p.rsphere;
+        A.inv_theta2 = (float)inv_theta2; A.fact = (float)fact; A.naccept = LAGPM_NACCEPT;
+        unsigned blocks = (unsigned)(box->walk_stack_entries / ((size_t)STACK_MAX * WALK_WAVES));
+        hipLaunchKernelGGL(walk_f32_kernel, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, A);
+        HIPCHK(ctx, hipGetLastError());
+    } else {
+        walk64_args A;
+        A.x = box->x; A.y = box->y; A.z = box->z; A.mass = box->p.mass; A.info = box->nd_info; A.child = box->nd_child;
+        A.nd = box->nd_d; A.cellroot = box->cellroot; A.gbox = box->gbox;
+        A.key = box->key; A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell); A.pgroup = box->pgroup; A.tab = box->tabd;
+        A.vx = box->vx; A.vy = box->vy; A.vz = box->vz; A.counters = box->counters; A.np = box->np; A.mcell = box->mcell;
+        A.box = (double)box->p.n; A.cellw = cellw; A.rc2 = box->p.rsphere * box->p.rsphere; A.rcut = box->p.rsphere;
+        A.inv_theta2 = inv_theta2; A.fact = fact; A.naccept = LAGPM_NACCEPT;
+        hipLaunchKernelGGL(walk_f64_kernel, dim3(blocks_for(box->np, 128)), dim3(128), 0, ctx->stream, A);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    return LAGPM_OK;
+}
+
+size_t lagpm_walk_stack_entries(int num_cu) {
+    size_t blocks = (size_t)std::max(1, num_cu) * 4;  // 4 workgroups (16 waves) per CU
+    return blocks * WALK_WAVES * STACK_MAX;
+}

@@ -1,0 +1,352 @@
+"""Python mirror of the reference's TreePM gravity entry points over the C ABI (include/lagpm.h).
+
+Names follow the reference interface they replace:
+  pmmain          Cosmos/pmmain.c:390        Box.pmmain(evolfact, pmstatus)
+  TSC             Cosmos/pmmain.c:112,188    Box.tsc()
+  psolver         Cosmos/psolver.c:13        Box.psolver()
+  fda4            Cosmos/fda4.c:31           Box.fda4(evolfact, pmstatus)
+  treecorrection  Cosmos/treecorrection.c:225 Box.treecorrection(evolfact)
+  i_force_spline  OST/Model4TreeForce.c:48   i_force_spline(params)
+  GetEvolFactor   Cosmos/CosmosEvolFactor.c:230 get_evol_factor(params, anow, astep)
+
+Errors: the reference exits the process on error; here every failing call raises LagpmError
+carrying the library's status code and message. The shared library is loaded from this
+package directory only; a missing library raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+KICK, PUSH, PULL = 0, 1, 2
+LAW_PMFIT, LAW_ERFC = 0, 1
+PREC_FP32, PREC_FP64 = 0, 1
+NSPLINE = 4096
+
+_STATUS = {
+    0: "OK", -1: "LAGPM_E_INVAL", -2: "LAGPM_E_NOMEM", -3: "LAGPM_E_HIP", -4: "LAGPM_E_FFT",
+    -5: "LAGPM_E_STATE", -6: "LAGPM_E_OVERFLOW",
+}
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblagpm.so")
+
+
+class LagpmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32), ("law", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("deconvolve", ctypes.c_int32), ("mass", ctypes.c_double), ("efold", ctypes.c_double),
+        ("epsilon", ctypes.c_double), ("rsphere", ctypes.c_double), ("theta", ctypes.c_double),
+        ("amax", ctypes.c_double), ("omep", ctypes.c_double), ("omeplam", ctypes.c_double),
+    ]
+
+
+class EvolFact(ctypes.Structure):
+    """Same fields as EvolFact, Cosmos/CosmosEvolFactor.h:1-6."""
+    _fields_ = [(k, ctypes.c_float) for k in (
+        "afact", "bfact", "pfact", "fact1", "fact2", "fact1_push", "fact2_push", "fact1_pull", "fact2_pull")]
+
+
+class TreeStats(ctypes.Structure):
+    _fields_ = [("n_pp", ctypes.c_uint64), ("n_pc", ctypes.c_uint64), ("n_nodes", ctypes.c_uint64),
+                ("n_groups", ctypes.c_uint64), ("ms_build", ctypes.c_double), ("ms_walk", ctypes.c_double)]
+
+
+class Timings(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in (
+        "ms_sort", "ms_tsc", "ms_fft_r2c", "ms_green", "ms_fft_c2r", "ms_grad", "ms_interp",
+        "ms_tree_sort", "ms_build", "ms_walk")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# every symbol of include/lagpm.h with its ctypes signature
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
+SIGNATURES = {
+    "lagpm_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int]),
+    "lagpm_destroy": (ctypes.c_int, [_P]),
+    "lagpm_last_error": (ctypes.c_char_p, [_P]),
+    "lagpm_abi_version": (ctypes.c_int, []),
+    "lagpm_stream": (_P, [_P]),
+    "lagpm_synchronize": (ctypes.c_int, [_P]),
+    "lagpm_box_create": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "lagpm_box_destroy": (ctypes.c_int, [_P, _P]),
+    "lagpm_box_params": (ctypes.c_int, [_P, ctypes.POINTER(Params)]),
+    "lagpm_box_np": (ctypes.c_size_t, [_P]),
+    "lagpm_box_device_bytes": (ctypes.c_size_t, [_P]),
+    "lagpm_estimate_box_bytes": (ctypes.c_size_t, [ctypes.POINTER(Params), ctypes.c_size_t]),
+    "lagpm_box_upload": (ctypes.c_int, [_P, _P, _D, _D, _D, _D, _D, _D]),
+    "lagpm_box_download": (ctypes.c_int, [_P, _P, _D, _D, _D, _D, _D, _D]),
+    "lagpm_box_upload_eunha": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.c_long]),
+    "lagpm_box_download_eunha_vel": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "lagpm_pmmain": (ctypes.c_int, [_P, _P, ctypes.POINTER(EvolFact), ctypes.c_int]),
+    "lagpm_tsc": (ctypes.c_int, [_P, _P, _F]),
+    "lagpm_tsc_fixed": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]),
+    "lagpm_psolver": (ctypes.c_int, [_P, _P, _F]),
+    "lagpm_green_apply": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_int, _F]),
+    "lagpm_fda4": (ctypes.c_int, [_P, _P, ctypes.POINTER(EvolFact), ctypes.c_int]),
+    "lagpm_set_potential": (ctypes.c_int, [_P, _P, _F]),
+    "lagpm_get_gradient": (ctypes.c_int, [_P, _P, _F]),
+    "lagpm_force_spline": (ctypes.c_int, [ctypes.POINTER(Params), _D, _D]),
+    "lagpm_treecorrection": (ctypes.c_int, [_P, _P, ctypes.POINTER(EvolFact)]),
+    "lagpm_tree_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(TreeStats)]),
+    "lagpm_box_timings": (ctypes.c_int, [_P, _P, ctypes.POINTER(Timings)]),
+    "lagpm_evolfactor": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_double, ctypes.c_double,
+                                        ctypes.POINTER(EvolFact)]),
+    "lagpm_hubble_drag": (ctypes.c_int, [_P, _P, ctypes.c_double]),
+    "lagpm_drift": (ctypes.c_int, [_P, _P, ctypes.c_double]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load liblagpm.so from the package directory; raise when it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LagpmError(-5, f"{path} is missing: build it with __graft_entry__.build() "
+                             "(make -C lageunha_amd/csrc)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D) if a is not None else None
+
+
+def _fp(a):
+    return a.ctypes.data_as(_F) if a is not None else None
+
+
+def make_params(n, *, law=LAW_ERFC, precision=PREC_FP32, deconvolve=1, mass=1.0, efold=2.5,
+                epsilon=0.1, rsphere=5.625, theta=0.5, amax=48.0, omep=0.26, omeplam=0.74) -> Params:
+    return Params(n, law, precision, deconvolve, mass, efold, epsilon, rsphere, theta, amax, omep, omeplam)
+
+
+def reference_params(n, *, precision=PREC_FP32, theta=0.3, mass=1.0) -> Params:
+    """The reference defaults: efold 0.9, eps 0.1, RSPHERE 4, pmcf law, no deconvolution
+    (Params/header.c:501-505,651-653). efold/eps/theta/rsphere are floats there."""
+    f = lambda v: float(np.float32(v))  # noqa: E731
+    return Params(n, LAW_PMFIT, precision, 0, mass, f(0.9), f(0.1), f(4.0), f(theta), 48.0, 0.26, 0.74)
+
+
+def evolfact(fact2=1.0, fact1=1.0, **kw) -> EvolFact:
+    ef = EvolFact()
+    ef.fact1, ef.fact2 = fact1, fact2
+    ef.fact1_push = kw.get("fact1_push", 1.0)
+    ef.fact2_push = kw.get("fact2_push", fact2)
+    ef.fact1_pull = kw.get("fact1_pull", 1.0)
+    ef.fact2_pull = kw.get("fact2_pull", fact2)
+    ef.pfact = kw.get("pfact", 0.0)
+    return ef
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = _P()
+        rc = self.lib.lagpm_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise LagpmError(rc, f"lagpm_create(device={device}) failed")
+        self.h = h
+
+    def check(self, rc):
+        if rc != 0:
+            raise LagpmError(rc, self.lib.lagpm_last_error(self.h).decode())
+
+    def synchronize(self):
+        self.check(self.lib.lagpm_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lagpm_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Box:
+    """One periodic box with resident particle, mesh and tree state on the device."""
+
+    def __init__(self, ctx: Context, params: Params, np_: int):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.params = params
+        self.np = int(np_)
+        self.n = params.n
+        h = _P()
+        ctx.check(self.lib.lagpm_box_create(ctx.h, ctypes.byref(params), self.np, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lagpm_box_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def device_bytes(self):
+        return self.lib.lagpm_box_device_bytes(self.h)
+
+    def upload(self, x, y, z, vx=None, vy=None, vz=None):
+        a = [np.ascontiguousarray(v, dtype=np.float64) if v is not None else None for v in (x, y, z, vx, vy, vz)]
+        for v in a:
+            if v is not None and v.shape != (self.np,):
+                raise ValueError(f"expected arrays of shape ({self.np},)")
+        self.ctx.check(self.lib.lagpm_box_upload(self.ctx.h, self.h, *[_dp(v) for v in a]))
+
+    def upload_pos(self, pos, vel=None):
+        pos = np.asarray(pos, dtype=np.float64)
+        if vel is None:
+            self.upload(pos[:, 0], pos[:, 1], pos[:, 2])
+        else:
+            vel = np.asarray(vel, dtype=np.float64)
+            self.upload(pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0], vel[:, 1], vel[:, 2])
+
+    def download(self):
+        out = [np.empty(self.np) for _ in range(6)]
+        self.ctx.check(self.lib.lagpm_box_download(self.ctx.h, self.h, *[_dp(v) for v in out]))
+        return np.stack(out[:3], axis=1), np.stack(out[3:], axis=1)
+
+    def velocities(self):
+        return self.download()[1]
+
+    def upload_eunha(self, records: np.ndarray):
+        records = np.ascontiguousarray(records)
+        self.ctx.check(self.lib.lagpm_box_upload_eunha(self.ctx.h, self.h, records.ctypes.data_as(_P),
+                                                       records.shape[0], self.n))
+
+    def download_eunha_vel(self, records: np.ndarray):
+        self.ctx.check(self.lib.lagpm_box_download_eunha_vel(self.ctx.h, self.h, records.ctypes.data_as(_P),
+                                                             records.shape[0]))
+
+    # ---- the drop-in entry points ----
+    def pmmain(self, ef: EvolFact, pmstatus: int = KICK):
+        self.ctx.check(self.lib.lagpm_pmmain(self.ctx.h, self.h, ctypes.byref(ef), pmstatus))
+
+    def tsc(self):
+        den = np.empty(self.n ** 3, dtype=np.float32)
+        self.ctx.check(self.lib.lagpm_tsc(self.ctx.h, self.h, _fp(den)))
+        return den.reshape(self.n, self.n, self.n)  # [z][y][x]
+
+    def tsc_fixed(self):
+        acc = np.empty(self.n ** 3, dtype=np.int64)
+        s = ctypes.c_int()
+        self.ctx.check(self.lib.lagpm_tsc_fixed(self.ctx.h, self.h, acc.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                ctypes.byref(s)))
+        return acc, s.value
+
+    def psolver(self):
+        phi = np.empty(self.n ** 3, dtype=np.float32)
+        self.ctx.check(self.lib.lagpm_psolver(self.ctx.h, self.h, _fp(phi)))
+        return phi
+
+    def fda4(self, ef: EvolFact, pmstatus: int = KICK):
+        self.ctx.check(self.lib.lagpm_fda4(self.ctx.h, self.h, ctypes.byref(ef), pmstatus))
+
+    def set_potential(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float32).ravel()
+        self.ctx.check(self.lib.lagpm_set_potential(self.ctx.h, self.h, _fp(phi)))
+
+    def get_gradient(self):
+        g = np.empty(4 * self.n ** 3, dtype=np.float32)
+        self.ctx.check(self.lib.lagpm_get_gradient(self.ctx.h, self.h, _fp(g)))
+        return g.reshape(-1, 4)
+
+    def treecorrection(self, ef: EvolFact):
+        self.ctx.check(self.lib.lagpm_treecorrection(self.ctx.h, self.h, ctypes.byref(ef)))
+
+    def tree_stats(self) -> TreeStats:
+        s = TreeStats()
+        self.lib.lagpm_tree_stats_get(self.h, ctypes.byref(s))
+        return s
+
+    def timings(self) -> Timings:
+        t = Timings()
+        self.ctx.check(self.lib.lagpm_box_timings(self.ctx.h, self.h, ctypes.byref(t)))
+        return t
+
+    def hubble_drag(self, fact1: float):
+        self.ctx.check(self.lib.lagpm_hubble_drag(self.ctx.h, self.h, fact1))
+
+    def drift(self, pfact: float):
+        self.ctx.check(self.lib.lagpm_drift(self.ctx.h, self.h, pfact))
+
+
+def green_apply(ctx: Context, n: int, efold: float, deconvolve: int, spec: np.ndarray) -> np.ndarray:
+    """Green multiply on a host spectrum (complex64, shape (n, n, n//2+1))."""
+    s = np.ascontiguousarray(spec, dtype=np.complex64).copy()
+    ctx.check(ctx.lib.lagpm_green_apply(ctx.h, n, efold, deconvolve, s.view(np.float32).ctypes.data_as(_F)))
+    return s
+
+
+def i_force_spline(params: Params):
+    """FP64 tables diff[4096][3], slope[4096][3] (OST/Model4TreeForce.c:48-91)."""
+    lib = load()
+    diff = np.empty((NSPLINE, 3))
+    slope = np.empty((NSPLINE, 3))
+    rc = lib.lagpm_force_spline(ctypes.byref(params), _dp(diff), _dp(slope))
+    if rc != 0:
+        raise LagpmError(rc, "lagpm_force_spline failed")
+    return diff, slope
+
+
+def get_evol_factor(params: Params, anow: float, astep: float) -> EvolFact:
+    lib = load()
+    ef = EvolFact()
+    rc = lib.lagpm_evolfactor(ctypes.byref(params), anow, astep, ctypes.byref(ef))
+    if rc != 0:
+        raise LagpmError(rc, "lagpm_evolfactor failed")
+    return ef
+
+
+def estimate_box_bytes(params: Params, np_: int) -> int:
+    return int(load().lagpm_estimate_box_bytes(ctypes.byref(params), int(np_)))
+
+
+@dataclass
+class ForceResult:
+    pm: np.ndarray
+    tree: np.ndarray
+
+    @property
+    def total(self):
+        return self.pm + self.tree
+
+
+def treepm_accel(ctx: Context, params: Params, pos: np.ndarray) -> ForceResult:
+    """Convenience: PM and tree accelerations (fact2 = 1, zero velocities) in caller order."""
+    with Box(ctx, params, pos.shape[0]) as b:
+        b.upload_pos(pos)
+        b.pmmain(evolfact(1.0), KICK)
+        v_pm = b.velocities()
+        b.upload_pos(pos)
+        b.treecorrection(evolfact(1.0))
+        v_tree = b.velocities()
+    return ForceResult(v_pm, v_tree)

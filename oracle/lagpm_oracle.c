@@ -1,0 +1,1040 @@
+/*
+ * lagpm_oracle.c — CPU restatement of the GOTPM TreePM gravity step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the timed CPU baseline.
+ * Nothing in the product (lageunha_amd/, liblagpm.so) links, loads or calls this file.
+ *
+ * Written from the behaviour of the reference files cited per function; no reference
+ * source is copied. Built with -ffp-contract=off so float expressions round exactly as
+ * the reference's C statements do (one rounding per operation, C promotion rules).
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   TSC deposit       partition of unity, exact mass, closed-form weights, whole-cell
+ *                     translation, brute-force per-particle stencil, fixed == float sum
+ *   FFT               numpy.fft on random data
+ *   psolver           single Fourier mode -> analytic Green factor, w=0 rule, +-k symmetry
+ *   fda4              sine mode -> (4/3) sin k - (1/6) sin 2k transfer function; constant -> 0;
+ *                     momentum sum; brute-force stencil on tiny meshes
+ *   PM total          point mass -> erf long-range law; zero self force; Ewald long-range
+ *   force tables      closed-form pmcf / erfc-law evaluation; derivative consistency
+ *   direct pair sum   analytic FP64 law; antisymmetry (Newton's third law)
+ *   reference walk    theta -> 0 equals the direct pair sum; quadrupole error O((s/d)^3)
+ *   Ewald             independence of the splitting parameter alpha; lattice symmetry
+ *   kick factors      Einstein-de Sitter closed forms; scipy quadrature for LCDM
+ */
+#include "lagpm_oracle.h"
+
+#include <float.h>
+#include <limits.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+void orc_set_threads(int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+
+static inline long wrapl(long i, long n) {
+    long r = i % n;
+    return r < 0 ? r + n : r;
+}
+/* icopysign, Cosmos/cosmology.h:1 — sign bit decides, so +0 -> +1 and -0 -> -1 */
+static inline int sgn_of(double d) { return signbit(d) ? -1 : 1; }
+
+/* ------------------------------------------------------------------------------------
+ * TSC deposit — Cosmos/pmmain.h:3-78 (Get_TSC_Den): nearest node by rint, weights
+ * w1 = 0.75 - d^2, w3 = 0.5 (0.25 + d (d - 1)), w2 = w3 + d at near, near+s, near-s,
+ * mass folded into the x weights. Positions are in [0,n); indices wrap periodically,
+ * which equals the ghost fold of RkS/gmigrate.c:212-275 on one rank.
+ * ------------------------------------------------------------------------------------ */
+typedef struct tsc_axis { long i[3]; float w[3]; } tsc_axis;
+
+static inline void tsc_axis_mass(double xp, float pmas0, tsc_axis* a) {
+    long near = (long)rint(xp);
+    double xmin = xp - (double)near;
+    int s = sgn_of(xmin);
+    float xd1 = (float)fabs(xmin);
+    float p05 = (float)((double)pmas0 * 0.5);
+    float w1 = (float)((0.75 - (double)(xd1 * xd1)) * (double)pmas0);
+    float w3 = (float)((double)p05 * (0.25 + (double)xd1 * ((double)xd1 - 1.0)));
+    float w2 = w3 + pmas0 * xd1;
+    a->i[0] = near; a->i[1] = near + s; a->i[2] = near - s;
+    a->w[0] = w1; a->w[1] = w2; a->w[2] = w3;
+}
+static inline void tsc_axis_unit(double xp, tsc_axis* a) {
+    long near = (long)rint(xp);
+    double xmin = xp - (double)near;
+    int s = sgn_of(xmin);
+    float xd1 = (float)fabs(xmin);
+    float w1 = (float)(0.75 - (double)(xd1 * xd1));
+    float w3 = (float)(0.5 * (0.25 + (double)xd1 * ((double)xd1 - 1.0)));
+    float w2 = w3 + xd1;
+    a->i[0] = near; a->i[1] = near + s; a->i[2] = near - s;
+    a->w[0] = w1; a->w[1] = w2; a->w[2] = w3;
+}
+
+int orc_fixed_scale_bits(double total_mass) {
+    /* largest S with total_mass * 2^S < 2^61: no cell sum can overflow int64 */
+    int e = 0;
+    if (total_mass > 0) e = (int)ceil(log2(total_mass));
+    int s = 61 - e;
+    if (s > 40) s = 40;
+    if (s < 8) s = 8;
+    return s;
+}
+
+void orc_tsc_fixed(int n, size_t np, const double* x, const double* y, const double* z,
+                   float mass, int scale_bits, int64_t* acc) {
+    size_t m = (size_t)n * n * n;
+    memset(acc, 0, m * sizeof(int64_t));
+    for (size_t p = 0; p < np; p++) {
+        tsc_axis ax, ay, az;
+        tsc_axis_mass(x[p], mass, &ax);
+        tsc_axis_unit(y[p], &ay);
+        tsc_axis_unit(z[p], &az);
+        for (int c = 0; c < 3; c++) {
+            long kk = wrapl(az.i[c], n);
+            for (int b = 0; b < 3; b++) {
+                long jj = wrapl(ay.i[b], n);
+                for (int a = 0; a < 3; a++) {
+                    long ii = wrapl(ax.i[a], n);
+                    float wxy = ax.w[a] * ay.w[b];
+                    float cval = wxy * az.w[c];
+                    acc[ii + (size_t)n * (jj + (size_t)n * kk)] += llrint(ldexp((double)cval, scale_bits));
+                }
+            }
+        }
+    }
+}
+
+void orc_fixed_to_density(int n, const int64_t* acc, int scale_bits, float* den) {
+    size_t m = (size_t)n * n * n;
+    for (size_t i = 0; i < m; i++) den[i] = (float)(ldexp((double)acc[i], -scale_bits) - 1.0);
+}
+
+/* Float deposit into a ghosted (n+3)^3 grid starting at -1 (MakeDenGrid4TSC with
+ * buffers 1,2: Cosmos/pmmain.c:75-110,436), then fold onto n^3 initialised to -1
+ * (Cosmos/pmmain.c:304). */
+void orc_tsc_float(int n, size_t np, const double* x, const double* y, const double* z,
+                   float mass, float* den) {
+    long g = n + 3;
+    float* gh = (float*)calloc((size_t)g * g * g, sizeof(float));
+    for (size_t p = 0; p < np; p++) {
+        tsc_axis ax, ay, az;
+        tsc_axis_mass(x[p] + 1.0, mass, &ax);
+        tsc_axis_unit(y[p] + 1.0, &ay);
+        tsc_axis_unit(z[p] + 1.0, &az);
+        for (int c = 0; c < 3; c++)
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++) {
+                    float wxy = ax.w[a] * ay.w[b];
+                    gh[ax.i[a] + g * (ay.i[b] + g * az.i[c])] += wxy * az.w[c];
+                }
+    }
+    size_t m = (size_t)n * n * n;
+    for (size_t i = 0; i < m; i++) den[i] = -1.0f;
+    for (long k = 0; k < g; k++)
+        for (long j = 0; j < g; j++)
+            for (long i = 0; i < g; i++) {
+                long ii = wrapl(i - 1, n), jj = wrapl(j - 1, n), kk = wrapl(k - 1, n);
+                den[ii + (size_t)n * (jj + (size_t)n * kk)] += gh[i + g * (j + g * k)];
+            }
+    free(gh);
+}
+
+/* ------------------------------------------------------------------------------------
+ * FFT: iterative radix-2 for powers of two, direct DFT otherwise. Unnormalised like
+ * FFTW (the reference scales by 1/n^3 inside psolver, Cosmos/psolver.c:31,71-72).
+ * ------------------------------------------------------------------------------------ */
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+static void fft1d(int n, double* re, double* im, int sign, double* twr, double* twi, double* sr, double* si) {
+    if (is_pow2(n)) {
+        for (int i = 1, j = 0; i < n; i++) {
+            int bit = n >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) {
+                double t = re[i]; re[i] = re[j]; re[j] = t;
+                t = im[i]; im[i] = im[j]; im[j] = t;
+            }
+        }
+        for (int len = 2; len <= n; len <<= 1) {
+            int step = n / len;
+            for (int i = 0; i < n; i += len)
+                for (int k = 0; k < len / 2; k++) {
+                    double wr = twr[k * step], wi = sign * twi[k * step];
+                    double ur = re[i + k], ui = im[i + k];
+                    double vr = re[i + k + len / 2] * wr - im[i + k + len / 2] * wi;
+                    double vi = re[i + k + len / 2] * wi + im[i + k + len / 2] * wr;
+                    re[i + k] = ur + vr; im[i + k] = ui + vi;
+                    re[i + k + len / 2] = ur - vr; im[i + k + len / 2] = ui - vi;
+                }
+        }
+    } else {
+        for (int k = 0; k < n; k++) {
+            double ar = 0, ai = 0;
+            for (int j = 0; j < n; j++) {
+                int t = (int)(((long)j * k) % n);
+                double wr = twr[t], wi = sign * twi[t];
+                ar += re[j] * wr - im[j] * wi;
+                ai += re[j] * wi + im[j] * wr;
+            }
+            sr[k] = ar; si[k] = ai;
+        }
+        memcpy(re, sr, n * sizeof(double));
+        memcpy(im, si, n * sizeof(double));
+    }
+}
+
+void orc_fft3d(int n, double* data, int sign) {
+    double* twr = (double*)malloc(n * sizeof(double));
+    double* twi = (double*)malloc(n * sizeof(double));
+    for (int t = 0; t < n; t++) { twr[t] = cos(2 * M_PI * t / n); twi[t] = sin(2 * M_PI * t / n); }
+    size_t nn = (size_t)n * n;
+    for (int axis = 0; axis < 3; axis++) {
+#pragma omp parallel
+        {
+            double* re = (double*)malloc(n * sizeof(double));
+            double* im = (double*)malloc(n * sizeof(double));
+            double* sr = (double*)malloc(n * sizeof(double));
+            double* si = (double*)malloc(n * sizeof(double));
+#pragma omp for schedule(static)
+            for (long line = 0; line < (long)nn; line++) {
+                size_t base, stride;
+                long a = line % n, b = line / n;
+                if (axis == 0) { base = (size_t)n * line; stride = 1; }
+                else if (axis == 1) { base = a + nn * b; stride = n; }
+                else { base = a + (size_t)n * b; stride = nn; }
+                for (int t = 0; t < n; t++) { re[t] = data[2 * (base + t * stride)]; im[t] = data[2 * (base + t * stride) + 1]; }
+                fft1d(n, re, im, sign, twr, twi, sr, si);
+                for (int t = 0; t < n; t++) { data[2 * (base + t * stride)] = re[t]; data[2 * (base + t * stride) + 1] = im[t]; }
+            }
+            free(re); free(im); free(sr); free(si);
+        }
+    }
+    free(twr); free(twi);
+}
+
+void orc_r2c(int n, const double* in, double* out) {
+    size_t m = (size_t)n * n * n;
+    double* c = (double*)malloc(2 * m * sizeof(double));
+    for (size_t i = 0; i < m; i++) { c[2 * i] = in[i]; c[2 * i + 1] = 0; }
+    orc_fft3d(n, c, -1);
+    int h = n / 2 + 1;
+    for (long k = 0; k < n; k++)
+        for (long j = 0; j < n; j++)
+            for (long i = 0; i < h; i++) {
+                size_t s = i + (size_t)n * (j + (size_t)n * k), d = i + (size_t)h * (j + (size_t)n * k);
+                out[2 * d] = c[2 * s]; out[2 * d + 1] = c[2 * s + 1];
+            }
+    free(c);
+}
+
+void orc_c2r(int n, const double* in, double* out) {
+    size_t m = (size_t)n * n * n;
+    int h = n / 2 + 1;
+    double* c = (double*)malloc(2 * m * sizeof(double));
+    for (long k = 0; k < n; k++)
+        for (long j = 0; j < n; j++)
+            for (long i = 0; i < n; i++) {
+                size_t d = i + (size_t)n * (j + (size_t)n * k);
+                if (i < h) {
+                    size_t s = i + (size_t)h * (j + (size_t)n * k);
+                    c[2 * d] = in[2 * s]; c[2 * d + 1] = in[2 * s + 1];
+                } else { /* Hermitian symmetry: X[i,j,k] = conj X[-i,-j,-k] */
+                    size_t s = (n - i) + (size_t)h * (wrapl(-j, n) + (size_t)n * wrapl(-k, n));
+                    c[2 * d] = in[2 * s]; c[2 * d + 1] = -in[2 * s + 1];
+                }
+            }
+    orc_fft3d(n, c, +1);
+    for (size_t i = 0; i < m; i++) out[i] = c[2 * i];
+    free(c);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Green's function, Cosmos/psolver.c:13-86: w = i^2 + wy^2 + wz^2 (integer modes,
+ * wrapped above n/2), factor = (-1/pi)/w * exp(-w (pi efold/n)^2), times 1/n^3, DC = 0.
+ * deconvolve (this build's option): divide by the TSC window squared per axis,
+ * W(k) = sinc^3(k/2), k = 2 pi m / n.
+ * ------------------------------------------------------------------------------------ */
+static double tsc_w2(int n, double m) {
+    if (m == 0) return 1.0;
+    double h = M_PI * m / n; /* k/2 */
+    double s = sin(h) / h;
+    double s3 = s * s * s;
+    return s3 * s3;
+}
+
+static void green_loop(int n, double efold, int deconvolve, float* fs, double* ds) {
+    double rn = n;
+    double fact = -1.0 / M_PI;
+    double sfact = -pow(M_PI * efold / rn, 2);
+    double irngc = 1.0 / (rn * rn * rn);
+    int h = n / 2 + 1;
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < n; k++) {
+        float wz = (float)k;
+        if (wz > rn / 2) wz = wz - (float)rn;
+        for (long j = 0; j < n; j++) {
+            float wy = (float)j;
+            if (wy > rn / 2) wy = wy - (float)rn;
+            for (long i = 0; i < h; i++) {
+                size_t q = i + (size_t)h * (j + (size_t)n * k);
+                double wave = (double)((float)(i * i) + wy * wy + wz * wz);
+                if (wave == 0.0) {
+                    if (fs) { fs[2 * q] = 0; fs[2 * q + 1] = 0; }
+                    else { ds[2 * q] = 0; ds[2 * q + 1] = 0; }
+                    continue;
+                }
+                double factr = fact / wave;
+                double factor = factr * exp(wave * sfact);
+                if (deconvolve) factor /= tsc_w2(n, (double)i) * tsc_w2(n, (double)wy) * tsc_w2(n, (double)wz);
+                if (fs) {
+                    fs[2 * q] = (float)(factor * (double)fs[2 * q] * irngc);
+                    fs[2 * q + 1] = (float)(factor * (double)fs[2 * q + 1] * irngc);
+                } else {
+                    ds[2 * q] = factor * ds[2 * q] * irngc;
+                    ds[2 * q + 1] = factor * ds[2 * q + 1] * irngc;
+                }
+            }
+        }
+    }
+}
+void orc_psolver_f32(int n, double efold, int deconvolve, float* spec) { green_loop(n, efold, deconvolve, spec, NULL); }
+void orc_psolver_f64(int n, double efold, int deconvolve, double* spec) { green_loop(n, efold, deconvolve, NULL, spec); }
+
+/* ------------------------------------------------------------------------------------
+ * FDA4 gradient, Cosmos/fda4.c:46-47 and fda4.h:302-384:
+ * F = (2/3)(phi[i-1] - phi[i+1]) + (-1/12)(phi[i-2] - phi[i+2]), periodic.
+ * ------------------------------------------------------------------------------------ */
+#define PHI(a, b, c) phi[wrapl(a, n) + (size_t)n * (wrapl(b, n) + (size_t)n * wrapl(c, n))]
+static inline float fda4_f(float m1, float p1, float m2, float p2) {
+    const float ffact1 = (float)((4. / 3.) / 2.);
+    const float ffact2 = (float)((-1. / 3.) / 4.);
+    float a = ffact1 * (m1 - p1);
+    float b = ffact2 * (m2 - p2);
+    return a + b;
+}
+void orc_fda4_gradient(int n, const float* phi, float* grad4) {
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < n; k++)
+        for (long j = 0; j < n; j++)
+            for (long i = 0; i < n; i++) {
+                size_t q = i + (size_t)n * (j + (size_t)n * k);
+                grad4[4 * q + 0] = fda4_f(PHI(i - 1, j, k), PHI(i + 1, j, k), PHI(i - 2, j, k), PHI(i + 2, j, k));
+                grad4[4 * q + 1] = fda4_f(PHI(i, j - 1, k), PHI(i, j + 1, k), PHI(i, j - 2, k), PHI(i, j + 2, k));
+                grad4[4 * q + 2] = fda4_f(PHI(i, j, k - 1), PHI(i, j, k + 1), PHI(i, j, k - 2), PHI(i, j, k + 2));
+                grad4[4 * q + 3] = phi[q];
+            }
+}
+
+/* Interpolation weights of FDA4_HEAD (fda4.h:34-102): xmin is stored as float there. */
+static inline void fda4_axis(double xp, tsc_axis* a) {
+    long near = (long)rint(xp);
+    float xmin = (float)(xp - (double)near);
+    int s = sgn_of((double)xmin);
+    float xd1 = (float)fabs((double)xmin);
+    float w1 = (float)(0.75 - (double)(xd1 * xd1));
+    float w3 = (float)(0.5 * (0.25 + (double)xd1 * ((double)xd1 - 1.0)));
+    float w2 = w3 + xd1;
+    a->i[0] = near; a->i[1] = near + s; a->i[2] = near - s;
+    a->w[0] = w1; a->w[1] = w2; a->w[2] = w3;
+}
+
+void orc_fda4_kick(int n, const float* phi, size_t np, const double* x, const double* y,
+                   const double* z, float fact2, float* dvx, float* dvy, float* dvz) {
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < (long)np; p++) {
+        tsc_axis ax, ay, az;
+        fda4_axis(x[p], &ax);
+        fda4_axis(y[p], &ay);
+        fda4_axis(z[p], &az);
+        /* wt order: k outer, j middle, i inner (wt1..wt27 of FDA4_HEAD) */
+        float fx = 0, fy = 0, fz = 0;
+        int first = 1;
+        for (int c = 0; c < 3; c++)
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++) {
+                    float wxy = ax.w[a] * ay.w[b];
+                    float wt = wxy * az.w[c];
+                    long i = ax.i[a], j = ay.i[b], k = az.i[c];
+                    float gx = fda4_f(PHI(i - 1, j, k), PHI(i + 1, j, k), PHI(i - 2, j, k), PHI(i + 2, j, k));
+                    float gy = fda4_f(PHI(i, j - 1, k), PHI(i, j + 1, k), PHI(i, j - 2, k), PHI(i, j + 2, k));
+                    float gz = fda4_f(PHI(i, j, k - 1), PHI(i, j, k + 1), PHI(i, j, k - 2), PHI(i, j, k + 2));
+                    if (first) { fx = wt * gx; fy = wt * gy; fz = wt * gz; first = 0; }
+                    else { fx = fx + wt * gx; fy = fy + wt * gy; fz = fz + wt * gz; }
+                }
+        dvx[p] = fact2 * fx;
+        dvy[p] = fact2 * fy;
+        dvz[p] = fact2 * fz;
+    }
+}
+#undef PHI
+
+#define PHID(a, b, c) phi[wrapl(a, n) + (size_t)n * (wrapl(b, n) + (size_t)n * wrapl(c, n))]
+static inline double fda4_d(double m1, double p1, double m2, double p2) {
+    return (2.0 / 3.0) * (m1 - p1) + (-1.0 / 12.0) * (m2 - p2);
+}
+static inline void axis_d(double xp, long* idx, double* w) {
+    long near = (long)rint(xp);
+    double xmin = xp - (double)near;
+    int s = sgn_of(xmin);
+    double d = fabs(xmin);
+    w[0] = 0.75 - d * d; w[2] = 0.5 * (0.25 + d * (d - 1.0)); w[1] = w[2] + d;
+    idx[0] = near; idx[1] = near + s; idx[2] = near - s;
+}
+void orc_fda4_kick_f64(int n, const double* phi, size_t np, const double* x, const double* y,
+                       const double* z, double fact2, double* dv) {
+#pragma omp parallel for schedule(static)
+    for (long p = 0; p < (long)np; p++) {
+        long ix[3], iy[3], iz[3];
+        double wx[3], wy[3], wz[3];
+        axis_d(x[p], ix, wx); axis_d(y[p], iy, wy); axis_d(z[p], iz, wz);
+        double fx = 0, fy = 0, fz = 0;
+        for (int c = 0; c < 3; c++)
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++) {
+                    double wt = wx[a] * wy[b] * wz[c];
+                    long i = ix[a], j = iy[b], k = iz[c];
+                    fx += wt * fda4_d(PHID(i - 1, j, k), PHID(i + 1, j, k), PHID(i - 2, j, k), PHID(i + 2, j, k));
+                    fy += wt * fda4_d(PHID(i, j - 1, k), PHID(i, j + 1, k), PHID(i, j - 2, k), PHID(i, j + 2, k));
+                    fz += wt * fda4_d(PHID(i, j, k - 1), PHID(i, j, k + 1), PHID(i, j, k - 2), PHID(i, j, k + 2));
+                }
+        dv[3 * p] = fact2 * fx; dv[3 * p + 1] = fact2 * fy; dv[3 * p + 2] = fact2 * fz;
+    }
+}
+#undef PHID
+
+/* Full PM chain, Cosmos/pmmain.c:390-527 (deposit, fold, -1, r2c, psolver, c2r, fda4). */
+void orc_pm_accel(int n, size_t np, const double* x, const double* y, const double* z,
+                  double mass, double efold, int deconvolve, int prec, double* acc) {
+    size_t m = (size_t)n * n * n;
+    int h = n / 2 + 1;
+    size_t K = (size_t)h * n * n;
+    double* rho = (double*)malloc(m * sizeof(double));
+    double* spec = (double*)malloc(2 * K * sizeof(double));
+    if (prec == 0) {
+        int S = orc_fixed_scale_bits((double)np * mass);
+        int64_t* a = (int64_t*)malloc(m * sizeof(int64_t));
+        float* den = (float*)malloc(m * sizeof(float));
+        orc_tsc_fixed(n, np, x, y, z, (float)mass, S, a);
+        orc_fixed_to_density(n, a, S, den);
+        for (size_t i = 0; i < m; i++) rho[i] = den[i];
+        orc_r2c(n, rho, spec);
+        float* fs = (float*)malloc(2 * K * sizeof(float));
+        for (size_t i = 0; i < 2 * K; i++) fs[i] = (float)spec[i];
+        orc_psolver_f32(n, efold, deconvolve, fs);
+        for (size_t i = 0; i < 2 * K; i++) spec[i] = fs[i];
+        orc_c2r(n, spec, rho);
+        for (size_t i = 0; i < m; i++) den[i] = (float)rho[i];
+        float* dv = (float*)malloc(3 * np * sizeof(float));
+        orc_fda4_kick(n, den, np, x, y, z, 1.0f, dv, dv + np, dv + 2 * np);
+        for (size_t p = 0; p < np; p++) {
+            acc[3 * p] = dv[p]; acc[3 * p + 1] = dv[np + p]; acc[3 * p + 2] = dv[2 * np + p];
+        }
+        free(dv); free(fs); free(a); free(den);
+    } else {
+        /* FP64: deposit with double weights, double FFT, double gradient */
+        memset(rho, 0, m * sizeof(double));
+        for (size_t p = 0; p < np; p++) {
+            long ix[3], iy[3], iz[3];
+            double wx[3], wy[3], wz[3];
+            axis_d(x[p], ix, wx); axis_d(y[p], iy, wy); axis_d(z[p], iz, wz);
+            for (int c = 0; c < 3; c++)
+                for (int b = 0; b < 3; b++)
+                    for (int a = 0; a < 3; a++)
+                        rho[wrapl(ix[a], n) + (size_t)n * (wrapl(iy[b], n) + (size_t)n * wrapl(iz[c], n))] +=
+                            mass * wx[a] * wy[b] * wz[c];
+        }
+        for (size_t i = 0; i < m; i++) rho[i] -= 1.0;
+        orc_r2c(n, rho, spec);
+        orc_psolver_f64(n, efold, deconvolve, spec);
+        orc_c2r(n, spec, rho);
+        orc_fda4_kick_f64(n, rho, np, x, y, z, 1.0, acc);
+    }
+    free(rho); free(spec);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Force tables, OST/Model4TreeForce.c:48-116. Delta = 6/4096; for i >= 1, r = i Delta:
+ * fr = law(r) / n^2; fp, fpp by central differences with dr = 0.1 Delta;
+ * diff0 = -fr/r, diff1 = -(fp r - fr)/(2 r^3), diff2 = -(fpp r^2 - 3 fp r + 3 fr)/(2 r^5);
+ * slopes are forward differences; entry 0 := entry 1; entry 4095 := 0.
+ * ------------------------------------------------------------------------------------ */
+static const double PMFIT[8] = {1.0 / 256.0 / 256.0, 0.78224e+00, 0.37971e-06, -0.60338e+00,
+                                -0.34419e-07, -0.39741e+01, -0.10607e+01, -0.38145e+00};
+
+double orc_pmcf(int n, double eps, double r) {
+    double a1 = PMFIT[1];
+    double ch = cosh(a1 * r);
+    double newton = r / pow(r * r + eps * eps, 1.5);
+    double fit = 1.0 / (r * r) * tanh(a1 * r) - a1 / (ch * ch * r)
+                 - 2.0 * PMFIT[2] / PMFIT[0] * r * exp(PMFIT[3] * r * r)
+                 - 2.0 * PMFIT[2] * PMFIT[3] / PMFIT[0] * r * r * r * exp(PMFIT[3] * r * r)
+                 + PMFIT[4] / PMFIT[0] * (1.0 + PMFIT[5] * r * r + PMFIT[6] * r * r * r * r) * exp(PMFIT[7] * r * r);
+    return (newton - fit) / ((double)n * (double)n);
+}
+
+/* erfc law: Plummer Newton minus the long-range force of a Gaussian-filtered point mass,
+ * F_long(r) = [erf(r/2rs) - (r/(rs sqrt(pi))) exp(-r^2/4rs^2)] / r^2. */
+double orc_erfc_fr(int n, double eps, double rs, double r) {
+    double newton = r / pow(r * r + eps * eps, 1.5);
+    double u = r / (2.0 * rs);
+    double flong = (erf(u) - (r / (rs * sqrt(M_PI))) * exp(-u * u)) / (r * r);
+    return (newton - flong) / ((double)n * (double)n);
+}
+
+void orc_force_spline(int law, int n, double eps, double rs, double* diff, double* slope) {
+    const double ran = 6.0 / (double)ORC_NSPLINE;
+    const double dr = ran * 0.1;
+    memset(diff, 0, 3 * ORC_NSPLINE * sizeof(double));
+    memset(slope, 0, 3 * ORC_NSPLINE * sizeof(double));
+#define LAW(rr) (law == 0 ? orc_pmcf(n, eps, (rr)) : orc_erfc_fr(n, eps, rs, (rr)))
+    for (int i = 1; i < ORC_NSPLINE; i++) {
+        double r = (double)i * ran;
+        double rm1 = r - dr, rm2 = r - 2 * dr, rp1 = r + dr, rp2 = r + 2 * dr;
+        double fp = (LAW(rp1) - LAW(rm1)) / (dr + dr);
+        double fpp = (LAW(rp2) - 2 * LAW(r) + LAW(rm2)) / (4 * dr * dr);
+        double fr = LAW(r);
+        diff[3 * i + 0] = -(fr / r);
+        diff[3 * i + 1] = -(0.5 * (fp * r - fr) / r / r / r);
+        diff[3 * i + 2] = -(0.5 * (fpp * r * r - 3 * fp * r + 3 * fr) / r / r / r / r / r);
+    }
+#undef LAW
+    for (int i = 1; i < ORC_NSPLINE - 1; i++)
+        for (int c = 0; c < 3; c++) slope[3 * i + c] = (diff[3 * (i + 1) + c] - diff[3 * i + c]) / ran;
+    for (int c = 0; c < 3; c++) {
+        slope[c] = slope[3 + c];
+        diff[c] = diff[3 + c];
+        diff[3 * (ORC_NSPLINE - 1) + c] = 0;
+        slope[3 * (ORC_NSPLINE - 1) + c] = 0;
+    }
+}
+
+/* GetPointGrav / GetCellGrav, OST/Model4TreeForce.c:34-46: float index arithmetic,
+ * double table evaluation, float result. GetPointGrav clamps here too (no caller
+ * reaches r >= 6 with a cutoff below 6). */
+static const float RAN_F = (float)(6.0 / (double)ORC_NSPLINE);
+static float inv_ran_f(void) { return (float)(1.0 / (double)RAN_F); }
+float orc_point_grav(const double* diff, const double* slope, float dist) {
+    int nt = (int)(dist * inv_ran_f());
+    if (nt > ORC_NSPLINE - 1) nt = ORC_NSPLINE - 1;
+    float t = dist - (float)nt * RAN_F;
+    return (float)(diff[3 * nt] + slope[3 * nt] * (double)t);
+}
+void orc_cell_grav(const double* diff, const double* slope, float dist, float* f1, float* f2, float* f3) {
+    int nt = (int)(dist * inv_ran_f());
+    float t = dist - (float)nt * RAN_F;
+    if (nt > ORC_NSPLINE - 1) nt = ORC_NSPLINE - 1;
+    *f1 = (float)(diff[3 * nt + 0] + slope[3 * nt + 0] * (double)t);
+    *f2 = (float)(diff[3 * nt + 1] + slope[3 * nt + 1] * (double)t);
+    *f3 = (float)(diff[3 * nt + 2] + slope[3 * nt + 2] * (double)t);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Direct short-range sum with a chaining mesh of width >= rcut, minimum image.
+ * acc = sum_s (x_t - x_s) * f0(|x_t - x_s|) * m_s over |x_t - x_s| <= rcut.
+ * ------------------------------------------------------------------------------------ */
+typedef struct cellgrid { int mc; double w; long* head; long* next; } cellgrid;
+static void cellgrid_build(cellgrid* g, int n, double rcut, size_t np, const double* x, const double* y, const double* z) {
+    int mc = (int)floor((double)n / rcut);
+    if (mc < 3) mc = 1;
+    g->mc = mc; g->w = (double)n / mc;
+    size_t nc = (size_t)mc * mc * mc;
+    g->head = (long*)malloc(nc * sizeof(long));
+    g->next = (long*)malloc(np * sizeof(long));
+    for (size_t c = 0; c < nc; c++) g->head[c] = -1;
+    for (size_t p = np; p-- > 0;) {
+        int ci = (int)(x[p] / g->w), cj = (int)(y[p] / g->w), ck = (int)(z[p] / g->w);
+        if (ci >= mc) ci = mc - 1;
+        if (cj >= mc) cj = mc - 1;
+        if (ck >= mc) ck = mc - 1;
+        size_t c = ci + (size_t)mc * (cj + (size_t)mc * ck);
+        g->next[p] = g->head[c];
+        g->head[c] = (long)p;
+    }
+}
+static void cellgrid_free(cellgrid* g) { free(g->head); free(g->next); }
+static inline double min_image(double d, double L) {
+    if (d > 0.5 * L) d -= L;
+    else if (d < -0.5 * L) d += L;
+    return d;
+}
+
+static void pp_direct_impl(int mode, int law, int n, double eps, double rs, size_t np,
+                           const double* x, const double* y, const double* z, double mass,
+                           const double* diff, const double* slope, double rcut,
+                           size_t ntarget, const int64_t* targets, double* acc) {
+    cellgrid g;
+    cellgrid_build(&g, n, rcut, np, x, y, z);
+    double L = n, rc2 = rcut * rcut;
+    int span = g.mc >= 3 ? 1 : 0;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long t = 0; t < (long)ntarget; t++) {
+        long p = targets ? (long)targets[t] : t;
+        int ci = (int)(x[p] / g.w), cj = (int)(y[p] / g.w), ck = (int)(z[p] / g.w);
+        if (ci >= g.mc) ci = g.mc - 1;
+        if (cj >= g.mc) cj = g.mc - 1;
+        if (ck >= g.mc) ck = g.mc - 1;
+        double ax = 0, ay = 0, az = 0;
+        for (int dk = -span; dk <= span; dk++)
+            for (int dj = -span; dj <= span; dj++)
+                for (int di = -span; di <= span; di++) {
+                    size_t c = wrapl(ci + di, g.mc) + (size_t)g.mc * (wrapl(cj + dj, g.mc) + (size_t)g.mc * wrapl(ck + dk, g.mc));
+                    for (long s = g.head[c]; s >= 0; s = g.next[s]) {
+                        double dx = min_image(x[p] - x[s], L), dy = min_image(y[p] - y[s], L), dz = min_image(z[p] - z[s], L);
+                        double r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 > rc2 || r2 == 0.0) continue;
+                        double f;
+                        if (mode == 0) {
+                            f = (double)orc_point_grav(diff, slope, (float)sqrt(r2)) * mass;
+                        } else {
+                            double r = sqrt(r2);
+                            double fr = law == 0 ? orc_pmcf(n, eps, r) : orc_erfc_fr(n, eps, rs, r);
+                            f = -fr / r * mass;
+                        }
+                        ax += dx * f; ay += dy * f; az += dz * f;
+                    }
+                }
+        acc[3 * t] = ax; acc[3 * t + 1] = ay; acc[3 * t + 2] = az;
+    }
+    cellgrid_free(&g);
+}
+void orc_pp_direct(int n, size_t np, const double* x, const double* y, const double* z,
+                   float mass, const double* diff, const double* slope, double rcut,
+                   size_t ntarget, const int64_t* targets, double* acc) {
+    pp_direct_impl(0, 0, n, 0, 0, np, x, y, z, mass, diff, slope, rcut, ntarget, targets, acc);
+}
+void orc_pp_direct_analytic(int law, int n, double eps, double rs, size_t np,
+                            const double* x, const double* y, const double* z, double mass,
+                            double rcut, size_t ntarget, const int64_t* targets, double* acc) {
+    pp_direct_impl(1, law, n, eps, rs, np, x, y, z, mass, NULL, NULL, rcut, ntarget, targets, acc);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Reference tree correction (Cosmos/treecorrection.c:225-475 with OST/ost.ver2.c).
+ *  - periodic ghosts within width cellw of the faces (ppadding, RkS/mpirks.c:1058-1200)
+ *  - chaining mesh from -cellw with width cellw (BasicCell.h:1-31, treecorrection.c:263-273)
+ *  - per interior cell: targets = its particles relative to the cell centre
+ *    cellw*(ix - 1/2) + lmin (treecorrection.c:53-98); sources = 3x3x3 cells (:99-148)
+ *  - ngrv >= 16: Make_Tree3/divide_node3 (ost.ver2.c:36-196) and treeforcesearch3
+ *    (:296-351) per target; else DirectSummation (treecorrection.c:177-219)
+ * References in the threaded tree: >= 0 particle index, < 0 node -(k+1), NIL end.
+ * ------------------------------------------------------------------------------------ */
+#define NIL INT_MIN
+typedef struct rptl { double x, y, z; float mass; int sibling; } rptl;
+typedef struct rnode {
+    int sibling, daughter, nparticle;
+    double nodesize, monox, monoy, monoz;
+    float mass, dist_over_thetasq, quad[6], trQ;
+} rnode;
+typedef struct rtree { rptl* p; rnode* nd; int nfree; int quirks; } rtree;
+
+static inline int is_node_ref(int r) { return r < 0 && r != NIL; }
+static inline int node_of(int r) { return -(r + 1); }
+static inline int node_ref(int k) { return -(k + 1); }
+static inline void set_sibling(rtree* t, int r, int s) {
+    if (is_node_ref(r)) t->nd[node_of(r)].sibling = s;
+    else t->p[r].sibling = s;
+}
+
+/* DivideNode, ost.ver2.c:32 with MINCELLWIDTH 1e-5 and MIN_CELL_PARTICLE_NUM 10 (ost.h:4-5) */
+static inline int divide_ok(const rnode* nd, int cnt) { return nd->nodesize > 0.5 * 1.0e-5 && cnt >= 10; }
+
+static void divide_node3_r(rtree* t, int k, float thetasq) {
+    rnode* nd = &t->nd[k];
+    int head = nd->daughter;
+    nd->mass = 0;
+    nd->monox = nd->monoy = nd->monoz = 0;
+    for (int q = head; q != NIL; q = t->p[q].sibling) {
+        float pm = t->p[q].mass;
+        nd->mass += pm;
+        nd->monox += (double)pm * t->p[q].x;
+        nd->monoy += (double)pm * t->p[q].y;
+        nd->monoz += (double)pm * t->p[q].z;
+    }
+    nd->monox /= (double)nd->mass;
+    nd->monoy /= (double)nd->mass;
+    nd->monoz /= (double)nd->mass;
+    double distmax = -1.0e20;
+    float q0 = 0, q1 = 0, q2 = 0, q3 = 0, q4 = 0, q5 = 0;
+    for (int q = head; q != NIL; q = t->p[q].sibling) {
+        double pm = t->p[q].mass;
+        double tx = t->p[q].x - nd->monox, ty = t->p[q].y - nd->monoy, tz = t->p[q].z - nd->monoz;
+        double d2 = tx * tx + ty * ty + tz * tz;
+        if (d2 > distmax) distmax = d2;
+        q0 = (float)((double)q0 + pm * tx * tx);
+        q1 = (float)((double)q1 + pm * ty * ty);
+        q2 = (float)((double)q2 + pm * tz * tz);
+        q3 = (float)((double)q3 + pm * tx * ty);
+        q4 = (float)((double)q4 + pm * tx * tz);
+        q5 = (float)((double)q5 + pm * ty * ((t->quirks & ORC_QUIRK_D2) ? tx : tz));
+    }
+    nd->nodesize = sqrt(distmax);
+    nd->quad[0] = q0; nd->quad[1] = q1; nd->quad[2] = q2;
+    nd->quad[3] = q3; nd->quad[4] = q4; nd->quad[5] = q5;
+    nd->trQ = q0 + q1 + q2;
+    nd->dist_over_thetasq = (float)(distmax / (double)thetasq);
+
+    int ohead[8], ocnt[8];
+    for (int o = 0; o < 8; o++) { ohead[o] = NIL; ocnt[o] = 0; }
+    for (int q = head; q != NIL;) {
+        int nx = q;
+        int after = t->p[q].sibling;
+        int mx = t->p[q].x > nd->monox, my = t->p[q].y > nd->monoy, mz = t->p[q].z > nd->monoz;
+        int o = mx + 2 * (my + 2 * mz);
+        ocnt[o]++;
+        t->p[nx].sibling = ohead[o];
+        ohead[o] = nx;
+        q = after;
+    }
+    int first = 0;
+    for (; first < 8; first++) if (ocnt[first] > 0) break;
+    int first_daughter = t->nfree;
+    int now = t->nfree;
+    int split[8];
+    for (int o = 0; o < 8; o++) split[o] = divide_ok(nd, ocnt[o]);
+    nd->daughter = split[first] ? node_ref(first_daughter) : ohead[first];
+    int left = NIL;
+    for (int o = 0; o < 8; o++) {
+        if (split[o]) {
+            rnode* c = &t->nd[now];
+            memset(c, 0, sizeof(*c));
+            c->daughter = ohead[o];
+            c->nparticle = ocnt[o];
+            c->sibling = NIL;
+            if (left != NIL) set_sibling(t, left, node_ref(now));
+            left = node_ref(now);
+            now++;
+        } else if (ocnt[o] > 0) {
+            int q = ohead[o];
+            if (left != NIL) set_sibling(t, left, q);
+            for (; q != NIL; q = t->p[q].sibling) left = q;
+        }
+    }
+    set_sibling(t, left, t->nd[k].sibling);
+    t->nfree = now;
+    int job = first_daughter;
+    for (int o = 0; o < 8; o++)
+        if (split[o]) { divide_node3_r(t, job, thetasq); job++; }
+}
+
+static void make_tree3(rtree* t, int np, float thetasq) {
+    for (int i = 0; i < np; i++) t->p[i].sibling = i + 1;
+    t->p[np - 1].sibling = NIL;
+    memset(&t->nd[0], 0, sizeof(rnode));
+    t->nd[0].sibling = NIL;
+    t->nd[0].daughter = 0;
+    t->nd[0].nparticle = np;
+    t->nfree = 1;
+    divide_node3_r(t, 0, thetasq);
+}
+
+static void tree_walk3(const rtree* t, double px, double py, double pz, const double* diff,
+                       const double* slope, float rspheresq, float vfact2, int quirks, float* a) {
+    int ptr = node_ref(0);
+    while (ptr != NIL) {
+        if (is_node_ref(ptr)) {
+            const rnode* nd = &t->nd[node_of(ptr)];
+            float tx = (float)(px - nd->monox), ty = (float)(py - nd->monoy), tz = (float)(pz - nd->monoz);
+            float d2 = tx * tx + ty * ty + tz * tz;
+            if (d2 < nd->dist_over_thetasq) { ptr = nd->daughter; continue; }
+            if (d2 <= rspheresq) {
+                float dist = (float)sqrt((double)d2);
+                float Qxx = nd->quad[0], Qyy = nd->quad[1], Qzz = nd->quad[2];
+                float Qxy = nd->quad[3], Qxz = nd->quad[4], Qyz = nd->quad[5];
+                float qxx1 = Qxx * tx + Qxy * ty + Qxz * tz;
+                float qxx2 = Qxy * tx + Qyy * ty + Qyz * tz;
+                float qxx3 = Qxz * tx + Qyz * ty + Qzz * tz;
+                float qxy = Qxx * tx * tx + Qyy * ty * ty + Qzz * tz * tz + 2 * (Qxy * tx * ty + Qxz * tx * tz + Qyz * ty * tz);
+                float f1, f2, f3;
+                orc_cell_grav(diff, slope, dist, &f1, &f2, &f3);
+                float tmp = nd->mass * f1 + nd->trQ * f2 + qxy * f3;
+                float two2 = 2 * f2;
+                if (quirks & ORC_QUIRK_D1) {
+                    float v = vfact2 * (tx * tmp + qxx1 * two2);
+                    a[0] += v; a[0] += v; a[0] += v;
+                } else {
+                    a[0] += vfact2 * (tx * tmp + qxx1 * two2);
+                    a[1] += vfact2 * (ty * tmp + qxx2 * two2);
+                    a[2] += vfact2 * (tz * tmp + qxx3 * two2);
+                }
+            }
+            ptr = nd->sibling;
+        } else {
+            const rptl* s = &t->p[ptr];
+            float tx = (float)(px - s->x), ty = (float)(py - s->y), tz = (float)(pz - s->z);
+            float d2 = tx * tx + ty * ty + tz * tz;
+            if (d2 <= rspheresq) {
+                float dist = (float)sqrt((double)d2);
+                float f = orc_point_grav(diff, slope, dist) * s->mass;
+                a[0] += tx * f * vfact2;
+                a[1] += ty * f * vfact2;
+                a[2] += tz * f * vfact2;
+            }
+            ptr = s->sibling;
+        }
+    }
+}
+
+int orc_treecorrection_ref(int n, size_t np, const double* x, const double* y,
+                           const double* z, float mass, const double* diff,
+                           const double* slope, double rsphere, double theta,
+                           double cellw, float fact2, int quirks, float* dv) {
+    const double L = n;
+    const double width = cellw; /* ghost width = cell width >= rsphere */
+    if (cellw < rsphere) return -1;
+    /* ghosts: every periodic image within `width` of the box */
+    size_t cap = np + np / 2 + 1024, ng = 0;
+    double* gx = (double*)malloc(cap * sizeof(double));
+    double* gy = (double*)malloc(cap * sizeof(double));
+    double* gz = (double*)malloc(cap * sizeof(double));
+    long* gid = (long*)malloc(cap * sizeof(long));
+    for (size_t p = 0; p < np; p++)
+        for (int sk = -1; sk <= 1; sk++)
+            for (int sj = -1; sj <= 1; sj++)
+                for (int si = -1; si <= 1; si++) {
+                    double xx = x[p] + si * L, yy = y[p] + sj * L, zz = z[p] + sk * L;
+                    if (xx < -width || xx >= L + width || yy < -width || yy >= L + width || zz < -width || zz >= L + width) continue;
+                    if (ng == cap) {
+                        cap = cap * 2;
+                        gx = (double*)realloc(gx, cap * sizeof(double)); gy = (double*)realloc(gy, cap * sizeof(double));
+                        gz = (double*)realloc(gz, cap * sizeof(double)); gid = (long*)realloc(gid, cap * sizeof(long));
+                    }
+                    gx[ng] = xx; gy[ng] = yy; gz[ng] = zz;
+                    gid[ng] = (si == 0 && sj == 0 && sk == 0) ? (long)p : -1 - (long)p;
+                    ng++;
+                }
+    double lmin = -cellw;
+    int mc = (int)ceil((L + 2 * cellw) / cellw);
+    size_t nc = (size_t)mc * mc * mc;
+    long* head = (long*)malloc(nc * sizeof(long));
+    long* next = (long*)malloc(ng * sizeof(long));
+    long* nmem = (long*)calloc(nc, sizeof(long));
+    for (size_t c = 0; c < nc; c++) head[c] = -1;
+    /* push-front linking in array order (BasicCell.h:5-31) */
+    for (size_t q = 0; q < ng; q++) {
+        long ci = (long)((gx[q] - lmin) / cellw), cj = (long)((gy[q] - lmin) / cellw), ck = (long)((gz[q] - lmin) / cellw);
+        size_t c = ci + (size_t)mc * (cj + (size_t)mc * ck);
+        next[q] = head[c];
+        head[c] = (long)q;
+        nmem[c]++;
+    }
+    size_t mpos = 0;
+    for (size_t c = 0; c < nc; c++) if ((size_t)nmem[c] > mpos) mpos = nmem[c];
+    float rsq = (float)(rsphere * rsphere);
+    float th2 = (float)(theta * theta);
+    memset(dv, 0, 3 * np * sizeof(float));
+    int status = 0;
+#pragma omp parallel
+    {
+        size_t scap = 27 * mpos + 16;
+        rtree t;
+        t.p = (rptl*)malloc(scap * sizeof(rptl));
+        t.nd = (rnode*)malloc(scap * sizeof(rnode));
+        t.quirks = quirks;
+        long* tid = (long*)malloc((mpos + 1) * sizeof(long));
+        double* tpos = (double*)malloc(3 * (mpos + 1) * sizeof(double));
+#pragma omp for schedule(dynamic)
+        for (long ii = 0; ii < (long)nc; ii++) {
+            long iz = ii / ((long)mc * mc), iy = (ii % ((long)mc * mc)) / mc, ix = ii % mc;
+            if (ix == 0 || iy == 0 || iz == 0 || ix == mc - 1 || iy == mc - 1 || iz == mc - 1) continue;
+            double cx = cellw * (ix - 0.5) + 0.0, cy = cellw * (iy - 0.5) + 0.0, cz = cellw * (iz - 0.5) + 0.0;
+            int npos = 0;
+            for (long q = head[ii]; q >= 0; q = next[q]) {
+                if (gid[q] < 0) continue;
+                tid[npos] = gid[q];
+                tpos[3 * npos] = gx[q] - cx; tpos[3 * npos + 1] = gy[q] - cy; tpos[3 * npos + 2] = gz[q] - cz;
+                npos++;
+            }
+            if (npos == 0) continue;
+            int ngrv = 0;
+            for (long k = iz - 1; k <= iz + 1; k++)
+                for (long j = iy - 1; j <= iy + 1; j++)
+                    for (long i = ix - 1; i <= ix + 1; i++) {
+                        size_t c = i + (size_t)mc * (j + (size_t)mc * k);
+                        for (long q = head[c]; q >= 0; q = next[q]) {
+                            t.p[ngrv].x = gx[q] - cx; t.p[ngrv].y = gy[q] - cy; t.p[ngrv].z = gz[q] - cz;
+                            t.p[ngrv].mass = mass;
+                            ngrv++;
+                        }
+                    }
+            if (ngrv >= 16) {
+                make_tree3(&t, ngrv, th2);
+                for (int q = 0; q < npos; q++) {
+                    float a[3] = {0, 0, 0};
+                    tree_walk3(&t, tpos[3 * q], tpos[3 * q + 1], tpos[3 * q + 2], diff, slope, rsq, fact2, quirks, a);
+                    long p = tid[q];
+                    dv[3 * p] += a[0]; dv[3 * p + 1] += a[1]; dv[3 * p + 2] += a[2];
+                }
+            } else {
+                for (int q = 0; q < npos; q++) {
+                    float ax = 0, ay = 0, az = 0;
+                    for (int s = 0; s < ngrv; s++) {
+                        double tx = tpos[3 * q] - t.p[s].x, ty = tpos[3 * q + 1] - t.p[s].y, tz = tpos[3 * q + 2] - t.p[s].z;
+                        double d2 = tx * tx + ty * ty + tz * tz;
+                        if (d2 <= (double)rsq) {
+                            float dist = (float)sqrt(d2);
+                            float f = orc_point_grav(diff, slope, dist) * t.p[s].mass;
+                            ax = (float)((double)ax + tx * (double)f);
+                            ay = (float)((double)ay + ty * (double)f);
+                            az = (float)((double)az + tz * (double)f);
+                        }
+                    }
+                    long p = (quirks & ORC_QUIRK_D3) ? tid[0] : tid[q];
+                    dv[3 * p] += fact2 * ax; dv[3 * p + 1] += fact2 * ay; dv[3 * p + 2] += fact2 * az;
+                }
+            }
+        }
+        free(t.p); free(t.nd); free(tid); free(tpos);
+    }
+    free(gx); free(gy); free(gz); free(gid); free(head); free(next); free(nmem);
+    return status;
+}
+
+/* The reference quadrupole node interaction (AddNode) for a node built from the given
+ * sources (monopole + quad about the COM as divide_node3 computes them). */
+void orc_node_accel(const double* diff, const double* slope, double rcut,
+                    size_t ns, const double* sx, const double* sy, const double* sz,
+                    const float* sm, const double* target, int quirks, float* acc3) {
+    rtree t;
+    t.p = (rptl*)malloc((ns + 1) * sizeof(rptl));
+    t.nd = (rnode*)malloc(sizeof(rnode));
+    t.quirks = quirks;
+    for (size_t i = 0; i < ns; i++) { t.p[i].x = sx[i]; t.p[i].y = sy[i]; t.p[i].z = sz[i]; t.p[i].mass = sm[i]; t.p[i].sibling = (int)i + 1; }
+    t.p[ns - 1].sibling = NIL;
+    rnode* nd = &t.nd[0];
+    memset(nd, 0, sizeof(*nd));
+    nd->daughter = 0; nd->sibling = NIL;
+    /* moments exactly as divide_node3 (first half), then force with theta -> infinity */
+    double mx = 0, my = 0, mz = 0; float m = 0;
+    for (size_t i = 0; i < ns; i++) { m += sm[i]; mx += (double)sm[i] * sx[i]; my += (double)sm[i] * sy[i]; mz += (double)sm[i] * sz[i]; }
+    nd->mass = m; nd->monox = mx / m; nd->monoy = my / m; nd->monoz = mz / m;
+    float q[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t i = 0; i < ns; i++) {
+        double tx = sx[i] - nd->monox, ty = sy[i] - nd->monoy, tz = sz[i] - nd->monoz, pm = sm[i];
+        q[0] = (float)((double)q[0] + pm * tx * tx); q[1] = (float)((double)q[1] + pm * ty * ty);
+        q[2] = (float)((double)q[2] + pm * tz * tz); q[3] = (float)((double)q[3] + pm * tx * ty);
+        q[4] = (float)((double)q[4] + pm * tx * tz);
+        q[5] = (float)((double)q[5] + pm * ty * ((quirks & ORC_QUIRK_D2) ? tx : tz));
+    }
+    memcpy(nd->quad, q, sizeof(q));
+    nd->trQ = q[0] + q[1] + q[2];
+    nd->dist_over_thetasq = 0.0f; /* never open */
+    acc3[0] = acc3[1] = acc3[2] = 0;
+    tree_walk3(&t, target[0], target[1], target[2], diff, slope, (float)(rcut * rcut), 1.0f, quirks, acc3);
+    free(t.p); free(t.nd);
+}
+
+/* ------------------------------------------------------------------------------------
+ * Ewald sum: a_i = -G sum_j m_j [ sum_n r (erfc(a r) + 2 a r/sqrt(pi) e^{-a^2 r^2}) / r^3
+ *            + (4 pi / L^3) sum_{k != 0} k / k^2 exp(-k^2 / 4a^2) sin(k . r_ij) ],
+ * r = x_i - x_j + n L. Independent of alpha when converged.
+ * ------------------------------------------------------------------------------------ */
+void orc_ewald_accel(size_t np, const double* x, const double* y, const double* z,
+                     const double* m, double L, double G, double alpha, int nreal, int kmax,
+                     double* acc) {
+    double sqpi = sqrt(M_PI);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long i = 0; i < (long)np; i++) {
+        double ax = 0, ay = 0, az = 0;
+        for (size_t j = 0; j < np; j++) {
+            double dx0 = min_image(x[i] - x[j], L), dy0 = min_image(y[i] - y[j], L), dz0 = min_image(z[i] - z[j], L);
+            for (int a = -nreal; a <= nreal; a++)
+                for (int b = -nreal; b <= nreal; b++)
+                    for (int c = -nreal; c <= nreal; c++) {
+                        if ((size_t)i == j && a == 0 && b == 0 && c == 0) continue;
+                        double dx = dx0 + a * L, dy = dy0 + b * L, dz = dz0 + c * L;
+                        double r2 = dx * dx + dy * dy + dz * dz;
+                        double r = sqrt(r2);
+                        double f = (erfc(alpha * r) + 2 * alpha * r / sqpi * exp(-alpha * alpha * r2)) / (r2 * r);
+                        ax -= G * m[j] * f * dx; ay -= G * m[j] * f * dy; az -= G * m[j] * f * dz;
+                    }
+        }
+        double kf = 2 * M_PI / L;
+        for (int a = -kmax; a <= kmax; a++)
+            for (int b = -kmax; b <= kmax; b++)
+                for (int c = -kmax; c <= kmax; c++) {
+                    if (a == 0 && b == 0 && c == 0) continue;
+                    if (a * a + b * b + c * c > kmax * kmax) continue;
+                    double kx = kf * a, ky = kf * b, kz = kf * c;
+                    double k2 = kx * kx + ky * ky + kz * kz;
+                    double w = 4 * M_PI / (L * L * L) * exp(-k2 / (4 * alpha * alpha)) / k2;
+                    double s = 0;
+                    for (size_t j = 0; j < np; j++) s += m[j] * sin(kx * (x[i] - x[j]) + ky * (y[i] - y[j]) + kz * (z[i] - z[j]));
+                    ax -= G * w * kx * s; ay -= G * w * ky * s; az -= G * w * kz * s;
+                }
+        acc[3 * i] = ax; acc[3 * i + 1] = ay; acc[3 * i + 2] = az;
+    }
+}
+
+/* ------------------------------------------------------------------------------------
+ * Kick factors (Cosmos/CosmosEvolFactor.c:35-65,230-256; Cosmos/FLRW.c:60-71), FP64,
+ * cosmological constant only (w = -1, so the dark-energy growth term is 1):
+ *   1+z = amax/a, H^2 = (1+z)^3 Om + (1+z)^2 (1-Om-OL) + OL,
+ *   ap = sqrt(8 pi/3 / (amax Om) / (1+z)^2 * H^2),
+ *   fact1 = a1^3 H1 / (a2^3 H2), fact2 = n / (a2^2 ap(a2)) * int_{a1}^{a2} da / (a ap(a)).
+ * Adaptive Simpson quadrature to 1e-14 relative.
+ * ------------------------------------------------------------------------------------ */
+double orc_hofez(double omep, double omeplam, double amax, double a) {
+    double zp1 = amax / a;
+    return sqrt(zp1 * zp1 * zp1 * omep + zp1 * zp1 * (1 - omep - omeplam) + omeplam);
+}
+double orc_ap(double omep, double omeplam, double amax, double a) {
+    double zp1 = amax / a;
+    double h = orc_hofez(omep, omeplam, amax, a);
+    return sqrt(8.0 * M_PI / 3.0 / (amax * omep) / (zp1 * zp1) * h * h);
+}
+typedef struct cosmo { double om, ol, amax; } cosmo;
+static double integrand(const cosmo* c, double a) { return 1.0 / (a * orc_ap(c->om, c->ol, c->amax, a)); }
+static double simpson_rec(const cosmo* c, double a, double b, double fa, double fm, double fb, double whole, double tol, int depth) {
+    double m = 0.5 * (a + b), lm = 0.5 * (a + m), rm = 0.5 * (m + b);
+    double flm = integrand(c, lm), frm = integrand(c, rm);
+    double left = (m - a) / 6 * (fa + 4 * flm + fm), right = (b - m) / 6 * (fm + 4 * frm + fb);
+    if (depth <= 0 || fabs(left + right - whole) <= 15 * tol) return left + right + (left + right - whole) / 15;
+    return simpson_rec(c, a, m, fa, flm, fm, left, tol / 2, depth - 1) + simpson_rec(c, m, b, fm, frm, fb, right, tol / 2, depth - 1);
+}
+static double integrate(const cosmo* c, double a, double b) {
+    if (a == b) return 0;
+    double fa = integrand(c, a), fb = integrand(c, b), fm = integrand(c, 0.5 * (a + b));
+    double whole = (b - a) / 6 * (fa + 4 * fm + fb);
+    return simpson_rec(c, a, b, fa, fm, fb, whole, 1e-15 * fabs(whole) + 1e-300, 40);
+}
+double orc_fact1(double omep, double omeplam, double amax, double a1, double a2) {
+    double h1 = orc_hofez(omep, omeplam, amax, a1), h2 = orc_hofez(omep, omeplam, amax, a2);
+    return (a1 * a1 * a1 * h1) / (a2 * a2 * a2 * h2);
+}
+double orc_fact2(double omep, double omeplam, double amax, int n, double a1, double a2) {
+    cosmo c = {omep, omeplam, amax};
+    return 1.0 / (a2 * a2 * orc_ap(omep, omeplam, amax, a2)) * integrate(&c, a1, a2) * n;
+}
+void orc_evolfactor(double omep, double omeplam, double amax, int n, double anow,
+                    double astep, orc_evolfact* out) {
+    out->afact = out->bfact = 0;
+    out->fact1 = orc_fact1(omep, omeplam, amax, anow - 0.5 * astep, anow + 0.5 * astep);
+    out->fact2 = orc_fact2(omep, omeplam, amax, n, anow - 0.5 * astep, anow + 0.5 * astep);
+    out->pfact = astep * n;
+    out->fact1_push = orc_fact1(omep, omeplam, amax, anow, anow + 0.5 * astep);
+    out->fact1_pull = orc_fact1(omep, omeplam, amax, anow - 0.5 * astep, anow);
+    out->fact2_push = orc_fact2(omep, omeplam, amax, n, anow, anow + 0.5 * astep);
+    out->fact2_pull = orc_fact2(omep, omeplam, amax, n, anow - 0.5 * astep, anow);
+}

@@ -1,0 +1,65 @@
+"""The C-ABI library loads, exports every symbol include/lagpm.h declares, and its host-only
+entry points (force tables, kick factors, size estimate) agree with the oracle. CPU only."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from lageunha_amd import lagpm
+from oracle import pyoracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "lagpm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lagpm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(lagpm.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # and the Python mirror binds exactly those
+    assert sorted(lagpm.SIGNATURES) == syms
+
+
+def test_abi_version():
+    assert lagpm.load().lagpm_abi_version() == 1
+
+
+def test_force_spline_matches_oracle_tables():
+    for law, p in ((0, lagpm.reference_params(256)), (1, lagpm.make_params(256, efold=2.5))):
+        diff, slope = lagpm.i_force_spline(p)
+        od, os_ = orc.force_spline(law, 256, p.epsilon, p.efold / 2)
+        assert np.allclose(diff, od, rtol=1e-13, atol=0)
+        assert np.allclose(slope, os_, rtol=1e-9, atol=1e-9 * np.abs(os_).max())
+
+
+def test_evolfactor_matches_oracle_quadrature():
+    p = lagpm.make_params(256, omep=0.26, omeplam=0.74, amax=48.0)
+    for anow in (1.0, 5.0, 30.0):
+        ef = lagpm.get_evol_factor(p, anow, 0.25)
+        o = orc.evolfactor(0.26, 0.74, 48.0, 256, anow, 0.25)
+        for k in ("fact1", "fact2", "fact1_push", "fact2_push", "fact1_pull", "fact2_pull", "pfact"):
+            assert np.isclose(getattr(ef, k), getattr(o, k), rtol=2e-7), k
+
+
+def test_invalid_parameters_are_rejected_without_a_device():
+    bad = lagpm.make_params(256, rsphere=7.0)
+    d = np.empty((4096, 3))
+    s = np.empty((4096, 3))
+    rc = lagpm.load().lagpm_force_spline(ctypes.byref(bad), d.ctypes.data_as(lagpm._D), s.ctypes.data_as(lagpm._D))
+    assert rc == -1
+
+
+def test_box_byte_estimate_scales():
+    p = lagpm.make_params(256)
+    b1 = lagpm.estimate_box_bytes(p, 256 ** 3)
+    b2 = lagpm.estimate_box_bytes(lagpm.make_params(128), 128 ** 3)
+    assert 6 < b1 / b2 < 10
+    assert 2e9 < b1 < 20e9

@@ -1,0 +1,150 @@
+"""GPU parity of the PM half through the C ABI against the oracle (run with -m gpu)."""
+import numpy as np
+import pytest
+
+from lageunha_amd import ics, lagpm
+from oracle import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def edge_positions(n):
+    g = np.random.default_rng(21)
+    pos = g.random((3000, 3)) * n
+    pos[:8] = [[0, 0, 0], [n - 1e-12, n - 1e-12, n - 1e-12], [0.5, 1.5, 2.5], [n - 0.5, 3.0, 4.0],
+               [1.25, 2.75, n - 0.25], [7.0, 7.0, 7.0], [n / 2, n / 2, n / 2], [0.4999999999, 0.5000000001, 1e-9]]
+    return pos
+
+
+def cases(n):
+    return {
+        "uniform": ics.uniform_random(20000, n, seed=5),
+        "lattice": ics.perturbed_lattice(n, rms=0.3, seed=1),
+        "clustered": ics.clustered_box(n, halo_fraction=0.3, nhalo=40, seed=3),
+        "edges": edge_positions(n),
+    }
+
+
+@pytest.mark.parametrize("prec", [lagpm.PREC_FP32, lagpm.PREC_FP64])
+@pytest.mark.parametrize("case", ["uniform", "lattice", "clustered", "edges"])
+def test_tsc_deposit_bit_exact(ctx, case, prec):
+    n = 32
+    pos = cases(n)[case]
+    print(f"deposit case={case} prec={prec} N={len(pos)}", flush=True)
+    p = lagpm.make_params(n, precision=prec)
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.upload_pos(pos)
+        acc, bits = b.tsc_fixed()
+        den = b.tsc().ravel()
+    if prec == lagpm.PREC_FP32:
+        oacc, obits = orc.tsc_fixed(n, pos, mass=1.0)
+        assert bits == obits
+        assert np.array_equal(acc, oacc)
+        assert np.array_equal(den, orc.fixed_to_density(n, oacc, obits))
+    else:
+        # FP64 weights: compare with the FP32-weight oracle within float rounding
+        oacc, obits = orc.tsc_fixed(n, pos, mass=1.0)
+        assert np.allclose(acc * 2.0 ** -bits, oacc * 2.0 ** -obits, atol=2e-5)
+    assert abs(acc.sum() * 2.0 ** -bits - len(pos)) < 1e-3
+
+
+def test_tsc_deposit_dense_halo_falls_back_and_stays_exact(ctx):
+    n = 32
+    g = np.random.default_rng(9)
+    # one chunk spans the whole box (forces the global-atomic path) + a dense core
+    pos = np.concatenate([g.random((700, 3)) * n, 16 + g.standard_normal((30000, 3)) * 0.05])
+    with lagpm.Box(ctx, lagpm.make_params(n), len(pos)) as b:
+        b.upload_pos(pos)
+        acc, bits = b.tsc_fixed()
+    oacc, _ = orc.tsc_fixed(n, pos)
+    assert np.array_equal(acc, oacc)
+
+
+def test_green_multiply_bit_exact(ctx):
+    n = 32
+    g = np.random.default_rng(22)
+    spec = (g.standard_normal((n, n, n // 2 + 1)) + 1j * g.standard_normal((n, n, n // 2 + 1))).astype(np.complex64)
+    for efold, dec in ((0.9, 0), (2.5, 1)):
+        out = lagpm.green_apply(ctx, n, efold, dec, spec)
+        ref = orc.psolver_f32(n, efold, dec, spec)
+        assert np.array_equal(out, ref), (efold, dec)
+
+
+def test_fda4_gradient_and_kick_bit_exact(ctx):
+    n = 32
+    g = np.random.default_rng(23)
+    phi = np.fft.irfftn(np.fft.rfftn(g.standard_normal((n, n, n))) * np.exp(-np.arange(n // 2 + 1) / 4.0)).astype(np.float32)
+    pos = edge_positions(n)
+    with lagpm.Box(ctx, lagpm.make_params(n), len(pos)) as b:
+        b.upload_pos(pos)
+        b.set_potential(phi)
+        b.fda4(lagpm.evolfact(0.75), lagpm.KICK)
+        grad = b.get_gradient()
+        dv = b.velocities()
+    assert np.array_equal(grad, orc.fda4_gradient(n, phi))
+    ref = orc.fda4_kick(n, phi, pos, fact2=0.75).astype(np.float64)
+    assert np.array_equal(dv, ref)
+
+
+def test_pm_factor_selection(ctx):
+    """KICK uses fact2, PULL fact2_pull, PUSH fact2_push / fact1_push (Cosmos/fda4.c:49-63)."""
+    n = 32
+    pos = ics.uniform_random(5000, n, seed=8)
+    out = {}
+    for st, ef in ((lagpm.KICK, lagpm.evolfact(2.0)),
+                   (lagpm.PULL, lagpm.evolfact(9.0, fact2_pull=0.5)),
+                   (lagpm.PUSH, lagpm.evolfact(9.0, fact2_push=1.5, fact1_push=3.0))):
+        with lagpm.Box(ctx, lagpm.make_params(n), len(pos)) as b:
+            b.upload_pos(pos)
+            b.pmmain(ef, st)
+            out[st] = b.velocities()
+    assert np.allclose(out[lagpm.PULL] * 4.0, out[lagpm.KICK], rtol=1e-6, atol=1e-12)
+    assert np.allclose(out[lagpm.PUSH] * 4.0, out[lagpm.KICK], rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", ["reference", "accurate"])
+def test_pmmain_matches_oracle_chain(ctx, mode):
+    n = 64
+    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=60, seed=4)
+    if mode == "reference":
+        p = lagpm.reference_params(n)
+        efold, dec = p.efold, 0
+    else:
+        p = lagpm.make_params(n, efold=2.5, deconvolve=1)
+        efold, dec = 2.5, 1
+    print(f"pmmain mode={mode} N={len(pos)}", flush=True)
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.upload_pos(pos)
+        b.pmmain(lagpm.evolfact(1.0), lagpm.KICK)
+        a = b.velocities()
+    ref = orc.pm_accel(n, pos, mass=1.0, efold=efold, deconvolve=dec, prec=0)
+    err = np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2))
+    print(f"  rms rel vs oracle FP32-storage chain: {err:.3e}")
+    assert err < 2e-5
+    ref64 = orc.pm_accel(n, pos, mass=1.0, efold=efold, deconvolve=dec, prec=1)
+    err64 = np.sqrt(np.sum((a - ref64) ** 2) / np.sum(ref64 ** 2))
+    print(f"  rms rel vs oracle FP64 chain: {err64:.3e}")
+    assert err64 < 1e-4
+
+
+def test_pmmain_fp64_matches_oracle_fp64(ctx):
+    n = 32
+    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=30, seed=6)
+    p = lagpm.make_params(n, efold=2.5, deconvolve=1, precision=lagpm.PREC_FP64)
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.upload_pos(pos)
+        b.pmmain(lagpm.evolfact(1.0), lagpm.KICK)
+        a = b.velocities()
+    ref = orc.pm_accel(n, pos, mass=1.0, efold=2.5, deconvolve=1, prec=1)
+    err = np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2))
+    assert err < 1e-9, err
+
+
+def test_pm_momentum_conservation_on_device(ctx):
+    n = 64
+    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=50, seed=10)
+    with lagpm.Box(ctx, lagpm.make_params(n), len(pos)) as b:
+        b.upload_pos(pos)
+        b.pmmain(lagpm.evolfact(1.0), lagpm.KICK)
+        a = b.velocities()
+    assert np.max(np.abs(a.sum(axis=0))) < 1e-4 * np.abs(a).sum() / len(a) * np.sqrt(len(a))

@@ -1,0 +1,136 @@
+"""GPU parity of the tree correction through the C ABI (run with -m gpu).
+
+theta -> 0 must reproduce the oracle's exact short-range pair sum; at theta = 0.5 the FP32
+wavefront walk must agree with the plain FP64 walk of the same tree to well below 1e-3
+and stay within the reference walk's own approximation error of the exact sum."""
+import numpy as np
+import pytest
+
+from lageunha_amd import ics, lagpm
+from oracle import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+def tree_accel(ctx, params, pos, fact=1.0):
+    with lagpm.Box(ctx, params, len(pos)) as b:
+        b.upload_pos(pos)
+        b.treecorrection(lagpm.evolfact(fact))
+        v = b.velocities()
+        st = b.tree_stats()
+    return v, st
+
+
+def workload(n, kind, seed=3):
+    if kind == "uniform":
+        return ics.uniform_random(n ** 3, n, seed=seed)
+    if kind == "lattice":
+        return ics.perturbed_lattice(n, rms=0.3, seed=seed)
+    return ics.clustered_box(n, halo_fraction=0.3, nhalo=max(8, n ** 3 // 2000), seed=seed)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "clustered"])
+@pytest.mark.parametrize("prec", [lagpm.PREC_FP32, lagpm.PREC_FP64])
+def test_tree_theta_zero_is_the_direct_sum(ctx, kind, prec):
+    n = 32
+    pos = workload(n, kind)
+    p = lagpm.make_params(n, theta=1e-6, precision=prec)
+    print(f"tree theta0 kind={kind} prec={prec} N={len(pos)}", flush=True)
+    a, st = tree_accel(ctx, p, pos)
+    diff, slope = orc.force_spline(1, n, p.epsilon, p.efold / 2)
+    ref = orc.pp_direct(n, pos, diff, slope, p.rsphere)
+    err = rel_rms(a, ref)
+    print(f"  rms rel {err:.3e}  n_pp {st.n_pp} n_pc {st.n_pc} nodes {st.n_nodes} groups {st.n_groups}")
+    assert st.n_pc == 0
+    assert err < (2e-6 if prec == lagpm.PREC_FP32 else 1e-6)
+
+
+def test_tree_reference_law_theta_zero(ctx):
+    n = 32
+    pos = workload(n, "clustered", seed=5)
+    p = lagpm.reference_params(n, theta=1e-6)
+    a, _ = tree_accel(ctx, p, pos)
+    diff, slope = orc.force_spline(0, n, p.epsilon, 0.45)
+    ref = orc.pp_direct(n, pos, diff, slope, p.rsphere)
+    assert rel_rms(a, ref) < 2e-6
+
+
+@pytest.mark.parametrize("kind", ["lattice", "clustered"])
+def test_tree_theta_half_fp32_vs_fp64_and_exact(ctx, kind):
+    n = 48
+    pos = workload(n, kind, seed=7)
+    p32 = lagpm.make_params(n, theta=0.5)
+    p64 = lagpm.make_params(n, theta=0.5, precision=lagpm.PREC_FP64)
+    a32, st32 = tree_accel(ctx, p32, pos)
+    a64, st64 = tree_accel(ctx, p64, pos)
+    diff, slope = orc.force_spline(1, n, p32.epsilon, p32.efold / 2)
+    exact = orc.pp_direct(n, pos, diff, slope, p32.rsphere)
+    e_prec = rel_rms(a32, a64)
+    e_tree = rel_rms(a64, exact)
+    ref_walk = orc.treecorrection_ref(n, pos, diff, slope, p32.rsphere, 0.5)
+    e_ref = rel_rms(ref_walk, exact)
+    print(f"theta=0.5 {kind}: fp32 vs fp64 {e_prec:.2e}; gpu tree vs exact {e_tree:.2e}; "
+          f"reference walk vs exact {e_ref:.2e}; pp {st32.n_pp} pc {st32.n_pc}", flush=True)
+    assert e_prec < 1e-4
+    assert e_tree < max(2 * e_ref, 1e-3)
+    assert st32.n_pc > 0
+
+
+def test_tree_reference_mode_theta_03_close_to_reference_walk(ctx):
+    n = 32
+    pos = workload(n, "clustered", seed=9)
+    p = lagpm.reference_params(n, theta=0.3)
+    a, _ = tree_accel(ctx, p, pos)
+    diff, slope = orc.force_spline(0, n, p.epsilon, 0.45)
+    exact = orc.pp_direct(n, pos, diff, slope, p.rsphere)
+    ref_walk = orc.treecorrection_ref(n, pos, diff, slope, p.rsphere, 0.3)
+    assert rel_rms(a, exact) < max(2 * rel_rms(ref_walk, exact), 1e-4)
+    assert rel_rms(a, ref_walk) < 5e-3
+
+
+def test_tree_is_deterministic_and_scales_with_fact(ctx):
+    n = 32
+    pos = workload(n, "clustered", seed=11)
+    p = lagpm.make_params(n, theta=0.5)
+    a1, _ = tree_accel(ctx, p, pos, fact=1.0)
+    a2, _ = tree_accel(ctx, p, pos, fact=1.0)
+    assert np.array_equal(a1, a2)
+    a3, _ = tree_accel(ctx, p, pos, fact=0.5)
+    assert np.allclose(a3, 0.5 * a1, rtol=1e-6, atol=1e-12)
+
+
+def test_tree_small_and_empty_inputs(ctx):
+    n = 24
+    p = lagpm.make_params(n, theta=0.5)
+    for npart in (0, 1, 2, 17, 100):
+        pos = ics.uniform_random(npart, n, seed=npart + 1)
+        a, st = tree_accel(ctx, p, pos)
+        assert a.shape == (npart, 3)
+        if npart:
+            diff, slope = orc.force_spline(1, n, p.epsilon, p.efold / 2)
+            ref = orc.pp_direct(n, pos, diff, slope, p.rsphere)
+            assert np.allclose(a, ref, rtol=1e-4, atol=1e-7 * max(1e-30, np.abs(ref).max()))
+
+
+def test_tree_rejects_box_smaller_than_four_cutoffs(ctx):
+    p = lagpm.make_params(16, rsphere=5.625)
+    with lagpm.Box(ctx, p, 10) as b:
+        b.upload_pos(ics.uniform_random(10, 16))
+        with pytest.raises(lagpm.LagpmError):
+            b.treecorrection(lagpm.evolfact(1.0))
+
+
+def test_coincident_particles_beyond_group_size(ctx):
+    """200 particles at one point (identical Morton keys at depth 16) plus a background."""
+    n = 32
+    g = np.random.default_rng(4)
+    pos = np.concatenate([np.tile([[10.3, 11.7, 12.1]], (200, 1)), g.random((3000, 3)) * n])
+    p = lagpm.make_params(n, theta=1e-6)
+    a, st = tree_accel(ctx, p, pos)
+    diff, slope = orc.force_spline(1, n, p.epsilon, p.efold / 2)
+    ref = orc.pp_direct(n, pos, diff, slope, p.rsphere)
+    assert rel_rms(a, ref) < 2e-6

@@ -1,0 +1,203 @@
+"""Pins of the oracle's PM chain (TSC deposit, FFT, Green, FDA4) against closed forms,
+invariants and brute force. CPU only."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as orc
+
+
+def tsc_w(d):
+    """TSC weights for offset d = x - rint(x) in [-0.5, 0.5]: near, near+sign, near-sign."""
+    a = abs(d)
+    return np.array([0.75 - a * a, 0.5 * (0.5 - a) ** 2 + a, 0.5 * (0.5 - a) ** 2])
+
+
+def brute_tsc(n, pos, mass):
+    den = np.zeros((n, n, n))
+    for p in pos:
+        idx, w = [], []
+        for a in range(3):
+            near = np.rint(p[a])
+            d = p[a] - near
+            s = -1 if np.signbit(d) else 1
+            idx.append([int(near), int(near) + s, int(near) - s])
+            w.append(tsc_w(d))
+        for c in range(3):
+            for b in range(3):
+                for a in range(3):
+                    den[idx[2][c] % n, idx[1][b] % n, idx[0][a] % n] += mass * w[0][a] * w[1][b] * w[2][c]
+    return den
+
+
+@pytest.mark.parametrize("d", [0.0, 0.25, -0.25, 0.49, -0.4999, 0.1])
+def test_tsc_closed_form_single_particle(d):
+    n = 8
+    pos = np.array([[3.0 + d, 4.0 - d / 2, 5.0 + d / 3]])
+    acc, bits = orc.tsc_fixed(n, pos, mass=1.0)
+    den = acc.astype(np.float64) * 2.0 ** -bits
+    ref = brute_tsc(n, pos, 1.0).ravel()
+    assert np.allclose(den, ref, atol=2e-7)
+    # closed-form values at the nearest node: (0.75 - d^2) per axis
+    w = [0.75 - d * d, 0.75 - (d / 2) ** 2, 0.75 - (d / 3) ** 2]
+    assert abs(den[3 + n * (4 + n * 5)] - np.prod(w)) < 2e-7
+
+
+def test_tsc_partition_of_unity_and_mass():
+    n = 16
+    pos = orc_pos = np.random.default_rng(5).random((4000, 3)) * n
+    acc, bits = orc.tsc_fixed(n, orc_pos, mass=1.5)
+    total = acc.sum() * 2.0 ** -bits
+    assert abs(total - 1.5 * len(pos)) < 1e-4 * len(pos) * 1e-3
+    den = orc.fixed_to_density(n, acc, bits)
+    assert abs((den.astype(np.float64) + 1).sum() - 1.5 * len(pos)) < 1e-2
+
+
+def test_tsc_brute_force_and_periodic_wrap():
+    n = 8
+    rng = np.random.default_rng(1)
+    pos = rng.random((7, 3)) * n
+    pos[0] = [n - 1e-9, 0.0, n - 0.5]       # wraps on every axis, exact half-integer
+    pos[1] = [0.5, 2.5, 7.5]                 # rint ties to even
+    acc, bits = orc.tsc_fixed(n, pos, mass=1.0)
+    ref = brute_tsc(n, pos, 1.0).ravel()
+    assert np.allclose(acc * 2.0 ** -bits, ref, atol=1e-6)
+
+
+def test_tsc_whole_cell_translation_is_exact():
+    n = 16
+    pos = np.random.default_rng(2).random((3000, 3)) * (n - 4)
+    a1, b = orc.tsc_fixed(n, pos)
+    a2, _ = orc.tsc_fixed(n, pos + np.array([1.0, 2.0, 3.0]), bits=b)
+    rolled = np.roll(a1.reshape(n, n, n), shift=(3, 2, 1), axis=(0, 1, 2)).ravel()
+    assert np.array_equal(rolled, a2)
+
+
+def test_tsc_fixed_matches_float_reference_order():
+    n = 16
+    pos = np.random.default_rng(3).random((20000, 3)) * n
+    acc, bits = orc.tsc_fixed(n, pos)
+    den_fixed = orc.fixed_to_density(n, acc, bits)
+    den_float = orc.tsc_float(n, pos)
+    assert np.max(np.abs(den_fixed - den_float)) < 2e-5
+
+
+def test_fft_matches_numpy_and_round_trips():
+    for n in (8, 12, 16):
+        x = np.random.default_rng(n).standard_normal(n ** 3)
+        spec = orc.r2c(n, x)
+        ref = np.fft.rfftn(x.reshape(n, n, n))
+        assert np.allclose(spec, ref, rtol=1e-10, atol=1e-9)
+        back = orc.c2r(n, spec)
+        assert np.allclose(back, x * n ** 3, rtol=1e-10, atol=1e-8)
+
+
+def green_closed(n, efold, i, j, k):
+    wy = j if j <= n / 2 else j - n
+    wz = k if k <= n / 2 else k - n
+    w = i * i + wy * wy + wz * wz
+    if w == 0:
+        return 0.0
+    return (-1.0 / np.pi) / w * np.exp(-w * (np.pi * efold / n) ** 2) / n ** 3
+
+
+@pytest.mark.parametrize("mode", [(1, 0, 0), (0, 3, 0), (2, 5, 7), (8, 8, 8), (0, 0, 0), (3, 13, 1)])
+def test_psolver_single_mode(mode):
+    n, efold = 16, 0.9
+    i, j, k = mode
+    spec = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+    spec[k, j, i] = 1.0 - 0.5j
+    out = orc.psolver_f64(n, efold, 0, spec)
+    g = green_closed(n, efold, i, j, k)
+    assert np.isclose(out[k, j, i], g * (1.0 - 0.5j), rtol=1e-13, atol=0)
+    mask = np.ones_like(out, dtype=bool)
+    mask[k, j, i] = False
+    assert np.all(out[mask] == 0)
+    # symmetry under (j, k) -> (-j, -k)
+    spec2 = np.zeros_like(spec)
+    spec2[(-k) % n, (-j) % n, i] = 1.0 - 0.5j
+    out2 = orc.psolver_f64(n, efold, 0, spec2)
+    assert np.isclose(out2[(-k) % n, (-j) % n, i], out[k, j, i], rtol=1e-14)
+
+
+def test_psolver_f32_rounds_the_f64_product():
+    n = 16
+    rng = np.random.default_rng(4)
+    spec = (rng.standard_normal((n, n, n // 2 + 1)) + 1j * rng.standard_normal((n, n, n // 2 + 1))).astype(np.complex64)
+    a = orc.psolver_f32(n, 0.9, 0, spec)
+    b = orc.psolver_f64(n, 0.9, 0, spec.astype(np.complex128))
+    assert np.array_equal(a, b.astype(np.complex64))
+
+
+def test_psolver_deconvolution_divides_by_tsc_window():
+    n = 16
+    spec = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+    spec[2, 1, 3] = 1.0
+    plain = orc.psolver_f64(n, 2.5, 0, spec)[2, 1, 3]
+    dec = orc.psolver_f64(n, 2.5, 1, spec)[2, 1, 3]
+    w = 1.0
+    for m in (3, 1, 2):
+        h = np.pi * m / n
+        w *= (np.sin(h) / h) ** 6
+    assert np.isclose(dec, plain / w, rtol=1e-13)
+
+
+@pytest.mark.parametrize("m", [1, 2, 5])
+def test_fda4_transfer_function(m):
+    n = 32
+    x = np.arange(n)
+    k = 2 * np.pi * m / n
+    phi = np.broadcast_to(np.sin(k * x)[None, None, :], (n, n, n)).astype(np.float32)
+    g = orc.fda4_gradient(n, phi).reshape(n, n, n, 4)
+    expect = -np.cos(k * x) * ((4 / 3) * np.sin(k) - (1 / 6) * np.sin(2 * k))
+    assert np.allclose(g[0, 0, :, 0], expect, atol=2e-6)
+    assert np.all(g[..., 1] == 0) and np.all(g[..., 2] == 0)
+
+
+def test_fda4_constant_field_gives_zero():
+    n = 16
+    g = orc.fda4_gradient(n, np.full(n ** 3, 3.25, dtype=np.float32))
+    assert np.all(g[:, :3] == 0)
+
+
+def test_pm_momentum_and_zero_self_force():
+    n = 16
+    pos = np.random.default_rng(6).random((500, 3)) * n
+    acc = orc.pm_accel(n, pos, mass=n ** 3 / 500, efold=0.9, prec=1)
+    assert np.max(np.abs(acc.sum(axis=0))) < 1e-10 * np.max(np.abs(acc)) * len(pos) + 1e-12
+    one = orc.pm_accel(n, np.array([[5.3, 7.7, 2.2]]), mass=1.0, efold=0.9, prec=1)
+    assert np.max(np.abs(one)) < 1e-12
+
+
+def long_range_force(r, rs, n):
+    from scipy.special import erf
+    u = r / (2 * rs)
+    return (erf(u) - r / (rs * np.sqrt(np.pi)) * np.exp(-u * u)) / r ** 2 / n ** 2
+
+
+def test_pm_point_mass_follows_gaussian_long_range_law():
+    """Two unit masses along x at separations 2..8 cells: the PM force with TSC^2
+    deconvolution follows the erf long-range law (plus its periodic images)."""
+    n, efold = 64, 2.5
+    rs = efold / 2
+    for r in (2.0, 3.0, 5.0, 8.0):
+        pos = np.array([[20.2, 30.1, 25.7], [20.2 + r, 30.1, 25.7]])
+        acc = orc.pm_accel(n, pos, mass=1.0, efold=efold, deconvolve=1, prec=1)
+        fx = acc[0, 0]
+        # periodic images along x only matter at the 1e-4 level here; include the two nearest
+        expect = long_range_force(r, rs, n) - long_range_force(n - r, rs, n)
+        assert abs(fx - expect) < 0.02 * abs(expect), (r, fx, expect)
+
+
+def test_pmcf_fit_matches_measured_reference_pm_force():
+    """The reference law is Newton minus a fit of the PM force at efold 0.9 without
+    deconvolution (OST/Model4TreeForce.c:92-116). The oracle's own PM chain at those
+    settings must give that long-range force to a few percent at 1.5 <= r <= 4."""
+    n = 64
+    for r in (1.5, 2.5, 4.0):
+        fx = []
+        for off in (0.0, 0.25, 0.5):  # average over sub-cell placement
+            pos = np.array([[20.0 + off, 30.0, 25.0], [20.0 + off + r, 30.0, 25.0]])
+            fx.append(orc.pm_accel(n, pos, mass=1.0, efold=0.9, deconvolve=0, prec=1)[0, 0])
+        newton = 1.0 / r ** 2 / n ** 2
+        fit_long = newton - orc.pmcf(n, 0.0, r)
+        assert abs(np.mean(fx) - fit_long) < 0.06 * fit_long, (r, np.mean(fx), fit_long)
