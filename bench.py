@@ -131,6 +131,19 @@ def main():
         box.drift(ef.pfact)
         return t, s
 
+    # first step traced phase by phase (synchronised), so a slow phase names itself
+    tt = time.perf_counter()
+    box.hubble_drag(ef.fact1)
+    ctx.synchronize()
+    log(f"[bench] trace: drag done {time.perf_counter() - tt:.2f} s")
+    box.pmmain(ef, lagpm.KICK)
+    ctx.synchronize()
+    log(f"[bench] trace: pmmain done {time.perf_counter() - tt:.2f} s ({box.timings().as_dict()})")
+    box.treecorrection(ef)
+    log(f"[bench] trace: treecorrection done {time.perf_counter() - tt:.2f} s ({box.timings().as_dict()})")
+    box.drift(ef.pfact)
+    ctx.synchronize()
+    log(f"[bench] trace: first step done {time.perf_counter() - tt:.2f} s")
     for i in range(args.warmup):
         step()
         log(f"[bench] warmup step {i + 1}/{args.warmup} done")

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <unistd.h>
+#include <cstdlib>
 #include <vector>
 
 #include "lagpm_internal.h"
@@ -244,7 +246,18 @@ int lagpm_create(lagpm_ctx** out, int device) {
         return LAGPM_E_HIP;
     }
     for (int i = 0; i < 8; i++) (void)hipEventCreate(&ctx->ev[i]);
-    std::call_once(g_rocfft_once, []() { rocfft_setup(); });
+    std::call_once(g_rocfft_once, []() {
+        // rocFFT compiles some kernels at run time and caches them; give it a writable cache
+        // file when the caller did not choose one (the default lives under $HOME, which may
+        // not be writable for service accounts).
+        if (!std::getenv("ROCFFT_RTC_CACHE_PATH")) {
+            const char* tmp = std::getenv("TMPDIR");
+            std::string path = std::string(tmp && *tmp ? tmp : "/tmp") + "/lagpm_rocfft_" +
+                               std::to_string((long)getuid()) + ".db";
+            setenv("ROCFFT_RTC_CACHE_PATH", path.c_str(), 0);
+        }
+        rocfft_setup();
+    });
     *out = ctx;
     return LAGPM_OK;
 }

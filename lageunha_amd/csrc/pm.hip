@@ -415,7 +415,7 @@ int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
         HIPCHK(ctx, hipGetLastError());
     }
     const double inv_scale = std::ldexp(1.0, -box->scale_bits);
-    const double mean = (double)box->np * box->p.mass / (double)m;
+    const double mean = 1.0;  // the FFT grid starts at -1 before the fold (Cosmos/pmmain.c:304)
     if (f64)
         hipLaunchKernelGGL(convert_kernel<double>, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, box->acc, m, inv_scale,
                            mean, (double*)box->rho);

@@ -31,6 +31,7 @@ constexpr int PP_CAP = 1152;     // 63 left over + 64 lanes x 16 particles per a
 constexpr int PC_CAP = 128;
 constexpr int PP_TAKE = 16;
 constexpr int STACK_MAX = 8192;  // per-wave walk stack entries (global memory)
+constexpr int WALK_MAX_ITERS = 1 << 20;  // per-group bound on 64-node pop rounds
 constexpr float RAN_F = (float)(6.0 / 4096.0);
 
 __device__ inline uint64_t spread3(uint32_t v) {
@@ -352,6 +353,9 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// Exclusive wave prefix sum. The total is read from lane 63 into a scalar register, so every
+// loop or branch the walk takes on it is wave-uniform by construction (scalar branches): the
+// compiler can then never split the wave's lanes across different iterations of the walk.
 __device__ inline int wave_excl_scan(int v, int lane, int& total) {
     int incl = v;
 #pragma unroll
@@ -359,7 +363,7 @@ __device__ inline int wave_excl_scan(int v, int lane, int& total) {
         int t = __shfl_up(incl, o, 64);
         if (lane >= o) incl += t;
     }
-    total = __shfl(incl, 63, 64);
+    total = __builtin_amdgcn_readlane(incl, 63);
     return incl - v;
 }
 __device__ inline float wave_minf(float v) {
@@ -420,9 +424,12 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
     int* stack = A.stack + (size_t)(blockIdx.x * WALK_WAVES + wid) * STACK_MAX;
 
     for (;;) {
-        int g = 0;
-        if (lane == 0) g = (int)atomicAdd(&A.counters[2], 1ull);
-        g = __shfl(g, 0, 64);
+        // Group ticket. Every lane executes the atomic (lane 0 adds 1, the others 0) and the
+        // ticket is read from lane 0 into a scalar register: no lane-dependent branch sits at
+        // the loop head, so the compiler cannot thread the back edge differently per lane
+        // (which would let lanes 1..63 re-enter a group without a new ticket).
+        const unsigned long long ticket = atomicAdd(&A.counters[2], lane == 0 ? 1ull : 0ull);
+        const int g = __builtin_amdgcn_readlane((int)ticket, 0);
         if ((size_t)g >= A.ngroups) break;
         const int2 gr = A.groups[g];
         const bool active = lane < gr.y;
@@ -520,7 +527,12 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
             wave_sync();
         };
 
+        int iters = 0;
         while (sp > 0) {
+            if (++iters > WALK_MAX_ITERS) {  // a well-formed tree never gets here
+                overflow = true;
+                break;
+            }
             const int take = min(sp, 64);
             sp -= take;
             int e = lane < take ? stack[sp + lane] : -1;
