@@ -94,7 +94,7 @@ int fixed_scale_bits(double total_mass) {
 void free_box(lagpm_box* b) {
     dfree(b->x); dfree(b->y); dfree(b->z); dfree(b->vx); dfree(b->vy); dfree(b->vz); dfree(b->id);
     dfree(b->key); dfree(b->key2); dfree(b->perm); dfree(b->perm2); dfree(b->dtmp); dfree(b->utmp);
-    dfree(b->acc); dfree(b->rho); dfree(b->spec); dfree(b->phi); dfree(b->grad);
+    dfree(b->acc); dfree(b->dep_start); dfree(b->rho); dfree(b->spec); dfree(b->phi); dfree(b->grad);
     dfree(b->pp); dfree(b->ncount); dfree(b->noff); dfree(b->nd_info); dfree(b->nd_parent); dfree(b->nd_child);
     dfree(b->nd_com); dfree(b->nd_q0); dfree(b->nd_q1); dfree(b->nd_d); dfree(b->cellroot); dfree(b->gflag);
     dfree(b->goff); dfree(b->groups); dfree(b->gbox); dfree(b->pgroup); dfree(b->walk_stack); dfree(b->counters);

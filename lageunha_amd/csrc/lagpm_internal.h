@@ -63,6 +63,8 @@ struct lagpm_box {
     void* spec = nullptr;                // n*n*(n/2+1) complex
     void* phi = nullptr;                 // n^3 real
     void* grad = nullptr;                // n^3 x 4 real (fx, fy, fz, phi)
+    uint32_t* dep_start = nullptr;       // deposit chunk starts (+ sentinel)
+    size_t dep_cap = 0;
 
     // tree
     int level_cell = 0;       // chaining level L: 2^L cells per axis, width n/2^L >= rsphere

@@ -74,14 +74,32 @@ __global__ void zero_u64_kernel(unsigned long long* __restrict__ a, size_t m) {
     for (; i < m; i += stride) a[i] = 0ull;
 }
 
+// Deposit chunks: runs of at most DEP_CHUNK consecutive (Morton-sorted) particles that stay
+// inside one Morton block of <= 13 mesh cells per side, so every chunk's TSC footprint fits
+// the 4096-cell LDS box. A chunk starts where the block prefix changes or at a multiple of
+// DEP_CHUNK.
+__global__ void dep_flag_kernel(const uint64_t* __restrict__ K, size_t N, int sh, uint32_t* __restrict__ flag) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    flag[i] = (i == 0 || (i % DEP_CHUNK) == 0 || (K[i] >> sh) != (K[i - 1] >> sh)) ? 1u : 0u;
+}
+__global__ void dep_start_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off, size_t N,
+                                 uint32_t* __restrict__ start, uint32_t nchunk) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N && flag[i]) start[off[i]] = (uint32_t)i;
+    if (i == 0) start[nchunk] = (uint32_t)N;
+}
+
 template <bool F64>
 __global__ void __launch_bounds__(DEP_THREADS) deposit_kernel(const double* __restrict__ px, const double* __restrict__ py,
-                                                              const double* __restrict__ pz, size_t np, int n, float mass,
-                                                              double scale, unsigned long long* __restrict__ acc) {
+                                                              const double* __restrict__ pz, const uint32_t* __restrict__ start,
+                                                              int n, float mass, double scale,
+                                                              unsigned long long* __restrict__ acc) {
     __shared__ unsigned long long box[DEP_LDS];
     __shared__ int red[6][DEP_THREADS / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const size_t base = (size_t)blockIdx.x * DEP_CHUNK;
+    const size_t base = start[blockIdx.x];
+    const size_t np = start[blockIdx.x + 1];
 
     int ix[DEP_PPT], iy[DEP_PPT], iz[DEP_PPT], sx[DEP_PPT], sy[DEP_PPT], sz[DEP_PPT];
     float wx[DEP_PPT][3], wy[DEP_PPT][3], wz[DEP_PPT][3];
@@ -404,14 +422,32 @@ int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
     hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, box->acc, m);
     HIPCHK(ctx, hipGetLastError());
     const double scale = std::ldexp(1.0, box->scale_bits);
-    const unsigned nblk = (unsigned)((box->np + DEP_CHUNK - 1) / DEP_CHUNK);
-    if (nblk) {
+    if (box->np) {
+        // chunk list from the sorted keys (the caller sorted the particles)
+        int lvl = 0;
+        while (lvl < LAGPM_KEY_BITS && (double)n / (double)(1 << lvl) > 13.0) lvl++;
+        const int sh = 3 * (LAGPM_KEY_BITS - lvl);
+        const size_t np = box->np;
+        hipLaunchKernelGGL(dep_flag_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, ctx->stream, box->key, np, sh,
+                           box->ncount);
+        HIPCHK(ctx, hipGetLastError());
+        uint32_t nchunk = 0;
+        RCHK(lagpm_exclusive_scan_u32(ctx, box->ncount, box->noff, np, &nchunk));
+        if ((size_t)nchunk + 1 > box->dep_cap) {
+            if (box->dep_start) (void)hipFree(box->dep_start);
+            box->dep_start = nullptr;
+            box->dep_cap = (size_t)nchunk + nchunk / 4 + 1024;
+            HIPCHK(ctx, hipMalloc(&box->dep_start, box->dep_cap * sizeof(uint32_t)));
+        }
+        hipLaunchKernelGGL(dep_start_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, ctx->stream, box->ncount,
+                           box->noff, np, box->dep_start, nchunk);
+        HIPCHK(ctx, hipGetLastError());
         if (f64)
-            hipLaunchKernelGGL(deposit_kernel<true>, dim3(nblk), dim3(DEP_THREADS), 0, ctx->stream, box->x, box->y, box->z,
-                               box->np, n, (float)box->p.mass, scale, box->acc);
+            hipLaunchKernelGGL(deposit_kernel<true>, dim3(nchunk), dim3(DEP_THREADS), 0, ctx->stream, box->x, box->y, box->z,
+                               box->dep_start, n, (float)box->p.mass, scale, box->acc);
         else
-            hipLaunchKernelGGL(deposit_kernel<false>, dim3(nblk), dim3(DEP_THREADS), 0, ctx->stream, box->x, box->y, box->z,
-                               box->np, n, (float)box->p.mass, scale, box->acc);
+            hipLaunchKernelGGL(deposit_kernel<false>, dim3(nchunk), dim3(DEP_THREADS), 0, ctx->stream, box->x, box->y, box->z,
+                               box->dep_start, n, (float)box->p.mass, scale, box->acc);
         HIPCHK(ctx, hipGetLastError());
     }
     const double inv_scale = std::ldexp(1.0, -box->scale_bits);

@@ -25,11 +25,11 @@
 
 namespace {
 
-constexpr int WALK_THREADS = 256;
+constexpr int WALK_THREADS = 1024;  // one workgroup per CU (LDS: shared table + 16 wave lists)
 constexpr int WALK_WAVES = WALK_THREADS / 64;
-constexpr int PP_CAP = 1152;     // 63 left over + 64 lanes x 16 particles per append round
+constexpr int PP_TAKE = 8;          // particles a lane appends per round
+constexpr int PP_CAP = 64 * PP_TAKE + 64;  // one round + 63 left over
 constexpr int PC_CAP = 128;
-constexpr int PP_TAKE = 16;
 constexpr int STACK_MAX = 8192;  // per-wave walk stack entries (global memory)
 constexpr int WALK_MAX_ITERS = 1 << 20;  // per-group bound on 64-node pop rounds
 constexpr float RAN_F = (float)(6.0 / 4096.0);
@@ -410,10 +410,15 @@ struct walk_args {
 };
 
 __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
+    // One 1024-thread workgroup per CU: the 16 waves share one LDS copy of the particle force
+    // table (32 KB) next to their private interaction lists.
+    __shared__ float2 s_tab0[4096];
     __shared__ uint32_t s_ppidx[WALK_WAVES][PP_CAP];
     __shared__ int s_pcidx[WALK_WAVES][PC_CAP];
     __shared__ float4 s_pp[WALK_WAVES][64];
     __shared__ float4 s_pc[WALK_WAVES][3][64];
+    for (int i = threadIdx.x; i < 4096; i += WALK_THREADS) s_tab0[i] = A.tab0[i];
+    __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     uint32_t* ppidx = s_ppidx[wid];
     int* pcidx = s_pcidx[wid];
@@ -432,12 +437,20 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
         const int g = __builtin_amdgcn_readlane((int)ticket, 0);
         if ((size_t)g >= A.ngroups) break;
         const int2 gr = A.groups[g];
-        const bool active = lane < gr.y;
-        const int me = active ? gr.x + lane : gr.x;
+        const int cnt = __builtin_amdgcn_readfirstlane(gr.y);
+        const int gbeg = __builtin_amdgcn_readfirstlane(gr.x);
+        // Lane replication: a group of cnt <= 32 targets gets R = 64/Tw lanes per target; the
+        // R lanes of a target split each source chunk and are summed at the end.
+        int R = 1;
+        while (cnt * 2 * R <= 64) R <<= 1;
+        const int Tw = 64 / R;
+        const int tgt = lane & (Tw - 1), rep = lane / Tw;
+        const bool valid = tgt < cnt;
+        const int me = gbeg + (valid ? tgt : 0);
         const float4 xt = A.pp[me];
         const float lox = wave_minf(xt.x), loy = wave_minf(xt.y), loz = wave_minf(xt.z);
         const float hix = wave_maxf(xt.x), hiy = wave_maxf(xt.y), hiz = wave_maxf(xt.z);
-        const uint64_t cellkey = A.key[gr.x] >> A.cell_shift;
+        const uint64_t cellkey = A.key[gbeg] >> A.cell_shift;
         const int cx = (int)compact3(cellkey), cy = (int)compact3(cellkey >> 1), cz = (int)compact3(cellkey >> 2);
         float ax = 0.f, ay = 0.f, az = 0.f;
         unsigned long long npp = 0, npc = 0;
@@ -472,15 +485,15 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
                 cpp[lane] = make_float4(s.x + sx, s.y + sy, s.z + sz, s.w);
             }
             wave_sync();
-            for (int j = 0; j < m; j++) {
+            for (int j = rep; j < m; j += R) {
                 float4 s = cpp[j];
                 float dx = xt.x - s.x, dy = xt.y - s.y, dz = xt.z - s.z;
                 float r2 = dx * dx + dy * dy + dz * dz;
                 if (r2 <= A.rc2) {
-                    float r = __fsqrt_rn(r2);
+                    float r = __builtin_amdgcn_sqrtf(r2);
                     int nt = min((int)(r * (1.0f / RAN_F)), 4095);
                     float t = r - (float)nt * RAN_F;
-                    float2 tb = A.tab0[nt];
+                    float2 tb = s_tab0[nt];
                     float f = (tb.x + tb.y * t) * s.w;
                     ax += dx * f; ay += dy * f; az += dz * f;
                 }
@@ -500,15 +513,15 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
                 cpc2[lane] = A.q1[v];
             }
             wave_sync();
-            for (int j = 0; j < m; j++) {
+            for (int j = rep; j < m; j += R) {
                 float4 c = cpc0[j], qa = cpc1[j], qb = cpc2[j];
                 float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
                 float r2 = dx * dx + dy * dy + dz * dz;
                 if (r2 <= A.rc2) {
-                    float r = __fsqrt_rn(r2);
+                    float r = __builtin_amdgcn_sqrtf(r2);
                     int nt = min((int)(r * (1.0f / RAN_F)), 4095);
                     float t = r - (float)nt * RAN_F;
-                    float2 t0 = A.tab0[nt];
+                    float2 t0 = s_tab0[nt];
                     float4 t12 = A.tab12[nt];
                     float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
                     float qx1 = qa.x * dx + qa.w * dy + qb.x * dz;
@@ -626,16 +639,22 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
         }
         if (pc_n > 0) flush_pc(0, pc_n);
         if (pp_n > 0) flush_pp(0, pp_n);
+        // sum the R partial accumulators of each target (lanes tgt, tgt + Tw, ...)
+        for (int o = Tw; o < 64; o <<= 1) {
+            ax += __shfl_xor(ax, o, 64);
+            ay += __shfl_xor(ay, o, 64);
+            az += __shfl_xor(az, o, 64);
+        }
         if (overflow) {
             if (lane == 0) atomicAdd(&A.counters[3], 1ull);
-        } else if (active) {
+        } else if (valid && rep == 0) {
             A.vx[me] += (double)(A.fact * ax);
             A.vy[me] += (double)(A.fact * ay);
             A.vz[me] += (double)(A.fact * az);
         }
         if (lane == 0) {
-            atomicAdd(&A.counters[0], npp * (unsigned long long)gr.y);
-            atomicAdd(&A.counters[1], npc * (unsigned long long)gr.y);
+            atomicAdd(&A.counters[0], npp * (unsigned long long)cnt);
+            atomicAdd(&A.counters[1], npc * (unsigned long long)cnt);
         }
     }
 }
@@ -909,6 +928,6 @@ int lagpm_tree_walk(lagpm_ctx* ctx, lagpm_box* box, double fact) {
 }
 
 size_t lagpm_walk_stack_entries(int num_cu) {
-    size_t blocks = (size_t)std::max(1, num_cu) * 4;  // 4 workgroups (16 waves) per CU
+    size_t blocks = (size_t)std::max(1, num_cu);  // one 16-wave workgroup per CU
     return blocks * WALK_WAVES * STACK_MAX;
 }
