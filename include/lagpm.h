@@ -42,10 +42,11 @@ extern "C" {
 #define LAGPM_E_STATE    -5   /* call out of order (e.g. no particles uploaded) */
 #define LAGPM_E_OVERFLOW -6   /* a bounded device structure overflowed (walk stack, node pool) */
 
-/* PM status of the kick, Cosmos/globalT.c:113-131 and Cosmos/fda4.c:49-63 */
-#define LAGPM_KICK 0
-#define LAGPM_PUSH 1
-#define LAGPM_PULL 2
+/* PM status of the kick, same values as PUSH/PULL/KICK of eunha.h:69-77
+ * (selection in Cosmos/globalT.c:113-131 and Cosmos/fda4.c:49-63) */
+#define LAGPM_PUSH 0
+#define LAGPM_PULL 1
+#define LAGPM_KICK 2
 
 /* short-range force law of the tree correction */
 #define LAGPM_LAW_PMFIT 0  /* reference: Plummer Newton minus fitted PM force, OST/Model4TreeForce.c:92-116 */

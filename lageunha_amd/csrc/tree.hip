@@ -32,6 +32,9 @@ constexpr int PP_CAP = 64 * PP_TAKE + 64;  // one round + 63 left over
 constexpr int PC_CAP = 128;
 constexpr int STACK_MAX = 8192;  // per-wave walk stack entries (global memory)
 constexpr int WALK_MAX_ITERS = 1 << 20;  // per-group bound on 64-node pop rounds
+// Internal nodes up to this many particles get the exact max distance from their centre of
+// mass (the reference's nodesize); larger ones use the bound max(|d_child| + rmax_child).
+constexpr int RMAX_EXACT_MAX = 512;
 constexpr float RAN_F = (float)(6.0 / 4096.0);
 
 __device__ inline uint64_t spread3(uint32_t v) {
@@ -250,6 +253,15 @@ __global__ void moments_f32_kernel(const float4* __restrict__ pp, size_t nnodes,
             kahan(Q[5], cQ[5], b1.y + a.w * dy * dz);
             rmax = fmaxf(rmax, sqrtf(dx * dx + dy * dy + dz * dz) + b1.w);
         }
+        if (nd.y <= RMAX_EXACT_MAX) {  // exact max distance, as the reference's nodesize
+            float r2 = 0.f;
+            for (int p = nd.x; p < nd.x + nd.y; p++) {
+                float4 a = pp[p];
+                float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
+                r2 = fmaxf(r2, dx * dx + dy * dy + dz * dz);
+            }
+            rmax = sqrtf(r2);
+        }
     }
     com[v] = make_float4(cm.x, cm.y, cm.z, M);
     q0[v] = make_float4(Q[0], Q[1], Q[2], Q[3]);
@@ -287,6 +299,14 @@ __global__ void moments_f64_kernel(const double* __restrict__ x, const double* _
             Q[0] += a[4] + m * dx * dx; Q[1] += a[5] + m * dy * dy; Q[2] += a[6] + m * dz * dz;
             Q[3] += a[7] + m * dx * dy; Q[4] += a[8] + m * dx * dz; Q[5] += a[9] + m * dy * dz;
             rmax = fmax(rmax, sqrt(dx * dx + dy * dy + dz * dz) + a[11]);
+        }
+        if (nd.y <= RMAX_EXACT_MAX) {
+            double r2 = 0;
+            for (int p = nd.x; p < nd.x + nd.y; p++) {
+                double dx = x[p] - cxx, dy = y[p] - cyy, dz = z[p] - czz;
+                r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+            }
+            rmax = sqrt(r2);
         }
     }
     double* o = nd_d + 12 * v;
@@ -476,30 +496,50 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
             sy = image_shift(cy, (nb / 3) % 3 - 1, A.mcell, A.box);
             sz = image_shift(cz, nb / 9 - 1, A.mcell, A.box);
         };
-        auto flush_pp = [&](int start, int m) {
+        // Source chunks are staged in LDS as float4 (x, y, z, m); entries past m are far-away
+        // dummies of zero mass, so every lane runs the same unrolled trip count and the cutoff
+        // is a select (no branch, no exec-mask loop).
+        const float4 dummy = make_float4(1.0e4f, 1.0e4f, 1.0e4f, 0.f);
+        auto load_pp = [&](int start, int m) -> float4 {
+            float4 s = dummy;
             if (lane < m) {
                 uint32_t e = ppidx[start + lane];
                 float sx, sy, sz;
                 shift_of((int)(e & 31u), sx, sy, sz);
-                float4 s = A.pp[e >> 5];
-                cpp[lane] = make_float4(s.x + sx, s.y + sy, s.z + sz, s.w);
+                s = A.pp[e >> 5];
+                s.x += sx; s.y += sy; s.z += sz;
             }
-            wave_sync();
-            for (int j = rep; j < m; j += R) {
-                float4 s = cpp[j];
+            return s;
+        };
+        auto pp_chunk = [&]() {
+            const int iters = 64 / R;
+#pragma unroll 2
+            for (int k = 0; k < iters; k++) {
+                float4 s = cpp[rep + k * R];
                 float dx = xt.x - s.x, dy = xt.y - s.y, dz = xt.z - s.z;
                 float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 <= A.rc2) {
-                    float r = __builtin_amdgcn_sqrtf(r2);
-                    int nt = min((int)(r * (1.0f / RAN_F)), 4095);
-                    float t = r - (float)nt * RAN_F;
-                    float2 tb = s_tab0[nt];
-                    float f = (tb.x + tb.y * t) * s.w;
-                    ax += dx * f; ay += dy * f; az += dz * f;
-                }
+                if (!__any(r2 <= A.rc2)) continue;  // wave-uniform skip of far sources
+                float r = __builtin_amdgcn_sqrtf(r2);
+                int nt = min((int)(r * (1.0f / RAN_F)), 4095);
+                float t = r - (float)nt * RAN_F;
+                float2 tb = s_tab0[nt];
+                float f = (tb.x + tb.y * t) * s.w;
+                f = (r2 <= A.rc2) ? f : 0.f;
+                ax += dx * f; ay += dy * f; az += dz * f;
             }
-            npp += (unsigned long long)m;
-            wave_sync();
+        };
+        // flush `nchunk` consecutive chunks starting at list entry `start` (the last may be
+        // partial: `mlast` entries); the next chunk's global loads overlap this chunk's math
+        auto flush_pp = [&](int start, int nchunk, int mlast) {
+            float4 nxt = load_pp(start, nchunk == 1 ? mlast : 64);
+            for (int c = 0; c < nchunk; c++) {
+                cpp[lane] = nxt;
+                wave_sync();
+                if (c + 1 < nchunk) nxt = load_pp(start + 64 * (c + 1), c + 2 == nchunk ? mlast : 64);
+                pp_chunk();
+                npp += (unsigned long long)(c + 1 == nchunk ? mlast : 64);
+                wave_sync();
+            }
         };
         auto flush_pc = [&](int start, int m) {
             if (lane < m) {
@@ -511,30 +551,39 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
                 cpc0[lane] = make_float4(c.x + sx, c.y + sy, c.z + sz, c.w);
                 cpc1[lane] = A.q0[v];
                 cpc2[lane] = A.q1[v];
+            } else {
+                cpc0[lane] = dummy;
+                cpc1[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+                cpc2[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
             wave_sync();
-            for (int j = rep; j < m; j += R) {
+            const int iters = 64 / R;
+#pragma unroll 2
+            for (int k = 0; k < iters; k++) {
+                const int j = rep + k * R;
                 float4 c = cpc0[j], qa = cpc1[j], qb = cpc2[j];
                 float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
                 float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 <= A.rc2) {
-                    float r = __builtin_amdgcn_sqrtf(r2);
-                    int nt = min((int)(r * (1.0f / RAN_F)), 4095);
-                    float t = r - (float)nt * RAN_F;
-                    float2 t0 = s_tab0[nt];
-                    float4 t12 = A.tab12[nt];
-                    float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
-                    float qx1 = qa.x * dx + qa.w * dy + qb.x * dz;
-                    float qx2 = qa.w * dx + qa.y * dy + qb.y * dz;
-                    float qx3 = qb.x * dx + qb.y * dy + qa.z * dz;
-                    float qxy = qa.x * dx * dx + qa.y * dy * dy + qa.z * dz * dz +
-                                2.f * (qa.w * dx * dy + qb.x * dx * dz + qb.y * dy * dz);
-                    float tmp = c.w * f1 + qb.z * f2 + qxy * f3;
-                    float two = 2.f * f2;
-                    ax += dx * tmp + qx1 * two;
-                    ay += dy * tmp + qx2 * two;
-                    az += dz * tmp + qx3 * two;
-                }
+                if (!__any(r2 <= A.rc2)) continue;  // wave-uniform skip of far nodes
+                float r = __builtin_amdgcn_sqrtf(r2);
+                int nt = min((int)(r * (1.0f / RAN_F)), 4095);
+                float t = r - (float)nt * RAN_F;
+                float2 t0 = s_tab0[nt];
+                float4 t12 = A.tab12[nt];
+                float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
+                float qx1 = qa.x * dx + qa.w * dy + qb.x * dz;
+                float qx2 = qa.w * dx + qa.y * dy + qb.y * dz;
+                float qx3 = qb.x * dx + qb.y * dy + qa.z * dz;
+                float qxy = qa.x * dx * dx + qa.y * dy * dy + qa.z * dz * dz +
+                            2.f * (qa.w * dx * dy + qb.x * dx * dz + qb.y * dy * dz);
+                float tmp = c.w * f1 + qb.z * f2 + qxy * f3;
+                float two = 2.f * f2;
+                const bool in = r2 <= A.rc2;
+                tmp = in ? tmp : 0.f;
+                two = in ? two : 0.f;
+                ax += dx * tmp + qx1 * two;
+                ay += dy * tmp + qx2 * two;
+                az += dz * tmp + qx3 * two;
             }
             npc += (unsigned long long)m;
             wave_sync();
@@ -611,7 +660,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
                     if (tot == 0) break;
                     if (pp_n + tot > PP_CAP) {
                         int full = pp_n & ~63;
-                        for (int s = 0; s < full; s += 64) flush_pp(s, 64);
+                        if (full) flush_pp(0, full / 64, 64);
                         int left = pp_n - full;
                         uint32_t t = (lane < left) ? ppidx[full + lane] : 0u;
                         wave_sync();
@@ -627,7 +676,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
                 }
                 if (pp_n >= 64) {
                     int full = pp_n & ~63;
-                    for (int s = 0; s < full; s += 64) flush_pp(s, 64);
+                    flush_pp(0, full / 64, 64);
                     int left = pp_n - full;
                     uint32_t t = (lane < left) ? ppidx[full + lane] : 0u;
                     wave_sync();
@@ -638,7 +687,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
             }
         }
         if (pc_n > 0) flush_pc(0, pc_n);
-        if (pp_n > 0) flush_pp(0, pp_n);
+        if (pp_n > 0) flush_pp(0, 1, pp_n);
         // sum the R partial accumulators of each target (lanes tgt, tgt + Tw, ...)
         for (int o = Tw; o < 64; o <<= 1) {
             ax += __shfl_xor(ax, o, 64);

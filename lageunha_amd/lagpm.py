@@ -21,7 +21,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-KICK, PUSH, PULL = 0, 1, 2
+PUSH, PULL, KICK = 0, 1, 2  # eunha.h:69-77
 LAW_PMFIT, LAW_ERFC = 0, 1
 PREC_FP32, PREC_FP64 = 0, 1
 NSPLINE = 4096

@@ -148,3 +148,34 @@ def test_pm_momentum_conservation_on_device(ctx):
         b.pmmain(lagpm.evolfact(1.0), lagpm.KICK)
         a = b.velocities()
     assert np.max(np.abs(a.sum(axis=0))) < 1e-4 * np.abs(a).sum() / len(a) * np.sqrt(len(a))
+
+
+def test_eunha_layout_roundtrip(ctx):
+    """32-byte GOTPM+XYZDBL records (eunha.h:372-400): u64 lattice index in the low 56 bits with a
+    flag byte on top, f32 displacement, f32 velocity. Position = fmod(d + i + n, n) (eunha.h:1228-1232)."""
+    n = 16
+    g = np.random.default_rng(8)
+    npart = n ** 3
+    rec = np.zeros(npart, dtype=[("u4if", "<u8"), ("d", "<f4", 3), ("v", "<f4", 3)])
+    idx = g.permutation(npart).astype(np.uint64)
+    flags = g.integers(0, 256, npart).astype(np.uint64)
+    rec["u4if"] = idx | (flags << np.uint64(56))
+    rec["d"] = (g.standard_normal((npart, 3)) * 2.0).astype(np.float32)
+    rec["d"][:4] = [[-0.5, 0.0, 15.9], [-16.0, 16.0, -1e-7], [0.25, -0.75, 3.5], [-3.0, -15.99, 0.0]]
+    rec["v"] = g.standard_normal((npart, 3)).astype(np.float32)
+    ii = (idx % n).astype(np.float64)
+    jj = ((idx // n) % n).astype(np.float64)
+    kk = (idx // (n * n)).astype(np.float64)
+    d = rec["d"].astype(np.float64)
+    want = np.stack([np.fmod(d[:, 0] + ii + n, n), np.fmod(d[:, 1] + jj + n, n), np.fmod(d[:, 2] + kk + n, n)], 1)
+    p = lagpm.make_params(n)
+    with lagpm.Box(ctx, p, npart) as b:
+        b.upload_eunha(rec)
+        pos, vel = b.download()
+        assert np.array_equal(pos, want)
+        assert np.array_equal(vel, rec["v"].astype(np.float64))
+        b.hubble_drag(0.5)
+        out = rec.copy()
+        b.download_eunha_vel(out)
+    assert np.array_equal(out["u4if"], rec["u4if"]) and np.array_equal(out["d"], rec["d"])
+    assert np.array_equal(out["v"], (rec["v"].astype(np.float64) * 0.5).astype(np.float32))
