@@ -84,6 +84,157 @@ def cpu_baseline(args):
     }
 
 
+SMOOTH_SIZES = (32, 36, 48, 54, 64, 72, 96, 108, 128, 144, 162, 192)
+
+
+def smooth_round(n):
+    """Nearest mesh size with prime factors 2 and 3 only (rocFFT single-kernel lengths)."""
+    return min(SMOOTH_SIZES, key=lambda s: (abs(np.log(s / n)), s))
+
+
+def ensemble_specs(nbox, nmin, nmax, seed):
+    """Box sizes log-uniform in [nmin, nmax] rounded to 2,3-smooth meshes; Omega_m ~ U[0.25, 0.35]."""
+    g = np.random.default_rng(seed)
+    ns = [smooth_round(float(np.exp(g.uniform(np.log(nmin), np.log(nmax))))) for _ in range(nbox)]
+    oms = g.uniform(0.25, 0.35, nbox)
+    return list(zip(ns, oms))
+
+
+def ensemble(args, ctx, lagpm, ics):
+    """64 independent boxes, N_b = n_b^3 on perturbed lattices (seed b), n_b log-uniform in [32, 128],
+    flat LCDM with Omega_m,b ~ U[0.25, 0.35]; 10 GOTPM steps from a = 1 with da = 0.1.
+    plain: every box stepped alone (lagpm_kdk_step), one after another;
+    batched: all boxes per step in one lagpm_batch_step (fused walk, concurrent streams).
+    Both start from the same state; their final states are compared bit for bit."""
+    specs = ensemble_specs(args.ens_boxes, 32, 128, args.seed)
+    sets = {"plain": [], "batched": []}
+    total = 0
+    t0 = time.perf_counter()
+    for b, (n, om) in enumerate(specs):
+        pos = ics.perturbed_lattice(n, rms=0.3, seed=100 + b)
+        p = lagpm.make_params(n, law=lagpm.LAW_ERFC, precision=lagpm.PREC_FP32, deconvolve=1, efold=2.5, epsilon=0.1,
+                              rsphere=5.625, theta=args.theta, amax=48.0, omep=om, omeplam=1.0 - om)
+        for k in sets:
+            bx = lagpm.Box(ctx, p, len(pos))
+            bx.upload_pos(pos)
+            sets[k].append(bx)
+        total += len(pos)
+    log(f"[bench] ensemble: {len(specs)} boxes, {total} particles, sizes {sorted(set(n for n, _ in specs))} "
+        f"(setup {time.perf_counter() - t0:.1f} s)")
+    a0, da, steps = 1.0, 0.1, args.ens_steps
+    # warm-up step (plans, tables) in both modes
+    for bx in sets["plain"]:
+        bx.kdk_step(a0, da)
+    lagpm.batch_step(ctx, sets["batched"], a0, da)
+    lat = {"plain": [], "batched": []}
+    tt = {}
+    t0 = time.perf_counter()
+    for s in range(steps):
+        for bx in sets["plain"]:
+            t = time.perf_counter()
+            bx.kdk_step(a0 + da * (s + 1), da)
+            lat["plain"].append(time.perf_counter() - t)
+    tt["plain"] = time.perf_counter() - t0
+    log(f"[bench] ensemble plain: {tt['plain']:.2f} s")
+    t0 = time.perf_counter()
+    for s in range(steps):
+        t = time.perf_counter()
+        lagpm.batch_step(ctx, sets["batched"], a0 + da * (s + 1), da)
+        lat["batched"].append(time.perf_counter() - t)
+    tt["batched"] = time.perf_counter() - t0
+    log(f"[bench] ensemble batched: {tt['batched']:.2f} s")
+    same = True
+    for bp, bb in zip(sets["plain"], sets["batched"]):
+        xp, vp = bp.download()
+        xb, vb = bb.download()
+        same = same and np.array_equal(xp, xb) and np.array_equal(vp, vb)
+    for k in sets:
+        for bx in sets[k]:
+            bx.close()
+    out = {"boxes": len(specs), "particles": total, "steps": steps,
+           "sizes": sorted(n for n, _ in specs), "identical_plain_vs_batched": bool(same)}
+    for k in ("plain", "batched"):
+        L = np.array(lat[k]) * 1e3
+        out[k] = {"particle_steps_per_s": total * steps / tt[k], "seconds": tt[k],
+                  "step_latency_p50_ms": float(np.percentile(L, 50)), "step_latency_p99_ms": float(np.percentile(L, 99)),
+                  "latency_unit": "one box-step (plain) / one batch step of all boxes (batched)"}
+    out["batched_over_plain"] = out["batched"]["particle_steps_per_s"] / out["plain"]["particle_steps_per_s"]
+    # the plain FP64 kernels, box by box (one step each after a warm-up step)
+    t_fp64, ps = 0.0, 0
+    for b, (n, om) in enumerate(specs):
+        pos = ics.perturbed_lattice(n, rms=0.3, seed=100 + b)
+        p = lagpm.make_params(n, law=lagpm.LAW_ERFC, precision=lagpm.PREC_FP64, deconvolve=1, efold=2.5, epsilon=0.1,
+                              rsphere=5.625, theta=args.theta, amax=48.0, omep=om, omeplam=1.0 - om)
+        with lagpm.Box(ctx, p, len(pos)) as bx:
+            bx.upload_pos(pos)
+            bx.kdk_step(a0, da)
+            t = time.perf_counter()
+            bx.kdk_step(a0 + da, da)
+            t_fp64 += time.perf_counter() - t
+            ps += len(pos)
+    out["plain_fp64"] = {"particle_steps_per_s": ps / t_fp64, "seconds": t_fp64, "steps": 1}
+    out["batched_over_plain_fp64"] = out["batched"]["particle_steps_per_s"] / out["plain_fp64"]["particle_steps_per_s"]
+    log(f"[bench] ensemble plain FP64: {out['plain_fp64']['particle_steps_per_s']:.3e} particle-steps/s")
+    return out
+
+
+def serving(args, ctx, lagpm, ics):
+    """Poisson stream of box requests (rate 20/s over a window), n log-uniform in [32, 192]
+    rounded to 2,3-smooth meshes, 5..50 steps each, Omega_m ~ U[0.25, 0.35]; admission under a
+    200 GB HBM budget; continuous batching (one lagpm_batch_step over all active boxes per
+    iteration). Initial conditions: one perturbed lattice per mesh size, prepared before the
+    clock starts, shifted by a random whole-cell offset per request."""
+    from lageunha_amd.serve import Request, Server
+    g = np.random.default_rng(args.seed + 7)
+    t, reqs = 0.0, []
+    while True:
+        t += g.exponential(1.0 / args.serve_rate)
+        if t > args.serve_window:
+            break
+        n = smooth_round(float(np.exp(g.uniform(np.log(32), np.log(192)))))
+        reqs.append(Request(rid=len(reqs), n=n, steps=int(g.integers(5, 51)), arrival=t,
+                            omep=float(g.uniform(0.25, 0.35)), a_start=1.0, da=0.1, seed=int(g.integers(1 << 30))))
+    t0 = time.perf_counter()
+    base = {n: ics.perturbed_lattice(n, rms=0.3, seed=n) for n in sorted({r.n for r in reqs})}
+    log(f"[bench] serving: {len(reqs)} requests over {args.serve_window:.0f} s, ICs for sizes {sorted(base)} "
+        f"({time.perf_counter() - t0:.1f} s)")
+
+    def params_for(r):
+        return lagpm.make_params(r.n, omep=r.omep, omeplam=1 - r.omep, theta=args.theta)
+
+    def positions_for(r):
+        off = np.random.default_rng(r.seed).integers(0, r.n, 3)
+        return np.mod(base[r.n] + off, r.n)
+
+    # warm the FFT plans of every size (not timed)
+    for n in base:
+        with lagpm.Box(ctx, params_for(Request(-1, n, 1, 0.0)), n ** 3) as b:
+            b.upload_pos(base[n])
+            b.kdk_step(1.0, 0.1)
+    start = time.perf_counter()
+    clock = lambda: time.perf_counter() - start  # noqa: E731
+    srv = Server(ctx, params_for, positions_for, hbm_budget=args.serve_budget_gb * 1e9,
+                 max_active_particles=args.serve_max_particles or None, clock=clock)
+    for r in reqs:
+        srv.submit(r)
+    last = 0.0
+    while srv.queue or srv.active:
+        if not srv.iterate():
+            if srv.queue:
+                time.sleep(max(0.0, min(0.01, srv.queue[0].arrival - clock())))
+        if clock() - last > 5.0:
+            last = clock()
+            log(f"[bench] serving t={last:.1f} s: done {len(srv.done)}/{len(reqs)}, active {len(srv.active)}, "
+                f"queued {len(srv.queue)}")
+    wall = clock()
+    st = srv.stats()
+    st.update({"wall_seconds": wall, "particle_steps_per_s": st["particle_steps"] / wall,
+               "arrival_rate_per_s": args.serve_rate, "window_s": args.serve_window,
+               "hbm_budget_gb": args.serve_budget_gb, "max_active_particles": args.serve_max_particles or None,
+               "latency_unit": "one box-step = the lagpm_batch_step iteration that carried it"})
+    return st
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of known size: FETCH_SIZE / WRITE_SIZE read for it give the
     factor that turns those counters into HBM bytes for the TreePM kernels."""
@@ -110,6 +261,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-boxes", type=int, default=64)
+    ap.add_argument("--ens-steps", type=int, default=10)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-window", type=float, default=8.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-max-particles", type=int, default=0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -247,6 +406,19 @@ def main():
                         "speedup_vs_fp64_plain": t64 / max(t32, 1e-12)})
             log(f"[bench] accuracy vs FP64 plain: rms {rms:.2e} p99 {p99:.2e}; FP64 plain force eval {t64:.3f} s")
         out["accuracy"] = acc
+
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble(args, ctx, lagpm, ics)
+        e = out["ensemble"]
+        log(f"[bench] ensemble: plain {e['plain']['particle_steps_per_s']:.3e} / batched "
+            f"{e['batched']['particle_steps_per_s']:.3e} particle-steps/s, x{e['batched_over_plain']:.2f}, "
+            f"identical {e['identical_plain_vs_batched']}")
+
+    if not args.no_serve:
+        out["serving"] = serving(args, ctx, lagpm, ics)
+        sv = out["serving"]
+        log(f"[bench] serving: {sv['requests']} requests, {sv['particle_steps_per_s']:.3e} particle-steps/s, "
+            f"step p50 {sv.get('step_latency_p50_ms', 0):.1f} ms p99 {sv.get('step_latency_p99_ms', 0):.1f} ms")
 
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline at {args.cpu_n}^3 ...")

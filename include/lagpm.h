@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define LAGPM_ABI_VERSION 1
+#define LAGPM_ABI_VERSION 2
 
 /* status codes */
 #define LAGPM_OK          0
@@ -185,6 +185,19 @@ int lagpm_evolfactor(const lagpm_params* p, double anow, double astep, lagpm_evo
 int lagpm_hubble_drag(lagpm_ctx* ctx, lagpm_box* box, double fact1);
 /* x += pfact * v, wrapped into [0,n) (Cosmos/stepevolv.c:19-113). */
 int lagpm_drift(lagpm_ctx* ctx, lagpm_box* box, double pfact);
+
+/* ---- whole steps ------------------------------------------------------------------ */
+/* One ordinary GOTPM global step of a box with PMSTATUS = KICK (the loop body of
+ * Cosmos/globalT.c:221-306: PM_PULL_AND_PUSH :114-134, then TreeAndPreFoFmain :38-95, then
+ * onestepforwardposition): v *= fact1; v += fact2 * (F_PM + F_tree); x += pfact * v, with the
+ * factors of GetEvolFactor(anow, astep) (Cosmos/CosmosEvolFactor.c:230-256) from the box's own
+ * cosmology. Returns when the step is complete; tree statistics and phase timings are filled. */
+int lagpm_kdk_step(lagpm_ctx* ctx, lagpm_box* box, double anow, double astep);
+/* The same step for nbox independent boxes (box i at anow[i], astep[i]); no counterpart in
+ * the reference, which evolves one box per MPI job. The boxes' PM phases and tree builds run
+ * concurrently on a pool of streams, and the tree walks of all FP32 boxes with the same force
+ * law run as one persistent launch. Each box's result equals its lagpm_kdk_step result. */
+int lagpm_batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* anow, const double* astep);
 
 #ifdef __cplusplus
 }

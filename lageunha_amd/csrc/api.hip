@@ -12,8 +12,6 @@
 
 #include "lagpm_internal.h"
 
-size_t lagpm_walk_stack_entries(int num_cu);
-
 int lagpm_fail(lagpm_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;
     return code;
@@ -47,11 +45,13 @@ __global__ void scale3_kernel(double* __restrict__ a, double* __restrict__ b, do
     b[i] *= f;
     c[i] *= f;
 }
+// skip (optional): a box whose tree build overflowed got no tree kick yet, so it must not
+// drift until the host has redone the kick
 __global__ void drift_kernel(double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
                              const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
-                             size_t np, double pfact, double L) {
+                             size_t np, double pfact, double L, const uint32_t* __restrict__ skip) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
+    if (i >= np || (skip && *skip)) return;
     double p[3] = {x[i] + pfact * vx[i], y[i] + pfact * vy[i], z[i] + pfact * vz[i]};
     for (int a = 0; a < 3; a++) {
         double v = fmod(p[a], L);
@@ -97,8 +97,8 @@ void free_box(lagpm_box* b) {
     dfree(b->acc); dfree(b->dep_start); dfree(b->rho); dfree(b->spec); dfree(b->phi); dfree(b->grad);
     dfree(b->pp); dfree(b->ncount); dfree(b->noff); dfree(b->nd_info); dfree(b->nd_parent); dfree(b->nd_child);
     dfree(b->nd_com); dfree(b->nd_q0); dfree(b->nd_q1); dfree(b->nd_d); dfree(b->cellroot); dfree(b->gflag);
-    dfree(b->goff); dfree(b->groups); dfree(b->gbox); dfree(b->pgroup); dfree(b->walk_stack); dfree(b->counters);
-    dfree(b->tab0); dfree(b->tab12); dfree(b->tabd);
+    dfree(b->goff); dfree(b->groups); dfree(b->gbox); dfree(b->pgroup); dfree(b->counters); dfree(b->dsz);
+    dfree(b->scan_tmp);
 }
 
 int check_params(lagpm_ctx* ctx, const lagpm_params* p) {
@@ -195,30 +195,46 @@ void lagpm_build_tables(const lagpm_params* p, double* diff, double* slope) {
     }
 }
 
-int lagpm_tables_upload(lagpm_ctx* ctx, lagpm_box* box) {
-    const int NS = LAGPM_NSPLINE;
-    std::vector<double> diff(3 * NS), slope(3 * NS);
-    lagpm_build_tables(&box->p, diff.data(), slope.data());
-    if (box->p.precision == LAGPM_PREC_FP64) {
-        std::vector<double> t(6 * NS);
-        for (int i = 0; i < NS; i++)
-            for (int c = 0; c < 3; c++) {
-                t[6 * i + 2 * c] = diff[3 * i + c];
-                t[6 * i + 2 * c + 1] = slope[3 * i + c];
-            }
-        HIPCHK(ctx, hipMemcpyAsync(box->tabd, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    } else {
-        std::vector<float2> t0(NS);
-        std::vector<float4> t12(NS);
-        for (int i = 0; i < NS; i++) {
-            t0[i] = make_float2((float)diff[3 * i], (float)slope[3 * i]);
-            t12[i] = make_float4((float)diff[3 * i + 1], (float)slope[3 * i + 1], (float)diff[3 * i + 2], (float)slope[3 * i + 2]);
-        }
-        HIPCHK(ctx, hipMemcpyAsync(box->tab0, t0.data(), NS * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(box->tab12, t12.data(), NS * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+int lagpm_dalloc_bytes(lagpm_ctx* ctx, void** p, size_t bytes) {
+    HIPCHK(ctx, hipMalloc(p, std::max<size_t>(bytes, 1)));
+    return LAGPM_OK;
+}
+
+// Device force tables of a law, built once per context for n = 1 (the law scales as 1/n^2;
+// the walk multiplies by the box's 1/n^2 with the kick factor).
+int lagpm_get_tables(lagpm_ctx* ctx, const lagpm_params* p, const law_tables** out) {
+    const auto key = std::make_tuple(p->law, p->epsilon, p->efold);
+    auto it = ctx->tables.find(key);
+    if (it != ctx->tables.end()) {
+        *out = &it->second;
+        return LAGPM_OK;
     }
+    const int NS = LAGPM_NSPLINE;
+    lagpm_params q = *p;
+    q.n = 1;
+    std::vector<double> diff(3 * NS), slope(3 * NS);
+    lagpm_build_tables(&q, diff.data(), slope.data());
+    std::vector<double> t(6 * NS);
+    std::vector<float2> t0(NS);
+    std::vector<float4> t12(NS);
+    for (int i = 0; i < NS; i++) {
+        for (int c = 0; c < 3; c++) {
+            t[6 * i + 2 * c] = diff[3 * i + c];
+            t[6 * i + 2 * c + 1] = slope[3 * i + c];
+        }
+        t0[i] = make_float2((float)diff[3 * i], (float)slope[3 * i]);
+        t12[i] = make_float4((float)diff[3 * i + 1], (float)slope[3 * i + 1], (float)diff[3 * i + 2], (float)slope[3 * i + 2]);
+    }
+    law_tables lt;
+    RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tab0, NS * sizeof(float2)));
+    RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tab12, NS * sizeof(float4)));
+    RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tabd, 6 * NS * sizeof(double)));
+    HIPCHK(ctx, hipMemcpyAsync(lt.tab0, t0.data(), NS * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(lt.tab12, t12.data(), NS * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(lt.tabd, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    box->tables_ready = true;
+    auto res = ctx->tables.emplace(key, lt);
+    *out = &res.first->second;
     return LAGPM_OK;
 }
 
@@ -245,7 +261,14 @@ int lagpm_create(lagpm_ctx** out, int device) {
         delete ctx;
         return LAGPM_E_HIP;
     }
-    for (int i = 0; i < 8; i++) (void)hipEventCreate(&ctx->ev[i]);
+    for (int i = 0; i < 4; i++) (void)hipEventCreate(&ctx->ev[i]);
+    ctx->walk_stack_entries = lagpm_walk_stack_entries(ctx->num_cu);
+    if (hipMalloc((void**)&ctx->walk_stack, ctx->walk_stack_entries * sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&ctx->walk_ticket, sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipGetLastError();
+        lagpm_destroy(ctx);
+        return LAGPM_E_NOMEM;
+    }
     std::call_once(g_rocfft_once, []() {
         // rocFFT compiles some kernels at run time and caches them; give it a writable cache
         // file when the caller did not choose one (the default lives under $HOME, which may
@@ -265,12 +288,21 @@ int lagpm_create(lagpm_ctx** out, int device) {
 int lagpm_destroy(lagpm_ctx* ctx) {
     if (!ctx) return LAGPM_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipDeviceSynchronize();
     lagpm_fft_release(ctx);
-    if (ctx->scan_tmp) (void)hipFree(ctx->scan_tmp);
-    for (int i = 0; i < 8; i++)
+    for (auto& kv : ctx->tables) {
+        dfree(kv.second.tab0);
+        dfree(kv.second.tab12);
+        dfree(kv.second.tabd);
+    }
+    dfree(ctx->walk_stack);
+    dfree(ctx->walk_ticket);
+    dfree(ctx->walk_boxes);
+    dfree(ctx->walk_prefix);
+    for (int i = 0; i < 4; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
-    (void)hipStreamDestroy(ctx->stream);
+    for (hipStream_t s : ctx->pool) (void)hipStreamDestroy(s);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LAGPM_OK;
 }
@@ -284,15 +316,20 @@ int lagpm_synchronize(lagpm_ctx* ctx) {
 }
 
 size_t lagpm_estimate_box_bytes(const lagpm_params* p, size_t np) {
-    if (!p) return 0;
+    if (!p || p->n < 1) return 0;
     const size_t n = (size_t)p->n, m = n * n * n, K = (n / 2 + 1) * n * n;
-    const size_t real = p->precision == LAGPM_PREC_FP64 ? 8 : 4;
-    size_t b = np * (7 * 8 + 4 + 2 * 8 + 2 * 4 + 4 + 2 * 4);  // particles, keys, perm, tmp, node counts
-    b += m * 8 + m * real * 2 + K * 2 * real + m * 4 * real;   // acc, rho, phi, spec, grad
-    if (p->precision == LAGPM_PREC_FP32) b += np * 16;
-    else b += np * 4;
-    b += (np / 4 + 1024) * (16 + 4 + 32 + 8 + (p->precision == LAGPM_PREC_FP64 ? 96 : 48));  // nodes (estimate)
-    b += (np / 8 + 1024) * 8;                                                                   // groups
+    const bool f64 = p->precision == LAGPM_PREC_FP64;
+    const size_t real = f64 ? 8 : 4;
+    const int L = lagpm_choose_cell_level(p->n, p->rsphere);
+    const size_t mcell = (size_t)1 << L;
+    size_t b = np * (6 * 8 + 4 * 4 + 2 * 8 + 8 + 2 * 4);  // x..vz, id/perm/perm2/utmp, keys, dtmp, node counts
+    b += np * (f64 ? 4 : 16);                             // pgroup / pp
+    b += m * 8 + 2 * m * real + K * 2 * real + 4 * m * real;  // acc, rho, phi, spec, grad
+    b += 4 * mcell * mcell * mcell + 4 * 8 + DSZ_COUNT * 4;  // cellroot, counters, sizes
+    b += 4 * (lagpm_deposit_chunk_cap(p->n, np) + 1);     // deposit chunk starts
+    const size_t cap = lagpm_tree_node_cap_guess(np, (int)mcell), gcap = cap + np / 64 + 1;
+    b += cap * (16 + 4 + 32 + 4 + 4 + (f64 ? 96 : 48)) + gcap * (8 + (f64 ? 48 : 0));
+    b += std::max(lagpm_scan_tmp_bytes(cap), std::max(lagpm_scan_tmp_bytes(np), lagpm_sort_tmp_bytes(np)));
     return b;
 }
 
@@ -328,32 +365,41 @@ int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box
     ALLOC(box->ncount, np); ALLOC(box->noff, np);
     ALLOC(box->cellroot, (size_t)box->mcell * box->mcell * box->mcell);
     ALLOC(box->counters, 4);
-    if (f64) {
-        ALLOC(box->tabd, 6 * LAGPM_NSPLINE);
-        ALLOC(box->pgroup, np);
-    } else {
-        ALLOC(box->pp, np);
-        ALLOC(box->tab0, LAGPM_NSPLINE);
-        ALLOC(box->tab12, LAGPM_NSPLINE);
-        box->walk_stack_entries = lagpm_walk_stack_entries(ctx->num_cu);
-        ALLOC(box->walk_stack, box->walk_stack_entries);
-    }
+    ALLOC(box->dsz, DSZ_COUNT);
+    box->dep_cap = lagpm_deposit_chunk_cap(p->n, np) + 1;
+    ALLOC(box->dep_start, box->dep_cap);
+    if (f64) ALLOC(box->pgroup, np);
+    else ALLOC(box->pp, np);
 #undef ALLOC
+    if (rc == LAGPM_OK) {
+        // node arrays and scan scratch (grown later if an input needs more nodes)
+        rc = lagpm_tree_alloc(ctx, box, lagpm_tree_node_cap_guess(np, box->mcell));
+        if (rc == LAGPM_OK) {
+            box->bytes += box->scan_tmp_bytes + box->node_cap * (16 + 4 + 32 + 4 + 4 + (f64 ? 96 : 48)) +
+                          box->group_cap * (8 + (f64 ? 48 : 0));
+            // the node arrays may grow later (dense input); the estimate is what admission uses
+            box->bytes = std::max(box->bytes, lagpm_estimate_box_bytes(p, np));
+        }
+    }
+    if (rc == LAGPM_OK) rc = lagpm_get_tables(ctx, p, &box->tab);
     if (rc != LAGPM_OK) {
         free_box(box);
         delete box;
         return rc;
     }
+    box->st = ctx->stream;
     for (int i = 0; i < 12; i++) (void)hipEventCreate(&box->ev[i]);
     (void)hipMemsetAsync(box->vx, 0, np * sizeof(double), ctx->stream);
     (void)hipMemsetAsync(box->vy, 0, np * sizeof(double), ctx->stream);
     (void)hipMemsetAsync(box->vz, 0, np * sizeof(double), ctx->stream);
+    (void)hipMemsetAsync(box->dsz, 0, DSZ_COUNT * sizeof(uint32_t), ctx->stream);
     *out = box;
     return LAGPM_OK;
 }
 
 int lagpm_box_destroy(lagpm_ctx* ctx, lagpm_box* box) {
     if (!box) return LAGPM_OK;
+    if (ctx) (void)hipStreamSynchronize(box->st);
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
     for (int i = 0; i < 12; i++)
         if (box->ev[i]) (void)hipEventDestroy(box->ev[i]);
@@ -378,21 +424,21 @@ int lagpm_box_upload(lagpm_ctx* ctx, lagpm_box* box, const double* x, const doub
         box->uploaded = true;
         return LAGPM_OK;
     }
-    HIPCHK(ctx, hipMemcpyAsync(box->x, x, b, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(box->y, y, b, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(box->z, z, b, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(box->x, x, b, hipMemcpyHostToDevice, box->st));
+    HIPCHK(ctx, hipMemcpyAsync(box->y, y, b, hipMemcpyHostToDevice, box->st));
+    HIPCHK(ctx, hipMemcpyAsync(box->z, z, b, hipMemcpyHostToDevice, box->st));
     double* vs[3] = {box->vx, box->vy, box->vz};
     const double* hs[3] = {vx, vy, vz};
     for (int a = 0; a < 3; a++) {
-        if (hs[a]) HIPCHK(ctx, hipMemcpyAsync(vs[a], hs[a], b, hipMemcpyHostToDevice, ctx->stream));
-        else HIPCHK(ctx, hipMemsetAsync(vs[a], 0, b, ctx->stream));
+        if (hs[a]) HIPCHK(ctx, hipMemcpyAsync(vs[a], hs[a], b, hipMemcpyHostToDevice, box->st));
+        else HIPCHK(ctx, hipMemsetAsync(vs[a], 0, b, box->st));
     }
     double* ps[3] = {box->x, box->y, box->z};
     for (int a = 0; a < 3; a++)
-        hipLaunchKernelGGL(wrap_kernel, dim3(nblocks(np, 256)), dim3(256), 0, ctx->stream, ps[a], np, (double)box->p.n);
-    hipLaunchKernelGGL(iota_kernel, dim3(nblocks(np, 256)), dim3(256), 0, ctx->stream, box->id, np);
+        hipLaunchKernelGGL(wrap_kernel, dim3(nblocks(np, 256)), dim3(256), 0, box->st, ps[a], np, (double)box->p.n);
+    hipLaunchKernelGGL(iota_kernel, dim3(nblocks(np, 256)), dim3(256), 0, box->st, box->id, np);
     HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(box->st));
     box->uploaded = true;
     box->sorted = false;
     return LAGPM_OK;
@@ -405,10 +451,10 @@ int lagpm_box_download(lagpm_ctx* ctx, lagpm_box* box, double* x, double* y, dou
     double* dst[6] = {x, y, z, vx, vy, vz};
     for (int a = 0; a < 6; a++) {
         if (!dst[a] || np == 0) continue;
-        hipLaunchKernelGGL(scatter_kernel, dim3(nblocks(np, 256)), dim3(256), 0, ctx->stream, src[a], box->id, np, box->dtmp);
+        hipLaunchKernelGGL(scatter_kernel, dim3(nblocks(np, 256)), dim3(256), 0, box->st, src[a], box->id, np, box->dtmp);
         HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipMemcpyAsync(dst[a], box->dtmp, np * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(dst[a], box->dtmp, np * sizeof(double), hipMemcpyDeviceToHost, box->st));
+        HIPCHK(ctx, hipStreamSynchronize(box->st));
     }
     return LAGPM_OK;
 }
@@ -485,14 +531,14 @@ int lagpm_tsc(lagpm_ctx* ctx, lagpm_box* box, float* den) {
         const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
         if (box->p.precision == LAGPM_PREC_FP64) {
             std::vector<double> t(m);
-            HIPCHK(ctx, hipMemcpyAsync(t.data(), box->rho, m * 8, hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(t.data(), box->rho, m * 8, hipMemcpyDeviceToHost, box->st));
+            HIPCHK(ctx, hipStreamSynchronize(box->st));
             for (size_t i = 0; i < m; i++) den[i] = (float)t[i];
         } else {
-            HIPCHK(ctx, hipMemcpyAsync(den, box->rho, m * 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(den, box->rho, m * 4, hipMemcpyDeviceToHost, box->st));
         }
     }
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(box->st));
     return LAGPM_OK;
 }
 
@@ -501,9 +547,9 @@ int lagpm_tsc_fixed(lagpm_ctx* ctx, lagpm_box* box, int64_t* acc, int* scale_bit
     if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
     RCHK(lagpm_pm_deposit(ctx, box));
     const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
-    if (acc) HIPCHK(ctx, hipMemcpyAsync(acc, box->acc, m * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (acc) HIPCHK(ctx, hipMemcpyAsync(acc, box->acc, m * 8, hipMemcpyDeviceToHost, box->st));
     if (scale_bits) *scale_bits = box->scale_bits;
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(box->st));
     return LAGPM_OK;
 }
 
@@ -514,14 +560,14 @@ int lagpm_psolver(lagpm_ctx* ctx, lagpm_box* box, float* phi) {
         const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
         if (box->p.precision == LAGPM_PREC_FP64) {
             std::vector<double> t(m);
-            HIPCHK(ctx, hipMemcpyAsync(t.data(), box->phi, m * 8, hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(t.data(), box->phi, m * 8, hipMemcpyDeviceToHost, box->st));
+            HIPCHK(ctx, hipStreamSynchronize(box->st));
             for (size_t i = 0; i < m; i++) phi[i] = (float)t[i];
         } else {
-            HIPCHK(ctx, hipMemcpyAsync(phi, box->phi, m * 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(phi, box->phi, m * 4, hipMemcpyDeviceToHost, box->st));
         }
     }
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(box->st));
     return LAGPM_OK;
 }
 
@@ -533,7 +579,7 @@ int lagpm_green_apply(lagpm_ctx* ctx, int n, double efold, int deconvolve, float
     HIPCHK(ctx, hipMalloc(&d, K * 2 * sizeof(float)));
     int rc = LAGPM_OK;
     if (hipMemcpyAsync(d, spec_host, K * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
-    if (rc == LAGPM_OK) rc = lagpm_green_device(ctx, n, LAGPM_PREC_FP32, efold, deconvolve, d);
+    if (rc == LAGPM_OK) rc = lagpm_green_device(ctx, ctx->stream, n, LAGPM_PREC_FP32, efold, deconvolve, d);
     if (rc == LAGPM_OK && hipMemcpyAsync(spec_host, d, K * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(d);
@@ -547,7 +593,7 @@ int lagpm_fda4(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef, int pms
     RCHK(pm_factor(ctx, ef, pmstatus, &f));
     RCHK(lagpm_pm_gradient(ctx, box));
     RCHK(lagpm_pm_interp(ctx, box, f));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(box->st));
     return LAGPM_OK;
 }
 
@@ -556,11 +602,11 @@ int lagpm_set_potential(lagpm_ctx* ctx, lagpm_box* box, const float* phi) {
     const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
     if (box->p.precision == LAGPM_PREC_FP64) {
         std::vector<double> t(phi, phi + m);
-        HIPCHK(ctx, hipMemcpyAsync(box->phi, t.data(), m * 8, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(box->phi, t.data(), m * 8, hipMemcpyHostToDevice, box->st));
+        HIPCHK(ctx, hipStreamSynchronize(box->st));
     } else {
-        HIPCHK(ctx, hipMemcpyAsync(box->phi, phi, m * 4, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(box->phi, phi, m * 4, hipMemcpyHostToDevice, box->st));
+        HIPCHK(ctx, hipStreamSynchronize(box->st));
     }
     return LAGPM_OK;
 }
@@ -570,12 +616,12 @@ int lagpm_get_gradient(lagpm_ctx* ctx, lagpm_box* box, float* grad4) {
     const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
     if (box->p.precision == LAGPM_PREC_FP64) {
         std::vector<double> t(4 * m);
-        HIPCHK(ctx, hipMemcpyAsync(t.data(), box->grad, 4 * m * 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(t.data(), box->grad, 4 * m * 8, hipMemcpyDeviceToHost, box->st));
+        HIPCHK(ctx, hipStreamSynchronize(box->st));
         for (size_t i = 0; i < 4 * m; i++) grad4[i] = (float)t[i];
     } else {
-        HIPCHK(ctx, hipMemcpyAsync(grad4, box->grad, 4 * m * 4, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(grad4, box->grad, 4 * m * 4, hipMemcpyDeviceToHost, box->st));
+        HIPCHK(ctx, hipStreamSynchronize(box->st));
     }
     return LAGPM_OK;
 }
@@ -585,16 +631,16 @@ int lagpm_pmmain(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef, int p
     double f;
     RCHK(pm_factor(ctx, ef, pmstatus, &f));
     hipEvent_t* ev = box->ev;
-    HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ev[0], box->st));
     if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
-    HIPCHK(ctx, hipEventRecord(ev[1], ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ev[1], box->st));
     RCHK(lagpm_pm_deposit(ctx, box));
-    HIPCHK(ctx, hipEventRecord(ev[2], ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ev[2], box->st));
     RCHK(lagpm_pm_solve(ctx, box, ev + 3));
     RCHK(lagpm_pm_gradient(ctx, box));
-    HIPCHK(ctx, hipEventRecord(ev[6], ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ev[6], box->st));
     RCHK(lagpm_pm_interp(ctx, box, f));
-    HIPCHK(ctx, hipEventRecord(ev[7], ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ev[7], box->st));
     box->pm_timed = true;
     return LAGPM_OK;
 }
@@ -606,36 +652,60 @@ int lagpm_force_spline(const lagpm_params* p, double* diff, double* slope) {
     return LAGPM_OK;
 }
 
-int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef) {
-    RCHK(need_particles(ctx, box));
-    if (!ef) return lagpm_fail(ctx, LAGPM_E_INVAL, "evolfact is NULL");
-    if (box->level_cell < 2)
-        return lagpm_fail(ctx, LAGPM_E_INVAL, "box too small for the cutoff: need n >= 4 * rsphere");
-    if (box->np == 0) return LAGPM_OK;
-    if (!box->tables_ready) RCHK(lagpm_tables_upload(ctx, box));
+// Queues sort (when needed), build and walk of one box. The FP32 walk runs on the context
+// stream (its stacks are shared), so it waits for the box stream and the box stream for it.
+static int tree_enqueue(lagpm_ctx* ctx, lagpm_box* box, double fact) {
     hipEvent_t* ev = box->ev + 8;
-    HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ev[0], box->st));
     if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
-    HIPCHK(ctx, hipEventRecord(ev[1], ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ev[1], box->st));
     RCHK(lagpm_tree_build(ctx, box));
-    if (box->nnodes >= (size_t)(1u << 26)) return lagpm_fail(ctx, LAGPM_E_OVERFLOW, "more than 2^26 tree nodes");
-    HIPCHK(ctx, hipEventRecord(ev[2], ctx->stream));
-    RCHK(lagpm_tree_walk(ctx, box, (double)ef->fact2));
-    HIPCHK(ctx, hipEventRecord(ev[3], ctx->stream));
-    unsigned long long c[4];
-    HIPCHK(ctx, hipMemcpyAsync(c, box->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ev[2], box->st));
+    if (box->p.precision == LAGPM_PREC_FP64) {
+        RCHK(lagpm_tree_walk_f64(ctx, box, fact));
+        HIPCHK(ctx, hipEventRecord(ev[3], box->st));
+    } else {
+        if (box->st != ctx->stream) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ev[2], 0));
+        RCHK(lagpm_tree_walk_f32(ctx, &box, &fact, 1));
+        HIPCHK(ctx, hipEventRecord(ev[3], ctx->stream));
+        if (box->st != ctx->stream) HIPCHK(ctx, hipStreamWaitEvent(box->st, ev[3], 0));
+    }
+    box->tree_timed = true;
+    return LAGPM_OK;
+}
+
+static void tree_times(lagpm_box* box) {
+    hipEvent_t* ev = box->ev + 8;
     float t01 = 0, t12 = 0, t23 = 0;
+    (void)hipEventSynchronize(ev[3]);
     (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
     (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
     (void)hipEventElapsedTime(&t23, ev[2], ev[3]);
-    box->tree_timed = true;
-    box->stats.n_pp = c[0];
-    box->stats.n_pc = c[1];
     box->stats.ms_build = t01 + t12;
     box->stats.ms_walk = t23;
-    if (c[3]) return lagpm_fail(ctx, LAGPM_E_OVERFLOW, "tree walk stack overflow in " + std::to_string(c[3]) + " groups");
+}
+
+static int tree_check(lagpm_ctx* ctx, lagpm_box* box) {
+    if (!box->tab) return lagpm_fail(ctx, LAGPM_E_STATE, "box has no force tables");
+    if (box->level_cell < 2)
+        return lagpm_fail(ctx, LAGPM_E_INVAL, "box too small for the cutoff: need n >= 4 * rsphere");
     return LAGPM_OK;
+}
+
+int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef) {
+    RCHK(need_particles(ctx, box));
+    if (!ef) return lagpm_fail(ctx, LAGPM_E_INVAL, "evolfact is NULL");
+    RCHK(tree_check(ctx, box));
+    if (box->np == 0) return LAGPM_OK;
+    for (int attempt = 0;; attempt++) {
+        RCHK(tree_enqueue(ctx, box, (double)ef->fact2));
+        bool retry = false;
+        int rc = lagpm_tree_finish(ctx, box, &retry);
+        tree_times(box);
+        if (rc == LAGPM_OK) return LAGPM_OK;
+        // a build that overflowed its node arrays applied no kick: grow once and redo it
+        if (!retry || attempt > 0) return rc == LAGPM_E_OVERFLOW ? lagpm_fail(ctx, rc, "tree node arrays overflowed") : rc;
+    }
 }
 
 int lagpm_tree_stats_get(const lagpm_box* box, lagpm_tree_stats* out) {
@@ -689,7 +759,7 @@ int lagpm_evolfactor(const lagpm_params* p, double anow, double astep, lagpm_evo
 int lagpm_hubble_drag(lagpm_ctx* ctx, lagpm_box* box, double fact1) {
     RCHK(need_particles(ctx, box));
     if (box->np == 0) return LAGPM_OK;
-    hipLaunchKernelGGL(scale3_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, ctx->stream, box->vx, box->vy, box->vz,
+    hipLaunchKernelGGL(scale3_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, box->st, box->vx, box->vy, box->vz,
                        box->np, fact1);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
@@ -698,11 +768,165 @@ int lagpm_hubble_drag(lagpm_ctx* ctx, lagpm_box* box, double fact1) {
 int lagpm_drift(lagpm_ctx* ctx, lagpm_box* box, double pfact) {
     RCHK(need_particles(ctx, box));
     if (box->np == 0) return LAGPM_OK;
-    hipLaunchKernelGGL(drift_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, ctx->stream, box->x, box->y, box->z, box->vx,
-                       box->vy, box->vz, box->np, pfact, (double)box->p.n);
+    hipLaunchKernelGGL(drift_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx,
+                       box->vy, box->vz, box->np, pfact, (double)box->p.n, (const uint32_t*)nullptr);
     HIPCHK(ctx, hipGetLastError());
     box->sorted = false;
     return LAGPM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int ensure_pool(lagpm_ctx* ctx, size_t want) {
+    while (ctx->pool.size() < want) {
+        hipStream_t s = nullptr;
+        HIPCHK(ctx, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        ctx->pool.push_back(s);
+    }
+    return LAGPM_OK;
+}
+
+// Everything of one box's step up to the tree build, queued on box->st.
+int step_front(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact& ef) {
+    const size_t np = box->np;
+    hipLaunchKernelGGL(scale3_kernel, dim3(nblocks(np, 256)), dim3(256), 0, box->st, box->vx, box->vy, box->vz, np,
+                       (double)ef.fact1);
+    HIPCHK(ctx, hipGetLastError());
+    hipEvent_t* ev = box->ev;
+    HIPCHK(ctx, hipEventRecord(ev[0], box->st));
+    if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
+    HIPCHK(ctx, hipEventRecord(ev[1], box->st));
+    RCHK(lagpm_pm_deposit(ctx, box));
+    HIPCHK(ctx, hipEventRecord(ev[2], box->st));
+    RCHK(lagpm_pm_solve(ctx, box, ev + 3));
+    RCHK(lagpm_pm_gradient(ctx, box));
+    HIPCHK(ctx, hipEventRecord(ev[6], box->st));
+    RCHK(lagpm_pm_interp(ctx, box, (double)ef.fact2));
+    HIPCHK(ctx, hipEventRecord(ev[7], box->st));
+    box->pm_timed = true;
+    HIPCHK(ctx, hipEventRecord(box->ev[8], box->st));
+    HIPCHK(ctx, hipEventRecord(box->ev[9], box->st));  // already sorted: no tree sort
+    RCHK(lagpm_tree_build(ctx, box));
+    HIPCHK(ctx, hipEventRecord(box->ev[10], box->st));
+    return LAGPM_OK;
+}
+
+int drift_enqueue(lagpm_ctx* ctx, lagpm_box* box, double pfact, bool guarded) {
+    hipLaunchKernelGGL(drift_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx,
+                       box->vy, box->vz, box->np, pfact, (double)box->p.n,
+                       guarded ? (const uint32_t*)(box->dsz + DSZ_OVERFLOW) : (const uint32_t*)nullptr);
+    HIPCHK(ctx, hipGetLastError());
+    box->sorted = false;
+    return LAGPM_OK;
+}
+
+int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* anow, const double* astep) {
+    if (!ctx || (nbox > 0 && (!boxes || !anow || !astep)) || nbox < 0)
+        return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument or negative box count");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::vector<lagpm_evolfact> ef((size_t)nbox);
+    std::vector<lagpm_box*> live;
+    for (int i = 0; i < nbox; i++) {
+        lagpm_box* b = boxes[i];
+        if (!b) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL box");
+        if (!b->uploaded) return lagpm_fail(ctx, LAGPM_E_STATE, "no particles uploaded");
+        RCHK(tree_check(ctx, b));
+        for (int j = 0; j < i; j++)
+            if (boxes[j] == b) return lagpm_fail(ctx, LAGPM_E_INVAL, "a box appears twice in one batch");
+        if (lagpm_evolfactor(&b->p, anow[i], astep[i], &ef[(size_t)i]) != LAGPM_OK)
+            return lagpm_fail(ctx, LAGPM_E_INVAL, "bad scale factor or step for box " + std::to_string(i));
+        if (b->np) live.push_back(b);
+    }
+    if (live.empty()) return LAGPM_OK;
+    const size_t S = std::min<size_t>(live.size(), 8);
+    RCHK(ensure_pool(ctx, S));
+    // the pool streams start after everything queued on the context stream so far
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    for (size_t k = 0; k < S; k++) HIPCHK(ctx, hipStreamWaitEvent(ctx->pool[k], ctx->ev[0], 0));
+    std::vector<double> fact((size_t)nbox);
+    int rc = LAGPM_OK;
+    size_t k = 0;
+    for (int i = 0; i < nbox && rc == LAGPM_OK; i++) {
+        lagpm_box* b = boxes[i];
+        fact[(size_t)i] = ef[(size_t)i].fact2;
+        if (!b->np) continue;
+        b->st = ctx->pool[k++ % S];
+        rc = step_front(ctx, b, ef[(size_t)i]);
+    }
+    // FP32 walks: one persistent launch per force table on the context stream
+    std::vector<int> done((size_t)nbox, 0);
+    for (int i = 0; i < nbox && rc == LAGPM_OK; i++) {
+        lagpm_box* b = boxes[i];
+        if (!b->np || done[(size_t)i]) continue;
+        if (b->p.precision == LAGPM_PREC_FP64) {
+            rc = lagpm_tree_walk_f64(ctx, b, fact[(size_t)i]);
+            if (rc == LAGPM_OK && hipEventRecord(b->ev[11], b->st) != hipSuccess) rc = LAGPM_E_HIP;
+            done[(size_t)i] = 1;
+            continue;
+        }
+        std::vector<lagpm_box*> cls;
+        std::vector<double> cf;
+        for (int j = i; j < nbox; j++) {
+            lagpm_box* c = boxes[j];
+            if (!c->np || done[(size_t)j] || c->p.precision != LAGPM_PREC_FP32 || c->tab != b->tab) continue;
+            if (hipStreamWaitEvent(ctx->stream, c->ev[10], 0) != hipSuccess) rc = LAGPM_E_HIP;
+            cls.push_back(c);
+            cf.push_back(fact[(size_t)j]);
+            done[(size_t)j] = 1;
+        }
+        if (rc == LAGPM_OK) rc = lagpm_tree_walk_f32(ctx, cls.data(), cf.data(), (int)cls.size());
+        if (rc == LAGPM_OK && hipEventRecord(ctx->ev[1], ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
+        for (lagpm_box* c : cls) {
+            if (rc != LAGPM_OK) break;
+            if (hipEventRecord(c->ev[11], ctx->stream) != hipSuccess || hipStreamWaitEvent(c->st, ctx->ev[1], 0) != hipSuccess)
+                rc = LAGPM_E_HIP;
+        }
+    }
+    for (int i = 0; i < nbox && rc == LAGPM_OK; i++) {
+        lagpm_box* b = boxes[i];
+        if (!b->np) continue;
+        b->tree_timed = true;
+        rc = drift_enqueue(ctx, b, (double)ef[(size_t)i].pfact, true);
+    }
+    // completion: sizes and counters of every box; a box whose node arrays overflowed got
+    // neither tree kick nor drift, so it is rebuilt with larger arrays and finished here
+    int first_err = rc;
+    for (int i = 0; i < nbox; i++) {
+        lagpm_box* b = boxes[i];
+        if (!b->np) continue;
+        if (first_err == LAGPM_OK) {
+            bool retry = false;
+            int r = lagpm_tree_finish(ctx, b, &retry);
+            if (retry) {
+                b->st = ctx->stream;
+                HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+                r = tree_enqueue(ctx, b, fact[(size_t)i]);
+                if (r == LAGPM_OK) r = lagpm_tree_finish(ctx, b, &retry);
+                if (r == LAGPM_OK) r = drift_enqueue(ctx, b, (double)ef[(size_t)i].pfact, false);
+                if (r != LAGPM_OK && retry) r = lagpm_fail(ctx, LAGPM_E_OVERFLOW, "tree node arrays overflowed");
+            }
+            if (r == LAGPM_OK) tree_times(b);
+            if (r != LAGPM_OK) first_err = r;
+        }
+        b->st = ctx->stream;
+    }
+    for (size_t q = 0; q < S; q++) (void)hipStreamSynchronize(ctx->pool[q]);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return first_err;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lagpm_batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* anow, const double* astep) {
+    return batch_step(ctx, boxes, nbox, anow, astep);
+}
+
+int lagpm_kdk_step(lagpm_ctx* ctx, lagpm_box* box, double anow, double astep) {
+    return batch_step(ctx, &box, 1, &anow, &astep);
 }
 
 }  // extern "C"

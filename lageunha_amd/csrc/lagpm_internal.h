@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <map>
 #include <string>
+#include <tuple>
+#include <vector>
 
 #include "../../include/lagpm.h"
 
@@ -13,23 +15,42 @@
 #define LAGPM_KEY_BITS 16    // Morton bits per axis; octree depth <= 16
 #define LAGPM_MAX_INDEX (1u << 27)  // walk lists pack (index << 5 | neighbour)
 
-struct fft_plans {
-    rocfft_plan fwd = nullptr, inv = nullptr;
+struct fft_exec {
     rocfft_execution_info info = nullptr;
     void* work = nullptr;
-    size_t work_bytes = 0;
 };
+struct fft_plans {
+    rocfft_plan fwd = nullptr, inv = nullptr;
+    size_t work_bytes = 0;
+    std::map<hipStream_t, fft_exec> exec;  // one execution info + work buffer per stream
+};
+
+// Force-law tables normalised to n = 1 (the law scales as 1/n^2), shared by every box of a
+// context that has the same law, softening and split scale.
+struct law_tables {
+    float2* tab0 = nullptr;   // (d0, s0) x 4096: particle term
+    float4* tab12 = nullptr;  // (d1, s1, d2, s2) x 4096: quadrupole terms
+    double* tabd = nullptr;   // 6 x 4096: d0 s0 d1 s1 d2 s2 (FP64 walk)
+};
+
+struct walk_box;  // device view of one box for the walk (tree.hip)
 
 struct lagpm_ctx {
     int device = 0;
     int num_cu = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;            // default stream of the context
+    std::vector<hipStream_t> pool;           // per-box streams of batched steps
     std::string err;
     std::map<std::pair<int, int>, fft_plans> plans;  // (n, precision) -> plans
-    hipEvent_t ev[8] = {};
-    // device scratch for scans (grown on demand)
-    void* scan_tmp = nullptr;
-    size_t scan_tmp_bytes = 0;
+    std::map<std::tuple<int, double, double>, law_tables> tables;  // (law, epsilon, efold)
+    // walk resources (every walk runs on `stream`, so one set serves all boxes)
+    int* walk_stack = nullptr;
+    size_t walk_stack_entries = 0;
+    unsigned long long* walk_ticket = nullptr;
+    walk_box* walk_boxes = nullptr;          // descriptor array of the last launch
+    uint32_t* walk_prefix = nullptr;         // group prefix over the descriptors
+    size_t walk_box_cap = 0;
+    hipEvent_t ev[4] = {};
 };
 
 // Tunables of the tree correction (chosen per box; the autotuner may override).
@@ -39,6 +60,10 @@ struct tree_tune {
     int walk_waves = 0;    // persistent walk waves (0: auto)
 };
 
+// Device-side sizes of a box's tree and deposit, written by the kernels that produce them so
+// that a step never waits for the host. Indices into lagpm_box::dsz.
+enum { DSZ_NODES = 0, DSZ_GROUPS = 1, DSZ_CHUNKS = 2, DSZ_OVERFLOW = 3, DSZ_COUNT = 8 };
+
 struct lagpm_box {
     lagpm_params p{};
     size_t np = 0;
@@ -46,6 +71,7 @@ struct lagpm_box {
     bool uploaded = false;
     bool sorted = false;
     tree_tune tune;
+    hipStream_t st = nullptr;  // stream this box's work is queued on (context stream or a pool stream)
 
     // particle state, kept in the current (sorted after a sort) order
     double *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
@@ -55,6 +81,8 @@ struct lagpm_box {
     uint32_t *perm = nullptr, *perm2 = nullptr;
     double* dtmp = nullptr;
     uint32_t* utmp = nullptr;
+    uint32_t* scan_tmp = nullptr;  // scan / radix-sort scratch of this box
+    size_t scan_tmp_bytes = 0;
 
     // PM meshes
     int scale_bits = 32;
@@ -72,8 +100,7 @@ struct lagpm_box {
     float4* pp = nullptr;     // sorted FP32 positions + mass
     uint32_t* ncount = nullptr;   // per-particle node count, then offsets
     uint32_t* noff = nullptr;
-    size_t node_cap = 0;
-    size_t nnodes = 0;
+    size_t node_cap = 0;          // capacity of the node arrays (device count may not exceed it)
     int4* nd_info = nullptr;      // begin, count, nchild, level
     int* nd_parent = nullptr;
     int* nd_child = nullptr;      // 8 per node
@@ -84,18 +111,13 @@ struct lagpm_box {
     int* cellroot = nullptr;      // mcell^3
     uint32_t* gflag = nullptr;    // per-node group count, then offsets
     uint32_t* goff = nullptr;
-    size_t ngroups = 0;
     size_t group_cap = 0;
     int2* groups = nullptr;       // begin, count
     double* gbox = nullptr;       // FP64 group bbox: 6 per group
     int* pgroup = nullptr;        // group of each particle (FP64 plain walk)
-    int* walk_stack = nullptr;
-    size_t walk_stack_entries = 0;
-    unsigned long long* counters = nullptr;  // [0]=n_pp [1]=n_pc [2]=group ticket [3]=overflow
-    float2* tab0 = nullptr;       // (d0, s0) x 4096: particle law
-    float4* tab12 = nullptr;      // (d1, s1, d2, s2) x 4096: quadrupole terms
-    double* tabd = nullptr;       // 6 x 4096: d0 s0 d1 s1 d2 s2
-    bool tables_ready = false;
+    uint32_t* dsz = nullptr;      // DSZ_* device sizes
+    unsigned long long* counters = nullptr;  // [0]=n_pp [1]=n_pc [2]=walk stack overflow
+    const law_tables* tab = nullptr;
     lagpm_tree_stats stats{};
     // phase events: [0..7] pmmain (start, sort, tsc, r2c, green, c2r, grad, interp),
     //               [8..11] treecorrection (start, sort, build, walk)
@@ -118,25 +140,42 @@ int lagpm_fail(lagpm_ctx* ctx, int code, const std::string& msg);
     } while (0)
 
 // ---- scans / sort (sort.hip) ----
-int lagpm_exclusive_scan_u32(lagpm_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, uint32_t* total_host);
-int lagpm_radix_sort_pairs(lagpm_ctx* ctx, uint64_t* key, uint64_t* key_alt, uint32_t* val, uint32_t* val_alt,
-                           size_t n, int bits, bool* result_in_alt);
+// Scratch bytes a box needs for scans of n entries and for a radix sort of n keys.
+size_t lagpm_scan_tmp_bytes(size_t n);
+size_t lagpm_sort_tmp_bytes(size_t n);
+// Exclusive scan of n u32 values on stream s (in and out must not alias); the total is written
+// to total_dev (device memory) when not null. tmp: lagpm_scan_tmp_bytes(n) bytes.
+int lagpm_exclusive_scan_u32(lagpm_ctx* ctx, hipStream_t s, uint32_t* tmp, const uint32_t* in, uint32_t* out, size_t n,
+                             uint32_t* total_dev);
+int lagpm_radix_sort_pairs(lagpm_ctx* ctx, hipStream_t s, uint32_t* tmp, uint64_t* key, uint64_t* key_alt, uint32_t* val,
+                           uint32_t* val_alt, size_t n, int bits, bool* result_in_alt);
 
 // ---- PM (pm.hip) ----
 int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box);          // fills acc and rho
 int lagpm_pm_solve(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev = nullptr);  // rho -> phi (r2c, Green, c2r)
 int lagpm_pm_gradient(lagpm_ctx* ctx, lagpm_box* box);         // phi -> grad
 int lagpm_pm_interp(lagpm_ctx* ctx, lagpm_box* box, double fact);  // v += fact * F(grad)
-int lagpm_green_device(lagpm_ctx* ctx, int n, int prec, double efold, int deconvolve, void* spec);
+int lagpm_green_device(lagpm_ctx* ctx, hipStream_t s, int n, int prec, double efold, int deconvolve, void* spec);
+size_t lagpm_deposit_chunk_cap(int n, size_t np);
 int lagpm_fft_release(lagpm_ctx* ctx);
 
 // ---- tree (tree.hip) ----
 int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box);
+// Queues the octree build on box->st; sizes stay on the device (dsz).
 int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box);
-int lagpm_tree_walk(lagpm_ctx* ctx, lagpm_box* box, double fact);
-int lagpm_tables_upload(lagpm_ctx* ctx, lagpm_box* box);
+// One walk launch on ctx->stream over all groups of the given boxes (FP32 boxes sharing one
+// law table); facts[i] is the kick factor of box i. Waits are the caller's business.
+int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* facts, int nbox);
+int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact);
+// Reads the device sizes and counters after the walk (synchronises box->st); grows the node
+// arrays and returns LAGPM_E_OVERFLOW with *retry set when they were too small.
+int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry);
+int lagpm_tree_alloc(lagpm_ctx* ctx, lagpm_box* box, size_t node_cap);
+size_t lagpm_tree_node_cap_guess(size_t np, int mcell);
 int lagpm_choose_cell_level(int n, double rsphere);
+size_t lagpm_walk_stack_entries(int num_cu);
 
-// ---- host helpers (api.cpp) ----
+// ---- host helpers (api.hip) ----
 void lagpm_build_tables(const lagpm_params* p, double* diff, double* slope);
-size_t lagpm_tree_node_cap(size_t np, int nleaf);
+int lagpm_get_tables(lagpm_ctx* ctx, const lagpm_params* p, const law_tables** out);
+int lagpm_dalloc_bytes(lagpm_ctx* ctx, void** p, size_t bytes);

@@ -84,19 +84,21 @@ __global__ void dep_flag_kernel(const uint64_t* __restrict__ K, size_t N, int sh
     flag[i] = (i == 0 || (i % DEP_CHUNK) == 0 || (K[i] >> sh) != (K[i - 1] >> sh)) ? 1u : 0u;
 }
 __global__ void dep_start_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off, size_t N,
-                                 uint32_t* __restrict__ start, uint32_t nchunk) {
+                                 uint32_t* __restrict__ start, const uint32_t* __restrict__ nchunk) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N && flag[i]) start[off[i]] = (uint32_t)i;
-    if (i == 0) start[nchunk] = (uint32_t)N;
+    if (i == 0) start[*nchunk] = (uint32_t)N;
 }
 
 template <bool F64>
 __global__ void __launch_bounds__(DEP_THREADS) deposit_kernel(const double* __restrict__ px, const double* __restrict__ py,
                                                               const double* __restrict__ pz, const uint32_t* __restrict__ start,
-                                                              int n, float mass, double scale,
-                                                              unsigned long long* __restrict__ acc) {
+                                                              const uint32_t* __restrict__ nchunk, int n, float mass,
+                                                              double scale, unsigned long long* __restrict__ acc) {
     __shared__ unsigned long long box[DEP_LDS];
     __shared__ int red[6][DEP_THREADS / 64];
+    // the grid is sized for the largest possible chunk count; the actual count is on the device
+    if (blockIdx.x >= *nchunk) return;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const size_t base = start[blockIdx.x];
     const size_t np = start[blockIdx.x + 1];
@@ -390,14 +392,28 @@ int get_plans(lagpm_ctx* ctx, int n, int prec, fft_plans** out) {
     RCHK(fft_status(ctx, rocfft_plan_get_work_buffer_size(fp.fwd, &w1), "work_size"));
     RCHK(fft_status(ctx, rocfft_plan_get_work_buffer_size(fp.inv, &w2), "work_size"));
     fp.work_bytes = w1 > w2 ? w1 : w2;
-    RCHK(fft_status(ctx, rocfft_execution_info_create(&fp.info), "execution_info_create"));
-    if (fp.work_bytes) {
-        HIPCHK(ctx, hipMalloc(&fp.work, fp.work_bytes));
-        RCHK(fft_status(ctx, rocfft_execution_info_set_work_buffer(fp.info, fp.work, fp.work_bytes), "set_work_buffer"));
-    }
-    RCHK(fft_status(ctx, rocfft_execution_info_set_stream(fp.info, ctx->stream), "set_stream"));
     auto res = ctx->plans.emplace(key, fp);
     *out = &res.first->second;
+    return LAGPM_OK;
+}
+
+// Execution info (stream + work buffer) of a plan pair for one stream: boxes transformed
+// concurrently on different streams never share a work buffer.
+int get_exec(lagpm_ctx* ctx, fft_plans* fp, hipStream_t st, rocfft_execution_info* out) {
+    auto it = fp->exec.find(st);
+    if (it != fp->exec.end()) {
+        *out = it->second.info;
+        return LAGPM_OK;
+    }
+    fft_exec ex;
+    RCHK(fft_status(ctx, rocfft_execution_info_create(&ex.info), "execution_info_create"));
+    if (fp->work_bytes) {
+        HIPCHK(ctx, hipMalloc(&ex.work, fp->work_bytes));
+        RCHK(fft_status(ctx, rocfft_execution_info_set_work_buffer(ex.info, ex.work, fp->work_bytes), "set_work_buffer"));
+    }
+    RCHK(fft_status(ctx, rocfft_execution_info_set_stream(ex.info, st), "set_stream"));
+    fp->exec.emplace(st, ex);
+    *out = ex.info;
     return LAGPM_OK;
 }
 
@@ -408,70 +424,80 @@ int lagpm_fft_release(lagpm_ctx* ctx) {
         fft_plans& fp = kv.second;
         if (fp.fwd) rocfft_plan_destroy(fp.fwd);
         if (fp.inv) rocfft_plan_destroy(fp.inv);
-        if (fp.info) rocfft_execution_info_destroy(fp.info);
-        if (fp.work) (void)hipFree(fp.work);
+        for (auto& ex : fp.exec) {
+            if (ex.second.info) rocfft_execution_info_destroy(ex.second.info);
+            if (ex.second.work) (void)hipFree(ex.second.work);
+        }
     }
     ctx->plans.clear();
     return LAGPM_OK;
+}
+
+// Morton level whose blocks are at most 13 cells wide (the LDS box of a deposit chunk).
+static int deposit_block_level(int n) {
+    int lvl = 0;
+    while (lvl < LAGPM_KEY_BITS && (double)n / (double)(1 << lvl) > 13.0) lvl++;
+    return lvl;
+}
+
+size_t lagpm_deposit_chunk_cap(int n, size_t np) {
+    const int lvl = deposit_block_level(n);
+    const size_t blocks = lvl >= 10 ? np : std::min(np, (size_t)1 << (3 * lvl));
+    return np / DEP_CHUNK + blocks + 2;
 }
 
 int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
     const int n = box->p.n;
     const size_t m = (size_t)n * n * n;
     const bool f64 = box->p.precision == LAGPM_PREC_FP64;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, box->acc, m);
+    hipStream_t st = box->st;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(m, 256)), dim3(256), 0, st, box->acc, m);
     HIPCHK(ctx, hipGetLastError());
     const double scale = std::ldexp(1.0, box->scale_bits);
     if (box->np) {
-        // chunk list from the sorted keys (the caller sorted the particles)
-        int lvl = 0;
-        while (lvl < LAGPM_KEY_BITS && (double)n / (double)(1 << lvl) > 13.0) lvl++;
-        const int sh = 3 * (LAGPM_KEY_BITS - lvl);
+        // chunk list from the sorted keys (the caller sorted the particles); the chunk count
+        // stays on the device and the deposit grid covers its upper bound
+        const int sh = 3 * (LAGPM_KEY_BITS - deposit_block_level(n));
         const size_t np = box->np;
-        hipLaunchKernelGGL(dep_flag_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, ctx->stream, box->key, np, sh,
+        hipLaunchKernelGGL(dep_flag_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, st, box->key, np, sh,
                            box->ncount);
         HIPCHK(ctx, hipGetLastError());
-        uint32_t nchunk = 0;
-        RCHK(lagpm_exclusive_scan_u32(ctx, box->ncount, box->noff, np, &nchunk));
-        if ((size_t)nchunk + 1 > box->dep_cap) {
-            if (box->dep_start) (void)hipFree(box->dep_start);
-            box->dep_start = nullptr;
-            box->dep_cap = (size_t)nchunk + nchunk / 4 + 1024;
-            HIPCHK(ctx, hipMalloc(&box->dep_start, box->dep_cap * sizeof(uint32_t)));
-        }
-        hipLaunchKernelGGL(dep_start_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, ctx->stream, box->ncount,
+        uint32_t* nchunk = box->dsz + DSZ_CHUNKS;
+        RCHK(lagpm_exclusive_scan_u32(ctx, st, box->scan_tmp, box->ncount, box->noff, np, nchunk));
+        hipLaunchKernelGGL(dep_start_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, st, box->ncount,
                            box->noff, np, box->dep_start, nchunk);
         HIPCHK(ctx, hipGetLastError());
+        const unsigned grid = (unsigned)(box->dep_cap - 1);
         if (f64)
-            hipLaunchKernelGGL(deposit_kernel<true>, dim3(nchunk), dim3(DEP_THREADS), 0, ctx->stream, box->x, box->y, box->z,
-                               box->dep_start, n, (float)box->p.mass, scale, box->acc);
+            hipLaunchKernelGGL(deposit_kernel<true>, dim3(grid), dim3(DEP_THREADS), 0, st, box->x, box->y, box->z,
+                               box->dep_start, nchunk, n, (float)box->p.mass, scale, box->acc);
         else
-            hipLaunchKernelGGL(deposit_kernel<false>, dim3(nchunk), dim3(DEP_THREADS), 0, ctx->stream, box->x, box->y, box->z,
-                               box->dep_start, n, (float)box->p.mass, scale, box->acc);
+            hipLaunchKernelGGL(deposit_kernel<false>, dim3(grid), dim3(DEP_THREADS), 0, st, box->x, box->y, box->z,
+                               box->dep_start, nchunk, n, (float)box->p.mass, scale, box->acc);
         HIPCHK(ctx, hipGetLastError());
     }
     const double inv_scale = std::ldexp(1.0, -box->scale_bits);
     const double mean = 1.0;  // the FFT grid starts at -1 before the fold (Cosmos/pmmain.c:304)
     if (f64)
-        hipLaunchKernelGGL(convert_kernel<double>, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, box->acc, m, inv_scale,
+        hipLaunchKernelGGL(convert_kernel<double>, dim3(grid_for(m, 256)), dim3(256), 0, st, box->acc, m, inv_scale,
                            mean, (double*)box->rho);
     else
-        hipLaunchKernelGGL(convert_kernel<float>, dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream, box->acc, m, inv_scale,
+        hipLaunchKernelGGL(convert_kernel<float>, dim3(grid_for(m, 256)), dim3(256), 0, st, box->acc, m, inv_scale,
                            mean, (float*)box->rho);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }
 
-int lagpm_green_device(lagpm_ctx* ctx, int n, int prec, double efold, int deconvolve, void* spec) {
+int lagpm_green_device(lagpm_ctx* ctx, hipStream_t st, int n, int prec, double efold, int deconvolve, void* spec) {
     const double rn = n;
     const double sfact = -std::pow(M_PI * efold / rn, 2);
     const double irngc = 1.0 / (rn * rn * rn);
     const size_t K = (size_t)(n / 2 + 1) * n * n;
     if (prec == LAGPM_PREC_FP64)
-        hipLaunchKernelGGL(green_kernel<double>, dim3(grid_for(K, 256)), dim3(256), 0, ctx->stream, (double*)spec, n, sfact,
+        hipLaunchKernelGGL(green_kernel<double>, dim3(grid_for(K, 256)), dim3(256), 0, st, (double*)spec, n, sfact,
                            irngc, deconvolve);
     else
-        hipLaunchKernelGGL(green_kernel<float>, dim3(grid_for(K, 256)), dim3(256), 0, ctx->stream, (float*)spec, n, sfact,
+        hipLaunchKernelGGL(green_kernel<float>, dim3(grid_for(K, 256)), dim3(256), 0, st, (float*)spec, n, sfact,
                            irngc, deconvolve);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
@@ -480,16 +506,18 @@ int lagpm_green_device(lagpm_ctx* ctx, int n, int prec, double efold, int deconv
 int lagpm_pm_solve(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     fft_plans* fp = nullptr;
     RCHK(get_plans(ctx, box->p.n, box->p.precision, &fp));
+    rocfft_execution_info info = nullptr;
+    RCHK(get_exec(ctx, fp, box->st, &info));
     void* in[1] = {box->rho};
     void* out[1] = {box->spec};
-    RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, out, fp->info), "execute(r2c)"));
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
-    RCHK(lagpm_green_device(ctx, box->p.n, box->p.precision, box->p.efold, box->p.deconvolve, box->spec));
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[1], ctx->stream));
+    RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, out, info), "execute(r2c)"));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[0], box->st));
+    RCHK(lagpm_green_device(ctx, box->st, box->p.n, box->p.precision, box->p.efold, box->p.deconvolve, box->spec));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[1], box->st));
     void* in2[1] = {box->spec};
     void* out2[1] = {box->phi};
-    RCHK(fft_status(ctx, rocfft_execute(fp->inv, in2, out2, fp->info), "execute(c2r)"));
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[2], ctx->stream));
+    RCHK(fft_status(ctx, rocfft_execute(fp->inv, in2, out2, info), "execute(c2r)"));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[2], box->st));
     return LAGPM_OK;
 }
 
@@ -497,10 +525,10 @@ int lagpm_pm_gradient(lagpm_ctx* ctx, lagpm_box* box) {
     const int n = box->p.n;
     const size_t m = (size_t)n * n * n;
     if (box->p.precision == LAGPM_PREC_FP64)
-        hipLaunchKernelGGL((gradient_kernel<double, double4>), dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL((gradient_kernel<double, double4>), dim3(grid_for(m, 256)), dim3(256), 0, box->st,
                            (const double*)box->phi, n, (double4*)box->grad);
     else
-        hipLaunchKernelGGL((gradient_kernel<float, float4>), dim3(grid_for(m, 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL((gradient_kernel<float, float4>), dim3(grid_for(m, 256)), dim3(256), 0, box->st,
                            (const float*)box->phi, n, (float4*)box->grad);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
@@ -510,10 +538,10 @@ int lagpm_pm_interp(lagpm_ctx* ctx, lagpm_box* box, double fact) {
     if (box->np == 0) return LAGPM_OK;
     const unsigned nblk = (unsigned)((box->np + 255) / 256);
     if (box->p.precision == LAGPM_PREC_FP64)
-        hipLaunchKernelGGL(interp_kick_f64, dim3(nblk), dim3(256), 0, ctx->stream, box->x, box->y, box->z, box->vx, box->vy,
+        hipLaunchKernelGGL(interp_kick_f64, dim3(nblk), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx, box->vy,
                            box->vz, box->np, box->p.n, (const double4*)box->grad, fact);
     else
-        hipLaunchKernelGGL(interp_kick_f32, dim3(nblk), dim3(256), 0, ctx->stream, box->x, box->y, box->z, box->vx, box->vy,
+        hipLaunchKernelGGL(interp_kick_f32, dim3(nblk), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx, box->vy,
                            box->vz, box->np, box->p.n, (const float4*)box->grad, (float)fact);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;

@@ -4,6 +4,8 @@
 // an exclusive scan of the digit-major histogram, and a scatter kernel that ranks keys
 // stably inside each wavefront with ballot-based digit matching (64-lane masks), then
 // orders wavefronts of a tile by their position. Deterministic and stable.
+#include <algorithm>
+
 #include "lagpm_internal.h"
 
 namespace {
@@ -148,28 +150,18 @@ __global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* 
     }
 }
 
-int ensure_scan_tmp(lagpm_ctx* ctx, size_t bytes) {
-    if (ctx->scan_tmp_bytes >= bytes) return LAGPM_OK;
-    if (ctx->scan_tmp) (void)hipFree(ctx->scan_tmp);
-    ctx->scan_tmp = nullptr;
-    ctx->scan_tmp_bytes = 0;
-    HIPCHK(ctx, hipMalloc(&ctx->scan_tmp, bytes));
-    ctx->scan_tmp_bytes = bytes;
-    return LAGPM_OK;
-}
-
 // recursive exclusive scan using tmp (enough room for all levels)
-int scan_rec(lagpm_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp) {
+int scan_rec(lagpm_ctx* ctx, hipStream_t st, const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp) {
     size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb == 0) return LAGPM_OK;
     uint32_t* sums = tmp;
     uint32_t* sums_scan = tmp + nb;
-    hipLaunchKernelGGL(scan_tile_kernel, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n,
+    hipLaunchKernelGGL(scan_tile_kernel, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, in, out, n,
                        nb > 1 ? sums : (uint32_t*)nullptr);
     HIPCHK(ctx, hipGetLastError());
     if (nb > 1) {
-        RCHK(scan_rec(ctx, sums, sums_scan, nb, tmp + 2 * nb));
-        hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, out, n,
+        RCHK(scan_rec(ctx, st, sums, sums_scan, nb, tmp + 2 * nb));
+        hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n,
                            sums_scan);
         HIPCHK(ctx, hipGetLastError());
     }
@@ -187,37 +179,33 @@ size_t scan_tmp_elems(size_t n) {
 
 }  // namespace
 
-// Exclusive scan of n u32 values (in and out must not alias). total_host (optional)
-// receives the sum; that copy synchronises the stream.
-int lagpm_exclusive_scan_u32(lagpm_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, uint32_t* total_host) {
-    if (n == 0) {
-        if (total_host) *total_host = 0;
-        return LAGPM_OK;
-    }
-    size_t elems = scan_tmp_elems(n) + 4;
-    RCHK(ensure_scan_tmp(ctx, elems * sizeof(uint32_t)));
-    uint32_t* tmp = (uint32_t*)ctx->scan_tmp;
-    RCHK(scan_rec(ctx, in, out, n, tmp + 4));
-    if (total_host) {
-        hipLaunchKernelGGL(scan_total_kernel, dim3(1), dim3(64), 0, ctx->stream, in, out, n, tmp);
+size_t lagpm_scan_tmp_bytes(size_t n) { return (scan_tmp_elems(n) + 4) * sizeof(uint32_t); }
+
+size_t lagpm_sort_tmp_bytes(size_t n) {
+    size_t nblocks = (n + RS_TILE - 1) / RS_TILE;
+    size_t hist_elems = (size_t)256 * std::max<size_t>(nblocks, 1);
+    return (scan_tmp_elems(hist_elems) + 4 + 2 * hist_elems) * sizeof(uint32_t);
+}
+
+int lagpm_exclusive_scan_u32(lagpm_ctx* ctx, hipStream_t st, uint32_t* tmp, const uint32_t* in, uint32_t* out, size_t n,
+                             uint32_t* total_dev) {
+    if (n > 0) RCHK(scan_rec(ctx, st, in, out, n, tmp + 4));
+    if (total_dev) {
+        hipLaunchKernelGGL(scan_total_kernel, dim3(1), dim3(64), 0, st, in, out, n, total_dev);
         HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipMemcpyAsync(total_host, tmp, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     }
     return LAGPM_OK;
 }
 
 // Stable sort of (key, val) pairs on the low `bits` bits of key (rounded up to 8).
-int lagpm_radix_sort_pairs(lagpm_ctx* ctx, uint64_t* key, uint64_t* key_alt, uint32_t* val, uint32_t* val_alt,
-                           size_t n, int bits, bool* result_in_alt) {
+int lagpm_radix_sort_pairs(lagpm_ctx* ctx, hipStream_t st, uint32_t* base, uint64_t* key, uint64_t* key_alt, uint32_t* val,
+                           uint32_t* val_alt, size_t n, int bits, bool* result_in_alt) {
     *result_in_alt = false;
     if (n <= 1) return LAGPM_OK;
     uint32_t nblocks = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
     size_t hist_elems = (size_t)256 * nblocks;
     // histogram + its scan live after the scan temp area
     size_t scan_elems = scan_tmp_elems(hist_elems) + 4;
-    RCHK(ensure_scan_tmp(ctx, (scan_elems + 2 * hist_elems) * sizeof(uint32_t)));
-    uint32_t* base = (uint32_t*)ctx->scan_tmp;
     uint32_t* ghist = base + scan_elems;
     uint32_t* goff = ghist + hist_elems;
     uint64_t *ki = key, *ko = key_alt;
@@ -225,10 +213,10 @@ int lagpm_radix_sort_pairs(lagpm_ctx* ctx, uint64_t* key, uint64_t* key_alt, uin
     int passes = (bits + 7) / 8;
     for (int p = 0; p < passes; p++) {
         int shift = 8 * p;
-        hipLaunchKernelGGL(rs_hist_kernel, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, ki, n, shift, ghist, nblocks);
+        hipLaunchKernelGGL(rs_hist_kernel, dim3(nblocks), dim3(RS_THREADS), 0, st, ki, n, shift, ghist, nblocks);
         HIPCHK(ctx, hipGetLastError());
-        RCHK(scan_rec(ctx, ghist, goff, hist_elems, base + 4));
-        hipLaunchKernelGGL(rs_scatter_kernel, dim3(nblocks), dim3(RS_THREADS), 0, ctx->stream, ki, vi, n, shift, goff,
+        RCHK(scan_rec(ctx, st, ghist, goff, hist_elems, base + 4));
+        hipLaunchKernelGGL(rs_scatter_kernel, dim3(nblocks), dim3(RS_THREADS), 0, st, ki, vi, n, shift, goff,
                            nblocks, ko, vo);
         HIPCHK(ctx, hipGetLastError());
         std::swap(ki, ko);

@@ -145,14 +145,20 @@ __global__ void node_count_kernel(const uint64_t* __restrict__ K, size_t N, int 
 }
 
 __global__ void node_write_kernel(const uint64_t* __restrict__ K, size_t N, int L, int nleaf, const uint32_t* __restrict__ off,
-                                  int4* __restrict__ info, int* __restrict__ cellroot, int mcell) {
+                                  int4* __restrict__ info, int* __restrict__ cellroot, int mcell, size_t cap,
+                                  uint32_t* __restrict__ dsz) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     uint32_t ends[LAGPM_KEY_BITS + 1];
     int l0;
     int c = node_levels(K, N, i, L, nleaf, &l0, ends);
     uint32_t v0 = off[i];
+    // Every node index below min(total, cap) is written (each belongs to one (particle, t)),
+    // so the later kernels see a consistent partial tree when the arrays are too small; the
+    // flag makes the walk skip the box and the host grow the arrays and rebuild.
+    if ((size_t)v0 + c > cap) dsz[DSZ_OVERFLOW] = 1u;
     for (int t = 0; t < c; t++) {
+        if ((size_t)v0 + t >= cap) break;
         int l = l0 + t;
         info[v0 + t] = make_int4((int)i, (int)(ends[t] - i), 0, l);
         if (l == L) {
@@ -164,9 +170,17 @@ __global__ void node_write_kernel(const uint64_t* __restrict__ K, size_t N, int 
 }
 
 // Child lists: children of internal node (l, b) are the level-(l+1) runs inside [b, b+count).
-__global__ void children_kernel(const uint64_t* __restrict__ K, size_t nnodes, int L, int nleaf, const uint32_t* __restrict__ off,
-                                int4* __restrict__ info, int* __restrict__ child, int* __restrict__ parent) {
+// node count on the device, clamped to the array capacity
+__device__ inline size_t dev_count(const uint32_t* __restrict__ dsz, int which, size_t cap) {
+    size_t c = dsz[which];
+    return c < cap ? c : cap;
+}
+
+__global__ void children_kernel(const uint64_t* __restrict__ K, const uint32_t* __restrict__ dsz, size_t cap, int L, int nleaf,
+                                const uint32_t* __restrict__ off, int4* __restrict__ info, int* __restrict__ child,
+                                int* __restrict__ parent) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nnodes = dev_count(dsz, DSZ_NODES, cap);
     if (v >= nnodes) return;
     int4 nd = info[v];
     if (nd.w == L) parent[v] = -1;
@@ -199,11 +213,11 @@ __device__ inline void kahan(float& s, float& c, float v) {
 }
 
 // FP32 moments with Kahan-compensated sums, one level per launch (deepest first).
-__global__ void moments_f32_kernel(const float4* __restrict__ pp, size_t nnodes, int level, const int4* __restrict__ info,
-                                   const int* __restrict__ child, float4* __restrict__ com, float4* __restrict__ q0,
-                                   float4* __restrict__ q1) {
+__global__ void moments_f32_kernel(const float4* __restrict__ pp, const uint32_t* __restrict__ dsz, size_t cap, int level,
+                                   const int4* __restrict__ info, const int* __restrict__ child, float4* __restrict__ com,
+                                   float4* __restrict__ q0, float4* __restrict__ q1) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nnodes) return;
+    if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
     int4 nd = info[v];
     if (nd.w != level) return;
     float M = 0, cM = 0, Sx = 0, cx = 0, Sy = 0, cy = 0, Sz = 0, cz = 0;
@@ -270,10 +284,10 @@ __global__ void moments_f32_kernel(const float4* __restrict__ pp, size_t nnodes,
 
 // FP64 moments for the plain FP64 path (12 doubles per node).
 __global__ void moments_f64_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-                                   double mass, size_t nnodes, int level, const int4* __restrict__ info,
-                                   const int* __restrict__ child, double* __restrict__ nd_d) {
+                                   double mass, const uint32_t* __restrict__ dsz, size_t cap, int level,
+                                   const int4* __restrict__ info, const int* __restrict__ child, double* __restrict__ nd_d) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nnodes) return;
+    if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
     int4 nd = info[v];
     if (nd.w != level) return;
     double M = 0, Sx = 0, Sy = 0, Sz = 0, Q[6] = {0, 0, 0, 0, 0, 0}, rmax = 0;
@@ -318,10 +332,14 @@ __global__ void moments_f64_kernel(const double* __restrict__ x, const double* _
 
 // Groups: the first node of size <= G on each root-to-leaf path; max-depth leaves larger
 // than G (coincident keys) are cut into chunks of G.
-__global__ void group_count_kernel(size_t nnodes, int L, int G, const int4* __restrict__ info, const int* __restrict__ parent,
-                                   uint32_t* __restrict__ gcnt) {
+__global__ void group_count_kernel(const uint32_t* __restrict__ dsz, size_t cap, int L, int G, const int4* __restrict__ info,
+                                   const int* __restrict__ parent, uint32_t* __restrict__ gcnt) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nnodes) return;
+    if (v >= cap) return;
+    if (v >= dev_count(dsz, DSZ_NODES, cap)) {  // entries past the node count scan as zero
+        gcnt[v] = 0u;
+        return;
+    }
     int4 nd = info[v];
     int pv = parent[v];
     bool top = (nd.w == L) || pv < 0 || (info[pv].y > G);
@@ -332,12 +350,17 @@ __global__ void group_count_kernel(size_t nnodes, int L, int G, const int4* __re
     }
     gcnt[v] = c;
 }
-__global__ void group_write_kernel(size_t nnodes, int G, const int4* __restrict__ info, const uint32_t* __restrict__ gcnt,
-                                   const uint32_t* __restrict__ goff, int2* __restrict__ groups) {
+__global__ void group_write_kernel(uint32_t* __restrict__ dsz, size_t cap, size_t gcap, int G, const int4* __restrict__ info,
+                                   const uint32_t* __restrict__ gcnt, const uint32_t* __restrict__ goff,
+                                   int2* __restrict__ groups) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nnodes) return;
+    if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
     uint32_t c = gcnt[v];
     if (!c) return;
+    if ((size_t)goff[v] + c > gcap) {
+        dsz[DSZ_OVERFLOW] = 1u;
+        return;
+    }
     int4 nd = info[v];
     for (uint32_t t = 0; t < c; t++) {
         int b = nd.x + (int)t * G;
@@ -346,10 +369,10 @@ __global__ void group_write_kernel(size_t nnodes, int G, const int4* __restrict_
     }
 }
 __global__ void group_box_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-                                 size_t ngroups, const int2* __restrict__ groups, double* __restrict__ gbox,
-                                 int* __restrict__ pgroup) {
+                                 const uint32_t* __restrict__ dsz, size_t gcap, const int2* __restrict__ groups,
+                                 double* __restrict__ gbox, int* __restrict__ pgroup) {
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
+    if (dsz[DSZ_OVERFLOW] || g >= dev_count(dsz, DSZ_GROUPS, gcap)) return;
     int2 gr = groups[g];
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int p = gr.x; p < gr.x + gr.y; p++) {
@@ -364,6 +387,13 @@ __global__ void group_box_kernel(const double* __restrict__ x, const double* __r
 __global__ void fill_int_kernel(int* __restrict__ a, size_t m, int v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < m) a[i] = v;
+}
+// start of a tree build: clear the size words and the interaction counters (a kernel, not a
+// memset, so the step can be captured in a graph)
+__global__ void tree_reset_kernel(uint32_t* __restrict__ dsz, unsigned long long* __restrict__ counters) {
+    int t = threadIdx.x;
+    if (t == DSZ_NODES || t == DSZ_GROUPS || t == DSZ_OVERFLOW) dsz[t] = 0u;
+    if (t < 4) counters[t] = 0ull;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -402,7 +432,11 @@ __device__ inline float image_shift(int c, int d, int mcell, float box) {
     return t < 0 ? -box : (t >= mcell ? box : 0.0f);
 }
 
-struct walk_args {
+}  // namespace
+
+// Device view of one box for the walk. The force tables are per unit n^2 (shared by boxes of
+// equal law, softening and split), so `fact` carries the box's kick factor times 1/n^2.
+struct walk_box {
     const float4* pp;
     const int4* info;
     const int* child;
@@ -412,24 +446,40 @@ struct walk_args {
     const int* cellroot;
     const int2* groups;
     const uint64_t* key;
-    int cell_shift;
-    const float2* tab0;
-    const float4* tab12;
     double *vx, *vy, *vz;
-    int* stack;
-    unsigned long long* counters;
-    size_t ngroups;
+    unsigned long long* counters;  // [0] n_pp, [1] n_pc, [2] walk stack overflow
+    const uint32_t* dsz;
+    int cell_shift;
     int mcell;
+    int naccept;
     float box;       // n
-    float inv_cellw;
     float rc2;
     float rcut;
     float inv_theta2;
     float fact;
-    int naccept;
 };
 
-__global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
+namespace {
+
+// Group prefix over the boxes of one walk launch (a box whose build overflowed contributes no
+// groups and so gets no kick) and the reset of the shared group ticket.
+__global__ void walk_prefix_kernel(const walk_box* __restrict__ B, int nbox, uint32_t* __restrict__ prefix,
+                                   unsigned long long* __restrict__ ticket) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t s = 0;
+    for (int b = 0; b < nbox; b++) {
+        prefix[b] = s;
+        const uint32_t* d = B[b].dsz;
+        s += d[DSZ_OVERFLOW] ? 0u : d[DSZ_GROUPS];
+    }
+    prefix[nbox] = s;
+    *ticket = 0ull;
+}
+
+__global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* __restrict__ B, const uint32_t* __restrict__ gprefix,
+                                                                int nbox, const float2* __restrict__ tab0,
+                                                                const float4* __restrict__ tab12, int* __restrict__ stacks,
+                                                                unsigned long long* __restrict__ ticket_ctr) {
     // One 1024-thread workgroup per CU: the 16 waves share one LDS copy of the particle force
     // table (32 KB) next to their private interaction lists.
     __shared__ float2 s_tab0[4096];
@@ -437,7 +487,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
     __shared__ int s_pcidx[WALK_WAVES][PC_CAP];
     __shared__ float4 s_pp[WALK_WAVES][64];
     __shared__ float4 s_pc[WALK_WAVES][3][64];
-    for (int i = threadIdx.x; i < 4096; i += WALK_THREADS) s_tab0[i] = A.tab0[i];
+    for (int i = threadIdx.x; i < 4096; i += WALK_THREADS) s_tab0[i] = tab0[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     uint32_t* ppidx = s_ppidx[wid];
@@ -446,17 +496,26 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
     float4* cpc0 = s_pc[wid][0];
     float4* cpc1 = s_pc[wid][1];
     float4* cpc2 = s_pc[wid][2];
-    int* stack = A.stack + (size_t)(blockIdx.x * WALK_WAVES + wid) * STACK_MAX;
+    int* stack = stacks + (size_t)(blockIdx.x * WALK_WAVES + wid) * STACK_MAX;
+    const int gtotal = __builtin_amdgcn_readfirstlane((int)gprefix[nbox]);
 
     for (;;) {
         // Group ticket. Every lane executes the atomic (lane 0 adds 1, the others 0) and the
         // ticket is read from lane 0 into a scalar register: no lane-dependent branch sits at
         // the loop head, so the compiler cannot thread the back edge differently per lane
         // (which would let lanes 1..63 re-enter a group without a new ticket).
-        const unsigned long long ticket = atomicAdd(&A.counters[2], lane == 0 ? 1ull : 0ull);
+        const unsigned long long ticket = atomicAdd(ticket_ctr, lane == 0 ? 1ull : 0ull);
         const int g = __builtin_amdgcn_readlane((int)ticket, 0);
-        if ((size_t)g >= A.ngroups) break;
-        const int2 gr = A.groups[g];
+        if (g >= gtotal) break;
+        // the box that owns global group g: largest b with gprefix[b] <= g (wave-uniform search)
+        int blo = 0, bhi = nbox;
+        while (bhi - blo > 1) {
+            const int mid = (blo + bhi) >> 1;
+            if (__builtin_amdgcn_readfirstlane((int)gprefix[mid]) <= g) blo = mid;
+            else bhi = mid;
+        }
+        const walk_box A = B[blo];
+        const int2 gr = A.groups[g - __builtin_amdgcn_readfirstlane((int)gprefix[blo])];
         const int cnt = __builtin_amdgcn_readfirstlane(gr.y);
         const int gbeg = __builtin_amdgcn_readfirstlane(gr.x);
         // Lane replication: a group of cnt <= 32 targets gets R = 64/Tw lanes per target; the
@@ -569,7 +628,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
                 int nt = min((int)(r * (1.0f / RAN_F)), 4095);
                 float t = r - (float)nt * RAN_F;
                 float2 t0 = s_tab0[nt];
-                float4 t12 = A.tab12[nt];
+                float4 t12 = tab12[nt];
                 float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
                 float qx1 = qa.x * dx + qa.w * dy + qb.x * dz;
                 float qx2 = qa.w * dx + qa.y * dy + qb.y * dz;
@@ -695,7 +754,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(walk_args A) {
             az += __shfl_xor(az, o, 64);
         }
         if (overflow) {
-            if (lane == 0) atomicAdd(&A.counters[3], 1ull);
+            if (lane == 0) atomicAdd(&A.counters[2], 1ull);
         } else if (valid && rep == 0) {
             A.vx[me] += (double)(A.fact * ax);
             A.vy[me] += (double)(A.fact * ay);
@@ -723,6 +782,7 @@ struct walk64_args {
     const double* tab;
     double *vx, *vy, *vz;
     unsigned long long* counters;
+    const uint32_t* dsz;
     size_t np;
     int mcell;
     double box, cellw, rc2, rcut, inv_theta2, fact;
@@ -731,7 +791,7 @@ struct walk64_args {
 
 __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.np) return;
+    if (i >= A.np || A.dsz[DSZ_OVERFLOW]) return;
     const double px0 = A.x[i], py0 = A.y[i], pz0 = A.z[i];
     const int g = A.pgroup[i];
     const double lo[3] = {A.gbox[6 * g], A.gbox[6 * g + 1], A.gbox[6 * g + 2]};
@@ -808,7 +868,7 @@ __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
         }
     }
     if (overflow) {
-        atomicAdd(&A.counters[3], 1ull);
+        atomicAdd(&A.counters[2], 1ull);
         return;
     }
     A.vx[i] += A.fact * ax;
@@ -836,27 +896,61 @@ int lagpm_choose_cell_level(int n, double rsphere) {
     return L;
 }
 
+size_t lagpm_tree_node_cap_guess(size_t np, int mcell) {
+    // nleaf = 16 gives about np/6 nodes for uniform and np/4 for strongly clustered input
+    return np / 3 + (size_t)mcell * mcell * mcell + 4096;
+}
+
+int lagpm_tree_alloc(lagpm_ctx* ctx, lagpm_box* box, size_t cap) {
+    const bool f64 = box->p.precision == LAGPM_PREC_FP64;
+    const size_t gcap = cap + box->np / 64 + 1;
+    RCHK(regrow(ctx, &box->nd_info, cap));
+    RCHK(regrow(ctx, &box->nd_parent, cap));
+    RCHK(regrow(ctx, &box->nd_child, 8 * cap));
+    RCHK(regrow(ctx, &box->gflag, cap));
+    RCHK(regrow(ctx, &box->goff, cap));
+    RCHK(regrow(ctx, &box->groups, gcap));
+    if (f64) {
+        RCHK(regrow(ctx, &box->nd_d, 12 * cap));
+        RCHK(regrow(ctx, &box->gbox, 6 * gcap));
+    } else {
+        RCHK(regrow(ctx, &box->nd_com, cap));
+        RCHK(regrow(ctx, &box->nd_q0, cap));
+        RCHK(regrow(ctx, &box->nd_q1, cap));
+    }
+    const size_t need = std::max(lagpm_scan_tmp_bytes(cap), std::max(lagpm_scan_tmp_bytes(box->np), lagpm_sort_tmp_bytes(box->np)));
+    if (need > box->scan_tmp_bytes) {
+        RCHK(regrow(ctx, (char**)&box->scan_tmp, need));
+        box->scan_tmp_bytes = need;
+    }
+    box->node_cap = cap;
+    box->group_cap = gcap;
+    return LAGPM_OK;
+}
+
 int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box) {
     const size_t np = box->np;
     if (np == 0) return LAGPM_OK;
+    hipStream_t st = box->st;
     const double s = 65536.0 / (double)box->p.n;
-    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, box->x, box->y, box->z, np, s,
+    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->x, box->y, box->z, np, s,
                        box->key, box->perm);
     HIPCHK(ctx, hipGetLastError());
     bool alt = false;
-    RCHK(lagpm_radix_sort_pairs(ctx, box->key, box->key2, box->perm, box->perm2, np, 3 * LAGPM_KEY_BITS, &alt));
+    RCHK(lagpm_radix_sort_pairs(ctx, st, box->scan_tmp, box->key, box->key2, box->perm, box->perm2, np, 3 * LAGPM_KEY_BITS,
+                                &alt));
     if (alt) {
         std::swap(box->key, box->key2);
         std::swap(box->perm, box->perm2);
     }
     double** arrs[6] = {&box->x, &box->y, &box->z, &box->vx, &box->vy, &box->vz};
     for (int a = 0; a < 6; a++) {
-        hipLaunchKernelGGL(gather_kernel<double>, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, *arrs[a], box->perm,
+        hipLaunchKernelGGL(gather_kernel<double>, dim3(blocks_for(np, 256)), dim3(256), 0, st, *arrs[a], box->perm,
                            np, box->dtmp);
         HIPCHK(ctx, hipGetLastError());
         std::swap(*arrs[a], box->dtmp);
     }
-    hipLaunchKernelGGL(gather_kernel<uint32_t>, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, box->id, box->perm, np,
+    hipLaunchKernelGGL(gather_kernel<uint32_t>, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->id, box->perm, np,
                        box->utmp);
     HIPCHK(ctx, hipGetLastError());
     std::swap(box->id, box->utmp);
@@ -870,109 +964,133 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     const int nleaf = box->tune.nleaf;
     const int G = box->tune.group;
     const bool f64 = box->p.precision == LAGPM_PREC_FP64;
+    hipStream_t st = box->st;
+    const size_t cap = box->node_cap, gcap = box->group_cap;
+    hipLaunchKernelGGL(tree_reset_kernel, dim3(1), dim3(64), 0, st, box->dsz, box->counters);
+    HIPCHK(ctx, hipGetLastError());
+    if (np == 0) return LAGPM_OK;
     if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
     if (!f64) {
-        hipLaunchKernelGGL(pack_pp_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, box->x, box->y, box->z, np,
+        hipLaunchKernelGGL(pack_pp_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->x, box->y, box->z, np,
                            (float)box->p.mass, box->pp);
         HIPCHK(ctx, hipGetLastError());
     }
     const size_t ncell = (size_t)box->mcell * box->mcell * box->mcell;
-    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(ncell, 256)), dim3(256), 0, ctx->stream, box->cellroot, ncell, -1);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(ncell, 256)), dim3(256), 0, st, box->cellroot, ncell, -1);
     HIPCHK(ctx, hipGetLastError());
-    hipLaunchKernelGGL(node_count_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, box->key, np, L, nleaf,
+    hipLaunchKernelGGL(node_count_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->key, np, L, nleaf,
                        box->ncount);
     HIPCHK(ctx, hipGetLastError());
-    uint32_t total = 0;
-    RCHK(lagpm_exclusive_scan_u32(ctx, box->ncount, box->noff, np, &total));
-    box->nnodes = total;
-    if (total > box->node_cap) {
-        size_t cap = (size_t)total + total / 4 + 1024;
-        RCHK(regrow(ctx, &box->nd_info, cap));
-        RCHK(regrow(ctx, &box->nd_parent, cap));
-        RCHK(regrow(ctx, &box->nd_child, 8 * cap));
-        RCHK(regrow(ctx, &box->gflag, cap));
-        RCHK(regrow(ctx, &box->goff, cap));
-        if (f64) {
-            RCHK(regrow(ctx, &box->nd_d, 12 * cap));
-        } else {
-            RCHK(regrow(ctx, &box->nd_com, cap));
-            RCHK(regrow(ctx, &box->nd_q0, cap));
-            RCHK(regrow(ctx, &box->nd_q1, cap));
-        }
-        box->node_cap = cap;
-    }
-    const size_t nn = total;
-    hipLaunchKernelGGL(node_write_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, ctx->stream, box->key, np, L, nleaf,
-                       box->noff, box->nd_info, box->cellroot, box->mcell);
+    RCHK(lagpm_exclusive_scan_u32(ctx, st, box->scan_tmp, box->ncount, box->noff, np, box->dsz + DSZ_NODES));
+    hipLaunchKernelGGL(node_write_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->key, np, L, nleaf,
+                       box->noff, box->nd_info, box->cellroot, box->mcell, cap, box->dsz);
     HIPCHK(ctx, hipGetLastError());
-    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, box->nd_parent, nn, -1);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->nd_parent, cap, -1);
     HIPCHK(ctx, hipGetLastError());
-    hipLaunchKernelGGL(children_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, box->key, nn, L, nleaf,
+    hipLaunchKernelGGL(children_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->key, box->dsz, cap, L, nleaf,
                        box->noff, box->nd_info, box->nd_child, box->nd_parent);
     HIPCHK(ctx, hipGetLastError());
     for (int l = LAGPM_KEY_BITS; l >= L; l--) {
         if (f64)
-            hipLaunchKernelGGL(moments_f64_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, box->x, box->y,
-                               box->z, box->p.mass, nn, l, box->nd_info, box->nd_child, box->nd_d);
+            hipLaunchKernelGGL(moments_f64_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->x, box->y, box->z,
+                               box->p.mass, box->dsz, cap, l, box->nd_info, box->nd_child, box->nd_d);
         else
-            hipLaunchKernelGGL(moments_f32_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, box->pp, nn, l,
+            hipLaunchKernelGGL(moments_f32_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->pp, box->dsz, cap, l,
                                box->nd_info, box->nd_child, box->nd_com, box->nd_q0, box->nd_q1);
         HIPCHK(ctx, hipGetLastError());
     }
-    hipLaunchKernelGGL(group_count_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, nn, L, G, box->nd_info,
+    hipLaunchKernelGGL(group_count_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, L, G, box->nd_info,
                        box->nd_parent, box->gflag);
     HIPCHK(ctx, hipGetLastError());
-    uint32_t ng = 0;
-    RCHK(lagpm_exclusive_scan_u32(ctx, box->gflag, box->goff, nn, &ng));
-    if (ng > box->group_cap) {
-        size_t cap = (size_t)ng + ng / 4 + 1024;
-        RCHK(regrow(ctx, &box->groups, cap));
-        if (f64) RCHK(regrow(ctx, &box->gbox, 6 * cap));
-        box->group_cap = cap;
-    }
-    box->ngroups = ng;
-    hipLaunchKernelGGL(group_write_kernel, dim3(blocks_for(nn, 256)), dim3(256), 0, ctx->stream, nn, G, box->nd_info,
-                       box->gflag, box->goff, box->groups);
+    RCHK(lagpm_exclusive_scan_u32(ctx, st, box->scan_tmp, box->gflag, box->goff, cap, box->dsz + DSZ_GROUPS));
+    hipLaunchKernelGGL(group_write_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, gcap, G,
+                       box->nd_info, box->gflag, box->goff, box->groups);
     HIPCHK(ctx, hipGetLastError());
     if (f64) {
-        hipLaunchKernelGGL(group_box_kernel, dim3(blocks_for(ng, 256)), dim3(256), 0, ctx->stream, box->x, box->y, box->z,
-                           (size_t)ng, box->groups, box->gbox, box->pgroup);
+        hipLaunchKernelGGL(group_box_kernel, dim3(blocks_for(gcap, 256)), dim3(256), 0, st, box->x, box->y, box->z,
+                           box->dsz, gcap, box->groups, box->gbox, box->pgroup);
         HIPCHK(ctx, hipGetLastError());
     }
-    box->stats.n_nodes = nn;
-    box->stats.n_groups = ng;
     return LAGPM_OK;
 }
 
-int lagpm_tree_walk(lagpm_ctx* ctx, lagpm_box* box, double fact) {
-    const bool f64 = box->p.precision == LAGPM_PREC_FP64;
-    HIPCHK(ctx, hipMemsetAsync(box->counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
-    const double th = box->p.theta;
-    const double inv_theta2 = th > 0 ? 1.0 / (th * th) : 1e30;
-    const double cellw = (double)box->p.n / (double)box->mcell;
-    if (!f64) {
-        walk_args A;
-        A.pp = box->pp; A.info = box->nd_info; A.child = box->nd_child; A.com = box->nd_com; A.q0 = box->nd_q0; A.q1 = box->nd_q1;
-        A.cellroot = box->cellroot; A.groups = box->groups; A.tab0 = box->tab0; A.tab12 = box->tab12;
-        A.key = box->key; A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell);
-        A.vx = box->vx; A.vy = box->vy; A.vz = box->vz; A.stack = box->walk_stack; A.counters = box->counters;
-        A.ngroups = box->ngroups; A.mcell = box->mcell; A.box = (float)box->p.n; A.inv_cellw = (float)(1.0 / cellw);
-        A.rc2 = (float)(box->p.rsphere * box->p.rsphere); A.rcut = (float)box->p.rsphere;
-        A.inv_theta2 = (float)inv_theta2; A.fact = (float)fact; A.naccept = LAGPM_NACCEPT;
-        unsigned blocks = (unsigned)(box->walk_stack_entries / ((size_t)STACK_MAX * WALK_WAVES));
-        hipLaunchKernelGGL(walk_f32_kernel, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, A);
-        HIPCHK(ctx, hipGetLastError());
-    } else {
-        walk64_args A;
-        A.x = box->x; A.y = box->y; A.z = box->z; A.mass = box->p.mass; A.info = box->nd_info; A.child = box->nd_child;
-        A.nd = box->nd_d; A.cellroot = box->cellroot; A.gbox = box->gbox;
-        A.key = box->key; A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell); A.pgroup = box->pgroup; A.tab = box->tabd;
-        A.vx = box->vx; A.vy = box->vy; A.vz = box->vz; A.counters = box->counters; A.np = box->np; A.mcell = box->mcell;
-        A.box = (double)box->p.n; A.cellw = cellw; A.rc2 = box->p.rsphere * box->p.rsphere; A.rcut = box->p.rsphere;
-        A.inv_theta2 = inv_theta2; A.fact = fact; A.naccept = LAGPM_NACCEPT;
-        hipLaunchKernelGGL(walk_f64_kernel, dim3(blocks_for(box->np, 128)), dim3(128), 0, ctx->stream, A);
-        HIPCHK(ctx, hipGetLastError());
+int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* facts, int nbox) {
+    if (nbox <= 0) return LAGPM_OK;
+    if ((size_t)nbox > ctx->walk_box_cap) {
+        size_t cap = std::max<size_t>(64, (size_t)nbox * 2);
+        RCHK(regrow(ctx, &ctx->walk_boxes, cap));
+        RCHK(regrow(ctx, &ctx->walk_prefix, cap + 1));
+        ctx->walk_box_cap = cap;
     }
+    const law_tables* tab = boxes[0]->tab;
+    std::vector<walk_box> hb((size_t)nbox);
+    for (int b = 0; b < nbox; b++) {
+        lagpm_box* box = boxes[b];
+        if (box->tab != tab || box->p.precision != LAGPM_PREC_FP32)
+            return lagpm_fail(ctx, LAGPM_E_INVAL, "boxes of one walk launch must share the FP32 force table");
+        const double th = box->p.theta;
+        walk_box& A = hb[(size_t)b];
+        A.pp = box->pp; A.info = box->nd_info; A.child = box->nd_child; A.com = box->nd_com; A.q0 = box->nd_q0;
+        A.q1 = box->nd_q1; A.cellroot = box->cellroot; A.groups = box->groups; A.key = box->key;
+        A.vx = box->vx; A.vy = box->vy; A.vz = box->vz; A.counters = box->counters; A.dsz = box->dsz;
+        A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell);
+        A.mcell = box->mcell;
+        A.naccept = LAGPM_NACCEPT;
+        A.box = (float)box->p.n;
+        A.rc2 = (float)(box->p.rsphere * box->p.rsphere);
+        A.rcut = (float)box->p.rsphere;
+        A.inv_theta2 = (float)(th > 0 ? 1.0 / (th * th) : 1e30);
+        A.fact = (float)(facts[b] / ((double)box->p.n * (double)box->p.n));
+    }
+    // pageable source: the copy is staged before the call returns, so hb may go out of scope
+    HIPCHK(ctx, hipMemcpyAsync(ctx->walk_boxes, hb.data(), hb.size() * sizeof(walk_box), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(walk_prefix_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->walk_boxes, nbox, ctx->walk_prefix,
+                       ctx->walk_ticket);
+    HIPCHK(ctx, hipGetLastError());
+    const unsigned blocks = (unsigned)(ctx->walk_stack_entries / ((size_t)STACK_MAX * WALK_WAVES));
+    hipLaunchKernelGGL(walk_f32_kernel, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes, ctx->walk_prefix,
+                       nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
+    HIPCHK(ctx, hipGetLastError());
+    return LAGPM_OK;
+}
+
+int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact) {
+    const double th = box->p.theta;
+    const double cellw = (double)box->p.n / (double)box->mcell;
+    walk64_args A;
+    A.x = box->x; A.y = box->y; A.z = box->z; A.mass = box->p.mass; A.info = box->nd_info; A.child = box->nd_child;
+    A.nd = box->nd_d; A.cellroot = box->cellroot; A.gbox = box->gbox;
+    A.key = box->key; A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell); A.pgroup = box->pgroup; A.tab = box->tab->tabd;
+    A.vx = box->vx; A.vy = box->vy; A.vz = box->vz; A.counters = box->counters; A.dsz = box->dsz; A.np = box->np;
+    A.mcell = box->mcell; A.box = (double)box->p.n; A.cellw = cellw; A.rc2 = box->p.rsphere * box->p.rsphere;
+    A.rcut = box->p.rsphere; A.inv_theta2 = th > 0 ? 1.0 / (th * th) : 1e30;
+    A.fact = fact / ((double)box->p.n * (double)box->p.n);
+    A.naccept = LAGPM_NACCEPT;
+    if (box->np == 0) return LAGPM_OK;
+    hipLaunchKernelGGL(walk_f64_kernel, dim3(blocks_for(box->np, 128)), dim3(128), 0, box->st, A);
+    HIPCHK(ctx, hipGetLastError());
+    return LAGPM_OK;
+}
+
+int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry) {
+    *retry = false;
+    uint32_t d[DSZ_COUNT];
+    unsigned long long c[4];
+    HIPCHK(ctx, hipMemcpyAsync(d, box->dsz, sizeof(d), hipMemcpyDeviceToHost, box->st));
+    HIPCHK(ctx, hipMemcpyAsync(c, box->counters, sizeof(c), hipMemcpyDeviceToHost, box->st));
+    HIPCHK(ctx, hipStreamSynchronize(box->st));
+    box->stats.n_pp = c[0];
+    box->stats.n_pc = c[1];
+    box->stats.n_nodes = d[DSZ_NODES];
+    box->stats.n_groups = d[DSZ_GROUPS];
+    if (d[DSZ_OVERFLOW]) {
+        // grow to the node count this input needs (groups follow from nodes) and rebuild
+        size_t need = std::max<size_t>((size_t)d[DSZ_NODES], box->node_cap);
+        RCHK(lagpm_tree_alloc(ctx, box, need + need / 4 + 4096));
+        *retry = true;
+        return LAGPM_E_OVERFLOW;
+    }
+    if (c[2]) return lagpm_fail(ctx, LAGPM_E_OVERFLOW, "tree walk stack overflow in " + std::to_string(c[2]) + " groups");
     return LAGPM_OK;
 }
 

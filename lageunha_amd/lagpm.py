@@ -106,6 +106,8 @@ SIGNATURES = {
                                         ctypes.POINTER(EvolFact)]),
     "lagpm_hubble_drag": (ctypes.c_int, [_P, _P, ctypes.c_double]),
     "lagpm_drift": (ctypes.c_int, [_P, _P, ctypes.c_double]),
+    "lagpm_kdk_step": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double]),
+    "lagpm_batch_step": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int, _D, _D]),
 }
 
 _lib = None
@@ -297,6 +299,19 @@ class Box:
 
     def drift(self, pfact: float):
         self.ctx.check(self.lib.lagpm_drift(self.ctx.h, self.h, pfact))
+
+    def kdk_step(self, anow: float, astep: float):
+        """One GOTPM global step (drag, PM kick, tree kick, drift) at scale factor anow."""
+        self.ctx.check(self.lib.lagpm_kdk_step(self.ctx.h, self.h, anow, astep))
+
+
+def batch_step(ctx: Context, boxes, anow, astep):
+    """One GOTPM global step of every box in `boxes` (anow/astep: scalars or one per box)."""
+    nb = len(boxes)
+    a = np.broadcast_to(np.asarray(anow, dtype=np.float64), (nb,)).copy()
+    s = np.broadcast_to(np.asarray(astep, dtype=np.float64), (nb,)).copy()
+    hs = (_P * max(nb, 1))(*[b.h for b in boxes])
+    ctx.check(ctx.lib.lagpm_batch_step(ctx.h, hs, nb, _dp(a), _dp(s)))
 
 
 def green_apply(ctx: Context, n: int, efold: float, deconvolve: int, spec: np.ndarray) -> np.ndarray:

@@ -29,7 +29,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert lagpm.load().lagpm_abi_version() == 1
+    hdr = open(os.path.join(ROOT, "include", "lagpm.h")).read()
+    want = int(re.search(r"#define LAGPM_ABI_VERSION (\d+)", hdr).group(1))
+    assert lagpm.load().lagpm_abi_version() == want
 
 
 def test_force_spline_matches_oracle_tables():

@@ -1,0 +1,134 @@
+"""GPU tests of the whole-step entry points (run with -m gpu): lagpm_kdk_step equals the
+reference's step sequence (HubbleDragDM, pmmain KICK, treecorrection, drift; Cosmos/globalT.c:221-306)
+made of the single-phase entry points, and lagpm_batch_step gives every box exactly the result
+it gets alone, whatever the other boxes in the batch."""
+import numpy as np
+import pytest
+
+from lageunha_amd import ics, lagpm
+from oracle import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def lattice_box(n, seed, rms=0.3):
+    return ics.perturbed_lattice(n, rms=rms, seed=seed)
+
+
+def run_manual(ctx, p, pos, vel, anow, astep, steps):
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.upload_pos(pos, vel)
+        for s in range(steps):
+            ef = lagpm.get_evol_factor(p, anow + s * astep, astep)
+            b.hubble_drag(ef.fact1)
+            b.pmmain(ef, lagpm.KICK)
+            b.treecorrection(ef)
+            b.drift(ef.pfact)
+        return b.download()
+
+
+def run_kdk(ctx, p, pos, vel, anow, astep, steps):
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.upload_pos(pos, vel)
+        for s in range(steps):
+            b.kdk_step(anow + s * astep, astep)
+        st = b.tree_stats()
+        return b.download(), st
+
+
+def test_kdk_step_equals_the_reference_step_sequence(ctx):
+    n = 32
+    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=20, seed=2)
+    vel = np.random.default_rng(2).standard_normal(pos.shape) * 0.01
+    p = lagpm.make_params(n, omep=0.3, omeplam=0.7)
+    print(f"kdk vs manual: N={len(pos)} n={n}", flush=True)
+    (x1, v1) = run_manual(ctx, p, pos, vel, 1.0, 0.25, 3)
+    (x2, v2), st = run_kdk(ctx, p, pos, vel, 1.0, 0.25, 3)
+    assert st.n_pp > 0 and st.n_groups > 0
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2)
+
+
+def test_kdk_step_moves_particles_by_the_kicked_velocity(ctx):
+    """x_new = wrap(x + pfact * v_new), v_new = fact1 v + fact2 (a_PM + a_tree)."""
+    n = 32
+    pos = lattice_box(n, 4)
+    vel = np.random.default_rng(4).standard_normal(pos.shape) * 0.01
+    p = lagpm.make_params(n)
+    ef = lagpm.get_evol_factor(p, 2.0, 0.5)
+    (x1, v1), _ = run_kdk(ctx, p, pos, vel, 2.0, 0.5, 1)
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.upload_pos(pos)
+        b.pmmain(lagpm.evolfact(1.0), lagpm.KICK)
+        b.treecorrection(lagpm.evolfact(1.0))
+        acc = b.velocities()
+    v_want = np.float64(ef.fact1) * vel + np.float64(ef.fact2) * acc
+    assert np.allclose(v1, v_want, rtol=1e-5, atol=1e-9 * np.abs(v_want).max())
+    x_want = np.mod(pos + np.float64(ef.pfact) * v1, n)
+    d = np.abs(x1 - x_want)
+    d = np.minimum(d, n - d)
+    assert d.max() < 1e-9
+
+
+@pytest.mark.parametrize("prec", [lagpm.PREC_FP32, lagpm.PREC_FP64])
+def test_batch_equals_each_box_alone(ctx, prec):
+    # uneven boxes: different sizes, cosmologies, clustering and scale factors
+    specs = [(32, 0.25, 1.0, "clustered"), (40, 0.30, 1.5, "lattice"), (24, 0.35, 3.0, "clustered"),
+             (48, 0.28, 1.2, "lattice"), (32, 0.33, 2.0, "lattice")]
+    boxes, alone, inputs = [], [], []
+    for i, (n, om, a0, kind) in enumerate(specs):
+        pos = (ics.clustered_box(n, halo_fraction=0.3, nhalo=max(4, n ** 3 // 2000), seed=10 + i) if kind == "clustered"
+               else lattice_box(n, 10 + i))
+        vel = np.random.default_rng(i).standard_normal(pos.shape) * 0.01
+        p = lagpm.make_params(n, omep=om, omeplam=1 - om, precision=prec)
+        inputs.append((p, pos, vel, a0))
+    print(f"batch of {len(specs)} boxes, prec={prec}", flush=True)
+    for p, pos, vel, a0 in inputs:
+        (x, v), _ = run_kdk(ctx, p, pos, vel, a0, 0.1, 2)
+        alone.append((x, v))
+    try:
+        for p, pos, vel, a0 in inputs:
+            b = lagpm.Box(ctx, p, len(pos))
+            b.upload_pos(pos, vel)
+            boxes.append(b)
+        anow = np.array([a0 for *_, a0 in inputs])
+        for s in range(2):
+            lagpm.batch_step(ctx, boxes, anow + 0.1 * s, 0.1)
+        for b, (x, v) in zip(boxes, alone):
+            xb, vb = b.download()
+            assert np.array_equal(xb, x) and np.array_equal(vb, v)
+    finally:
+        for b in boxes:
+            b.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(lagpm.PREC_FP32, 3e-5), (lagpm.PREC_FP64, 1e-5)])
+def test_node_overflow_is_regrown_and_the_kick_is_exact(ctx, prec, tol):
+    """Tight 17-particle clumps make about one node per particle, beyond the initial node
+    capacity: the build must detect it, grow the arrays and still give the exact pair sum.
+    The FP32 walk sees FP32 positions (clump members 2e-4 cells apart differ in the 3rd
+    digit at that rounding), so its exact sum is taken over the FP32-rounded positions."""
+    n = 32
+    g = np.random.default_rng(12)
+    centres = g.random((400, 3)) * n
+    pos = (centres[:, None, :] + g.standard_normal((400, 17, 3)) * 2e-4).reshape(-1, 3) % n
+    p = lagpm.make_params(n, theta=1e-6, precision=prec)
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.upload_pos(pos)
+        b.treecorrection(lagpm.evolfact(1.0))
+        a = b.velocities()
+        st = b.tree_stats()
+        # the regrown arrays hold the tree: a second kick adds exactly the same acceleration
+        b.treecorrection(lagpm.evolfact(1.0))
+        assert np.array_equal(b.velocities(), 2 * a)
+    print(f"clumps prec={prec}: N={len(pos)} nodes {st.n_nodes} groups {st.n_groups}", flush=True)
+    assert st.n_nodes > len(pos) // 3
+    diff, slope = orc.force_spline(1, n, p.epsilon, p.efold / 2)
+    seen = pos.astype(np.float32).astype(np.float64) if prec == lagpm.PREC_FP32 else pos
+    ref = orc.pp_direct(n, seen, diff, slope, p.rsphere)
+    err = float(np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)))
+    print(f"  rms rel {err:.3e}", flush=True)
+    assert err < tol
+    # and the same input through a whole step (overflow inside the batched path)
+    (xk, vk), _ = run_kdk(ctx, p, pos, None, 1.0, 0.1, 1)
+    (xm, vm) = run_manual(ctx, p, pos, None, 1.0, 0.1, 1)
+    assert np.array_equal(xk, xm) and np.array_equal(vk, vm)
