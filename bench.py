@@ -143,6 +143,16 @@ def ensemble(args, ctx, lagpm, ics):
         lat["batched"].append(time.perf_counter() - t)
     tt["batched"] = time.perf_counter() - t0
     log(f"[bench] ensemble batched: {tt['batched']:.2f} s")
+    # the same batched steps with every kernel launched (no step graphs), for comparison
+    ctx.set_step_graphs(False)
+    t0 = time.perf_counter()
+    for s in range(2):
+        lagpm.batch_step(ctx, sets["batched"], a0 + da * (steps + 1 + s), da)
+    t_eager = (time.perf_counter() - t0) / 2
+    ctx.set_step_graphs(True)
+    for s in range(2):  # keep the plain set in step with the batched one
+        for bx in sets["plain"]:
+            bx.kdk_step(a0 + da * (steps + 1 + s), da)
     same = True
     for bp, bb in zip(sets["plain"], sets["batched"]):
         xp, vp = bp.download()
@@ -159,6 +169,7 @@ def ensemble(args, ctx, lagpm, ics):
                   "step_latency_p50_ms": float(np.percentile(L, 50)), "step_latency_p99_ms": float(np.percentile(L, 99)),
                   "latency_unit": "one box-step (plain) / one batch step of all boxes (batched)"}
     out["batched_over_plain"] = out["batched"]["particle_steps_per_s"] / out["plain"]["particle_steps_per_s"]
+    out["batched_without_graphs"] = {"particle_steps_per_s": total / t_eager, "step_ms": t_eager * 1e3}
     # the plain FP64 kernels, box by box (one step each after a warm-up step)
     t_fp64, ps = 0.0, 0
     for b, (n, om) in enumerate(specs):

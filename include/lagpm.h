@@ -166,6 +166,15 @@ int lagpm_set_potential(lagpm_ctx* ctx, lagpm_box* box, const float* phi);
 /* Download the gradient mesh (n^3 x 4 floats: fx, fy, fz, phi). */
 int lagpm_get_gradient(lagpm_ctx* ctx, lagpm_box* box, float* grad4);
 
+/* CorrMeasure (Cosmos/Correlation.c:13-116) of the box's current particles: TSC density minus
+ * one, r2c, then |delta_k|^2/n^6 over the modes with kx < n/2, divided by the squared TSC
+ * window and binned with linear weights between floor(|k|) and its neighbour (grid units).
+ * Outputs (n+1 entries each): pk (bin means), npk (bin weights), cor (xi at r = i cells);
+ * sigma8 for a top hat of 8 Mpc/h in a box of side boxsize Mpc/h; slope of log cor as the
+ * reference computes it from cor[2] and cor[3]. */
+int lagpm_power_spectrum(lagpm_ctx* ctx, lagpm_box* box, double boxsize, double* pk, double* npk, double* cor,
+                         double* sigma8, double* slope);
+
 /* ---- short-range tree correction --------------------------------------------------- */
 /* Builds FP64 force tables (NSPLINE=4096 entries over r in [0,6)): diff[i][3], slope[i][3]
  * exactly as OST/Model4TreeForce.c:48-91 (law PMFIT) or for the erfc law. Host only. */
@@ -198,6 +207,9 @@ int lagpm_kdk_step(lagpm_ctx* ctx, lagpm_box* box, double anow, double astep);
  * concurrently on a pool of streams, and the tree walks of all FP32 boxes with the same force
  * law run as one persistent launch. Each box's result equals its lagpm_kdk_step result. */
 int lagpm_batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* anow, const double* astep);
+/* Whole steps capture each box's step (up to the tree walk) in a hipGraph on first use and
+ * replay it (default on); off: every kernel is launched each step. Results are identical. */
+int lagpm_set_step_graphs(lagpm_ctx* ctx, int enable);
 
 #ifdef __cplusplus
 }

@@ -38,9 +38,18 @@ __global__ void scatter_kernel(const double* __restrict__ src, const uint32_t* _
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < np) dst[id[i]] = src[i];
 }
-__global__ void scale3_kernel(double* __restrict__ a, double* __restrict__ b, double* __restrict__ c, size_t np, double f) {
+// kick factors live in device memory (box->dfac) so that a captured step reads this step's values
+__global__ void set_factors_kernel(double* __restrict__ dfac, double f1, double f2, double pf, int mask) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (mask & 1) dfac[0] = f1;
+    if (mask & 2) dfac[1] = f2;
+    if (mask & 4) dfac[2] = pf;
+}
+__global__ void scale3_kernel(double* __restrict__ a, double* __restrict__ b, double* __restrict__ c, size_t np,
+                              const double* __restrict__ fac) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
+    const double f = fac[0];
     a[i] *= f;
     b[i] *= f;
     c[i] *= f;
@@ -49,9 +58,10 @@ __global__ void scale3_kernel(double* __restrict__ a, double* __restrict__ b, do
 // drift until the host has redone the kick
 __global__ void drift_kernel(double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
                              const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
-                             size_t np, double pfact, double L, const uint32_t* __restrict__ skip) {
+                             size_t np, const double* __restrict__ fac, double L, const uint32_t* __restrict__ skip) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np || (skip && *skip)) return;
+    const double pfact = fac[2];
     double p[3] = {x[i] + pfact * vx[i], y[i] + pfact * vy[i], z[i] + pfact * vz[i]};
     for (int a = 0; a < 3; a++) {
         double v = fmod(p[a], L);
@@ -93,7 +103,11 @@ int fixed_scale_bits(double total_mass) {
 
 void free_box(lagpm_box* b) {
     dfree(b->x); dfree(b->y); dfree(b->z); dfree(b->vx); dfree(b->vy); dfree(b->vz); dfree(b->id);
-    dfree(b->key); dfree(b->key2); dfree(b->perm); dfree(b->perm2); dfree(b->dtmp); dfree(b->utmp);
+    dfree(b->key); dfree(b->key2); dfree(b->perm); dfree(b->perm2);
+    for (int a = 0; a < 6; a++) dfree(b->alt[a]);
+    dfree(b->id_alt); dfree(b->dfac);
+    lagpm_box_fft_release(b);
+    lagpm_box_graphs_release(b);
     dfree(b->acc); dfree(b->dep_start); dfree(b->rho); dfree(b->spec); dfree(b->phi); dfree(b->grad);
     dfree(b->pp); dfree(b->ncount); dfree(b->noff); dfree(b->nd_info); dfree(b->nd_parent); dfree(b->nd_child);
     dfree(b->nd_com); dfree(b->nd_q0); dfree(b->nd_q1); dfree(b->nd_d); dfree(b->cellroot); dfree(b->gflag);
@@ -202,6 +216,52 @@ int lagpm_dalloc_bytes(lagpm_ctx* ctx, void** p, size_t bytes) {
 
 // Device force tables of a law, built once per context for n = 1 (the law scales as 1/n^2;
 // the walk multiplies by the box's 1/n^2 with the kick factor).
+// g(s) = fr(r)/r at s = r^2 for n = 1 (the short-range force per unit separation); in terms of
+// s the reference's table terms are f0 = -g, f1 = -(fp r - fr)/(2 r^3) = -dg/ds and
+// f2 = -(fpp r^2 - 3 fp r + 3 fr)/(2 r^5) = -2 d2g/ds2 (OST/Model4TreeForce.c:48-91).
+static double erfc_g(double eps, double rs, double s) {
+    const double r = std::sqrt(s);
+    const double newton = 1.0 / std::pow(s + eps * eps, 1.5);
+    const double u = r / (2.0 * rs);
+    double flong;
+    if (u < 1e-3) {  // series of (erf(u) - 2u/sqrt(pi) exp(-u^2)) / r^3, which is smooth in s
+        flong = (4.0 / (3.0 * std::sqrt(M_PI))) / (8.0 * rs * rs * rs) * (1.0 - 0.6 * u * u);
+    } else {
+        flong = (std::erf(u) - (r / (rs * std::sqrt(M_PI))) * std::exp(-u * u)) / (s * r);
+    }
+    return newton - flong;
+}
+
+static void erfc_log_tables(double eps, double rs, std::vector<double>& t /* 6 per bin */) {
+    const int NB = LAGPM_LOG_BINS;
+    auto node = [](int b) -> double {
+        uint32_t bits = (uint32_t)(b + (int)LAGPM_LOG_BASE) << LAGPM_LOG_SHIFT;
+        float f;
+        std::memcpy(&f, &bits, 4);
+        return (double)f;
+    };
+    auto terms = [&](double s, double out[3]) {
+        const double h = 1e-4 * std::max(s, eps * eps);
+        const double g0 = erfc_g(eps, rs, s), gp = erfc_g(eps, rs, s + h), gm = erfc_g(eps, rs, s - h);
+        out[0] = -g0;
+        out[1] = -(gp - gm) / (2 * h);
+        out[2] = -2.0 * (gp - 2 * g0 + gm) / (h * h);
+    };
+    t.assign(6 * (size_t)NB, 0.0);
+    for (int b = 0; b < NB; b++) {
+        const double s0 = node(b), s1 = node(b + 1);
+        double a[3], c[3];
+        terms(s0, a);
+        terms(s1, c);
+        for (int k = 0; k < 3; k++) {
+            t[6 * (size_t)b + 2 * k] = a[k];
+            t[6 * (size_t)b + 2 * k + 1] = (c[k] - a[k]) / (s1 - s0);
+        }
+    }
+}
+
+// Device force tables of a law, built once per context for n = 1 (the law scales as 1/n^2;
+// the walk multiplies by the box's mass / n^2 with the kick factor).
 int lagpm_get_tables(lagpm_ctx* ctx, const lagpm_params* p, const law_tables** out) {
     const auto key = std::make_tuple(p->law, p->epsilon, p->efold);
     auto it = ctx->tables.find(key);
@@ -209,28 +269,39 @@ int lagpm_get_tables(lagpm_ctx* ctx, const lagpm_params* p, const law_tables** o
         *out = &it->second;
         return LAGPM_OK;
     }
-    const int NS = LAGPM_NSPLINE;
-    lagpm_params q = *p;
-    q.n = 1;
-    std::vector<double> diff(3 * NS), slope(3 * NS);
-    lagpm_build_tables(&q, diff.data(), slope.data());
-    std::vector<double> t(6 * NS);
-    std::vector<float2> t0(NS);
-    std::vector<float4> t12(NS);
-    for (int i = 0; i < NS; i++) {
-        for (int c = 0; c < 3; c++) {
-            t[6 * i + 2 * c] = diff[3 * i + c];
-            t[6 * i + 2 * c + 1] = slope[3 * i + c];
-        }
-        t0[i] = make_float2((float)diff[3 * i], (float)slope[3 * i]);
-        t12[i] = make_float4((float)diff[3 * i + 1], (float)slope[3 * i + 1], (float)diff[3 * i + 2], (float)slope[3 * i + 2]);
-    }
     law_tables lt;
-    RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tab0, NS * sizeof(float2)));
-    RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tab12, NS * sizeof(float4)));
-    RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tabd, 6 * NS * sizeof(double)));
-    HIPCHK(ctx, hipMemcpyAsync(lt.tab0, t0.data(), NS * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(lt.tab12, t12.data(), NS * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+    std::vector<double> t;  // 6 per bin: d0 s0 d1 s1 d2 s2
+    if (p->law == LAGPM_LAW_ERFC) {
+        lt.logr2 = 1;
+        lt.nbins = LAGPM_LOG_BINS;
+        erfc_log_tables(p->epsilon, 0.5 * p->efold, t);
+    } else {
+        const int NS = LAGPM_NSPLINE;
+        lagpm_params q = *p;
+        q.n = 1;
+        std::vector<double> diff(3 * NS), slope(3 * NS);
+        lagpm_build_tables(&q, diff.data(), slope.data());
+        lt.logr2 = 0;
+        lt.nbins = NS;
+        t.resize(6 * (size_t)NS);
+        for (int i = 0; i < NS; i++)
+            for (int c = 0; c < 3; c++) {
+                t[6 * i + 2 * c] = diff[3 * i + c];
+                t[6 * i + 2 * c + 1] = slope[3 * i + c];
+            }
+    }
+    const int NB = lt.nbins;
+    std::vector<float2> t0(NB);
+    std::vector<float4> t12(NB);
+    for (int i = 0; i < NB; i++) {
+        t0[i] = make_float2((float)t[6 * i], (float)t[6 * i + 1]);
+        t12[i] = make_float4((float)t[6 * i + 2], (float)t[6 * i + 3], (float)t[6 * i + 4], (float)t[6 * i + 5]);
+    }
+    RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tab0, NB * sizeof(float2)));
+    RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tab12, NB * sizeof(float4)));
+    RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tabd, 6 * NB * sizeof(double)));
+    HIPCHK(ctx, hipMemcpyAsync(lt.tab0, t0.data(), NB * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(lt.tab12, t12.data(), NB * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(lt.tabd, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     auto res = ctx->tables.emplace(key, lt);
@@ -322,7 +393,7 @@ size_t lagpm_estimate_box_bytes(const lagpm_params* p, size_t np) {
     const size_t real = f64 ? 8 : 4;
     const int L = lagpm_choose_cell_level(p->n, p->rsphere);
     const size_t mcell = (size_t)1 << L;
-    size_t b = np * (6 * 8 + 4 * 4 + 2 * 8 + 8 + 2 * 4);  // x..vz, id/perm/perm2/utmp, keys, dtmp, node counts
+    size_t b = np * (12 * 8 + 4 * 4 + 2 * 8 + 2 * 4);  // two sets of x..vz, id/id_alt/perm/perm2, keys, node counts
     b += np * (f64 ? 4 : 16);                             // pgroup / pp
     b += m * 8 + 2 * m * real + K * 2 * real + 4 * m * real;  // acc, rho, phi, spec, grad
     b += 4 * mcell * mcell * mcell + 4 * 8 + DSZ_COUNT * 4;  // cellroot, counters, sizes
@@ -356,7 +427,8 @@ int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box
     ALLOC(box->x, np); ALLOC(box->y, np); ALLOC(box->z, np);
     ALLOC(box->vx, np); ALLOC(box->vy, np); ALLOC(box->vz, np);
     ALLOC(box->id, np); ALLOC(box->key, np); ALLOC(box->key2, np); ALLOC(box->perm, np); ALLOC(box->perm2, np);
-    ALLOC(box->dtmp, np); ALLOC(box->utmp, np);
+    for (int a = 0; a < 6; a++) ALLOC(box->alt[a], np);
+    ALLOC(box->id_alt, np); ALLOC(box->dfac, 4);
     ALLOC(box->acc, m);
     if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->rho, m * real);
     if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->spec, K * 2 * real);
@@ -451,9 +523,9 @@ int lagpm_box_download(lagpm_ctx* ctx, lagpm_box* box, double* x, double* y, dou
     double* dst[6] = {x, y, z, vx, vy, vz};
     for (int a = 0; a < 6; a++) {
         if (!dst[a] || np == 0) continue;
-        hipLaunchKernelGGL(scatter_kernel, dim3(nblocks(np, 256)), dim3(256), 0, box->st, src[a], box->id, np, box->dtmp);
+        hipLaunchKernelGGL(scatter_kernel, dim3(nblocks(np, 256)), dim3(256), 0, box->st, src[a], box->id, np, box->alt[0]);
         HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipMemcpyAsync(dst[a], box->dtmp, np * sizeof(double), hipMemcpyDeviceToHost, box->st));
+        HIPCHK(ctx, hipMemcpyAsync(dst[a], box->alt[0], np * sizeof(double), hipMemcpyDeviceToHost, box->st));
         HIPCHK(ctx, hipStreamSynchronize(box->st));
     }
     return LAGPM_OK;
@@ -592,7 +664,8 @@ int lagpm_fda4(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef, int pms
     double f;
     RCHK(pm_factor(ctx, ef, pmstatus, &f));
     RCHK(lagpm_pm_gradient(ctx, box));
-    RCHK(lagpm_pm_interp(ctx, box, f));
+    hipLaunchKernelGGL(set_factors_kernel, dim3(1), dim3(64), 0, box->st, box->dfac, 0.0, f, 0.0, 2);
+    RCHK(lagpm_pm_interp(ctx, box));
     HIPCHK(ctx, hipStreamSynchronize(box->st));
     return LAGPM_OK;
 }
@@ -639,9 +712,40 @@ int lagpm_pmmain(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef, int p
     RCHK(lagpm_pm_solve(ctx, box, ev + 3));
     RCHK(lagpm_pm_gradient(ctx, box));
     HIPCHK(ctx, hipEventRecord(ev[6], box->st));
-    RCHK(lagpm_pm_interp(ctx, box, f));
+    hipLaunchKernelGGL(set_factors_kernel, dim3(1), dim3(64), 0, box->st, box->dfac, 0.0, f, 0.0, 2);
+    RCHK(lagpm_pm_interp(ctx, box));
     HIPCHK(ctx, hipEventRecord(ev[7], box->st));
     box->pm_timed = true;
+    return LAGPM_OK;
+}
+
+int lagpm_power_spectrum(lagpm_ctx* ctx, lagpm_box* box, double boxsize, double* pk, double* npk, double* cor,
+                         double* sigma8, double* slope) {
+    RCHK(need_particles(ctx, box));
+    if (!pk || !npk || !cor || !sigma8 || !slope || !(boxsize > 0))
+        return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL output or non-positive box size");
+    const int n = box->p.n;
+    if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
+    RCHK(lagpm_pm_deposit(ctx, box));
+    RCHK(lagpm_pm_power(ctx, box, pk, npk));
+    // normalisation, sigma_8 and xi(r) as Cosmos/Correlation.c:80-103
+    const double rth = 8.0 / boxsize * n, fact = 2 * M_PI * rth / (double)n;
+    double var = 0;
+    for (int i = 1; i <= n; i++) {
+        pk[i] = npk[i] > 0 ? pk[i] / npk[i] : 0.0;
+        const double x = fact * i, u = std::sin(x) - x * std::cos(x);
+        var += pk[i] * (double)i * i * u * u / std::pow(x, 6);
+    }
+    pk[0] = npk[0] > 0 ? pk[0] / npk[0] : 0.0;
+    *sigma8 = std::sqrt(4 * M_PI * var * 9.0);
+    cor[0] = 0;
+    for (int i = 1; i <= n; i++) {
+        const double sep = 2 * M_PI * i / (double)n;
+        double c = 0;
+        for (int k = 1; k <= n; k++) c += std::sin(k * sep) * k * pk[k];
+        cor[i] = 2.0 * n * c / i;
+    }
+    *slope = (n >= 3 && cor[3] > 0 && cor[2] > 0) ? (std::log(cor[3]) - std::log(cor[2])) / (std::log(4.0) - std::log(3.0)) : 0.0;
     return LAGPM_OK;
 }
 
@@ -759,8 +863,9 @@ int lagpm_evolfactor(const lagpm_params* p, double anow, double astep, lagpm_evo
 int lagpm_hubble_drag(lagpm_ctx* ctx, lagpm_box* box, double fact1) {
     RCHK(need_particles(ctx, box));
     if (box->np == 0) return LAGPM_OK;
+    hipLaunchKernelGGL(set_factors_kernel, dim3(1), dim3(64), 0, box->st, box->dfac, fact1, 0.0, 0.0, 1);
     hipLaunchKernelGGL(scale3_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, box->st, box->vx, box->vy, box->vz,
-                       box->np, fact1);
+                       box->np, (const double*)box->dfac);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }
@@ -768,8 +873,9 @@ int lagpm_hubble_drag(lagpm_ctx* ctx, lagpm_box* box, double fact1) {
 int lagpm_drift(lagpm_ctx* ctx, lagpm_box* box, double pfact) {
     RCHK(need_particles(ctx, box));
     if (box->np == 0) return LAGPM_OK;
+    hipLaunchKernelGGL(set_factors_kernel, dim3(1), dim3(64), 0, box->st, box->dfac, 0.0, 0.0, pfact, 4);
     hipLaunchKernelGGL(drift_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx,
-                       box->vy, box->vz, box->np, pfact, (double)box->p.n, (const uint32_t*)nullptr);
+                       box->vy, box->vz, box->np, (const double*)box->dfac, (double)box->p.n, (const uint32_t*)nullptr);
     HIPCHK(ctx, hipGetLastError());
     box->sorted = false;
     return LAGPM_OK;
@@ -788,34 +894,106 @@ int ensure_pool(lagpm_ctx* ctx, size_t want) {
     return LAGPM_OK;
 }
 
-// Everything of one box's step up to the tree build, queued on box->st.
-int step_front(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact& ef) {
+// The kernels of one box's step up to the tree build, queued on box->st with the factors
+// already in box->dfac. With ev: phase events (eager runs only; a capture records none).
+int front_body(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     const size_t np = box->np;
     hipLaunchKernelGGL(scale3_kernel, dim3(nblocks(np, 256)), dim3(256), 0, box->st, box->vx, box->vy, box->vz, np,
-                       (double)ef.fact1);
+                       (const double*)box->dfac);
     HIPCHK(ctx, hipGetLastError());
-    hipEvent_t* ev = box->ev;
-    HIPCHK(ctx, hipEventRecord(ev[0], box->st));
-    if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
-    HIPCHK(ctx, hipEventRecord(ev[1], box->st));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[0], box->st));
+    RCHK(lagpm_sort_particles(ctx, box));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[1], box->st));
     RCHK(lagpm_pm_deposit(ctx, box));
-    HIPCHK(ctx, hipEventRecord(ev[2], box->st));
-    RCHK(lagpm_pm_solve(ctx, box, ev + 3));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[2], box->st));
+    RCHK(lagpm_pm_solve(ctx, box, ev ? ev + 3 : nullptr));
     RCHK(lagpm_pm_gradient(ctx, box));
-    HIPCHK(ctx, hipEventRecord(ev[6], box->st));
-    RCHK(lagpm_pm_interp(ctx, box, (double)ef.fact2));
-    HIPCHK(ctx, hipEventRecord(ev[7], box->st));
-    box->pm_timed = true;
-    HIPCHK(ctx, hipEventRecord(box->ev[8], box->st));
-    HIPCHK(ctx, hipEventRecord(box->ev[9], box->st));  // already sorted: no tree sort
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[6], box->st));
+    RCHK(lagpm_pm_interp(ctx, box));
+    if (ev) {
+        HIPCHK(ctx, hipEventRecord(ev[7], box->st));
+        HIPCHK(ctx, hipEventRecord(ev[8], box->st));
+        HIPCHK(ctx, hipEventRecord(ev[9], box->st));  // sorted above: no tree sort
+    }
     RCHK(lagpm_tree_build(ctx, box));
+    return LAGPM_OK;
+}
+
+struct box_ptrs {  // host-side pointer state a front changes (sort swaps)
+    double *x, *y, *z, *vx, *vy, *vz, *alt[6];
+    uint32_t *id, *id_alt;
+    int parity;
+    bool sorted;
+};
+box_ptrs save_ptrs(const lagpm_box* b) {
+    box_ptrs s{b->x, b->y, b->z, b->vx, b->vy, b->vz, {}, b->id, b->id_alt, b->parity, b->sorted};
+    for (int a = 0; a < 6; a++) s.alt[a] = b->alt[a];
+    return s;
+}
+void load_ptrs(lagpm_box* b, const box_ptrs& s) {
+    b->x = s.x; b->y = s.y; b->z = s.z; b->vx = s.vx; b->vy = s.vy; b->vz = s.vz;
+    for (int a = 0; a < 6; a++) b->alt[a] = s.alt[a];
+    b->id = s.id; b->id_alt = s.id_alt; b->parity = s.parity; b->sorted = s.sorted;
+}
+// what a front does to the host-side state, without queueing anything (graph replays)
+void front_host_effects(lagpm_box* b) {
+    std::swap(b->x, b->alt[0]); std::swap(b->y, b->alt[1]); std::swap(b->z, b->alt[2]);
+    std::swap(b->vx, b->alt[3]); std::swap(b->vy, b->alt[4]); std::swap(b->vz, b->alt[5]);
+    std::swap(b->id, b->id_alt);
+    b->parity ^= 1;
+    b->sorted = true;
+}
+
+// Everything of one box's step up to the tree build, queued on box->st. With graphs on, the
+// kernels are captured once per particle-set parity and replayed; factors are written first.
+int step_front(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact& ef) {
+    hipLaunchKernelGGL(set_factors_kernel, dim3(1), dim3(64), 0, box->st, box->dfac, (double)ef.fact1, (double)ef.fact2,
+                       (double)ef.pfact, 7);
+    HIPCHK(ctx, hipGetLastError());
+    if (!ctx->use_graphs || box->graph_failed) {
+        RCHK(front_body(ctx, box, box->ev));
+        box->pm_timed = true;
+        HIPCHK(ctx, hipEventRecord(box->ev[10], box->st));
+        return LAGPM_OK;
+    }
+    // everything a capture must not do (plan creation and its check synchronise) happens first
+    RCHK(lagpm_pm_prepare(ctx, box));
+    HIPCHK(ctx, hipEventRecord(box->ev[8], box->st));
+    HIPCHK(ctx, hipEventRecord(box->ev[9], box->st));
+    hipGraphExec_t& ex = box->gfront[box->parity];
+    if (!ex) {
+        const box_ptrs before = save_ptrs(box);
+        hipGraph_t g = nullptr;
+        bool ok = hipStreamBeginCapture(box->st, hipStreamCaptureModeRelaxed) == hipSuccess;
+        int rc = ok ? front_body(ctx, box, nullptr) : LAGPM_E_HIP;
+        if (ok) ok = hipStreamEndCapture(box->st, &g) == hipSuccess && rc == LAGPM_OK && g;
+        if (ok) ok = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) (void)hipGraphDestroy(g);
+        const box_ptrs after = save_ptrs(box);
+        load_ptrs(box, before);
+        (void)hipGetLastError();
+        if (!ok) {  // run eagerly from now on for this box
+            ex = nullptr;
+            box->graph_failed = true;
+            RCHK(front_body(ctx, box, box->ev));
+            box->pm_timed = true;
+            HIPCHK(ctx, hipEventRecord(box->ev[10], box->st));
+            return LAGPM_OK;
+        }
+        (void)after;
+    }
+    HIPCHK(ctx, hipGraphLaunch(ex, box->st));
+    front_host_effects(box);
+    box->pm_timed = false;  // no phase events inside a graph
     HIPCHK(ctx, hipEventRecord(box->ev[10], box->st));
     return LAGPM_OK;
 }
 
 int drift_enqueue(lagpm_ctx* ctx, lagpm_box* box, double pfact, bool guarded) {
+    if (!guarded)  // a redone step: the factor of this step is already in dfac[2] otherwise
+        hipLaunchKernelGGL(set_factors_kernel, dim3(1), dim3(64), 0, box->st, box->dfac, 0.0, 0.0, pfact, 4);
     hipLaunchKernelGGL(drift_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx,
-                       box->vy, box->vz, box->np, pfact, (double)box->p.n,
+                       box->vy, box->vz, box->np, (const double*)box->dfac, (double)box->p.n,
                        guarded ? (const uint32_t*)(box->dsz + DSZ_OVERFLOW) : (const uint32_t*)nullptr);
     HIPCHK(ctx, hipGetLastError());
     box->sorted = false;
@@ -919,10 +1097,24 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
 
 }  // namespace
 
+void lagpm_box_graphs_release(lagpm_box* box) {
+    for (int q = 0; q < 2; q++)
+        if (box->gfront[q]) {
+            (void)hipGraphExecDestroy(box->gfront[q]);
+            box->gfront[q] = nullptr;
+        }
+}
+
 extern "C" {
 
 int lagpm_batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* anow, const double* astep) {
     return batch_step(ctx, boxes, nbox, anow, astep);
+}
+
+int lagpm_set_step_graphs(lagpm_ctx* ctx, int enable) {
+    if (!ctx) return LAGPM_E_INVAL;
+    ctx->use_graphs = enable != 0;
+    return LAGPM_OK;
 }
 
 int lagpm_kdk_step(lagpm_ctx* ctx, lagpm_box* box, double anow, double astep) {

@@ -15,22 +15,26 @@
 #define LAGPM_KEY_BITS 16    // Morton bits per axis; octree depth <= 16
 #define LAGPM_MAX_INDEX (1u << 27)  // walk lists pack (index << 5 | neighbour)
 
-struct fft_exec {
-    rocfft_execution_info info = nullptr;
-    void* work = nullptr;
-};
 struct fft_plans {
     rocfft_plan fwd = nullptr, inv = nullptr;
     size_t work_bytes = 0;
-    std::map<hipStream_t, fft_exec> exec;  // one execution info + work buffer per stream
 };
 
 // Force-law tables normalised to n = 1 (the law scales as 1/n^2), shared by every box of a
 // context that has the same law, softening and split scale.
+// Two bin layouts: the reference's 4096 bins uniform in r over [0, 6) (LAW_PMFIT, so reference
+// mode interpolates exactly as OST/Model4TreeForce.c:34-46), and for LAW_ERFC bins uniform in
+// log r^2, 256 per octave over r^2 in [2^-14, 64), indexed by the bits of the FP32 r^2 (no
+// square root in the pair loop; about ten times smaller interpolation error).
+#define LAGPM_LOG_SHIFT 15                 // 23 - 8 mantissa bits kept: 256 bins per octave
+#define LAGPM_LOG_BASE 28928u              // bits(2^-14) >> 15
+#define LAGPM_LOG_BINS 5120                // 20 octaves: r^2 < 64
 struct law_tables {
-    float2* tab0 = nullptr;   // (d0, s0) x 4096: particle term
-    float4* tab12 = nullptr;  // (d1, s1, d2, s2) x 4096: quadrupole terms
-    double* tabd = nullptr;   // 6 x 4096: d0 s0 d1 s1 d2 s2 (FP64 walk)
+    int logr2 = 0;            // 1: log r^2 bins, 0: uniform r bins
+    int nbins = 0;
+    float2* tab0 = nullptr;   // (d0, s0) per bin: particle term
+    float4* tab12 = nullptr;  // (d1, s1, d2, s2) per bin: quadrupole terms
+    double* tabd = nullptr;   // 6 per bin: d0 s0 d1 s1 d2 s2 (FP64 walk)
 };
 
 struct walk_box;  // device view of one box for the walk (tree.hip)
@@ -50,6 +54,7 @@ struct lagpm_ctx {
     walk_box* walk_boxes = nullptr;          // descriptor array of the last launch
     uint32_t* walk_prefix = nullptr;         // group prefix over the descriptors
     size_t walk_box_cap = 0;
+    bool use_graphs = true;                  // capture box step fronts in hipGraphs
     hipEvent_t ev[4] = {};
 };
 
@@ -73,14 +78,25 @@ struct lagpm_box {
     tree_tune tune;
     hipStream_t st = nullptr;  // stream this box's work is queued on (context stream or a pool stream)
 
-    // particle state, kept in the current (sorted after a sort) order
+    // particle state, kept in the current (sorted after a sort) order. A sort gathers into the
+    // other set and swaps the two, so the pointers alternate with period two (a captured step
+    // graph exists per parity).
     double *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
     uint32_t* id = nullptr;  // caller index of each particle
+    double* alt[6] = {};     // the other set of x, y, z, vx, vy, vz
+    uint32_t* id_alt = nullptr;
+    int parity = 0;
     // sort scratch
     uint64_t *key = nullptr, *key2 = nullptr;
     uint32_t *perm = nullptr, *perm2 = nullptr;
-    double* dtmp = nullptr;
-    uint32_t* utmp = nullptr;
+    // kick factors read by the kernels: [0] drag fact1, [1] PM kick factor, [2] drift pfact
+    double* dfac = nullptr;
+    // FFT execution info and work buffer owned by the box (a captured graph refers to them)
+    rocfft_execution_info fft_info = nullptr;
+    void* fft_work = nullptr;
+    // captured front half of a step (drag .. tree build), one per particle-set parity
+    hipGraphExec_t gfront[2] = {nullptr, nullptr};
+    bool graph_failed = false;
     uint32_t* scan_tmp = nullptr;  // scan / radix-sort scratch of this box
     size_t scan_tmp_bytes = 0;
 
@@ -154,10 +170,14 @@ int lagpm_radix_sort_pairs(lagpm_ctx* ctx, hipStream_t s, uint32_t* tmp, uint64_
 int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box);          // fills acc and rho
 int lagpm_pm_solve(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev = nullptr);  // rho -> phi (r2c, Green, c2r)
 int lagpm_pm_gradient(lagpm_ctx* ctx, lagpm_box* box);         // phi -> grad
-int lagpm_pm_interp(lagpm_ctx* ctx, lagpm_box* box, double fact);  // v += fact * F(grad)
+int lagpm_pm_interp(lagpm_ctx* ctx, lagpm_box* box);  // v += dfac[1] * F(grad)
 int lagpm_green_device(lagpm_ctx* ctx, hipStream_t s, int n, int prec, double efold, int deconvolve, void* spec);
 size_t lagpm_deposit_chunk_cap(int n, size_t np);
 int lagpm_fft_release(lagpm_ctx* ctx);
+int lagpm_box_fft_release(lagpm_box* box);
+int lagpm_pm_prepare(lagpm_ctx* ctx, lagpm_box* box);
+int lagpm_pm_power(lagpm_ctx* ctx, lagpm_box* box, double* pk_host, double* npk_host);
+void lagpm_box_graphs_release(lagpm_box* box);
 
 // ---- tree (tree.hip) ----
 int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box);

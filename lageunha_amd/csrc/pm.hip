@@ -299,7 +299,8 @@ __device__ inline void fda4_axis(double xp, int& near, int& s, float w[3]) {
 __global__ void interp_kick_f32(const double* __restrict__ px, const double* __restrict__ py,
                                 const double* __restrict__ pz, double* __restrict__ vx, double* __restrict__ vy,
                                 double* __restrict__ vz, size_t np, int n, const float4* __restrict__ grad,
-                                float fact2) {
+                                const double* __restrict__ fac) {
+    const float fact2 = (float)fac[1];
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     int ix, iy, iz, sx, sy, sz;
@@ -336,7 +337,8 @@ __global__ void interp_kick_f32(const double* __restrict__ px, const double* __r
 __global__ void interp_kick_f64(const double* __restrict__ px, const double* __restrict__ py,
                                 const double* __restrict__ pz, double* __restrict__ vx, double* __restrict__ vy,
                                 double* __restrict__ vz, size_t np, int n, const double4* __restrict__ grad,
-                                double fact2) {
+                                const double* __restrict__ fac) {
+    const double fact2 = fac[1];
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     int ix, iy, iz, sx, sy, sz;
@@ -369,12 +371,60 @@ unsigned grid_for(size_t work, int threads, unsigned cap = 65535u * 4) {
     return (unsigned)g;
 }
 
+// ---- one-time check of each new FFT plan pair on the device ------------------------------
+// mode 0: a single Fourier mode (1, 2, 3); mode 1: a hashed pseudo-random field in [-1, 1)
+template <typename T>
+__global__ void fftchk_fill(T* __restrict__ x, int n, int mode) {
+    const size_t m = (size_t)n * n * n;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = q % n, j = (q / n) % n, k = q / ((size_t)n * n);
+        if (mode == 0) {
+            const size_t ph = (i + 2 * j + 3 * k) % (size_t)n;
+            x[q] = (T)cos(2.0 * M_PI * (double)ph / (double)n);
+        } else {
+            uint64_t h = (uint64_t)q * 0x9E3779B97F4A7C15ull;
+            h ^= h >> 29; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 32;
+            x[q] = (T)((double)(h >> 11) * (1.0 / 4503599627370496.0) - 1.0);
+        }
+    }
+}
+// wave maximum of a non-negative value, then one atomic per wave on its bit pattern (the
+// order of non-negative doubles is the order of their bits)
+__device__ inline void atomic_max_nonneg(double* a, double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax((unsigned long long*)a, (unsigned long long)__double_as_longlong(v));
+}
+// max |spec - expected| / (n^3 / 2) for the single mode of fftchk_fill(mode 0)
+template <typename T>
+__global__ void fftchk_mode(const T* __restrict__ spec, int n, double* __restrict__ err) {
+    const size_t h = n / 2 + 1, K = h * n * n;
+    const double half = 0.5 * (double)n * n * n;
+    double e = 0;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < K; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = q % h, j = (q / h) % n, k = q / (h * n);
+        const double want = (i == 1 && j == 2 % (size_t)n && k == 3 % (size_t)n) ? half : 0.0;
+        e = fmax(e, fmax(fabs((double)spec[2 * q] - want), fabs((double)spec[2 * q + 1])) / half);
+    }
+    atomic_max_nonneg(err, e);
+}
+// max |a * sa - b|
+template <typename T>
+__global__ void fftchk_cmp(const T* __restrict__ a, const T* __restrict__ b, size_t cnt, double sa, double* __restrict__ err) {
+    double e = 0;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += (size_t)gridDim.x * blockDim.x)
+        e = fmax(e, fabs((double)a[q] * sa - (double)b[q]));
+    atomic_max_nonneg(err, e);
+}
+
 int fft_status(lagpm_ctx* ctx, rocfft_status s, const char* what) {
     if (s == rocfft_status_success) return LAGPM_OK;
     return lagpm_fail(ctx, LAGPM_E_FFT, std::string("rocFFT ") + what + " failed, status " + std::to_string((int)s));
 }
 
-int get_plans(lagpm_ctx* ctx, int n, int prec, fft_plans** out) {
+int make_exec(lagpm_ctx* ctx, fft_plans* fp, hipStream_t st, rocfft_execution_info* info, void** work);
+int check_plans(lagpm_ctx* ctx, int n, int prec, fft_plans* fp, hipStream_t st);
+
+int get_plans(lagpm_ctx* ctx, int n, int prec, hipStream_t st, fft_plans** out) {
     auto key = std::make_pair(n, prec);
     auto it = ctx->plans.find(key);
     if (it != ctx->plans.end()) {
@@ -393,41 +443,202 @@ int get_plans(lagpm_ctx* ctx, int n, int prec, fft_plans** out) {
     RCHK(fft_status(ctx, rocfft_plan_get_work_buffer_size(fp.inv, &w2), "work_size"));
     fp.work_bytes = w1 > w2 ? w1 : w2;
     auto res = ctx->plans.emplace(key, fp);
-    *out = &res.first->second;
+    fft_plans* mine = &res.first->second;
+    const int rc = check_plans(ctx, n, prec, mine, st);
+    if (rc != LAGPM_OK) {
+        rocfft_plan_destroy(mine->fwd);
+        rocfft_plan_destroy(mine->inv);
+        ctx->plans.erase(key);
+        return rc;
+    }
+    *out = mine;
     return LAGPM_OK;
 }
 
-// Execution info (stream + work buffer) of a plan pair for one stream: boxes transformed
-// concurrently on different streams never share a work buffer.
-int get_exec(lagpm_ctx* ctx, fft_plans* fp, hipStream_t st, rocfft_execution_info* out) {
-    auto it = fp->exec.find(st);
-    if (it != fp->exec.end()) {
-        *out = it->second.info;
-        return LAGPM_OK;
+// The library trusts a plan pair only after it has transformed a single Fourier mode exactly,
+// given the same spectrum twice for the same input, and inverted a pseudo-random field.
+template <typename T>
+int check_plans_t(lagpm_ctx* ctx, int n, fft_plans* fp, hipStream_t st, double tol) {
+    const size_t m = (size_t)n * n * n, K = (size_t)(n / 2 + 1) * n * n;
+    rocfft_execution_info info = nullptr;
+    void* work = nullptr;
+    RCHK(make_exec(ctx, fp, st, &info, &work));
+    T *x = nullptr, *spec = nullptr, *spec2 = nullptr, *back = nullptr;
+    double* err = nullptr;
+    int rc = LAGPM_OK;
+    if (hipMalloc(&x, m * sizeof(T)) != hipSuccess || hipMalloc(&back, m * sizeof(T)) != hipSuccess ||
+        hipMalloc(&spec, 2 * K * sizeof(T)) != hipSuccess || hipMalloc(&spec2, 2 * K * sizeof(T)) != hipSuccess ||
+        hipMalloc(&err, 4 * sizeof(double)) != hipSuccess)
+        rc = lagpm_fail(ctx, LAGPM_E_NOMEM, "FFT check buffers");
+    double e[4] = {0, 0, 0, 0};
+    const unsigned g = grid_for(m, 256);
+    auto fwd = [&](T* in, T* out) {
+        void* ip[1] = {in};
+        void* op[1] = {out};
+        return fft_status(ctx, rocfft_execute(fp->fwd, ip, op, info), "execute(r2c check)");
+    };
+    if (rc == LAGPM_OK && hipMemsetAsync(err, 0, 4 * sizeof(double), st) != hipSuccess) rc = LAGPM_E_HIP;
+    if (rc == LAGPM_OK) {
+        hipLaunchKernelGGL(fftchk_fill<T>, dim3(g), dim3(256), 0, st, x, n, 0);
+        rc = fwd(x, spec);
     }
-    fft_exec ex;
-    RCHK(fft_status(ctx, rocfft_execution_info_create(&ex.info), "execution_info_create"));
+    if (rc == LAGPM_OK) {
+        hipLaunchKernelGGL(fftchk_mode<T>, dim3(g), dim3(256), 0, st, spec, n, err);
+        hipLaunchKernelGGL(fftchk_fill<T>, dim3(g), dim3(256), 0, st, x, n, 1);
+        rc = fwd(x, spec);
+    }
+    if (rc == LAGPM_OK) rc = fwd(x, spec2);
+    if (rc == LAGPM_OK) {
+        hipLaunchKernelGGL(fftchk_cmp<T>, dim3(g), dim3(256), 0, st, spec, spec2, 2 * K, 1.0, err + 1);
+        void* ip[1] = {spec2};
+        void* op[1] = {back};
+        rc = fft_status(ctx, rocfft_execute(fp->inv, ip, op, info), "execute(c2r check)");
+    }
+    if (rc == LAGPM_OK) {
+        hipLaunchKernelGGL(fftchk_cmp<T>, dim3(g), dim3(256), 0, st, back, x, m, 1.0 / (double)m, err + 2);
+        if (hipMemcpyAsync(e, err, sizeof(e), hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+            rc = lagpm_fail(ctx, LAGPM_E_HIP, "FFT check readback");
+    }
+    (void)hipFree(x); (void)hipFree(back); (void)hipFree(spec); (void)hipFree(spec2); (void)hipFree(err);
+    if (info) rocfft_execution_info_destroy(info);
+    if (work) (void)hipFree(work);
+    if (rc != LAGPM_OK) return rc;
+    if (!(e[0] <= tol && e[1] == 0.0 && e[2] <= tol))
+        return lagpm_fail(ctx, LAGPM_E_FFT,
+                          "rocFFT plan for n=" + std::to_string(n) + " failed its check (single mode error " + std::to_string(e[0]) +
+                              ", repeat difference " + std::to_string(e[1]) + ", round trip error " + std::to_string(e[2]) +
+                              "); this mesh size cannot be used with this rocFFT");
+    return LAGPM_OK;
+}
+
+int check_plans(lagpm_ctx* ctx, int n, int prec, fft_plans* fp, hipStream_t st) {
+    return prec == LAGPM_PREC_FP64 ? check_plans_t<double>(ctx, n, fp, st, 1e-9) : check_plans_t<float>(ctx, n, fp, st, 1e-4);
+}
+
+// Execution info + work buffer for a plan pair on a stream (caller owns both).
+int make_exec(lagpm_ctx* ctx, fft_plans* fp, hipStream_t st, rocfft_execution_info* info, void** work) {
+    *info = nullptr;
+    *work = nullptr;
+    RCHK(fft_status(ctx, rocfft_execution_info_create(info), "execution_info_create"));
     if (fp->work_bytes) {
-        HIPCHK(ctx, hipMalloc(&ex.work, fp->work_bytes));
-        RCHK(fft_status(ctx, rocfft_execution_info_set_work_buffer(ex.info, ex.work, fp->work_bytes), "set_work_buffer"));
+        HIPCHK(ctx, hipMalloc(work, fp->work_bytes));
+        RCHK(fft_status(ctx, rocfft_execution_info_set_work_buffer(*info, *work, fp->work_bytes), "set_work_buffer"));
     }
-    RCHK(fft_status(ctx, rocfft_execution_info_set_stream(ex.info, st), "set_stream"));
-    fp->exec.emplace(st, ex);
-    *out = ex.info;
+    RCHK(fft_status(ctx, rocfft_execution_info_set_stream(*info, st), "set_stream"));
     return LAGPM_OK;
 }
 
 }  // namespace
+
+// ---- CorrMeasure: binned power spectrum of the density contrast, Cosmos/Correlation.c:33-77 ----
+// One thread per half-complex mode with i < n/2. Each block accumulates its modes' linear bin
+// weights in LDS and adds them to the global bins once.
+__device__ inline double tsc_sinc3(double m, int n) {
+    if (m == 0.0) return 1.0;
+    const double x = M_PI * m / (double)n;
+    const double s = sin(x) / x;
+    return s * s * s;
+}
+template <typename T>
+__global__ void __launch_bounds__(256) pk_bin_kernel(const T* __restrict__ spec, int n, double* __restrict__ pk,
+                                                     double* __restrict__ npk) {
+    extern __shared__ double sh[];  // 2 * (n + 1)
+    double* lpk = sh;
+    double* lnpk = sh + n + 1;
+    for (int b = threadIdx.x; b <= n; b += blockDim.x) lpk[b] = lnpk[b] = 0.0;
+    __syncthreads();
+    const int h = n / 2 + 1, ngh = n / 2;
+    const size_t modes = (size_t)ngh * n * n;
+    const double rngm6 = 1.0 / ((double)n * n * n * (double)n * n * n);
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < modes; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % ngh);
+        const size_t r = q / ngh;
+        const int j = (int)(r % n), k = (int)(r / n);
+        const int wj = j > ngh ? j - n : j, wk = k > ngh ? k - n : k;
+        const double w = tsc_sinc3((double)i, n) * tsc_sinc3((double)wj, n) * tsc_sinc3((double)wk, n);
+        const size_t e = (size_t)i + (size_t)h * ((size_t)j + (size_t)n * k);
+        const double re = (double)spec[2 * e], im = (double)spec[2 * e + 1];
+        const double d = (re * re + im * im) * rngm6 / (w * w);
+        const double wave = sqrt((double)i * i + (double)wj * wj + (double)wk * wk);
+        const int b1 = (int)wave;
+        double t = wave - (double)b1;
+        const int b2 = b1 + (t > 0 ? 1 : -1);
+        t = fabs(t);
+        if (b1 <= n) {
+            atomicAdd(&lpk[b1], (1 - t) * d);
+            atomicAdd(&lnpk[b1], 1 - t);
+        }
+        if (b2 >= 0 && b2 <= n) {
+            atomicAdd(&lpk[b2], t * d);
+            atomicAdd(&lnpk[b2], t);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b <= n; b += blockDim.x) {
+        if (lnpk[b] != 0.0) {
+            atomicAdd(&pk[b], lpk[b]);
+            atomicAdd(&npk[b], lnpk[b]);
+        }
+    }
+}
+
+// Plans (with their one-time check) and the box's execution info exist before a step is
+// captured: both synchronise or allocate, which a capture must not contain.
+int lagpm_pm_prepare(lagpm_ctx* ctx, lagpm_box* box) {
+    fft_plans* fp = nullptr;
+    RCHK(get_plans(ctx, box->p.n, box->p.precision, box->st, &fp));
+    if (!box->fft_info) RCHK(make_exec(ctx, fp, box->st, &box->fft_info, &box->fft_work));
+    return LAGPM_OK;
+}
+
+// r2c of the resident density (rho) into box->spec, then the binned |delta_k|^2 / n^6
+int lagpm_pm_power(lagpm_ctx* ctx, lagpm_box* box, double* pk_host, double* npk_host) {
+    const int n = box->p.n;
+    RCHK(lagpm_pm_prepare(ctx, box));
+    fft_plans* fp = nullptr;
+    RCHK(get_plans(ctx, n, box->p.precision, box->st, &fp));
+    RCHK(fft_status(ctx, rocfft_execution_info_set_stream(box->fft_info, box->st), "set_stream"));
+    void* in[1] = {box->rho};
+    void* out[1] = {box->spec};
+    RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, out, box->fft_info), "execute(r2c)"));
+    double* d = nullptr;
+    HIPCHK(ctx, hipMalloc(&d, 2 * (size_t)(n + 1) * sizeof(double)));
+    int rc = LAGPM_OK;
+    if (hipMemsetAsync(d, 0, 2 * (size_t)(n + 1) * sizeof(double), box->st) != hipSuccess) rc = LAGPM_E_HIP;
+    const size_t modes = (size_t)(n / 2) * n * n;
+    const size_t shm = 2 * (size_t)(n + 1) * sizeof(double);
+    if (rc == LAGPM_OK) {
+        if (box->p.precision == LAGPM_PREC_FP64)
+            hipLaunchKernelGGL(pk_bin_kernel<double>, dim3(grid_for(modes, 256, 4096)), dim3(256), shm, box->st,
+                               (const double*)box->spec, n, d, d + n + 1);
+        else
+            hipLaunchKernelGGL(pk_bin_kernel<float>, dim3(grid_for(modes, 256, 4096)), dim3(256), shm, box->st,
+                               (const float*)box->spec, n, d, d + n + 1);
+        if (hipGetLastError() != hipSuccess) rc = LAGPM_E_HIP;
+    }
+    if (rc == LAGPM_OK && (hipMemcpyAsync(pk_host, d, (n + 1) * sizeof(double), hipMemcpyDeviceToHost, box->st) != hipSuccess ||
+                           hipMemcpyAsync(npk_host, d + n + 1, (n + 1) * sizeof(double), hipMemcpyDeviceToHost, box->st) !=
+                               hipSuccess ||
+                           hipStreamSynchronize(box->st) != hipSuccess))
+        rc = LAGPM_E_HIP;
+    (void)hipFree(d);
+    if (rc != LAGPM_OK) return lagpm_fail(ctx, rc, "power spectrum kernel failed");
+    return LAGPM_OK;
+}
+
+int lagpm_box_fft_release(lagpm_box* box) {
+    if (box->fft_info) rocfft_execution_info_destroy(box->fft_info);
+    if (box->fft_work) (void)hipFree(box->fft_work);
+    box->fft_info = nullptr;
+    box->fft_work = nullptr;
+    return LAGPM_OK;
+}
 
 int lagpm_fft_release(lagpm_ctx* ctx) {
     for (auto& kv : ctx->plans) {
         fft_plans& fp = kv.second;
         if (fp.fwd) rocfft_plan_destroy(fp.fwd);
         if (fp.inv) rocfft_plan_destroy(fp.inv);
-        for (auto& ex : fp.exec) {
-            if (ex.second.info) rocfft_execution_info_destroy(ex.second.info);
-            if (ex.second.work) (void)hipFree(ex.second.work);
-        }
     }
     ctx->plans.clear();
     return LAGPM_OK;
@@ -505,9 +716,11 @@ int lagpm_green_device(lagpm_ctx* ctx, hipStream_t st, int n, int prec, double e
 
 int lagpm_pm_solve(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     fft_plans* fp = nullptr;
-    RCHK(get_plans(ctx, box->p.n, box->p.precision, &fp));
-    rocfft_execution_info info = nullptr;
-    RCHK(get_exec(ctx, fp, box->st, &info));
+    RCHK(get_plans(ctx, box->p.n, box->p.precision, box->st, &fp));
+    if (!box->fft_info) RCHK(make_exec(ctx, fp, box->st, &box->fft_info, &box->fft_work));
+    // the box's execution info follows the stream its work is queued on
+    RCHK(fft_status(ctx, rocfft_execution_info_set_stream(box->fft_info, box->st), "set_stream"));
+    rocfft_execution_info info = box->fft_info;
     void* in[1] = {box->rho};
     void* out[1] = {box->spec};
     RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, out, info), "execute(r2c)"));
@@ -534,15 +747,15 @@ int lagpm_pm_gradient(lagpm_ctx* ctx, lagpm_box* box) {
     return LAGPM_OK;
 }
 
-int lagpm_pm_interp(lagpm_ctx* ctx, lagpm_box* box, double fact) {
+int lagpm_pm_interp(lagpm_ctx* ctx, lagpm_box* box) {
     if (box->np == 0) return LAGPM_OK;
     const unsigned nblk = (unsigned)((box->np + 255) / 256);
     if (box->p.precision == LAGPM_PREC_FP64)
         hipLaunchKernelGGL(interp_kick_f64, dim3(nblk), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx, box->vy,
-                           box->vz, box->np, box->p.n, (const double4*)box->grad, fact);
+                           box->vz, box->np, box->p.n, (const double4*)box->grad, (const double*)box->dfac);
     else
         hipLaunchKernelGGL(interp_kick_f32, dim3(nblk), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx, box->vy,
-                           box->vz, box->np, box->p.n, (const float4*)box->grad, (float)fact);
+                           box->vz, box->np, box->p.n, (const float4*)box->grad, (const double*)box->dfac);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }

@@ -36,6 +36,41 @@ constexpr int WALK_MAX_ITERS = 1 << 20;  // per-group bound on 64-node pop round
 // mass (the reference's nodesize); larger ones use the bound max(|d_child| + rmax_child).
 constexpr int RMAX_EXACT_MAX = 512;
 constexpr float RAN_F = (float)(6.0 / 4096.0);
+constexpr int TAB_LDS = LAGPM_LOG_BINS > 4096 ? LAGPM_LOG_BINS : 4096;
+
+// Table bin and offset of a pair at squared distance r2 (see law_tables): uniform r bins need
+// the square root; log r^2 bins take the exponent and 8 mantissa bits of r2.
+template <bool LOGT>
+__device__ inline int table_bin(float r2, float& t) {
+    if constexpr (LOGT) {
+        // clamping r2 into the table's range first keeps the bin in [0, LAGPM_LOG_BINS) with no
+        // integer clamps (pairs beyond the cutoff are discarded by the caller)
+        const float r2c = __builtin_amdgcn_fmed3f(r2, 0x1p-14f, 63.99f);
+        const uint32_t bits = __float_as_uint(r2c);
+        t = r2c - __uint_as_float(bits & ~((1u << LAGPM_LOG_SHIFT) - 1u));
+        return (int)__builtin_amdgcn_ubfe(bits, LAGPM_LOG_SHIFT, 32 - LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE;
+    } else {
+        const float r = __builtin_amdgcn_sqrtf(r2);
+        const int nt = min((int)(r * (1.0f / RAN_F)), 4095);
+        t = r - (float)nt * RAN_F;
+        return nt;
+    }
+}
+template <bool LOGT>
+__device__ inline int table_bin64(double r2, double& t) {
+    if constexpr (LOGT) {
+        const uint32_t bits = __float_as_uint((float)r2);
+        const int b = (int)(bits >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE;
+        t = r2 - (double)__uint_as_float(bits & ~((1u << LAGPM_LOG_SHIFT) - 1u));
+        return min(max(b, 0), LAGPM_LOG_BINS - 1);
+    } else {
+        const double ran = 6.0 / 4096.0;
+        const double r = sqrt(r2);
+        const int nt = min((int)(r / ran), 4095);
+        t = r - nt * ran;
+        return nt;
+    }
+}
 
 __device__ inline uint64_t spread3(uint32_t v) {
     uint64_t x = v & 0xffffu;
@@ -476,18 +511,20 @@ __global__ void walk_prefix_kernel(const walk_box* __restrict__ B, int nbox, uin
     *ticket = 0ull;
 }
 
+template <bool LOGT>
 __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* __restrict__ B, const uint32_t* __restrict__ gprefix,
                                                                 int nbox, const float2* __restrict__ tab0,
                                                                 const float4* __restrict__ tab12, int* __restrict__ stacks,
                                                                 unsigned long long* __restrict__ ticket_ctr) {
     // One 1024-thread workgroup per CU: the 16 waves share one LDS copy of the particle force
     // table (32 KB) next to their private interaction lists.
-    __shared__ float2 s_tab0[4096];
+    __shared__ float2 s_tab0[TAB_LDS];
     __shared__ uint32_t s_ppidx[WALK_WAVES][PP_CAP];
     __shared__ int s_pcidx[WALK_WAVES][PC_CAP];
     __shared__ float4 s_pp[WALK_WAVES][64];
     __shared__ float4 s_pc[WALK_WAVES][3][64];
-    for (int i = threadIdx.x; i < 4096; i += WALK_THREADS) s_tab0[i] = tab0[i];
+    constexpr int NBIN = LOGT ? LAGPM_LOG_BINS : 4096;
+    for (int i = threadIdx.x; i < NBIN; i += WALK_THREADS) s_tab0[i] = tab0[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     uint32_t* ppidx = s_ppidx[wid];
@@ -578,11 +615,10 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 float dx = xt.x - s.x, dy = xt.y - s.y, dz = xt.z - s.z;
                 float r2 = dx * dx + dy * dy + dz * dz;
                 if (!__any(r2 <= A.rc2)) continue;  // wave-uniform skip of far sources
-                float r = __builtin_amdgcn_sqrtf(r2);
-                int nt = min((int)(r * (1.0f / RAN_F)), 4095);
-                float t = r - (float)nt * RAN_F;
+                float t;
+                const int nt = table_bin<LOGT>(r2, t);
                 float2 tb = s_tab0[nt];
-                float f = (tb.x + tb.y * t) * s.w;
+                float f = tb.x + tb.y * t;  // particle masses are 1 here: the box mass is in fact
                 f = (r2 <= A.rc2) ? f : 0.f;
                 ax += dx * f; ay += dy * f; az += dz * f;
             }
@@ -624,9 +660,8 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
                 float r2 = dx * dx + dy * dy + dz * dz;
                 if (!__any(r2 <= A.rc2)) continue;  // wave-uniform skip of far nodes
-                float r = __builtin_amdgcn_sqrtf(r2);
-                int nt = min((int)(r * (1.0f / RAN_F)), 4095);
-                float t = r - (float)nt * RAN_F;
+                float t;
+                const int nt = table_bin<LOGT>(r2, t);
                 float2 t0 = s_tab0[nt];
                 float4 t12 = tab12[nt];
                 float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
@@ -789,6 +824,7 @@ struct walk64_args {
     int naccept;
 };
 
+template <bool LOGT>
 __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.np || A.dsz[DSZ_OVERFLOW]) return;
@@ -809,7 +845,6 @@ __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
     }
     double ax = 0, ay = 0, az = 0;
     unsigned long long npp = 0, npc = 0;
-    const double ran = 6.0 / 4096.0;
     bool overflow = false;
     while (sp > 0) {
         int e = stack[--sp];
@@ -832,9 +867,8 @@ __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
             double dx = px0 - qx, dy = py0 - qy, dz = pz0 - qz;
             double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 <= A.rc2) {
-                double r = sqrt(r2);
-                int nt = min((int)(r / ran), 4095);
-                double t = r - nt * ran;
+                double t;
+                const int nt = table_bin64<LOGT>(r2, t);
                 const double* tb = A.tab + 6 * nt;
                 double f1 = tb[0] + tb[1] * t, f2 = tb[2] + tb[3] * t, f3 = tb[4] + tb[5] * t;
                 double qx1 = m[4] * dx + m[7] * dy + m[8] * dz;
@@ -852,10 +886,9 @@ __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
                 double dx = px0 - (A.x[p] + sx), dy = py0 - (A.y[p] + sy), dz = pz0 - (A.z[p] + sz);
                 double r2 = dx * dx + dy * dy + dz * dz;
                 if (r2 <= A.rc2) {
-                    double r = sqrt(r2);
-                    int nt = min((int)(r / ran), 4095);
-                    double t = r - nt * ran;
-                    double f = (A.tab[6 * nt] + A.tab[6 * nt + 1] * t) * A.mass;
+                    double t;
+                    const int nt = table_bin64<LOGT>(r2, t);
+                    double f = A.tab[6 * nt] + A.tab[6 * nt + 1] * t;  // unit particle mass
                     ax += dx * f; ay += dy * f; az += dz * f;
                 }
             }
@@ -903,6 +936,7 @@ size_t lagpm_tree_node_cap_guess(size_t np, int mcell) {
 
 int lagpm_tree_alloc(lagpm_ctx* ctx, lagpm_box* box, size_t cap) {
     const bool f64 = box->p.precision == LAGPM_PREC_FP64;
+    lagpm_box_graphs_release(box);  // captured steps refer to the old arrays
     const size_t gcap = cap + box->np / 64 + 1;
     RCHK(regrow(ctx, &box->nd_info, cap));
     RCHK(regrow(ctx, &box->nd_parent, cap));
@@ -946,14 +980,15 @@ int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box) {
     double** arrs[6] = {&box->x, &box->y, &box->z, &box->vx, &box->vy, &box->vz};
     for (int a = 0; a < 6; a++) {
         hipLaunchKernelGGL(gather_kernel<double>, dim3(blocks_for(np, 256)), dim3(256), 0, st, *arrs[a], box->perm,
-                           np, box->dtmp);
+                           np, box->alt[a]);
         HIPCHK(ctx, hipGetLastError());
-        std::swap(*arrs[a], box->dtmp);
+        std::swap(*arrs[a], box->alt[a]);
     }
     hipLaunchKernelGGL(gather_kernel<uint32_t>, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->id, box->perm, np,
-                       box->utmp);
+                       box->id_alt);
     HIPCHK(ctx, hipGetLastError());
-    std::swap(box->id, box->utmp);
+    std::swap(box->id, box->id_alt);
+    box->parity ^= 1;
     box->sorted = true;
     return LAGPM_OK;
 }
@@ -971,8 +1006,9 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     if (np == 0) return LAGPM_OK;
     if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
     if (!f64) {
-        hipLaunchKernelGGL(pack_pp_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->x, box->y, box->z, np,
-                           (float)box->p.mass, box->pp);
+        // the tree works in units of the particle mass; the walk multiplies by it with the kick
+        hipLaunchKernelGGL(pack_pp_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->x, box->y, box->z, np, 1.0f,
+                           box->pp);
         HIPCHK(ctx, hipGetLastError());
     }
     const size_t ncell = (size_t)box->mcell * box->mcell * box->mcell;
@@ -993,7 +1029,7 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     for (int l = LAGPM_KEY_BITS; l >= L; l--) {
         if (f64)
             hipLaunchKernelGGL(moments_f64_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->x, box->y, box->z,
-                               box->p.mass, box->dsz, cap, l, box->nd_info, box->nd_child, box->nd_d);
+                               1.0, box->dsz, cap, l, box->nd_info, box->nd_child, box->nd_d);
         else
             hipLaunchKernelGGL(moments_f32_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->pp, box->dsz, cap, l,
                                box->nd_info, box->nd_child, box->nd_com, box->nd_q0, box->nd_q1);
@@ -1040,7 +1076,7 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
         A.rc2 = (float)(box->p.rsphere * box->p.rsphere);
         A.rcut = (float)box->p.rsphere;
         A.inv_theta2 = (float)(th > 0 ? 1.0 / (th * th) : 1e30);
-        A.fact = (float)(facts[b] / ((double)box->p.n * (double)box->p.n));
+        A.fact = (float)(facts[b] * box->p.mass / ((double)box->p.n * (double)box->p.n));
     }
     // pageable source: the copy is staged before the call returns, so hb may go out of scope
     HIPCHK(ctx, hipMemcpyAsync(ctx->walk_boxes, hb.data(), hb.size() * sizeof(walk_box), hipMemcpyHostToDevice, ctx->stream));
@@ -1048,8 +1084,12 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
                        ctx->walk_ticket);
     HIPCHK(ctx, hipGetLastError());
     const unsigned blocks = (unsigned)(ctx->walk_stack_entries / ((size_t)STACK_MAX * WALK_WAVES));
-    hipLaunchKernelGGL(walk_f32_kernel, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes, ctx->walk_prefix,
-                       nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
+    if (tab->logr2)
+        hipLaunchKernelGGL(walk_f32_kernel<true>, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes,
+                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
+    else
+        hipLaunchKernelGGL(walk_f32_kernel<false>, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes,
+                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }
@@ -1064,10 +1104,13 @@ int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact) {
     A.vx = box->vx; A.vy = box->vy; A.vz = box->vz; A.counters = box->counters; A.dsz = box->dsz; A.np = box->np;
     A.mcell = box->mcell; A.box = (double)box->p.n; A.cellw = cellw; A.rc2 = box->p.rsphere * box->p.rsphere;
     A.rcut = box->p.rsphere; A.inv_theta2 = th > 0 ? 1.0 / (th * th) : 1e30;
-    A.fact = fact / ((double)box->p.n * (double)box->p.n);
+    A.fact = fact * box->p.mass / ((double)box->p.n * (double)box->p.n);
     A.naccept = LAGPM_NACCEPT;
     if (box->np == 0) return LAGPM_OK;
-    hipLaunchKernelGGL(walk_f64_kernel, dim3(blocks_for(box->np, 128)), dim3(128), 0, box->st, A);
+    if (box->tab->logr2)
+        hipLaunchKernelGGL(walk_f64_kernel<true>, dim3(blocks_for(box->np, 128)), dim3(128), 0, box->st, A);
+    else
+        hipLaunchKernelGGL(walk_f64_kernel<false>, dim3(blocks_for(box->np, 128)), dim3(128), 0, box->st, A);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }

@@ -99,6 +99,7 @@ SIGNATURES = {
     "lagpm_set_potential": (ctypes.c_int, [_P, _P, _F]),
     "lagpm_get_gradient": (ctypes.c_int, [_P, _P, _F]),
     "lagpm_force_spline": (ctypes.c_int, [ctypes.POINTER(Params), _D, _D]),
+    "lagpm_power_spectrum": (ctypes.c_int, [_P, _P, ctypes.c_double, _D, _D, _D, _D, _D]),
     "lagpm_treecorrection": (ctypes.c_int, [_P, _P, ctypes.POINTER(EvolFact)]),
     "lagpm_tree_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(TreeStats)]),
     "lagpm_box_timings": (ctypes.c_int, [_P, _P, ctypes.POINTER(Timings)]),
@@ -108,6 +109,7 @@ SIGNATURES = {
     "lagpm_drift": (ctypes.c_int, [_P, _P, ctypes.c_double]),
     "lagpm_kdk_step": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double]),
     "lagpm_batch_step": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int, _D, _D]),
+    "lagpm_set_step_graphs": (ctypes.c_int, [_P, ctypes.c_int]),
 }
 
 _lib = None
@@ -173,6 +175,9 @@ class Context:
     def check(self, rc):
         if rc != 0:
             raise LagpmError(rc, self.lib.lagpm_last_error(self.h).decode())
+
+    def set_step_graphs(self, enable: bool):
+        self.check(self.lib.lagpm_set_step_graphs(self.h, 1 if enable else 0))
 
     def synchronize(self):
         self.check(self.lib.lagpm_synchronize(self.h))
@@ -299,6 +304,14 @@ class Box:
 
     def drift(self, pfact: float):
         self.ctx.check(self.lib.lagpm_drift(self.ctx.h, self.h, pfact))
+
+    def power_spectrum(self, boxsize: float):
+        """CorrMeasure of the current particles: (pk, npk, cor, sigma8, slope), n+1 bins."""
+        pk, npk, cor = np.empty(self.n + 1), np.empty(self.n + 1), np.empty(self.n + 1)
+        s8, sl = ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(self.lib.lagpm_power_spectrum(self.ctx.h, self.h, boxsize, _dp(pk), _dp(npk), _dp(cor),
+                                                     ctypes.byref(s8), ctypes.byref(sl)))
+        return pk, npk, cor, s8.value, sl.value
 
     def kdk_step(self, anow: float, astep: float):
         """One GOTPM global step (drag, PM kick, tree kick, drift) at scale factor anow."""

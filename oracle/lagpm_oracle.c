@@ -1038,3 +1038,62 @@ void orc_evolfactor(double omep, double omeplam, double amax, int n, double anow
     out->fact2_push = orc_fact2(omep, omeplam, amax, n, anow, anow + 0.5 * astep);
     out->fact2_pull = orc_fact2(omep, omeplam, amax, n, anow - 0.5 * astep, anow);
 }
+
+/* ---- CorrMeasure: power spectrum, sigma_8 and correlation function, Cosmos/Correlation.c:13-116 ----
+ * spec: r2c spectrum of the density contrast (complex interleaved, layout i + h*(j + n*k), i <= n/2).
+ * Modes with i < n/2 (the reference's ngh), all j, k; |delta_k|^2 / n^6 divided by the squared TSC
+ * window; each mode's weight is split linearly between the bins floor(|k|) and its neighbour
+ * (Correlation.c:55-75). pk/npk/cor: n+1 entries; pk[i] is the bin mean after normalisation
+ * (:84-87). sigma_8 with the top-hat of radius 8 Mpc/h (rth = 8/boxsize * n cells, :80-91);
+ * cor[i] = (2n/i) sum_k sin(2 pi k i / n) k pk[k] (:93-99); slope = dlog cor / dlog r between
+ * r = 2 and 3 cells (:100-103), 0 when undefined. */
+static double tsc_sinc3(double m, int n) {
+    if (m == 0.0) return 1.0;
+    double x = M_PI * m / (double)n;
+    double s = sin(x) / x;
+    return s * s * s;
+}
+
+void orc_power_spectrum(int n, const double* spec, double boxsize, double* pk, double* npk, double* cor,
+                        double* sigma8, double* slope) {
+    const int h = n / 2 + 1, ngh = n / 2;
+    const double rngm6 = 1.0 / ((double)n * n * n * (double)n * n * n);
+    for (int i = 0; i <= n; i++) pk[i] = npk[i] = cor[i] = 0.0;
+    for (int k = 0; k < n; k++) {
+        int wk = k > ngh ? k - n : k;
+        double cz = tsc_sinc3((double)wk, n);
+        for (int j = 0; j < n; j++) {
+            int wj = j > ngh ? j - n : j;
+            double cy = tsc_sinc3((double)wj, n);
+            for (int i = 0; i < ngh; i++) {
+                double cx = tsc_sinc3((double)i, n);
+                double wave = sqrt((double)i * i + (double)wj * wj + (double)wk * wk);
+                size_t q = (size_t)i + (size_t)h * ((size_t)j + (size_t)n * k);
+                double re = spec[2 * q], im = spec[2 * q + 1];
+                double w = cx * cy * cz;
+                double d = (re * re + im * im) * rngm6 / (w * w);
+                long b1 = (long)wave;
+                double t = wave - (double)b1;
+                long b2 = b1 + (t > 0 ? 1 : -1);
+                t = fabs(t);
+                if (b1 <= n) { pk[b1] += (1 - t) * d; npk[b1] += 1 - t; }
+                if (b2 >= 0 && b2 <= n) { pk[b2] += t * d; npk[b2] += t; }
+            }
+        }
+    }
+    const double rth = 8.0 / boxsize * n, fact = 2 * M_PI * rth / (double)n;
+    double var = 0;
+    for (int i = 1; i <= n; i++) {
+        pk[i] = npk[i] > 0 ? pk[i] / npk[i] : 0.0;
+        double x = fact * i, u = sin(x) - x * cos(x);
+        var += pk[i] * (double)i * i * u * u / pow(x, 6);
+    }
+    *sigma8 = sqrt(4 * M_PI * var * 9.0);
+    for (int i = 1; i <= n; i++) {
+        double sep = 2 * M_PI * i / (double)n, c = 0;
+        for (int k = 1; k <= n; k++) c += sin(k * sep) * k * pk[k];
+        cor[i] = 2.0 * n * c / i;
+    }
+    *slope = (cor[3] > 0 && cor[2] > 0) ? (log(cor[3]) - log(cor[2])) / (log(4.0) - log(3.0)) : 0.0;
+}
+

@@ -94,4 +94,8 @@ double orc_fact2(double omep, double omeplam, double amax, int n, double a1, dou
 void   orc_evolfactor(double omep, double omeplam, double amax, int n, double anow,
                       double astep, orc_evolfact* out);
 
+/* CorrMeasure (Cosmos/Correlation.c:13-116) on a density-contrast spectrum. */
+void   orc_power_spectrum(int n, const double* spec, double boxsize, double* pk /*n+1*/, double* npk /*n+1*/,
+                          double* cor /*n+1*/, double* sigma8, double* slope);
+
 #endif

@@ -61,6 +61,7 @@ def lib():
         "orc_hofez": (d, [d, d, d, d]), "orc_ap": (d, [d, d, d, d]),
         "orc_fact1": (d, [d, d, d, d, d]), "orc_fact2": (d, [d, d, d, i, d, d]),
         "orc_evolfactor": (None, [d, d, d, i, d, d, ctypes.POINTER(OrcEvolFact)]),
+        "orc_power_spectrum": (None, [i, _D, d, _D, _D, _D, _D, _D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -265,3 +266,13 @@ def fact1(omep, omeplam, amax, a1, a2):
 
 def ap(omep, omeplam, amax, a):
     return lib().orc_ap(omep, omeplam, amax, a)
+
+
+def power_spectrum(n, spec, boxsize):
+    """CorrMeasure restatement (Cosmos/Correlation.c:13-116) on an r2c spectrum of shape
+    (n, n, n//2+1) [k][j][i]: returns pk, npk, cor (n+1 each), sigma8, slope."""
+    sp = np.ascontiguousarray(np.asarray(spec, dtype=np.complex128)).view(np.float64).ravel()
+    pk, npk, cor = np.empty(n + 1), np.empty(n + 1), np.empty(n + 1)
+    s8, sl = ctypes.c_double(), ctypes.c_double()
+    lib().orc_power_spectrum(n, _dp(sp), boxsize, _dp(pk), _dp(npk), _dp(cor), ctypes.byref(s8), ctypes.byref(sl))
+    return pk, npk, cor, s8.value, sl.value

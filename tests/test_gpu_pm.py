@@ -179,3 +179,23 @@ def test_eunha_layout_roundtrip(ctx):
         b.download_eunha_vel(out)
     assert np.array_equal(out["u4if"], rec["u4if"]) and np.array_equal(out["d"], rec["d"])
     assert np.array_equal(out["v"], (rec["v"].astype(np.float64) * 0.5).astype(np.float32))
+
+
+def test_power_spectrum_matches_the_oracle(ctx):
+    """CorrMeasure on the device (deposit, r2c, binning; Cosmos/Correlation.c:13-116) against the
+    oracle's FP64 FFT of the same fixed-point density."""
+    n = 32
+    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=30, seed=17)
+    p = lagpm.make_params(n)
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.upload_pos(pos)
+        pk, npk, cor, s8, sl = b.power_spectrum(boxsize=64.0)
+    acc, bits = orc.tsc_fixed(n, pos)
+    den = orc.fixed_to_density(n, acc, bits).astype(np.float64)
+    opk, onpk, ocor, os8, osl = orc.power_spectrum(n, orc.r2c(n, den), 64.0)
+    print(f"power spectrum: sigma8 {s8:.6g} (oracle {os8:.6g}), slope {sl:.4g} ({osl:.4g})", flush=True)
+    assert np.allclose(npk, onpk, rtol=1e-12, atol=1e-9)
+    m = onpk > 0
+    assert np.allclose(pk[m], opk[m], rtol=2e-5, atol=1e-6 * np.abs(opk).max())
+    assert s8 == pytest.approx(os8, rel=1e-5)
+    assert np.allclose(cor, ocor, rtol=1e-4, atol=1e-5 * np.abs(ocor).max())

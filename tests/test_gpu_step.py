@@ -122,9 +122,8 @@ def test_node_overflow_is_regrown_and_the_kick_is_exact(ctx, prec, tol):
         assert np.array_equal(b.velocities(), 2 * a)
     print(f"clumps prec={prec}: N={len(pos)} nodes {st.n_nodes} groups {st.n_groups}", flush=True)
     assert st.n_nodes > len(pos) // 3
-    diff, slope = orc.force_spline(1, n, p.epsilon, p.efold / 2)
     seen = pos.astype(np.float32).astype(np.float64) if prec == lagpm.PREC_FP32 else pos
-    ref = orc.pp_direct(n, seen, diff, slope, p.rsphere)
+    ref = orc.pp_direct_analytic(1, n, p.epsilon, p.efold / 2, seen, p.rsphere)
     err = float(np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)))
     print(f"  rms rel {err:.3e}", flush=True)
     assert err < tol
@@ -132,3 +131,26 @@ def test_node_overflow_is_regrown_and_the_kick_is_exact(ctx, prec, tol):
     (xk, vk), _ = run_kdk(ctx, p, pos, None, 1.0, 0.1, 1)
     (xm, vm) = run_manual(ctx, p, pos, None, 1.0, 0.1, 1)
     assert np.array_equal(xk, xm) and np.array_equal(vk, vm)
+
+
+def test_graph_replays_equal_eager_steps_after_every_step(ctx):
+    """A box stepped through captured step graphs (replayed, both particle-set parities) equals
+    a box stepped with every kernel launched eagerly, compared after each step."""
+    n = 32
+    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=16, seed=21)
+    vel = np.random.default_rng(21).standard_normal(pos.shape) * 0.01
+    p = lagpm.make_params(n)
+    with lagpm.Box(ctx, p, len(pos)) as bg, lagpm.Box(ctx, p, len(pos)) as be:
+        bg.upload_pos(pos, vel)
+        be.upload_pos(pos, vel)
+        for s in range(5):
+            a = 1.0 + 0.1 * s
+            ctx.set_step_graphs(True)
+            bg.kdk_step(a, 0.1)
+            ctx.set_step_graphs(False)
+            be.kdk_step(a, 0.1)
+            ctx.synchronize()
+            xg, vg = bg.download()
+            xe, ve = be.download()
+            assert np.array_equal(xg, xe) and np.array_equal(vg, ve), f"step {s}"
+        ctx.set_step_graphs(True)

@@ -1,6 +1,7 @@
 """GPU parity of the tree correction through the C ABI (run with -m gpu).
 
-theta -> 0 must reproduce the oracle's exact short-range pair sum; at theta = 0.5 the FP32
+theta -> 0 must reproduce the oracle's exact short-range pair sum (the erfc law analytically in
+FP64; the reference's fitted law through its own 4096-entry table); at theta = 0.5 the FP32
 wavefront walk must agree with the plain FP64 walk of the same tree to well below 1e-3
 and stay within the reference walk's own approximation error of the exact sum."""
 import numpy as np
@@ -25,6 +26,11 @@ def tree_accel(ctx, params, pos, fact=1.0):
     return v, st
 
 
+def exact_erfc(n, p, pos):
+    """The erfc short-range law evaluated analytically in FP64 for every pair within the cutoff."""
+    return orc.pp_direct_analytic(1, n, p.epsilon, p.efold / 2, pos, p.rsphere)
+
+
 def workload(n, kind, seed=3):
     if kind == "uniform":
         return ics.uniform_random(n ** 3, n, seed=seed)
@@ -41,8 +47,7 @@ def test_tree_theta_zero_is_the_direct_sum(ctx, kind, prec):
     p = lagpm.make_params(n, theta=1e-6, precision=prec)
     print(f"tree theta0 kind={kind} prec={prec} N={len(pos)}", flush=True)
     a, st = tree_accel(ctx, p, pos)
-    diff, slope = orc.force_spline(1, n, p.epsilon, p.efold / 2)
-    ref = orc.pp_direct(n, pos, diff, slope, p.rsphere)
+    ref = exact_erfc(n, p, pos)
     err = rel_rms(a, ref)
     print(f"  rms rel {err:.3e}  n_pp {st.n_pp} n_pc {st.n_pc} nodes {st.n_nodes} groups {st.n_groups}")
     assert st.n_pc == 0
@@ -70,7 +75,7 @@ def test_tree_theta_half_fp32_vs_fp64_and_exact(ctx, kind):
     a32, st32 = tree_accel(ctx, p32, pos)
     a64, st64 = tree_accel(ctx, p64, pos)
     diff, slope = orc.force_spline(1, n, p32.epsilon, p32.efold / 2)
-    exact = orc.pp_direct(n, pos, diff, slope, p32.rsphere)
+    exact = exact_erfc(n, p32, pos)
     e_prec = rel_rms(a32, a64)
     e_tree = rel_rms(a64, exact)
     ref_walk = orc.treecorrection_ref(n, pos, diff, slope, p32.rsphere, 0.5)
@@ -113,8 +118,7 @@ def test_tree_small_and_empty_inputs(ctx):
         a, st = tree_accel(ctx, p, pos)
         assert a.shape == (npart, 3)
         if npart:
-            diff, slope = orc.force_spline(1, n, p.epsilon, p.efold / 2)
-            ref = orc.pp_direct(n, pos, diff, slope, p.rsphere)
+            ref = exact_erfc(n, p, pos)
             assert np.allclose(a, ref, rtol=1e-4, atol=1e-7 * max(1e-30, np.abs(ref).max()))
 
 
@@ -133,6 +137,4 @@ def test_coincident_particles_beyond_group_size(ctx):
     pos = np.concatenate([np.tile([[10.3, 11.7, 12.1]], (200, 1)), g.random((3000, 3)) * n])
     p = lagpm.make_params(n, theta=1e-6)
     a, st = tree_accel(ctx, p, pos)
-    diff, slope = orc.force_spline(1, n, p.epsilon, p.efold / 2)
-    ref = orc.pp_direct(n, pos, diff, slope, p.rsphere)
-    assert rel_rms(a, ref) < 3e-5
+    assert rel_rms(a, exact_erfc(n, p, pos)) < 3e-5
