@@ -44,9 +44,10 @@ def rel_errors(a, ref):
     return rms, float(np.percentile(per, 99)), float(np.median(per))
 
 
-def force_at(ctx, lagpm, params, pos):
+def force_at(ctx, lagpm, params, pos, tuning):
     """PM + tree acceleration at fixed positions (fact2 = 1, zero velocities)."""
     with lagpm.Box(ctx, params, len(pos)) as b:
+        b.set_tuning(*tuning)
         b.upload_pos(pos)
         t0 = time.perf_counter()
         b.pmmain(lagpm.evolfact(1.0), lagpm.KICK)
@@ -265,6 +266,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--halo-fraction", type=float, default=0.0, help="default 0.3 (0.4 at n >= 512)")
+    ap.add_argument("--nhalo", type=int, default=0, help="default 2000 scaled with volume beyond 256^3")
+    ap.add_argument("--acc-sample", type=float, default=1.0, help="fraction of particles in the accuracy statistics")
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-n", type=int, default=128)
@@ -272,6 +276,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--nleaf", type=int, default=0)
+    ap.add_argument("--group", type=int, default=0)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-boxes", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=10)
@@ -297,7 +304,9 @@ def main():
 
     n = args.n
     t_ic = time.perf_counter()
-    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=2000, seed=args.seed)
+    hfrac = args.halo_fraction or (0.4 if n >= 512 else 0.3)
+    nhalo = args.nhalo or max(2000, int(round(2000 * (n / 256) ** 3)))
+    pos = ics.clustered_box(n, halo_fraction=hfrac, nhalo=nhalo, seed=args.seed)
     N = len(pos)
     log(f"[bench] workload: {n}^3 box, {N} particles (IC {time.perf_counter() - t_ic:.1f} s)")
     params = lagpm.make_params(n, law=lagpm.LAW_ERFC, precision=lagpm.PREC_FP32, deconvolve=1, efold=2.5,
@@ -307,6 +316,13 @@ def main():
     ctx = lagpm.Context(0)
     box = lagpm.Box(ctx, params, N)
     box.upload_pos(pos)
+    if args.nleaf or args.group:
+        box.set_tuning(args.nleaf or 16, args.group or 64)
+    elif not args.no_autotune:
+        t_at = time.perf_counter()
+        nl, gs = box.autotune()
+        log(f"[bench] autotune: nleaf {nl}, group {gs} ({time.perf_counter() - t_at:.2f} s)")
+    tuning = box.tuning()
 
     phases, stats = [], []
 
@@ -395,10 +411,12 @@ def main():
         "dtype_detail": "FP32 mesh/FFT/pair arithmetic with FP32 accumulation, int64 fixed-point deposit, "
                         "FP64 positions/velocities and kick-drift",
         "data": "synthetic",
-        "config": {"workload": f"single box {n}^3 clustered (70% perturbed lattice + 30% in 2000 Plummer halos), "
+        "config": {"workload": f"single box {n}^3 clustered ({100 - 100 * hfrac:.0f}% perturbed lattice + "
+                               f"{100 * hfrac:.0f}% in {nhalo} Plummer halos), "
                                f"{n}^3 mesh, Rsplit 1.25, rcut 5.625, theta {args.theta}, one GOTPM step "
                                f"(drag + PM kick + tree kick + drift) per step",
-                   "n": n, "particles": N, "parallelism": "single GPU"},
+                   "n": n, "particles": N, "parallelism": "single GPU",
+                   "tree_tuning": {"nleaf": tuning[0], "group": tuning[1]}},
         "roofline": roof,
         "phases": phase_report,
     }
@@ -406,12 +424,16 @@ def main():
 
     if not args.no_accuracy:
         # forces at the initial positions: FP32 fast path vs the plain FP64 kernels
-        a32, t32 = force_at(ctx, lagpm, params, pos)
+        a32, t32 = force_at(ctx, lagpm, params, pos, tuning)
         acc = {"vs": "plain FP64 kernels, same tree/groups/criteria", "fp32_seconds": t32}
         if not args.no_fp64:
             p64 = lagpm.make_params(n, law=lagpm.LAW_ERFC, precision=lagpm.PREC_FP64, deconvolve=1, efold=2.5,
                                     epsilon=0.1, rsphere=5.625, theta=args.theta)
-            a64, t64 = force_at(ctx, lagpm, p64, pos)
+            a64, t64 = force_at(ctx, lagpm, p64, pos, tuning)
+            if args.acc_sample < 1.0:  # statistics on a seeded random subsample of the particles
+                sel = np.random.default_rng(args.seed + 11).random(N) < args.acc_sample
+                a32, a64 = a32[sel], a64[sel]
+                acc["sample_fraction"] = args.acc_sample
             rms, p99, med = rel_errors(a32, a64)
             acc.update({"rms_rel": rms, "p99_rel": p99, "median_rel": med, "fp64_plain_seconds": t64,
                         "speedup_vs_fp64_plain": t64 / max(t32, 1e-12)})

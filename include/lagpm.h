@@ -184,6 +184,15 @@ int lagpm_force_spline(const lagpm_params* p, double* diff /*[4096*3]*/, double*
  * Cosmos/treecorrection.c:227). Sorts, builds the octree, walks. */
 int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef);
 int lagpm_tree_stats_get(const lagpm_box* box, lagpm_tree_stats* out);
+/* Tree tuning of a box: nleaf (a node with more particles is split, 4..256) and the target
+ * group size (8..64, one wavefront walks one group). Defaults 16 and 64. */
+int lagpm_box_set_tuning(lagpm_box* box, int nleaf, int group);
+int lagpm_box_get_tuning(const lagpm_box* box, int* nleaf, int* group);
+/* Times the tree build + walk of the box's current particles for a set of (nleaf, group)
+ * candidates (the walk adds nothing to the velocities: factor 0), keeps the fastest, and
+ * returns it. Forces at theta > 0 depend slightly on the tree, so tune once per box, before
+ * stepping, and keep the choice. */
+int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group);
 /* Waits for the box's last recorded phase and fills the timings. */
 int lagpm_box_timings(lagpm_ctx* ctx, lagpm_box* box, lagpm_timings* out);
 

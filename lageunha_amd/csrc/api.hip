@@ -108,7 +108,8 @@ void free_box(lagpm_box* b) {
     dfree(b->id_alt); dfree(b->dfac);
     lagpm_box_fft_release(b);
     lagpm_box_graphs_release(b);
-    dfree(b->acc); dfree(b->dep_start); dfree(b->rho); dfree(b->spec); dfree(b->phi); dfree(b->grad);
+    dfree(b->acc); dfree(b->dep_start); dfree(b->rho); dfree(b->grad);
+    b->spec = b->phi = nullptr;  // the same padded mesh as rho
     dfree(b->pp); dfree(b->ncount); dfree(b->noff); dfree(b->nd_info); dfree(b->nd_parent); dfree(b->nd_child);
     dfree(b->nd_com); dfree(b->nd_q0); dfree(b->nd_q1); dfree(b->nd_d); dfree(b->cellroot); dfree(b->gflag);
     dfree(b->goff); dfree(b->groups); dfree(b->gbox); dfree(b->pgroup); dfree(b->counters); dfree(b->dsz);
@@ -214,20 +215,18 @@ int lagpm_dalloc_bytes(lagpm_ctx* ctx, void** p, size_t bytes) {
     return LAGPM_OK;
 }
 
-// Device force tables of a law, built once per context for n = 1 (the law scales as 1/n^2;
-// the walk multiplies by the box's 1/n^2 with the kick factor).
 // g(s) = fr(r)/r at s = r^2 for n = 1 (the short-range force per unit separation); in terms of
 // s the reference's table terms are f0 = -g, f1 = -(fp r - fr)/(2 r^3) = -dg/ds and
 // f2 = -(fpp r^2 - 3 fp r + 3 fr)/(2 r^5) = -2 d2g/ds2 (OST/Model4TreeForce.c:48-91).
 static double erfc_g(double eps, double rs, double s) {
-    const double r = std::sqrt(s);
     const double newton = 1.0 / std::pow(s + eps * eps, 1.5);
-    const double u = r / (2.0 * rs);
+    const double u2 = s / (4.0 * rs * rs);
     double flong;
-    if (u < 1e-3) {  // series of (erf(u) - 2u/sqrt(pi) exp(-u^2)) / r^3, which is smooth in s
-        flong = (4.0 / (3.0 * std::sqrt(M_PI))) / (8.0 * rs * rs * rs) * (1.0 - 0.6 * u * u);
+    if (u2 < 1e-6) {  // series of (erf(u) - 2u/sqrt(pi) exp(-u^2)) / r^3 in u^2 (also for s slightly < 0)
+        flong = (4.0 / (3.0 * std::sqrt(M_PI))) / (8.0 * rs * rs * rs) * (1.0 - 0.6 * u2 + (3.0 / 14.0) * u2 * u2);
     } else {
-        flong = (std::erf(u) - (r / (rs * std::sqrt(M_PI))) * std::exp(-u * u)) / (s * r);
+        const double r = std::sqrt(s), u = std::sqrt(u2);
+        flong = (std::erf(u) - (r / (rs * std::sqrt(M_PI))) * std::exp(-u2)) / (s * r);
     }
     return newton - flong;
 }
@@ -249,13 +248,15 @@ static void erfc_log_tables(double eps, double rs, std::vector<double>& t /* 6 p
     };
     t.assign(6 * (size_t)NB, 0.0);
     for (int b = 0; b < NB; b++) {
-        const double s0 = node(b), s1 = node(b + 1);
+        // bin 0 also serves every r^2 below its node: its line runs from r^2 = 0 to the next node
+        const double s0 = b == 0 ? 0.0 : node(b), s1 = node(b + 1);
         double a[3], c[3];
         terms(s0, a);
         terms(s1, c);
         for (int k = 0; k < 3; k++) {
-            t[6 * (size_t)b + 2 * k] = a[k];
-            t[6 * (size_t)b + 2 * k + 1] = (c[k] - a[k]) / (s1 - s0);
+            const double slope = (c[k] - a[k]) / (s1 - s0);
+            t[6 * (size_t)b + 2 * k] = a[k] + slope * (node(b) - s0);  // value at the bin's node
+            t[6 * (size_t)b + 2 * k + 1] = slope;
         }
     }
 }
@@ -395,7 +396,7 @@ size_t lagpm_estimate_box_bytes(const lagpm_params* p, size_t np) {
     const size_t mcell = (size_t)1 << L;
     size_t b = np * (12 * 8 + 4 * 4 + 2 * 8 + 2 * 4);  // two sets of x..vz, id/id_alt/perm/perm2, keys, node counts
     b += np * (f64 ? 4 : 16);                             // pgroup / pp
-    b += m * 8 + 2 * m * real + K * 2 * real + 4 * m * real;  // acc, rho, phi, spec, grad
+    b += m * 8 + K * 2 * real + 4 * m * real;              // acc, padded FFT mesh, grad
     b += 4 * mcell * mcell * mcell + 4 * 8 + DSZ_COUNT * 4;  // cellroot, counters, sizes
     b += 4 * (lagpm_deposit_chunk_cap(p->n, np) + 1);     // deposit chunk starts
     const size_t cap = lagpm_tree_node_cap_guess(np, (int)mcell), gcap = cap + np / 64 + 1;
@@ -430,9 +431,9 @@ int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box
     for (int a = 0; a < 6; a++) ALLOC(box->alt[a], np);
     ALLOC(box->id_alt, np); ALLOC(box->dfac, 4);
     ALLOC(box->acc, m);
-    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->rho, m * real);
-    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->spec, K * 2 * real);
-    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->phi, m * real);
+    // one in-place FFT mesh (padded rows of 2(n/2+1)): density, then spectrum, then potential
+    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->rho, K * 2 * real);
+    box->spec = box->phi = box->rho;
     if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->grad, m * 4 * real);
     ALLOC(box->ncount, np); ALLOC(box->noff, np);
     ALLOC(box->cellroot, (size_t)box->mcell * box->mcell * box->mcell);
@@ -595,6 +596,16 @@ static int need_particles(lagpm_ctx* ctx, lagpm_box* box) {
     return LAGPM_OK;
 }
 
+// n^3 host values <-> the padded FFT mesh (rows of n values every 2(n/2+1))
+static int mesh_copy(lagpm_ctx* ctx, lagpm_box* box, void* host, bool to_host, size_t elem) {
+    const size_t n = (size_t)box->p.n, mx = 2 * (n / 2 + 1);
+    if (to_host)
+        HIPCHK(ctx, hipMemcpy2DAsync(host, n * elem, box->rho, mx * elem, n * elem, n * n, hipMemcpyDeviceToHost, box->st));
+    else
+        HIPCHK(ctx, hipMemcpy2DAsync(box->rho, mx * elem, host, n * elem, n * elem, n * n, hipMemcpyHostToDevice, box->st));
+    return LAGPM_OK;
+}
+
 int lagpm_tsc(lagpm_ctx* ctx, lagpm_box* box, float* den) {
     RCHK(need_particles(ctx, box));
     if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
@@ -603,11 +614,11 @@ int lagpm_tsc(lagpm_ctx* ctx, lagpm_box* box, float* den) {
         const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
         if (box->p.precision == LAGPM_PREC_FP64) {
             std::vector<double> t(m);
-            HIPCHK(ctx, hipMemcpyAsync(t.data(), box->rho, m * 8, hipMemcpyDeviceToHost, box->st));
+            RCHK(mesh_copy(ctx, box, t.data(), true, 8));
             HIPCHK(ctx, hipStreamSynchronize(box->st));
             for (size_t i = 0; i < m; i++) den[i] = (float)t[i];
         } else {
-            HIPCHK(ctx, hipMemcpyAsync(den, box->rho, m * 4, hipMemcpyDeviceToHost, box->st));
+            RCHK(mesh_copy(ctx, box, den, true, 4));
         }
     }
     HIPCHK(ctx, hipStreamSynchronize(box->st));
@@ -632,11 +643,11 @@ int lagpm_psolver(lagpm_ctx* ctx, lagpm_box* box, float* phi) {
         const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
         if (box->p.precision == LAGPM_PREC_FP64) {
             std::vector<double> t(m);
-            HIPCHK(ctx, hipMemcpyAsync(t.data(), box->phi, m * 8, hipMemcpyDeviceToHost, box->st));
+            RCHK(mesh_copy(ctx, box, t.data(), true, 8));
             HIPCHK(ctx, hipStreamSynchronize(box->st));
             for (size_t i = 0; i < m; i++) phi[i] = (float)t[i];
         } else {
-            HIPCHK(ctx, hipMemcpyAsync(phi, box->phi, m * 4, hipMemcpyDeviceToHost, box->st));
+            RCHK(mesh_copy(ctx, box, phi, true, 4));
         }
     }
     HIPCHK(ctx, hipStreamSynchronize(box->st));
@@ -675,10 +686,10 @@ int lagpm_set_potential(lagpm_ctx* ctx, lagpm_box* box, const float* phi) {
     const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
     if (box->p.precision == LAGPM_PREC_FP64) {
         std::vector<double> t(phi, phi + m);
-        HIPCHK(ctx, hipMemcpyAsync(box->phi, t.data(), m * 8, hipMemcpyHostToDevice, box->st));
+        RCHK(mesh_copy(ctx, box, t.data(), false, 8));
         HIPCHK(ctx, hipStreamSynchronize(box->st));
     } else {
-        HIPCHK(ctx, hipMemcpyAsync(box->phi, phi, m * 4, hipMemcpyHostToDevice, box->st));
+        RCHK(mesh_copy(ctx, box, (void*)phi, false, 4));
         HIPCHK(ctx, hipStreamSynchronize(box->st));
     }
     return LAGPM_OK;
@@ -810,6 +821,53 @@ int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* e
         // a build that overflowed its node arrays applied no kick: grow once and redo it
         if (!retry || attempt > 0) return rc == LAGPM_E_OVERFLOW ? lagpm_fail(ctx, rc, "tree node arrays overflowed") : rc;
     }
+}
+
+int lagpm_box_set_tuning(lagpm_box* box, int nleaf, int group) {
+    if (!box || nleaf < 4 || nleaf > 256 || group < 8 || group > 64) return LAGPM_E_INVAL;
+    if (box->tune.nleaf != nleaf || box->tune.group != group) lagpm_box_graphs_release(box);
+    box->tune.nleaf = nleaf;
+    box->tune.group = group;
+    return LAGPM_OK;
+}
+
+int lagpm_box_get_tuning(const lagpm_box* box, int* nleaf, int* group) {
+    if (!box || !nleaf || !group) return LAGPM_E_INVAL;
+    *nleaf = box->tune.nleaf;
+    *group = box->tune.group;
+    return LAGPM_OK;
+}
+
+int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group) {
+    RCHK(need_particles(ctx, box));
+    RCHK(tree_check(ctx, box));
+    if (box->np == 0) return LAGPM_OK;
+    static const int cand[][2] = {{16, 64}, {32, 64}, {8, 64}, {16, 32}, {32, 32}, {8, 32}};
+    double best = 1e300;
+    int bl = box->tune.nleaf, bg = box->tune.group;
+    for (const auto& c : cand) {
+        RCHK(lagpm_box_set_tuning(box, c[0], c[1]));
+        double t = 1e300;
+        for (int rep = 0; rep < 2; rep++) {  // the first run of a candidate may grow the node arrays
+            RCHK(tree_enqueue(ctx, box, 0.0));
+            bool retry = false;
+            int rc = lagpm_tree_finish(ctx, box, &retry);
+            if (rc != LAGPM_OK && !retry) return rc;
+            if (rc == LAGPM_OK) {
+                tree_times(box);
+                t = std::min(t, box->stats.ms_build + box->stats.ms_walk);
+            }
+        }
+        if (t < best) {
+            best = t;
+            bl = c[0];
+            bg = c[1];
+        }
+    }
+    RCHK(lagpm_box_set_tuning(box, bl, bg));
+    if (nleaf) *nleaf = bl;
+    if (group) *group = bg;
+    return LAGPM_OK;
 }
 
 int lagpm_tree_stats_get(const lagpm_box* box, lagpm_tree_stats* out) {

@@ -103,9 +103,11 @@ struct lagpm_box {
     // PM meshes
     int scale_bits = 32;
     unsigned long long* acc = nullptr;  // int64 fixed-point deposit (two's complement)
-    void* rho = nullptr;                 // n^3 real (float or double)
-    void* spec = nullptr;                // n*n*(n/2+1) complex
-    void* phi = nullptr;                 // n^3 real
+    // one in-place FFT mesh, rows of n reals padded to 2(n/2+1): the density (rho), then its
+    // spectrum (spec, n*n*(n/2+1) complex), then the potential (phi); three names, one buffer
+    void* rho = nullptr;
+    void* spec = nullptr;
+    void* phi = nullptr;
     void* grad = nullptr;                // n^3 x 4 real (fx, fy, fz, phi)
     uint32_t* dep_start = nullptr;       // deposit chunk starts (+ sentinel)
     size_t dep_cap = 0;

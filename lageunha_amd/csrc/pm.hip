@@ -195,12 +195,16 @@ __global__ void __launch_bounds__(DEP_THREADS) deposit_kernel(const double* __re
     }
 }
 
+// The FFT mesh is the in-place real layout: rows of n values padded to mx = 2(n/2+1), the
+// layout of the reference's FFTW grid (Cosmos/pmmain.c:328-334). It holds the density, then
+// the spectrum, then the potential.
 template <typename T>
-__global__ void convert_kernel(const unsigned long long* __restrict__ acc, size_t m, double inv_scale, double mean,
+__global__ void convert_kernel(const unsigned long long* __restrict__ acc, int n, double inv_scale, double mean,
                                T* __restrict__ rho) {
+    const size_t m = (size_t)n * n * n, mx = 2 * (size_t)(n / 2 + 1);
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < m; i += stride) rho[i] = (T)((double)(long long)acc[i] * inv_scale - mean);
+    for (; i < m; i += stride) rho[i % n + mx * (i / n)] = (T)((double)(long long)acc[i] * inv_scale - mean);
 }
 
 __device__ inline double tsc_w2(int n, double m) {
@@ -260,14 +264,15 @@ __global__ void gradient_kernel(const T* __restrict__ phi, int n, T4* __restrict
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t nn = (size_t)n * n;
+    const size_t mx = 2 * (size_t)(n / 2 + 1), plane = mx * n;  // padded rows of the FFT mesh
     for (; q < m; q += stride) {
         int i = (int)(q % n);
         int j = (int)((q / n) % n);
         int k = (int)(q / nn);
-        size_t rowj = (size_t)n * j + nn * k;
+        size_t rowj = mx * j + plane * k;
         auto X = [&](int a) { return phi[(size_t)wrapi(a, n) + rowj]; };
-        auto Y = [&](int b) { return phi[(size_t)i + (size_t)n * wrapi(b, n) + nn * k]; };
-        auto Z = [&](int c) { return phi[(size_t)i + (size_t)n * j + nn * wrapi(c, n)]; };
+        auto Y = [&](int b) { return phi[(size_t)i + mx * wrapi(b, n) + plane * k]; };
+        auto Z = [&](int c) { return phi[(size_t)i + mx * j + plane * wrapi(c, n)]; };
         T4 g;
         if constexpr (sizeof(T) == 4) {
             g.x = fda4_f(X(i - 1), X(i + 1), X(i - 2), X(i + 2));
@@ -278,7 +283,7 @@ __global__ void gradient_kernel(const T* __restrict__ phi, int n, T4* __restrict
             g.y = fda4_d(Y(j - 1), Y(j + 1), Y(j - 2), Y(j + 2));
             g.z = fda4_d(Z(k - 1), Z(k + 1), Z(k - 2), Z(k + 2));
         }
-        g.w = phi[q];
+        g.w = phi[(size_t)i + rowj];
         grad[q] = g;
     }
 }
@@ -374,17 +379,18 @@ unsigned grid_for(size_t work, int threads, unsigned cap = 65535u * 4) {
 // ---- one-time check of each new FFT plan pair on the device ------------------------------
 // mode 0: a single Fourier mode (1, 2, 3); mode 1: a hashed pseudo-random field in [-1, 1)
 template <typename T>
-__global__ void fftchk_fill(T* __restrict__ x, int n, int mode) {
-    const size_t m = (size_t)n * n * n;
+__global__ void fftchk_fill(T* __restrict__ xp, int n, int mode) {
+    const size_t m = (size_t)n * n * n, mx = 2 * (size_t)(n / 2 + 1);
     for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < m; q += (size_t)gridDim.x * blockDim.x) {
         const size_t i = q % n, j = (q / n) % n, k = q / ((size_t)n * n);
+        const size_t at = mx * (q / n) + i;  // padded position of element q
         if (mode == 0) {
             const size_t ph = (i + 2 * j + 3 * k) % (size_t)n;
-            x[q] = (T)cos(2.0 * M_PI * (double)ph / (double)n);
+            xp[at] = (T)cos(2.0 * M_PI * (double)ph / (double)n);
         } else {
             uint64_t h = (uint64_t)q * 0x9E3779B97F4A7C15ull;
             h ^= h >> 29; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 32;
-            x[q] = (T)((double)(h >> 11) * (1.0 / 4503599627370496.0) - 1.0);
+            xp[at] = (T)((double)(h >> 11) * (1.0 / 4503599627370496.0) - 1.0);
         }
     }
 }
@@ -407,12 +413,15 @@ __global__ void fftchk_mode(const T* __restrict__ spec, int n, double* __restric
     }
     atomic_max_nonneg(err, e);
 }
-// max |a * sa - b|
+// max |a * sa - b| over cnt elements; row > 0: only the first `row` of every `pitch` elements
 template <typename T>
-__global__ void fftchk_cmp(const T* __restrict__ a, const T* __restrict__ b, size_t cnt, double sa, double* __restrict__ err) {
+__global__ void fftchk_cmp(const T* __restrict__ a, const T* __restrict__ b, size_t cnt, double sa, size_t row, size_t pitch,
+                           double* __restrict__ err) {
     double e = 0;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += (size_t)gridDim.x * blockDim.x)
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += (size_t)gridDim.x * blockDim.x) {
+        if (row && q % pitch >= row) continue;
         e = fmax(e, fabs((double)a[q] * sa - (double)b[q]));
+    }
     atomic_max_nonneg(err, e);
 }
 
@@ -434,10 +443,30 @@ int get_plans(lagpm_ctx* ctx, int n, int prec, hipStream_t st, fft_plans** out) 
     fft_plans fp;
     size_t lengths[3] = {(size_t)n, (size_t)n, (size_t)n};
     rocfft_precision rp = prec == LAGPM_PREC_FP64 ? rocfft_precision_double : rocfft_precision_single;
-    RCHK(fft_status(ctx, rocfft_plan_create(&fp.fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, rp, 3,
-                                            lengths, 1, nullptr), "plan_create(r2c)"));
-    RCHK(fft_status(ctx, rocfft_plan_create(&fp.inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, rp, 3,
-                                            lengths, 1, nullptr), "plan_create(c2r)"));
+    // In place on the padded mesh. (The out-of-place 3-D real transforms of this rocFFT gave
+    // wrong, run-to-run different results for n = 108 on MI355X; in place they are exact.)
+    const size_t h = (size_t)n / 2 + 1;
+    size_t rs[3] = {1, 2 * h, 2 * h * (size_t)n}, cs[3] = {1, h, h * (size_t)n};
+    const size_t rdist = 2 * h * (size_t)n * n, cdist = h * (size_t)n * n;
+    rocfft_plan_description dfw = nullptr, dinv = nullptr;
+    RCHK(fft_status(ctx, rocfft_plan_description_create(&dfw), "description_create"));
+    RCHK(fft_status(ctx, rocfft_plan_description_create(&dinv), "description_create"));
+    int rcd = fft_status(ctx, rocfft_plan_description_set_data_layout(dfw, rocfft_array_type_real,
+                                                                      rocfft_array_type_hermitian_interleaved, nullptr,
+                                                                      nullptr, 3, rs, rdist, 3, cs, cdist), "data_layout");
+    if (rcd == LAGPM_OK)
+        rcd = fft_status(ctx, rocfft_plan_description_set_data_layout(dinv, rocfft_array_type_hermitian_interleaved,
+                                                                      rocfft_array_type_real, nullptr, nullptr, 3, cs,
+                                                                      cdist, 3, rs, rdist), "data_layout");
+    if (rcd == LAGPM_OK)
+        rcd = fft_status(ctx, rocfft_plan_create(&fp.fwd, rocfft_placement_inplace, rocfft_transform_type_real_forward, rp,
+                                                 3, lengths, 1, dfw), "plan_create(r2c)");
+    if (rcd == LAGPM_OK)
+        rcd = fft_status(ctx, rocfft_plan_create(&fp.inv, rocfft_placement_inplace, rocfft_transform_type_real_inverse, rp,
+                                                 3, lengths, 1, dinv), "plan_create(c2r)");
+    rocfft_plan_description_destroy(dfw);
+    rocfft_plan_description_destroy(dinv);
+    RCHK(rcd);
     size_t w1 = 0, w2 = 0;
     RCHK(fft_status(ctx, rocfft_plan_get_work_buffer_size(fp.fwd, &w1), "work_size"));
     RCHK(fft_status(ctx, rocfft_plan_get_work_buffer_size(fp.inv, &w2), "work_size"));
@@ -459,47 +488,45 @@ int get_plans(lagpm_ctx* ctx, int n, int prec, hipStream_t st, fft_plans** out) 
 // given the same spectrum twice for the same input, and inverted a pseudo-random field.
 template <typename T>
 int check_plans_t(lagpm_ctx* ctx, int n, fft_plans* fp, hipStream_t st, double tol) {
-    const size_t m = (size_t)n * n * n, K = (size_t)(n / 2 + 1) * n * n;
+    const size_t m = (size_t)n * n * n, h = (size_t)n / 2 + 1, K = h * n * n, mx = 2 * h;
     rocfft_execution_info info = nullptr;
     void* work = nullptr;
     RCHK(make_exec(ctx, fp, st, &info, &work));
-    T *x = nullptr, *spec = nullptr, *spec2 = nullptr, *back = nullptr;
+    T *a = nullptr, *b = nullptr, *ref = nullptr;  // padded meshes (2K reals each)
     double* err = nullptr;
     int rc = LAGPM_OK;
-    if (hipMalloc(&x, m * sizeof(T)) != hipSuccess || hipMalloc(&back, m * sizeof(T)) != hipSuccess ||
-        hipMalloc(&spec, 2 * K * sizeof(T)) != hipSuccess || hipMalloc(&spec2, 2 * K * sizeof(T)) != hipSuccess ||
-        hipMalloc(&err, 4 * sizeof(double)) != hipSuccess)
+    if (hipMalloc(&a, 2 * K * sizeof(T)) != hipSuccess || hipMalloc(&b, 2 * K * sizeof(T)) != hipSuccess ||
+        hipMalloc(&ref, 2 * K * sizeof(T)) != hipSuccess || hipMalloc(&err, 4 * sizeof(double)) != hipSuccess)
         rc = lagpm_fail(ctx, LAGPM_E_NOMEM, "FFT check buffers");
     double e[4] = {0, 0, 0, 0};
     const unsigned g = grid_for(m, 256);
-    auto fwd = [&](T* in, T* out) {
-        void* ip[1] = {in};
-        void* op[1] = {out};
-        return fft_status(ctx, rocfft_execute(fp->fwd, ip, op, info), "execute(r2c check)");
+    auto run = [&](rocfft_plan plan, T* x) {
+        void* ip[1] = {x};
+        return fft_status(ctx, rocfft_execute(plan, ip, nullptr, info), "execute(check)");
     };
     if (rc == LAGPM_OK && hipMemsetAsync(err, 0, 4 * sizeof(double), st) != hipSuccess) rc = LAGPM_E_HIP;
     if (rc == LAGPM_OK) {
-        hipLaunchKernelGGL(fftchk_fill<T>, dim3(g), dim3(256), 0, st, x, n, 0);
-        rc = fwd(x, spec);
+        hipLaunchKernelGGL(fftchk_fill<T>, dim3(g), dim3(256), 0, st, a, n, 0);
+        rc = run(fp->fwd, a);
     }
     if (rc == LAGPM_OK) {
-        hipLaunchKernelGGL(fftchk_mode<T>, dim3(g), dim3(256), 0, st, spec, n, err);
-        hipLaunchKernelGGL(fftchk_fill<T>, dim3(g), dim3(256), 0, st, x, n, 1);
-        rc = fwd(x, spec);
+        hipLaunchKernelGGL(fftchk_mode<T>, dim3(g), dim3(256), 0, st, a, n, err);
+        hipLaunchKernelGGL(fftchk_fill<T>, dim3(g), dim3(256), 0, st, a, n, 1);
+        hipLaunchKernelGGL(fftchk_fill<T>, dim3(g), dim3(256), 0, st, b, n, 1);
+        hipLaunchKernelGGL(fftchk_fill<T>, dim3(g), dim3(256), 0, st, ref, n, 1);
+        rc = run(fp->fwd, a);
     }
-    if (rc == LAGPM_OK) rc = fwd(x, spec2);
+    if (rc == LAGPM_OK) rc = run(fp->fwd, b);
     if (rc == LAGPM_OK) {
-        hipLaunchKernelGGL(fftchk_cmp<T>, dim3(g), dim3(256), 0, st, spec, spec2, 2 * K, 1.0, err + 1);
-        void* ip[1] = {spec2};
-        void* op[1] = {back};
-        rc = fft_status(ctx, rocfft_execute(fp->inv, ip, op, info), "execute(c2r check)");
+        hipLaunchKernelGGL(fftchk_cmp<T>, dim3(g), dim3(256), 0, st, a, b, 2 * K, 1.0, (size_t)0, (size_t)1, err + 1);
+        rc = run(fp->inv, b);
     }
     if (rc == LAGPM_OK) {
-        hipLaunchKernelGGL(fftchk_cmp<T>, dim3(g), dim3(256), 0, st, back, x, m, 1.0 / (double)m, err + 2);
+        hipLaunchKernelGGL(fftchk_cmp<T>, dim3(g), dim3(256), 0, st, b, ref, 2 * K, 1.0 / (double)m, (size_t)n, mx, err + 2);
         if (hipMemcpyAsync(e, err, sizeof(e), hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
             rc = lagpm_fail(ctx, LAGPM_E_HIP, "FFT check readback");
     }
-    (void)hipFree(x); (void)hipFree(back); (void)hipFree(spec); (void)hipFree(spec2); (void)hipFree(err);
+    (void)hipFree(a); (void)hipFree(b); (void)hipFree(ref); (void)hipFree(err);
     if (info) rocfft_execution_info_destroy(info);
     if (work) (void)hipFree(work);
     if (rc != LAGPM_OK) return rc;
@@ -599,8 +626,7 @@ int lagpm_pm_power(lagpm_ctx* ctx, lagpm_box* box, double* pk_host, double* npk_
     RCHK(get_plans(ctx, n, box->p.precision, box->st, &fp));
     RCHK(fft_status(ctx, rocfft_execution_info_set_stream(box->fft_info, box->st), "set_stream"));
     void* in[1] = {box->rho};
-    void* out[1] = {box->spec};
-    RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, out, box->fft_info), "execute(r2c)"));
+    RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, nullptr, box->fft_info), "execute(r2c)"));
     double* d = nullptr;
     HIPCHK(ctx, hipMalloc(&d, 2 * (size_t)(n + 1) * sizeof(double)));
     int rc = LAGPM_OK;
@@ -690,10 +716,10 @@ int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
     const double inv_scale = std::ldexp(1.0, -box->scale_bits);
     const double mean = 1.0;  // the FFT grid starts at -1 before the fold (Cosmos/pmmain.c:304)
     if (f64)
-        hipLaunchKernelGGL(convert_kernel<double>, dim3(grid_for(m, 256)), dim3(256), 0, st, box->acc, m, inv_scale,
+        hipLaunchKernelGGL(convert_kernel<double>, dim3(grid_for(m, 256)), dim3(256), 0, st, box->acc, n, inv_scale,
                            mean, (double*)box->rho);
     else
-        hipLaunchKernelGGL(convert_kernel<float>, dim3(grid_for(m, 256)), dim3(256), 0, st, box->acc, m, inv_scale,
+        hipLaunchKernelGGL(convert_kernel<float>, dim3(grid_for(m, 256)), dim3(256), 0, st, box->acc, n, inv_scale,
                            mean, (float*)box->rho);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
@@ -721,15 +747,13 @@ int lagpm_pm_solve(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     // the box's execution info follows the stream its work is queued on
     RCHK(fft_status(ctx, rocfft_execution_info_set_stream(box->fft_info, box->st), "set_stream"));
     rocfft_execution_info info = box->fft_info;
-    void* in[1] = {box->rho};
-    void* out[1] = {box->spec};
-    RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, out, info), "execute(r2c)"));
+    void* in[1] = {box->rho};  // rho, spec and phi are one padded mesh
+    RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, nullptr, info), "execute(r2c)"));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], box->st));
     RCHK(lagpm_green_device(ctx, box->st, box->p.n, box->p.precision, box->p.efold, box->p.deconvolve, box->spec));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[1], box->st));
     void* in2[1] = {box->spec};
-    void* out2[1] = {box->phi};
-    RCHK(fft_status(ctx, rocfft_execute(fp->inv, in2, out2, info), "execute(c2r)"));
+    RCHK(fft_status(ctx, rocfft_execute(fp->inv, in2, nullptr, info), "execute(c2r)"));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[2], box->st));
     return LAGPM_OK;
 }

@@ -43,12 +43,13 @@ constexpr int TAB_LDS = LAGPM_LOG_BINS > 4096 ? LAGPM_LOG_BINS : 4096;
 template <bool LOGT>
 __device__ inline int table_bin(float r2, float& t) {
     if constexpr (LOGT) {
-        // clamping r2 into the table's range first keeps the bin in [0, LAGPM_LOG_BINS) with no
-        // integer clamps (pairs beyond the cutoff are discarded by the caller)
-        const float r2c = __builtin_amdgcn_fmed3f(r2, 0x1p-14f, 63.99f);
-        const uint32_t bits = __float_as_uint(r2c);
-        t = r2c - __uint_as_float(bits & ~((1u << LAGPM_LOG_SHIFT) - 1u));
-        return (int)__builtin_amdgcn_ubfe(bits, LAGPM_LOG_SHIFT, 32 - LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE;
+        // r2 above the table is clamped (the caller discards pairs beyond the cutoff); below the
+        // first node (r < 2^-7) the first bin's line is extrapolated to r2 = 0, where the law is
+        // smooth in r2 on the scale eps^2
+        const float r2c = fminf(r2, 63.99f);
+        const int b = max((int)(__float_as_uint(r2c) >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE, 0);
+        t = r2c - __uint_as_float((uint32_t)(b + (int)LAGPM_LOG_BASE) << LAGPM_LOG_SHIFT);
+        return b;
     } else {
         const float r = __builtin_amdgcn_sqrtf(r2);
         const int nt = min((int)(r * (1.0f / RAN_F)), 4095);
@@ -59,10 +60,10 @@ __device__ inline int table_bin(float r2, float& t) {
 template <bool LOGT>
 __device__ inline int table_bin64(double r2, double& t) {
     if constexpr (LOGT) {
-        const uint32_t bits = __float_as_uint((float)r2);
-        const int b = (int)(bits >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE;
-        t = r2 - (double)__uint_as_float(bits & ~((1u << LAGPM_LOG_SHIFT) - 1u));
-        return min(max(b, 0), LAGPM_LOG_BINS - 1);
+        const double r2c = fmin(r2, 63.99);
+        const int b = max((int)(__float_as_uint((float)r2c) >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE, 0);
+        t = r2c - (double)__uint_as_float((uint32_t)(b + (int)LAGPM_LOG_BASE) << LAGPM_LOG_SHIFT);
+        return b;
     } else {
         const double ran = 6.0 / 4096.0;
         const double r = sqrt(r2);

@@ -102,6 +102,9 @@ SIGNATURES = {
     "lagpm_power_spectrum": (ctypes.c_int, [_P, _P, ctypes.c_double, _D, _D, _D, _D, _D]),
     "lagpm_treecorrection": (ctypes.c_int, [_P, _P, ctypes.POINTER(EvolFact)]),
     "lagpm_tree_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(TreeStats)]),
+    "lagpm_box_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
+    "lagpm_box_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "lagpm_box_autotune": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "lagpm_box_timings": (ctypes.c_int, [_P, _P, ctypes.POINTER(Timings)]),
     "lagpm_evolfactor": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_double, ctypes.c_double,
                                         ctypes.POINTER(EvolFact)]),
@@ -304,6 +307,22 @@ class Box:
 
     def drift(self, pfact: float):
         self.ctx.check(self.lib.lagpm_drift(self.ctx.h, self.h, pfact))
+
+    def set_tuning(self, nleaf: int, group: int):
+        rc = self.lib.lagpm_box_set_tuning(self.h, nleaf, group)
+        if rc != 0:
+            raise LagpmError(rc, f"bad tuning nleaf={nleaf} group={group}")
+
+    def tuning(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self.lib.lagpm_box_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def autotune(self):
+        """Time tree build + walk for a set of (nleaf, group) and keep the fastest."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(self.lib.lagpm_box_autotune(self.ctx.h, self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def power_spectrum(self, boxsize: float):
         """CorrMeasure of the current particles: (pk, npk, cor, sigma8, slope), n+1 bins."""

@@ -23,7 +23,7 @@ def test_requests_in_the_mix_equal_requests_alone(ctx):
 
     biggest = max(lagpm.estimate_box_bytes(params_for(r), r.n ** 3) for r in reqs)
     it = {"t": 0}
-    srv = Server(ctx, params_for, positions_for, hbm_budget=2.5 * biggest, keep_results=True,
+    srv = Server(ctx, params_for, positions_for, hbm_budget=1.35 * biggest, keep_results=True,
                  clock=lambda: float(it["t"]))
     for r in reqs:
         srv.submit(r)
