@@ -231,7 +231,10 @@ static double erfc_g(double eps, double rs, double s) {
     return newton - flong;
 }
 
-static void erfc_log_tables(double eps, double rs, std::vector<double>& t /* 6 per bin */) {
+// Bins of the erfc law, uniform in log(lam r^2) with lam = 32 / rc2: bin b holds the line
+// c + s * r2 through the law at its node and the next node (bin 0 from r2 = 0). The node
+// lam r2 = 32 is the cutoff; bins from there on hold zeros.
+static void erfc_log_tables(double eps, double rs, double rc2, double lam, std::vector<double>& t /* 6 per bin */) {
     const int NB = LAGPM_LOG_BINS;
     auto node = [](int b) -> double {
         uint32_t bits = (uint32_t)(b + (int)LAGPM_LOG_BASE) << LAGPM_LOG_SHIFT;
@@ -248,23 +251,25 @@ static void erfc_log_tables(double eps, double rs, std::vector<double>& t /* 6 p
     };
     t.assign(6 * (size_t)NB, 0.0);
     for (int b = 0; b < NB; b++) {
+        if (node(b) >= 32.0) break;  // lam r2 >= 32 is r2 >= rc2: zeros
         // bin 0 also serves every r^2 below its node: its line runs from r^2 = 0 to the next node
-        const double s0 = b == 0 ? 0.0 : node(b), s1 = node(b + 1);
+        const double s0 = b == 0 ? 0.0 : node(b) / lam, s1 = node(b + 1) / lam;
         double a[3], c[3];
         terms(s0, a);
         terms(s1, c);
         for (int k = 0; k < 3; k++) {
             const double slope = (c[k] - a[k]) / (s1 - s0);
-            t[6 * (size_t)b + 2 * k] = a[k] + slope * (node(b) - s0);  // value at the bin's node
+            t[6 * (size_t)b + 2 * k] = a[k] - slope * s0;  // intercept at r2 = 0
             t[6 * (size_t)b + 2 * k + 1] = slope;
         }
     }
+    (void)rc2;
 }
 
 // Device force tables of a law, built once per context for n = 1 (the law scales as 1/n^2;
 // the walk multiplies by the box's mass / n^2 with the kick factor).
 int lagpm_get_tables(lagpm_ctx* ctx, const lagpm_params* p, const law_tables** out) {
-    const auto key = std::make_tuple(p->law, p->epsilon, p->efold);
+    const auto key = std::make_tuple(p->law, p->epsilon, p->efold, p->rsphere);
     auto it = ctx->tables.find(key);
     if (it != ctx->tables.end()) {
         *out = &it->second;
@@ -275,7 +280,10 @@ int lagpm_get_tables(lagpm_ctx* ctx, const lagpm_params* p, const law_tables** o
     if (p->law == LAGPM_LAW_ERFC) {
         lt.logr2 = 1;
         lt.nbins = LAGPM_LOG_BINS;
-        erfc_log_tables(p->epsilon, 0.5 * p->efold, t);
+        const double rc2 = p->rsphere * p->rsphere;
+        lt.lam = (float)(32.0 / rc2);
+        lt.rc2_eff = rc2;
+        erfc_log_tables(p->epsilon, 0.5 * p->efold, rc2, (double)lt.lam, t);
     } else {
         const int NS = LAGPM_NSPLINE;
         lagpm_params q = *p;
@@ -284,6 +292,7 @@ int lagpm_get_tables(lagpm_ctx* ctx, const lagpm_params* p, const law_tables** o
         lagpm_build_tables(&q, diff.data(), slope.data());
         lt.logr2 = 0;
         lt.nbins = NS;
+        lt.rc2_eff = p->rsphere * p->rsphere;
         t.resize(6 * (size_t)NS);
         for (int i = 0; i < NS; i++)
             for (int c = 0; c < 3; c++) {

@@ -24,14 +24,17 @@ struct fft_plans {
 // context that has the same law, softening and split scale.
 // Two bin layouts: the reference's 4096 bins uniform in r over [0, 6) (LAW_PMFIT, so reference
 // mode interpolates exactly as OST/Model4TreeForce.c:34-46), and for LAW_ERFC bins uniform in
-// log r^2, 256 per octave over r^2 in [2^-14, 64), indexed by the bits of the FP32 r^2 (no
-// square root in the pair loop; about ten times smaller interpolation error).
+// log(lam r^2), 256 per octave over lam r^2 in [2^-14, 64) with lam = 32 / rc^2, indexed by the
+// bits of the FP32 lam r^2 (no square root in the pair loop; about ten times smaller
+// interpolation error; the cutoff r^2 = rc^2 is the bin edge 32, beyond which bins are zero).
 #define LAGPM_LOG_SHIFT 15                 // 23 - 8 mantissa bits kept: 256 bins per octave
 #define LAGPM_LOG_BASE 28928u              // bits(2^-14) >> 15
 #define LAGPM_LOG_BINS 5120                // 20 octaves: r^2 < 64
 struct law_tables {
     int logr2 = 0;            // 1: log r^2 bins, 0: uniform r bins
     int nbins = 0;
+    double rc2_eff = 0;       // squared cutoff the table realises
+    float lam = 1.0f;         // log bins: r^2 is binned as lam * r^2, lam = 32 / rc^2 (the cutoff is a bin edge)
     float2* tab0 = nullptr;   // (d0, s0) per bin: particle term
     float4* tab12 = nullptr;  // (d1, s1, d2, s2) per bin: quadrupole terms
     double* tabd = nullptr;   // 6 per bin: d0 s0 d1 s1 d2 s2 (FP64 walk)
@@ -46,7 +49,7 @@ struct lagpm_ctx {
     std::vector<hipStream_t> pool;           // per-box streams of batched steps
     std::string err;
     std::map<std::pair<int, int>, fft_plans> plans;  // (n, precision) -> plans
-    std::map<std::tuple<int, double, double>, law_tables> tables;  // (law, epsilon, efold)
+    std::map<std::tuple<int, double, double, double>, law_tables> tables;  // (law, epsilon, efold, rsphere)
     // walk resources (every walk runs on `stream`, so one set serves all boxes)
     int* walk_stack = nullptr;
     size_t walk_stack_entries = 0;

@@ -41,15 +41,14 @@ constexpr int TAB_LDS = LAGPM_LOG_BINS > 4096 ? LAGPM_LOG_BINS : 4096;
 // Table bin and offset of a pair at squared distance r2 (see law_tables): uniform r bins need
 // the square root; log r^2 bins take the exponent and 8 mantissa bits of r2.
 template <bool LOGT>
-__device__ inline int table_bin(float r2, float& t) {
+__device__ inline int table_bin(float r2, float& t, float lam) {
     if constexpr (LOGT) {
-        // r2 above the table is clamped (the caller discards pairs beyond the cutoff); below the
-        // first node (r < 2^-7) the first bin's line is extrapolated to r2 = 0, where the law is
-        // smooth in r2 on the scale eps^2
-        const float r2c = fminf(r2, 63.99f);
-        const int b = max((int)(__float_as_uint(r2c) >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE, 0);
-        t = r2c - __uint_as_float((uint32_t)(b + (int)LAGPM_LOG_BASE) << LAGPM_LOG_SHIFT);
-        return b;
+        // Each bin holds the line c + s * r2 (bin 0 runs from r2 = 0, so it serves every r2 below
+        // its node); bins beyond the cutoff hold zeros, so far pairs and the far-away dummies of a
+        // partial chunk add nothing without a compare. r2 above the table is clamped to its last bin.
+        const float r2c = __builtin_amdgcn_fmed3f(lam * r2, 0.0f, 63.99f);  // >= 0: min(lam r2, 63.99), one op
+        t = r2;
+        return max((int)(__float_as_uint(r2c) >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE, 0);
     } else {
         const float r = __builtin_amdgcn_sqrtf(r2);
         const int nt = min((int)(r * (1.0f / RAN_F)), 4095);
@@ -58,12 +57,11 @@ __device__ inline int table_bin(float r2, float& t) {
     }
 }
 template <bool LOGT>
-__device__ inline int table_bin64(double r2, double& t) {
+__device__ inline int table_bin64(double r2, double& t, float lam) {
     if constexpr (LOGT) {
-        const double r2c = fmin(r2, 63.99);
-        const int b = max((int)(__float_as_uint((float)r2c) >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE, 0);
-        t = r2c - (double)__uint_as_float((uint32_t)(b + (int)LAGPM_LOG_BASE) << LAGPM_LOG_SHIFT);
-        return b;
+        const double r2c = fmin((double)lam * r2, 63.99);
+        t = r2;  // bins hold c + s * r2, zero beyond the cutoff (see table_bin)
+        return max((int)(__float_as_uint((float)r2c) >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE, 0);
     } else {
         const double ran = 6.0 / 4096.0;
         const double r = sqrt(r2);
@@ -516,7 +514,7 @@ template <bool LOGT>
 __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* __restrict__ B, const uint32_t* __restrict__ gprefix,
                                                                 int nbox, const float2* __restrict__ tab0,
                                                                 const float4* __restrict__ tab12, int* __restrict__ stacks,
-                                                                unsigned long long* __restrict__ ticket_ctr) {
+                                                                unsigned long long* __restrict__ ticket_ctr, float lam) {
     // One 1024-thread workgroup per CU: the 16 waves share one LDS copy of the particle force
     // table (32 KB) next to their private interaction lists.
     __shared__ float2 s_tab0[TAB_LDS];
@@ -608,21 +606,48 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             }
             return s;
         };
+        // Pair sums over one staged chunk. Sources are taken four at a time: their LDS loads,
+        // then their table loads, are issued together so one wave's LDS latency overlaps its
+        // own arithmetic (a loop of single sources waits on two dependent LDS reads per pair).
+        auto pp_one = [&](float4 s) {
+            float dx = xt.x - s.x, dy = xt.y - s.y, dz = xt.z - s.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float t;
+            const int nt = table_bin<LOGT>(r2, t, lam);
+            float2 tb = s_tab0[nt];
+            float f = tb.x + tb.y * t;  // particle masses are 1 here: the box mass is in fact
+            if constexpr (!LOGT) f = (r2 <= A.rc2) ? f : 0.f;
+            ax += dx * f; ay += dy * f; az += dz * f;
+        };
         auto pp_chunk = [&]() {
             const int iters = 64 / R;
-#pragma unroll 2
-            for (int k = 0; k < iters; k++) {
-                float4 s = cpp[rep + k * R];
-                float dx = xt.x - s.x, dy = xt.y - s.y, dz = xt.z - s.z;
-                float r2 = dx * dx + dy * dy + dz * dz;
-                if (!__any(r2 <= A.rc2)) continue;  // wave-uniform skip of far sources
-                float t;
-                const int nt = table_bin<LOGT>(r2, t);
-                float2 tb = s_tab0[nt];
-                float f = tb.x + tb.y * t;  // particle masses are 1 here: the box mass is in fact
-                f = (r2 <= A.rc2) ? f : 0.f;
-                ax += dx * f; ay += dy * f; az += dz * f;
+            int k = 0;
+            for (; k + 4 <= iters; k += 4) {
+                float4 s[4];
+                float dx[4], dy[4], dz[4], r2[4];
+                bool any = false;
+#pragma unroll
+                for (int u = 0; u < 4; u++) s[u] = cpp[rep + (k + u) * R];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    dx[u] = xt.x - s[u].x; dy[u] = xt.y - s[u].y; dz[u] = xt.z - s[u].z;
+                    r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+                    if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
+                }
+                if constexpr (!LOGT)
+                    if (!__any(any)) continue;  // wave-uniform skip of four far sources
+                float2 tb[4];
+                float t[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) tb[u] = s_tab0[table_bin<LOGT>(r2[u], t[u], lam)];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    float f = tb[u].x + tb[u].y * t[u];
+                    if constexpr (!LOGT) f = (r2[u] <= A.rc2) ? f : 0.f;
+                    ax += dx[u] * f; ay += dy[u] * f; az += dz[u] * f;
+                }
             }
+            for (; k < iters; k++) pp_one(cpp[rep + k * R]);
         };
         // flush `nchunk` consecutive chunks starting at list entry `start` (the last may be
         // partial: `mlast` entries); the next chunk's global loads overlap this chunk's math
@@ -653,18 +678,9 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 cpc2[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
             wave_sync();
-            const int iters = 64 / R;
-#pragma unroll 2
-            for (int k = 0; k < iters; k++) {
-                const int j = rep + k * R;
-                float4 c = cpc0[j], qa = cpc1[j], qb = cpc2[j];
-                float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
-                float r2 = dx * dx + dy * dy + dz * dz;
-                if (!__any(r2 <= A.rc2)) continue;  // wave-uniform skip of far nodes
-                float t;
-                const int nt = table_bin<LOGT>(r2, t);
-                float2 t0 = s_tab0[nt];
-                float4 t12 = tab12[nt];
+            // nodes two at a time, loads first, as for the particles
+            auto pc_eval = [&](float4 c, float4 qa, float4 qb, float2 t0, float4 t12, float t, float dx, float dy,
+                               float dz, bool in) {
                 float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
                 float qx1 = qa.x * dx + qa.w * dy + qb.x * dz;
                 float qx2 = qa.w * dx + qa.y * dy + qb.y * dz;
@@ -673,12 +689,54 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                             2.f * (qa.w * dx * dy + qb.x * dx * dz + qb.y * dy * dz);
                 float tmp = c.w * f1 + qb.z * f2 + qxy * f3;
                 float two = 2.f * f2;
-                const bool in = r2 <= A.rc2;
-                tmp = in ? tmp : 0.f;
-                two = in ? two : 0.f;
+                if constexpr (!LOGT) {
+                    tmp = in ? tmp : 0.f;
+                    two = in ? two : 0.f;
+                }
                 ax += dx * tmp + qx1 * two;
                 ay += dy * tmp + qx2 * two;
                 az += dz * tmp + qx3 * two;
+            };
+            const int iters = 64 / R;
+            int k = 0;
+            for (; k + 2 <= iters; k += 2) {
+                float4 c[2], qa[2], qb[2];
+                float dx[2], dy[2], dz[2], r2[2];
+                bool any = false;
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int j = rep + (k + u) * R;
+                    c[u] = cpc0[j]; qa[u] = cpc1[j]; qb[u] = cpc2[j];
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    dx[u] = xt.x - c[u].x; dy[u] = xt.y - c[u].y; dz[u] = xt.z - c[u].z;
+                    r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+                    if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
+                }
+                if constexpr (!LOGT)
+                    if (!__any(any)) continue;  // wave-uniform skip of two far nodes
+                float t[2];
+                int nt[2];
+                float2 t0[2];
+                float4 t12[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    nt[u] = table_bin<LOGT>(r2[u], t[u], lam);
+                    t0[u] = s_tab0[nt[u]];
+                    t12[u] = tab12[nt[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) pc_eval(c[u], qa[u], qb[u], t0[u], t12[u], t[u], dx[u], dy[u], dz[u], r2[u] <= A.rc2);
+            }
+            for (; k < iters; k++) {
+                const int j = rep + k * R;
+                float4 c = cpc0[j], qa = cpc1[j], qb = cpc2[j];
+                float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
+                float r2 = dx * dx + dy * dy + dz * dz;
+                float t;
+                const int nt = table_bin<LOGT>(r2, t, lam);
+                pc_eval(c, qa, qb, s_tab0[nt], tab12[nt], t, dx, dy, dz, r2 <= A.rc2);
             }
             npc += (unsigned long long)m;
             wave_sync();
@@ -816,6 +874,7 @@ struct walk64_args {
     const double* gbox;
     const int* pgroup;
     const double* tab;
+    float lam;
     double *vx, *vy, *vz;
     unsigned long long* counters;
     const uint32_t* dsz;
@@ -869,7 +928,7 @@ __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
             double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 <= A.rc2) {
                 double t;
-                const int nt = table_bin64<LOGT>(r2, t);
+                const int nt = table_bin64<LOGT>(r2, t, A.lam);
                 const double* tb = A.tab + 6 * nt;
                 double f1 = tb[0] + tb[1] * t, f2 = tb[2] + tb[3] * t, f3 = tb[4] + tb[5] * t;
                 double qx1 = m[4] * dx + m[7] * dy + m[8] * dz;
@@ -888,7 +947,7 @@ __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
                 double r2 = dx * dx + dy * dy + dz * dz;
                 if (r2 <= A.rc2) {
                     double t;
-                    const int nt = table_bin64<LOGT>(r2, t);
+                    const int nt = table_bin64<LOGT>(r2, t, A.lam);
                     double f = A.tab[6 * nt] + A.tab[6 * nt + 1] * t;  // unit particle mass
                     ax += dx * f; ay += dy * f; az += dz * f;
                 }
@@ -1087,10 +1146,10 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
     const unsigned blocks = (unsigned)(ctx->walk_stack_entries / ((size_t)STACK_MAX * WALK_WAVES));
     if (tab->logr2)
         hipLaunchKernelGGL(walk_f32_kernel<true>, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes,
-                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
+                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket, tab->lam);
     else
         hipLaunchKernelGGL(walk_f32_kernel<false>, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes,
-                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
+                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket, tab->lam);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }
@@ -1103,7 +1162,7 @@ int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact) {
     A.nd = box->nd_d; A.cellroot = box->cellroot; A.gbox = box->gbox;
     A.key = box->key; A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell); A.pgroup = box->pgroup; A.tab = box->tab->tabd;
     A.vx = box->vx; A.vy = box->vy; A.vz = box->vz; A.counters = box->counters; A.dsz = box->dsz; A.np = box->np;
-    A.mcell = box->mcell; A.box = (double)box->p.n; A.cellw = cellw; A.rc2 = box->p.rsphere * box->p.rsphere;
+    A.mcell = box->mcell; A.box = (double)box->p.n; A.cellw = cellw; A.rc2 = box->tab->rc2_eff; A.lam = box->tab->lam;
     A.rcut = box->p.rsphere; A.inv_theta2 = th > 0 ? 1.0 / (th * th) : 1e30;
     A.fact = fact * box->p.mass / ((double)box->p.n * (double)box->p.n);
     A.naccept = LAGPM_NACCEPT;
