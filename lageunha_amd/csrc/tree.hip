@@ -27,10 +27,11 @@ namespace {
 
 constexpr int WALK_THREADS = 1024;  // one workgroup per CU (LDS: shared table + 16 wave lists)
 constexpr int WALK_WAVES = WALK_THREADS / 64;
-constexpr int PP_TAKE = 8;          // particles a lane appends per round
+constexpr int PP_TAKE = 4;          // particles a lane appends per round
 constexpr int PP_CAP = 64 * PP_TAKE + 64;  // one round + 63 left over
 constexpr int PC_CAP = 128;
-constexpr int STACK_MAX = 8192;  // per-wave walk stack entries (global memory)
+constexpr int STACK_MAX = 8192;  // per-wave walk stack entries
+constexpr int LSTACK = 256;      // the first LSTACK entries of a wave's stack live in LDS, the rest in global memory
 constexpr int WALK_MAX_ITERS = 1 << 20;  // per-group bound on 64-node pop rounds
 // Internal nodes up to this many particles get the exact max distance from their centre of
 // mass (the reference's nodesize); larger ones use the bound max(|d_child| + rmax_child).
@@ -519,6 +520,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
     // table (32 KB) next to their private interaction lists.
     __shared__ float2 s_tab0[TAB_LDS];
     __shared__ uint32_t s_ppidx[WALK_WAVES][PP_CAP];
+    __shared__ int s_stack[WALK_WAVES][LSTACK];
     __shared__ int s_pcidx[WALK_WAVES][PC_CAP];
     __shared__ float4 s_pp[WALK_WAVES][64];
     __shared__ float4 s_pc[WALK_WAVES][3][64];
@@ -532,7 +534,14 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
     float4* cpc0 = s_pc[wid][0];
     float4* cpc1 = s_pc[wid][1];
     float4* cpc2 = s_pc[wid][2];
-    int* stack = stacks + (size_t)(blockIdx.x * WALK_WAVES + wid) * STACK_MAX;
+    int* gstack = stacks + (size_t)(blockIdx.x * WALK_WAVES + wid) * STACK_MAX;
+    int* lstack = s_stack[wid];
+    // stack entry i: LDS below LSTACK (96% of pop rounds stay there), global memory above
+    auto sget = [&](int i) -> int { return i < LSTACK ? lstack[i] : gstack[i]; };
+    auto sset = [&](int i, int v) {
+        if (i < LSTACK) lstack[i] = v;
+        else gstack[i] = v;
+    };
     const int gtotal = __builtin_amdgcn_readfirstlane((int)gprefix[nbox]);
 
     for (;;) {
@@ -581,7 +590,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             }
             int tot;
             int off = wave_excl_scan(e >= 0 ? 1 : 0, lane, tot);
-            if (e >= 0) stack[off] = e;
+            if (e >= 0) sset(off, e);
             sp = tot;
         }
         int pp_n = 0, pc_n = 0;
@@ -750,7 +759,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             }
             const int take = min(sp, 64);
             sp -= take;
-            int e = lane < take ? stack[sp + lane] : -1;
+            int e = lane < take ? sget(sp + lane) : -1;
             int kind = 0;  // 0 none, 1 accept, 2 leaf, 3 open
             int4 nd = make_int4(0, 0, 0, 0);
             int v = 0, nb = 0;
@@ -782,7 +791,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                     overflow = true;
                     break;
                 }
-                for (int c = 0; c < nch; c++) stack[sp + off + c] = (A.child[8 * v + c] << 5) | nb;
+                for (int c = 0; c < nch; c++) sset(sp + off + c, (A.child[8 * v + c] << 5) | nb);
                 sp += tot;
                 wave_sync();
             }
