@@ -345,7 +345,7 @@ int lagpm_create(lagpm_ctx** out, int device) {
     for (int i = 0; i < 4; i++) (void)hipEventCreate(&ctx->ev[i]);
     ctx->walk_stack_entries = lagpm_walk_stack_entries(ctx->num_cu);
     if (hipMalloc((void**)&ctx->walk_stack, ctx->walk_stack_entries * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&ctx->walk_ticket, sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc((void**)&ctx->walk_ticket, LAGPM_WALK_XCDS * sizeof(unsigned long long)) != hipSuccess) {
         (void)hipGetLastError();
         lagpm_destroy(ctx);
         return LAGPM_E_NOMEM;

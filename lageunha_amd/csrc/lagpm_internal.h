@@ -42,6 +42,9 @@ struct law_tables {
 
 struct walk_box;  // device view of one box for the walk (tree.hip)
 
+// XCDs of one MI355X in SPX mode: the walk splits its groups into this many ranges.
+constexpr int LAGPM_WALK_XCDS = 8;
+
 struct lagpm_ctx {
     int device = 0;
     int num_cu = 0;
@@ -53,7 +56,7 @@ struct lagpm_ctx {
     // walk resources (every walk runs on `stream`, so one set serves all boxes)
     int* walk_stack = nullptr;
     size_t walk_stack_entries = 0;
-    unsigned long long* walk_ticket = nullptr;
+    unsigned long long* walk_ticket = nullptr;  // LAGPM_WALK_XCDS group tickets, one per XCD range
     walk_box* walk_boxes = nullptr;          // descriptor array of the last launch
     uint32_t* walk_prefix = nullptr;         // group prefix over the descriptors
     size_t walk_box_cap = 0;

@@ -18,12 +18,14 @@
 // cell trees for its group with a group-conservative opening test, stages accepted nodes
 // and opened leaves in LDS, and evaluates the interactions lane-per-target.
 #include "lagpm_internal.h"
+#include "wave_ops.h"
 
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 namespace {
+using namespace lagpm_wave;
 
 constexpr int WALK_THREADS = 1024;  // one workgroup per CU (LDS: shared table + 16 wave lists)
 constexpr int WALK_WAVES = WALK_THREADS / 64;
@@ -31,6 +33,7 @@ constexpr int PP_TAKE = 4;          // particles a lane appends per round
 constexpr int PP_CAP = 64 * PP_TAKE + 64;  // one round + 63 left over
 constexpr int PC_CAP = 128;
 constexpr int STACK_MAX = 8192;  // per-wave walk stack entries
+constexpr int WALK_XCDS = LAGPM_WALK_XCDS;
 constexpr int LSTACK = 256;      // the first LSTACK entries of a wave's stack live in LDS, the rest in global memory
 constexpr int WALK_MAX_ITERS = 1 << 20;  // per-group bound on 64-node pop rounds
 // Internal nodes up to this many particles get the exact max distance from their centre of
@@ -47,9 +50,13 @@ __device__ inline int table_bin(float r2, float& t, float lam) {
         // Each bin holds the line c + s * r2 (bin 0 runs from r2 = 0, so it serves every r2 below
         // its node); bins beyond the cutoff hold zeros, so far pairs and the far-away dummies of a
         // partial chunk add nothing without a compare. r2 above the table is clamped to its last bin.
-        const float r2c = __builtin_amdgcn_fmed3f(lam * r2, 0.0f, 63.99f);  // >= 0: min(lam r2, 63.99), one op
+        // lam r2 >= 0, so its bit pattern orders like the value: one unsigned clamp of the bits to
+        // [bin 0's node, just below 64] (a v_med3_u32) replaces a float clamp and an integer max.
+        constexpr uint32_t LO = (uint32_t)LAGPM_LOG_BASE << LAGPM_LOG_SHIFT;
+        constexpr uint32_t HI = ((uint32_t)(LAGPM_LOG_BASE + LAGPM_LOG_BINS) << LAGPM_LOG_SHIFT) - 1u;
+        const uint32_t b = min(max(__float_as_uint(lam * r2), LO), HI);
         t = r2;
-        return max((int)(__float_as_uint(r2c) >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE, 0);
+        return (int)(b >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE;
     } else {
         const float r = __builtin_amdgcn_sqrtf(r2);
         const int nt = min((int)(r * (1.0f / RAN_F)), 4095);
@@ -433,33 +440,15 @@ __global__ void tree_reset_kernel(uint32_t* __restrict__ dsz, unsigned long long
 
 // ---------------------------------------------------------------------------------------
 // Walk helpers
+template <int V>
+struct ic {  // a compile-time int for generic lambdas
+    static constexpr int value = V;
+};
+
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-// Exclusive wave prefix sum. The total is read from lane 63 into a scalar register, so every
-// loop or branch the walk takes on it is wave-uniform by construction (scalar branches): the
-// compiler can then never split the wave's lanes across different iterations of the walk.
-__device__ inline int wave_excl_scan(int v, int lane, int& total) {
-    int incl = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
-    total = __builtin_amdgcn_readlane(incl, 63);
-    return incl - v;
-}
-__device__ inline float wave_minf(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline float wave_maxf(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
 }
 // periodic image offset of neighbour nb (0..26) of cell (cx,cy,cz)
 __device__ inline float image_shift(int c, int d, int mcell, float box) {
@@ -497,7 +486,7 @@ struct walk_box {
 namespace {
 
 // Group prefix over the boxes of one walk launch (a box whose build overflowed contributes no
-// groups and so gets no kick) and the reset of the shared group ticket.
+// groups and so gets no kick) and the reset of the group tickets (one per XCD range).
 __global__ void walk_prefix_kernel(const walk_box* __restrict__ B, int nbox, uint32_t* __restrict__ prefix,
                                    unsigned long long* __restrict__ ticket) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -508,7 +497,7 @@ __global__ void walk_prefix_kernel(const walk_box* __restrict__ B, int nbox, uin
         s += d[DSZ_OVERFLOW] ? 0u : d[DSZ_GROUPS];
     }
     prefix[nbox] = s;
-    *ticket = 0ull;
+    for (int x = 0; x < WALK_XCDS; x++) ticket[x] = 0ull;
 }
 
 template <bool LOGT>
@@ -543,15 +532,30 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         else gstack[i] = v;
     };
     const int gtotal = __builtin_amdgcn_readfirstlane((int)gprefix[nbox]);
+    // XCD-local group ranges. Workgroups are dispatched to the XCDs round robin (block b runs on
+    // XCD b % 8), and each XCD has its own L2. The groups are in Morton order, so giving XCD x
+    // the contiguous range x of 8 keeps the sources and nodes that neighbouring groups share in
+    // one L2; a single global ticket would spread neighbours over all eight caches. An XCD that
+    // has finished its range takes tickets from the next ranges in turn (work stealing), and a
+    // wave leaves after finding all eight ranges exhausted.
+    auto range_lo = [&](int x) -> int { return (int)(((long long)gtotal * x) / WALK_XCDS); };
+    int slot = (int)(blockIdx.x % WALK_XCDS), tries = 0;
 
     for (;;) {
         // Group ticket. Every lane executes the atomic (lane 0 adds 1, the others 0) and the
         // ticket is read from lane 0 into a scalar register: no lane-dependent branch sits at
         // the loop head, so the compiler cannot thread the back edge differently per lane
         // (which would let lanes 1..63 re-enter a group without a new ticket).
-        const unsigned long long ticket = atomicAdd(ticket_ctr, lane == 0 ? 1ull : 0ull);
-        const int g = __builtin_amdgcn_readlane((int)ticket, 0);
-        if (g >= gtotal) break;
+        const unsigned long long ticket = atomicAdd(ticket_ctr + slot, lane == 0 ? 1ull : 0ull);
+        const long long t = (long long)__builtin_amdgcn_readlane((int)ticket, 0) |
+                            ((long long)__builtin_amdgcn_readlane((int)(ticket >> 32), 0) << 32);
+        const int lo = range_lo(slot), hi = range_lo(slot + 1);
+        if (t >= (long long)(hi - lo)) {
+            if (++tries == WALK_XCDS) break;
+            slot = (slot + 1) % WALK_XCDS;
+            continue;
+        }
+        const int g = lo + (int)t;
         // the box that owns global group g: largest b with gprefix[b] <= g (wave-uniform search)
         int blo = 0, bhi = nbox;
         while (bhi - blo > 1) {
@@ -589,7 +593,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 if (r >= 0) e = (r << 5) | lane;
             }
             int tot;
-            int off = wave_excl_scan(e >= 0 ? 1 : 0, lane, tot);
+            int off = wave_excl_count(e >= 0, tot);
             if (e >= 0) sset(off, e);
             sp = tot;
         }
@@ -603,7 +607,8 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         // Source chunks are staged in LDS as float4 (x, y, z, m); entries past m are far-away
         // dummies of zero mass, so every lane runs the same unrolled trip count and the cutoff
         // is a select (no branch, no exec-mask loop).
-        const float4 dummy = make_float4(1.0e4f, 1.0e4f, 1.0e4f, 0.f);
+        // (8 box lengths away: sources lie within one box length of [0, n), and n >= 4 cutoffs)
+        const float4 dummy = make_float4(8.f * A.box, 8.f * A.box, 8.f * A.box, 0.f);
         auto load_pp = [&](int start, int m) -> float4 {
             float4 s = dummy;
             if (lane < m) {
@@ -615,9 +620,11 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             }
             return s;
         };
-        // Pair sums over one staged chunk. Sources are taken four at a time: their LDS loads,
-        // then their table loads, are issued together so one wave's LDS latency overlaps its
-        // own arithmetic (a loop of single sources waits on two dependent LDS reads per pair).
+        // Pair sums over one staged chunk. Lane (tgt, rep) takes the sources rep, rep + R, ...
+        // (neighbouring lanes read neighbouring entries: no LDS bank conflicts). R is a template
+        // value here, so the loads of one round are immediate offsets from one address. Sources
+        // are taken four at a time: their LDS loads, then their table loads, are issued together
+        // so one wave's LDS latency overlaps its own arithmetic.
         auto pp_one = [&](float4 s) {
             float dx = xt.x - s.x, dy = xt.y - s.y, dz = xt.z - s.z;
             float r2 = dx * dx + dy * dy + dz * dz;
@@ -628,47 +635,159 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             if constexpr (!LOGT) f = (r2 <= A.rc2) ? f : 0.f;
             ax += dx * f; ay += dy * f; az += dz * f;
         };
-        auto pp_chunk = [&]() {
-            const int iters = 64 / R;
-            int k = 0;
-            for (; k + 4 <= iters; k += 4) {
-                float4 s[4];
-                float dx[4], dy[4], dz[4], r2[4];
-                bool any = false;
+        auto pp_chunk_r = [&](auto rc) {
+            constexpr int RR = decltype(rc)::value;
+            constexpr int iters = 64 / RR;
+            const float4* src = cpp + rep;
+            if constexpr (iters >= 4) {
+#pragma unroll 2
+                for (int k = 0; k < iters; k += 4) {
+                    float4 s[4];
+                    float dx[4], dy[4], dz[4], r2[4];
+                    bool any = false;
 #pragma unroll
-                for (int u = 0; u < 4; u++) s[u] = cpp[rep + (k + u) * R];
+                    for (int u = 0; u < 4; u++) s[u] = src[(k + u) * RR];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    dx[u] = xt.x - s[u].x; dy[u] = xt.y - s[u].y; dz[u] = xt.z - s[u].z;
-                    r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-                    if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
+                    for (int u = 0; u < 4; u++) {
+                        dx[u] = xt.x - s[u].x; dy[u] = xt.y - s[u].y; dz[u] = xt.z - s[u].z;
+                        r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+                        if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
+                    }
+                    if constexpr (!LOGT)
+                        if (!__any(any)) continue;  // wave-uniform skip of four far sources
+                    float2 tb[4];
+                    float t[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) tb[u] = s_tab0[table_bin<LOGT>(r2[u], t[u], lam)];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        float f = tb[u].x + tb[u].y * t[u];
+                        if constexpr (!LOGT) f = (r2[u] <= A.rc2) ? f : 0.f;
+                        ax += dx[u] * f; ay += dy[u] * f; az += dz[u] * f;
+                    }
                 }
-                if constexpr (!LOGT)
-                    if (!__any(any)) continue;  // wave-uniform skip of four far sources
-                float2 tb[4];
-                float t[4];
+            } else {
 #pragma unroll
-                for (int u = 0; u < 4; u++) tb[u] = s_tab0[table_bin<LOGT>(r2[u], t[u], lam)];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    float f = tb[u].x + tb[u].y * t[u];
-                    if constexpr (!LOGT) f = (r2[u] <= A.rc2) ? f : 0.f;
-                    ax += dx[u] * f; ay += dy[u] * f; az += dz[u] * f;
-                }
+                for (int k = 0; k < iters; k++) pp_one(src[k * RR]);
             }
-            for (; k < iters; k++) pp_one(cpp[rep + k * R]);
+        };
+        // groups of more than 8 targets (R <= 4) take the unrolled loops; smaller ones a plain loop
+        auto pp_chunk = [&]() {
+            if (R == 1) pp_chunk_r(ic<1>{});
+            else if (R == 2) pp_chunk_r(ic<2>{});
+            else if (R == 4) pp_chunk_r(ic<4>{});
+            else
+                for (int k = 0; k < 64 / R; k++) pp_one(cpp[rep + k * R]);
         };
         // flush `nchunk` consecutive chunks starting at list entry `start` (the last may be
         // partial: `mlast` entries); the next chunk's global loads overlap this chunk's math
-        auto flush_pp = [&](int start, int nchunk, int mlast) {
-            float4 nxt = load_pp(start, nchunk == 1 ? mlast : 64);
-            for (int c = 0; c < nchunk; c++) {
-                cpp[lane] = nxt;
+        // Staging with a cut: a listed particle farther than the cutoff from the group's bounding
+        // box is beyond it from every target, so its pair terms are exactly zero (the table is zero
+        // there, and the reference law's select zeroes it); it is dropped before it costs a chunk
+        // slot. The margin keeps every dropped pair clear of the float-rounded table edge.
+        const float rc2_keep = A.rc2 * (1.0f + 1.0e-5f);
+        auto near_group = [&](float4 q) -> bool {
+            const float ddx = fmaxf(fmaxf(lox - q.x, q.x - hix), 0.f);
+            const float ddy = fmaxf(fmaxf(loy - q.y, q.y - hiy), 0.f);
+            const float ddz = fmaxf(fmaxf(loz - q.z, q.z - hiz), 0.f);
+            return ddx * ddx + ddy * ddy + ddz * ddz <= rc2_keep;
+        };
+        int fill = 0;  // staged particles in cpp[0, fill), wave-uniform
+        // append the kept particles of one 64-entry block to the chunk, computing the chunk when it
+        // fills; survivors past its end start the next chunk
+        auto stage_pp = [&](float4 q, bool keep) {
+            int tot;
+            const int off = wave_excl_count(keep, tot);
+            const int room = 64 - fill;
+            if (keep && off < room) cpp[fill + off] = q;
+            if (tot >= room) {
                 wave_sync();
-                if (c + 1 < nchunk) nxt = load_pp(start + 64 * (c + 1), c + 2 == nchunk ? mlast : 64);
                 pp_chunk();
-                npp += (unsigned long long)(c + 1 == nchunk ? mlast : 64);
+                npp += 64ull;
                 wave_sync();
+                if (keep && off >= room) cpp[off - room] = q;
+                fill = tot - room;
+            } else {
+                fill += tot;
+            }
+            wave_sync();
+        };
+        // stage `nblock` consecutive blocks of the list from entry `start` (the last may be
+        // partial: `mlast` entries); the next block's global loads overlap this block's chunk
+        auto flush_pp = [&](int start, int nblock, int mlast) {
+            float4 nxt = load_pp(start, nblock == 1 ? mlast : 64);
+            for (int c = 0; c < nblock; c++) {
+                const float4 cur = nxt;
+                if (c + 1 < nblock) nxt = load_pp(start + 64 * (c + 1), c + 2 == nblock ? mlast : 64);
+                stage_pp(cur, near_group(cur));  // entries past m are far dummies: never kept
+            }
+        };
+        // One node interaction: monopole plus quadrupole, Q d computed once and reused for
+        // d.Q.d (the quadratic form) and for the 2 f2 Q d term.
+        auto pc_eval = [&](float4 c, float4 qa, float4 qb, float2 t0, float4 t12, float t, float dx, float dy, float dz,
+                           bool in) {
+            float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
+            float qx1 = qa.x * dx + qa.w * dy + qb.x * dz;
+            float qx2 = qa.w * dx + qa.y * dy + qb.y * dz;
+            float qx3 = qb.x * dx + qb.y * dy + qa.z * dz;
+            float qxy = dx * qx1 + dy * qx2 + dz * qx3;
+            float tmp = c.w * f1 + qb.z * f2 + qxy * f3;
+            float two = 2.f * f2;
+            if constexpr (!LOGT) {
+                tmp = in ? tmp : 0.f;
+                two = in ? two : 0.f;
+            }
+            ax = fmaf(qx1, two, fmaf(dx, tmp, ax));
+            ay = fmaf(qx2, two, fmaf(dy, tmp, ay));
+            az = fmaf(qx3, two, fmaf(dz, tmp, az));
+        };
+        auto pc_chunk_r = [&](auto rc) {
+            constexpr int RR = decltype(rc)::value;
+            constexpr int iters = 64 / RR;
+            const float4* n0 = cpc0 + rep;
+            const float4* n1 = cpc1 + rep;
+            const float4* n2 = cpc2 + rep;
+            if constexpr (iters >= 2) {
+                // nodes two at a time, loads first, as for the particles
+#pragma unroll 2
+                for (int k = 0; k < iters; k += 2) {
+                    float4 c[2], qa[2], qb[2];
+                    float dx[2], dy[2], dz[2], r2[2];
+                    bool any = false;
+#pragma unroll
+                    for (int u = 0; u < 2; u++) {
+                        const int j = (k + u) * RR;
+                        c[u] = n0[j]; qa[u] = n1[j]; qb[u] = n2[j];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2; u++) {
+                        dx[u] = xt.x - c[u].x; dy[u] = xt.y - c[u].y; dz[u] = xt.z - c[u].z;
+                        r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+                        if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
+                    }
+                    if constexpr (!LOGT)
+                        if (!__any(any)) continue;  // wave-uniform skip of two far nodes
+                    float t[2];
+                    int nt[2];
+                    float2 t0[2];
+                    float4 t12[2];
+#pragma unroll
+                    for (int u = 0; u < 2; u++) {
+                        nt[u] = table_bin<LOGT>(r2[u], t[u], lam);
+                        t0[u] = s_tab0[nt[u]];
+                        t12[u] = tab12[nt[u]];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 2; u++)
+                        pc_eval(c[u], qa[u], qb[u], t0[u], t12[u], t[u], dx[u], dy[u], dz[u], r2[u] <= A.rc2);
+                }
+            } else {
+                const float4 c = n0[0], qa = n1[0], qb = n2[0];
+                float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
+                float r2 = dx * dx + dy * dy + dz * dz;
+                float t;
+                const int nt = table_bin<LOGT>(r2, t, lam);
+                pc_eval(c, qa, qb, s_tab0[nt], tab12[nt], t, dx, dy, dz, r2 <= A.rc2);
             }
         };
         auto flush_pc = [&](int start, int m) {
@@ -687,66 +806,19 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 cpc2[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
             wave_sync();
-            // nodes two at a time, loads first, as for the particles
-            auto pc_eval = [&](float4 c, float4 qa, float4 qb, float2 t0, float4 t12, float t, float dx, float dy,
-                               float dz, bool in) {
-                float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
-                float qx1 = qa.x * dx + qa.w * dy + qb.x * dz;
-                float qx2 = qa.w * dx + qa.y * dy + qb.y * dz;
-                float qx3 = qb.x * dx + qb.y * dy + qa.z * dz;
-                float qxy = qa.x * dx * dx + qa.y * dy * dy + qa.z * dz * dz +
-                            2.f * (qa.w * dx * dy + qb.x * dx * dz + qb.y * dy * dz);
-                float tmp = c.w * f1 + qb.z * f2 + qxy * f3;
-                float two = 2.f * f2;
-                if constexpr (!LOGT) {
-                    tmp = in ? tmp : 0.f;
-                    two = in ? two : 0.f;
+            if (R == 1) pc_chunk_r(ic<1>{});
+            else if (R == 2) pc_chunk_r(ic<2>{});
+            else if (R == 4) pc_chunk_r(ic<4>{});
+            else
+                for (int k = 0; k < 64 / R; k++) {
+                    const int j = rep + k * R;
+                    const float4 c = cpc0[j], qa = cpc1[j], qb = cpc2[j];
+                    const float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    float t;
+                    const int nt = table_bin<LOGT>(r2, t, lam);
+                    pc_eval(c, qa, qb, s_tab0[nt], tab12[nt], t, dx, dy, dz, r2 <= A.rc2);
                 }
-                ax += dx * tmp + qx1 * two;
-                ay += dy * tmp + qx2 * two;
-                az += dz * tmp + qx3 * two;
-            };
-            const int iters = 64 / R;
-            int k = 0;
-            for (; k + 2 <= iters; k += 2) {
-                float4 c[2], qa[2], qb[2];
-                float dx[2], dy[2], dz[2], r2[2];
-                bool any = false;
-#pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    const int j = rep + (k + u) * R;
-                    c[u] = cpc0[j]; qa[u] = cpc1[j]; qb[u] = cpc2[j];
-                }
-#pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    dx[u] = xt.x - c[u].x; dy[u] = xt.y - c[u].y; dz[u] = xt.z - c[u].z;
-                    r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-                    if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
-                }
-                if constexpr (!LOGT)
-                    if (!__any(any)) continue;  // wave-uniform skip of two far nodes
-                float t[2];
-                int nt[2];
-                float2 t0[2];
-                float4 t12[2];
-#pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    nt[u] = table_bin<LOGT>(r2[u], t[u], lam);
-                    t0[u] = s_tab0[nt[u]];
-                    t12[u] = tab12[nt[u]];
-                }
-#pragma unroll
-                for (int u = 0; u < 2; u++) pc_eval(c[u], qa[u], qb[u], t0[u], t12[u], t[u], dx[u], dy[u], dz[u], r2[u] <= A.rc2);
-            }
-            for (; k < iters; k++) {
-                const int j = rep + k * R;
-                float4 c = cpc0[j], qa = cpc1[j], qb = cpc2[j];
-                float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
-                float r2 = dx * dx + dy * dy + dz * dz;
-                float t;
-                const int nt = table_bin<LOGT>(r2, t, lam);
-                pc_eval(c, qa, qb, s_tab0[nt], tab12[nt], t, dx, dy, dz, r2 <= A.rc2);
-            }
             npc += (unsigned long long)m;
             wave_sync();
         };
@@ -762,6 +834,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             int e = lane < take ? sget(sp + lane) : -1;
             int kind = 0;  // 0 none, 1 accept, 2 leaf, 3 open
             int4 nd = make_int4(0, 0, 0, 0);
+            int4 ch0 = make_int4(0, 0, 0, 0), ch1 = ch0;
             int v = 0, nb = 0;
             if (e >= 0) {
                 v = e >> 5;
@@ -771,6 +844,10 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 float4 c = A.com[v];
                 float rmax = A.q1[v].w;
                 nd = A.info[v];
+                // the child slots (8 ints, 32-byte aligned) load with the node itself, so opening
+                // it needs no second dependent round trip to memory
+                ch0 = reinterpret_cast<const int4*>(A.child)[2 * v];
+                ch1 = reinterpret_cast<const int4*>(A.child)[2 * v + 1];
                 float px = c.x + sx, py = c.y + sy, pz = c.z + sz;
                 float ddx = fmaxf(fmaxf(lox - px, px - hix), 0.f);
                 float ddy = fmaxf(fmaxf(loy - py, py - hiy), 0.f);
@@ -791,14 +868,17 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                     overflow = true;
                     break;
                 }
-                for (int c = 0; c < nch; c++) sset(sp + off + c, (A.child[8 * v + c] << 5) | nb);
+                const int chv[8] = {ch0.x, ch0.y, ch0.z, ch0.w, ch1.x, ch1.y, ch1.z, ch1.w};
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+                    if (c < nch) sset(sp + off + c, (chv[c] << 5) | nb);
                 sp += tot;
                 wave_sync();
             }
             // accepted nodes
             {
                 int tot;
-                int off = wave_excl_scan(kind == 1 ? 1 : 0, lane, tot);
+                int off = wave_excl_count(kind == 1, tot);
                 if (kind == 1) pcidx[pc_n + off] = e;
                 pc_n += tot;
                 wave_sync();
@@ -850,6 +930,13 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         }
         if (pc_n > 0) flush_pc(0, pc_n);
         if (pp_n > 0) flush_pp(0, 1, pp_n);
+        if (fill > 0) {  // the last, partial chunk: far zero-mass dummies after the staged particles
+            if (lane >= fill) cpp[lane] = dummy;
+            wave_sync();
+            pp_chunk();
+            npp += (unsigned long long)fill;
+            wave_sync();
+        }
         // sum the R partial accumulators of each target (lanes tgt, tgt + Tw, ...)
         for (int o = Tw; o < 64; o <<= 1) {
             ax += __shfl_xor(ax, o, 64);
