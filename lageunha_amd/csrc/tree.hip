@@ -374,26 +374,62 @@ __global__ void moments_f64_kernel(const double* __restrict__ x, const double* _
 
 // Groups: the first node of size <= G on each root-to-leaf path; max-depth leaves larger
 // than G (coincident keys) are cut into chunks of G.
+// Target groups. A node of at most G particles whose parent holds more than G is a group, as is
+// a cell root of at most G; a leaf of more than G (coincident particles) splits into chunks of G.
+// Consecutive sibling groups are merged while their sum stays <= G: a parent of more than G packs
+// its small children, in order, into runs of contiguous particles. The children of a perturbed
+// lattice or a halo are uneven (many of a few particles each), so merging cuts the number of
+// groups, and with it of tree walks, to about half while keeping each group inside its parent.
+// The parent (or the big leaf, or the small cell root) owns its groups; group_pack visits them.
+template <typename F>
+__device__ inline uint32_t group_pack(const int4& nd, int L, int G, const int4* __restrict__ info,
+                                      const int* __restrict__ child, const int* __restrict__ parent, size_t v,
+                                      F&& emit) {
+    if (nd.y <= G) {
+        // a small node below the cell level belongs to its parent's runs (a node the child table
+        // does not link, parent -1, is a group of its own)
+        if (nd.w != L && parent[v] >= 0) return 0u;
+        emit(nd.x, nd.y);
+        return 1u;
+    }
+    if (nd.z == 0) {
+        const uint32_t c = (uint32_t)((nd.y + G - 1) / G);
+        for (uint32_t t = 0; t < c; t++) {
+            const int b = nd.x + (int)t * G;
+            emit(b, min(nd.x + nd.y, b + G) - b);
+        }
+        return c;
+    }
+    uint32_t c = 0;
+    int rb = 0, rn = 0;  // the open run: first particle, count
+    for (int k = 0; k < nd.z; k++) {
+        const int4 ci = info[child[8 * v + k]];
+        if (ci.y > G) {  // a big child owns its own groups and ends the run
+            if (rn) { emit(rb, rn); c++; rn = 0; }
+            continue;
+        }
+        if (rn && rn + ci.y > G) { emit(rb, rn); c++; rn = 0; }
+        if (!rn) rb = ci.x;
+        rn += ci.y;
+    }
+    if (rn) { emit(rb, rn); c++; }
+    return c;
+}
 __global__ void group_count_kernel(const uint32_t* __restrict__ dsz, size_t cap, int L, int G, const int4* __restrict__ info,
-                                   const int* __restrict__ parent, uint32_t* __restrict__ gcnt) {
+                                   const int* __restrict__ child, const int* __restrict__ parent,
+                                   uint32_t* __restrict__ gcnt) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= cap) return;
     if (v >= dev_count(dsz, DSZ_NODES, cap)) {  // entries past the node count scan as zero
         gcnt[v] = 0u;
         return;
     }
-    int4 nd = info[v];
-    int pv = parent[v];
-    bool top = (nd.w == L) || pv < 0 || (info[pv].y > G);
-    uint32_t c = 0;
-    if (top) {
-        if (nd.y <= G) c = 1;
-        else if (nd.z == 0) c = (uint32_t)((nd.y + G - 1) / G);
-    }
-    gcnt[v] = c;
+    gcnt[v] = group_pack(info[v], L, G, info, child, parent, v, [](int, int) {});
 }
-__global__ void group_write_kernel(uint32_t* __restrict__ dsz, size_t cap, size_t gcap, int G, const int4* __restrict__ info,
-                                   const uint32_t* __restrict__ gcnt, const uint32_t* __restrict__ goff,
+__global__ void group_write_kernel(uint32_t* __restrict__ dsz, size_t cap, size_t gcap, int L, int G,
+                                   const int4* __restrict__ info, const int* __restrict__ child,
+                                   const int* __restrict__ parent, const uint32_t* __restrict__ gcnt,
+                                   const uint32_t* __restrict__ goff,
                                    int2* __restrict__ groups) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
@@ -403,12 +439,8 @@ __global__ void group_write_kernel(uint32_t* __restrict__ dsz, size_t cap, size_
         dsz[DSZ_OVERFLOW] = 1u;
         return;
     }
-    int4 nd = info[v];
-    for (uint32_t t = 0; t < c; t++) {
-        int b = nd.x + (int)t * G;
-        int e = min(nd.x + nd.y, b + G);
-        groups[goff[v] + t] = make_int2(b, e - b);
-    }
+    uint32_t g = goff[v];
+    group_pack(info[v], L, G, info, child, parent, v, [&](int b, int m) { groups[g++] = make_int2(b, m); });
 }
 __global__ void group_box_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
                                  const uint32_t* __restrict__ dsz, size_t gcap, const int2* __restrict__ groups,
@@ -1192,11 +1224,11 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
         HIPCHK(ctx, hipGetLastError());
     }
     hipLaunchKernelGGL(group_count_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, L, G, box->nd_info,
-                       box->nd_parent, box->gflag);
+                       box->nd_child, box->nd_parent, box->gflag);
     HIPCHK(ctx, hipGetLastError());
     RCHK(lagpm_exclusive_scan_u32(ctx, st, box->scan_tmp, box->gflag, box->goff, cap, box->dsz + DSZ_GROUPS));
-    hipLaunchKernelGGL(group_write_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, gcap, G,
-                       box->nd_info, box->gflag, box->goff, box->groups);
+    hipLaunchKernelGGL(group_write_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, gcap, L, G,
+                       box->nd_info, box->nd_child, box->nd_parent, box->gflag, box->goff, box->groups);
     HIPCHK(ctx, hipGetLastError());
     if (f64) {
         hipLaunchKernelGGL(group_box_kernel, dim3(blocks_for(gcap, 256)), dim3(256), 0, st, box->x, box->y, box->z,
