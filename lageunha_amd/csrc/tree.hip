@@ -36,9 +36,6 @@ constexpr int STACK_MAX = 8192;  // per-wave walk stack entries
 constexpr int WALK_XCDS = LAGPM_WALK_XCDS;
 constexpr int LSTACK = 256;      // the first LSTACK entries of a wave's stack live in LDS, the rest in global memory
 constexpr int WALK_MAX_ITERS = 1 << 20;  // per-group bound on 64-node pop rounds
-// Internal nodes up to this many particles get the exact max distance from their centre of
-// mass (the reference's nodesize); larger ones use the bound max(|d_child| + rmax_child).
-constexpr int RMAX_EXACT_MAX = 512;
 constexpr float RAN_F = (float)(6.0 / 4096.0);
 constexpr int TAB_LDS = LAGPM_LOG_BINS > 4096 ? LAGPM_LOG_BINS : 4096;
 
@@ -130,6 +127,31 @@ __device__ inline size_t lower_bound_pfx(const uint64_t* __restrict__ K, size_t 
     return lo;
 }
 
+// Galloping searches from a particle i whose own prefix is pfx: the run of equal prefixes around
+// i is usually short (a node's particles), so doubling steps outward and then bisecting the last
+// step costs about 2 log2(run) loads instead of log2(N) over the whole key array.
+// First index in [i, hi) whose prefix exceeds pfx (hi if none); K[i] >> sh == pfx.
+__device__ inline size_t upper_gallop(const uint64_t* __restrict__ K, size_t i, size_t hi, uint64_t pfx, int sh) {
+    size_t lo = i + 1, step = 1, probe = i + 1;
+    while (probe < hi && (K[probe] >> sh) == pfx) {
+        lo = probe + 1;
+        step <<= 1;
+        probe = i + step;
+    }
+    return upper_bound_pfx(K, lo, probe < hi ? probe : hi, pfx, sh);
+}
+// First index in [lo0, i] whose prefix equals pfx; K[i] >> sh == pfx and earlier prefixes are <= pfx.
+__device__ inline size_t lower_gallop(const uint64_t* __restrict__ K, size_t lo0, size_t i, uint64_t pfx, int sh) {
+    size_t hi = i, step = 1;
+    for (;;) {
+        if (i < lo0 + step) return lower_bound_pfx(K, lo0, hi, pfx, sh);
+        const size_t probe = i - step;
+        if ((K[probe] >> sh) < pfx) return lower_bound_pfx(K, probe + 1, hi, pfx, sh);
+        hi = probe;
+        step <<= 1;
+    }
+}
+
 __global__ void keys_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
                             size_t np, double s, uint64_t* __restrict__ key, uint32_t* __restrict__ perm) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -162,15 +184,15 @@ __device__ inline int node_levels(const uint64_t* __restrict__ K, size_t N, size
     if (l0 > L) {
         int sh = key_shift(l0 - 1);
         uint64_t pfx = k >> sh;
-        size_t s = lower_bound_pfx(K, 0, i, pfx, sh);
-        size_t e = upper_bound_pfx(K, i, N, pfx, sh);
+        size_t s = lower_gallop(K, 0, i, pfx, sh);
+        size_t e = upper_gallop(K, i, N, pfx, sh);
         if (e - s <= (size_t)nleaf) return 0;  // parent is a leaf
         epar = e;
     }
     int c = 0;
     for (int l = l0; l <= LAGPM_KEY_BITS; l++) {
         int sh = key_shift(l);
-        size_t e = upper_bound_pfx(K, i, epar, k >> sh, sh);
+        size_t e = upper_gallop(K, i, epar, k >> sh, sh);
         if (ends) ends[c] = (uint32_t)e;
         c++;
         if (e - i <= (size_t)nleaf) break;
@@ -255,16 +277,15 @@ __device__ inline void kahan(float& s, float& c, float v) {
 }
 
 // FP32 moments with Kahan-compensated sums, one level per launch (deepest first).
-__global__ void moments_f32_kernel(const float4* __restrict__ pp, const uint32_t* __restrict__ dsz, size_t cap, int level,
-                                   const int4* __restrict__ info, const int* __restrict__ child, float4* __restrict__ com,
-                                   float4* __restrict__ q0, float4* __restrict__ q1) {
-    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
-    int4 nd = info[v];
-    if (nd.w != level) return;
+// Node moments, one node per thread: the com (Kahan sums relative to the first member), the
+// second moments about it (Kahan), and rmax. Leaves read their particles, inner nodes their
+// children, whose moments the previous level's launch wrote.
+__device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp, const int4* __restrict__ info,
+                                        const int* __restrict__ child, float4* __restrict__ com, float4* __restrict__ q0,
+                                        float4* __restrict__ q1) {
+    const int4 nd = info[v];
     float M = 0, cM = 0, Sx = 0, cx = 0, Sy = 0, cy = 0, Sz = 0, cz = 0;
     float Q[6] = {0, 0, 0, 0, 0, 0}, cQ[6] = {0, 0, 0, 0, 0, 0};
-    float rmax = 0;
     float3 cm;
     if (nd.z == 0) {
         const float4 r = pp[nd.x];
@@ -285,7 +306,6 @@ __global__ void moments_f32_kernel(const float4* __restrict__ pp, const uint32_t
             kahan(Q[3], cQ[3], a.w * dx * dy);
             kahan(Q[4], cQ[4], a.w * dx * dz);
             kahan(Q[5], cQ[5], a.w * dy * dz);
-            rmax = fmaxf(rmax, sqrtf(dx * dx + dy * dy + dz * dz));
         }
     } else {
         const float4 r = com[child[8 * v]];
@@ -307,32 +327,93 @@ __global__ void moments_f32_kernel(const float4* __restrict__ pp, const uint32_t
             kahan(Q[3], cQ[3], b0.w + a.w * dx * dy);
             kahan(Q[4], cQ[4], b1.x + a.w * dx * dz);
             kahan(Q[5], cQ[5], b1.y + a.w * dy * dz);
-            rmax = fmaxf(rmax, sqrtf(dx * dx + dy * dy + dz * dz) + b1.w);
-        }
-        if (nd.y <= RMAX_EXACT_MAX) {  // exact max distance, as the reference's nodesize
-            float r2 = 0.f;
-            for (int p = nd.x; p < nd.x + nd.y; p++) {
-                float4 a = pp[p];
-                float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
-                r2 = fmaxf(r2, dx * dx + dy * dy + dz * dz);
-            }
-            rmax = sqrtf(r2);
         }
     }
     com[v] = make_float4(cm.x, cm.y, cm.z, M);
     q0[v] = make_float4(Q[0], Q[1], Q[2], Q[3]);
-    q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], rmax);
+    q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], 0.f);  // rmax: rmax_f32_kernel
+}
+
+// Nodes grouped by level (level_count / level_offsets / level_scatter), so a level's moments
+// launch visits exactly its own nodes.
+// Reserve slots in the block's per-level counters h: one LDS atomic per distinct level in the
+// wave (a ballot per level, the lane's rank from mbcnt). Every lane of the wave calls it.
+__device__ inline uint32_t wave_level_reserve(int l, uint32_t* h) {
+    const int lane = threadIdx.x & 63;
+    bool pending = l >= 0;
+    uint32_t r = 0;
+    for (;;) {
+        const unsigned long long act = __ballot(pending);
+        if (!act) break;
+        const int leader = __ffsll((long long)act) - 1;
+        const int lead = __builtin_amdgcn_readlane(l, leader);
+        const bool mine = pending && l == lead;
+        const unsigned long long m = __ballot(mine);
+        uint32_t b0 = 0;
+        if (lane == leader) b0 = atomicAdd(&h[lead], (uint32_t)__popcll(m));
+        b0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, leader);
+        if (mine) {
+            r = b0 + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            pending = false;
+        }
+    }
+    return r;
+}
+__global__ void level_count_kernel(uint32_t* __restrict__ dsz, size_t cap, const int4* __restrict__ info) {
+    __shared__ uint32_t h[LAGPM_KEY_BITS + 1];
+    if (threadIdx.x <= LAGPM_KEY_BITS) h[threadIdx.x] = 0u;
+    __syncthreads();
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int l = -1;
+    if (v < dev_count(dsz, DSZ_NODES, cap)) {
+        l = info[v].w;
+        if (l < 0 || l > LAGPM_KEY_BITS) l = -1;
+    }
+    (void)wave_level_reserve(l, h);
+    __syncthreads();
+    if (threadIdx.x <= LAGPM_KEY_BITS && h[threadIdx.x]) atomicAdd(&dsz[DSZ_LVL + threadIdx.x], h[threadIdx.x]);
+}
+__global__ void level_offsets_kernel(uint32_t* __restrict__ dsz) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t o = 0;
+    for (int l = 0; l <= LAGPM_KEY_BITS; l++) {
+        dsz[DSZ_LOFF + l] = o;
+        o += dsz[DSZ_LVL + l];
+    }
+}
+__global__ void level_scatter_kernel(uint32_t* __restrict__ dsz, size_t cap, const int4* __restrict__ info,
+                                     uint32_t* __restrict__ order) {
+    __shared__ uint32_t h[LAGPM_KEY_BITS + 1], base[LAGPM_KEY_BITS + 1];
+    if (threadIdx.x <= LAGPM_KEY_BITS) h[threadIdx.x] = 0u;
+    __syncthreads();
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int l = -1;
+    if (v < dev_count(dsz, DSZ_NODES, cap)) {
+        l = info[v].w;
+        if (l < 0 || l > LAGPM_KEY_BITS) l = -1;
+    }
+    const uint32_t r = wave_level_reserve(l, h);
+    __syncthreads();
+    if (threadIdx.x <= LAGPM_KEY_BITS && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&dsz[DSZ_LCUR + threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (l >= 0) order[dsz[DSZ_LOFF + l] + base[l] + r] = (uint32_t)v;
+}
+constexpr int LEVEL_BLOCKS = 2048;  // grid of the per-level launches (grid-stride over the level)
+__global__ void moments_f32_kernel(const float4* __restrict__ pp, const uint32_t* __restrict__ dsz, int level,
+                                   const uint32_t* __restrict__ order, const int4* __restrict__ info,
+                                   const int* __restrict__ child, float4* __restrict__ com, float4* __restrict__ q0,
+                                   float4* __restrict__ q1) {
+    const uint32_t n = dsz[DSZ_LVL + level], o = dsz[DSZ_LOFF + level];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        moments_f32_node(order[o + i], pp, info, child, com, q0, q1);
 }
 
 // FP64 moments for the plain FP64 path (12 doubles per node).
-__global__ void moments_f64_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-                                   double mass, const uint32_t* __restrict__ dsz, size_t cap, int level,
-                                   const int4* __restrict__ info, const int* __restrict__ child, double* __restrict__ nd_d) {
-    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
-    int4 nd = info[v];
-    if (nd.w != level) return;
-    double M = 0, Sx = 0, Sy = 0, Sz = 0, Q[6] = {0, 0, 0, 0, 0, 0}, rmax = 0;
+__device__ inline void moments_f64_node(size_t v, const double* __restrict__ x, const double* __restrict__ y,
+                                        const double* __restrict__ z, double mass, const int4* __restrict__ info,
+                                        const int* __restrict__ child, double* __restrict__ nd_d) {
+    const int4 nd = info[v];
+    double M = 0, Sx = 0, Sy = 0, Sz = 0, Q[6] = {0, 0, 0, 0, 0, 0};
     double cxx, cyy, czz;
     if (nd.z == 0) {
         for (int p = nd.x; p < nd.x + nd.y; p++) { M += mass; Sx += mass * x[p]; Sy += mass * y[p]; Sz += mass * z[p]; }
@@ -341,7 +422,6 @@ __global__ void moments_f64_kernel(const double* __restrict__ x, const double* _
             double dx = x[p] - cxx, dy = y[p] - cyy, dz = z[p] - czz;
             Q[0] += mass * dx * dx; Q[1] += mass * dy * dy; Q[2] += mass * dz * dz;
             Q[3] += mass * dx * dy; Q[4] += mass * dx * dz; Q[5] += mass * dy * dz;
-            rmax = fmax(rmax, sqrt(dx * dx + dy * dy + dz * dz));
         }
     } else {
         for (int c = 0; c < nd.z; c++) {
@@ -354,26 +434,65 @@ __global__ void moments_f64_kernel(const double* __restrict__ x, const double* _
             double dx = a[0] - cxx, dy = a[1] - cyy, dz = a[2] - czz, m = a[3];
             Q[0] += a[4] + m * dx * dx; Q[1] += a[5] + m * dy * dy; Q[2] += a[6] + m * dz * dz;
             Q[3] += a[7] + m * dx * dy; Q[4] += a[8] + m * dx * dz; Q[5] += a[9] + m * dy * dz;
-            rmax = fmax(rmax, sqrt(dx * dx + dy * dy + dz * dz) + a[11]);
-        }
-        if (nd.y <= RMAX_EXACT_MAX) {
-            double r2 = 0;
-            for (int p = nd.x; p < nd.x + nd.y; p++) {
-                double dx = x[p] - cxx, dy = y[p] - cyy, dz = z[p] - czz;
-                r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
-            }
-            rmax = sqrt(r2);
         }
     }
     double* o = nd_d + 12 * v;
     o[0] = cxx; o[1] = cyy; o[2] = czz; o[3] = M;
     for (int k = 0; k < 6; k++) o[4 + k] = Q[k];
     o[10] = Q[0] + Q[1] + Q[2];
-    o[11] = rmax;
+    o[11] = 0.0;  // rmax: rmax_f64_kernel
+}
+__global__ void moments_f64_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                                   double mass, const uint32_t* __restrict__ dsz, int level, const uint32_t* __restrict__ order,
+                                   const int4* __restrict__ info, const int* __restrict__ child, double* __restrict__ nd_d) {
+    const uint32_t n = dsz[DSZ_LVL + level], o = dsz[DSZ_LOFF + level];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        moments_f64_node(order[o + i], x, y, z, mass, info, child, nd_d);
 }
 
-// Groups: the first node of size <= G on each root-to-leaf path; max-depth leaves larger
-// than G (coincident keys) are cut into chunks of G.
+// Node sizes: rmax is the exact largest distance of a node's particles from its com, as the
+// reference's node size. One thread per leaf takes the largest distance of its particles from the
+// com of the leaf and of each ancestor and combines it with an atomic max (positive floats order
+// like their bit patterns), so no thread loops over a big node's particles serially. The moments
+// kernels leave rmax at 0; these run after every level's com is final.
+__global__ void rmax_f32_kernel(const float4* __restrict__ pp, const uint32_t* __restrict__ dsz, size_t cap,
+                                const int4* __restrict__ info, const int* __restrict__ parent,
+                                const float4* __restrict__ com, float4* __restrict__ q1) {
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
+    const int4 nd = info[v];
+    if (nd.z != 0) return;
+    for (int a = (int)v; a >= 0; a = parent[a]) {
+        const float4 c = com[a];
+        float r2 = 0.f;
+        for (int p = nd.x; p < nd.x + nd.y; p++) {
+            const float4 q = pp[p];
+            const float dx = q.x - c.x, dy = q.y - c.y, dz = q.z - c.z;
+            r2 = fmaxf(r2, dx * dx + dy * dy + dz * dz);
+        }
+        atomicMax(reinterpret_cast<unsigned int*>(&q1[a]) + 3, __float_as_uint(sqrtf(r2)));
+    }
+}
+__global__ void rmax_f64_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                                const uint32_t* __restrict__ dsz, size_t cap, const int4* __restrict__ info,
+                                const int* __restrict__ parent, double* __restrict__ nd_d) {
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
+    const int4 nd = info[v];
+    if (nd.z != 0) return;
+    for (int a = (int)v; a >= 0; a = parent[a]) {
+        const double* c = nd_d + 12 * (size_t)a;
+        const double cx = c[0], cy = c[1], cz = c[2];
+        double r2 = 0.0;
+        for (int p = nd.x; p < nd.x + nd.y; p++) {
+            const double dx = x[p] - cx, dy = y[p] - cy, dz = z[p] - cz;
+            r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+        }
+        atomicMax(reinterpret_cast<unsigned long long*>(nd_d + 12 * (size_t)a + 11),
+                  (unsigned long long)__double_as_longlong(sqrt(r2)));
+    }
+}
+
 // Target groups. A node of at most G particles whose parent holds more than G is a group, as is
 // a cell root of at most G; a leaf of more than G (coincident particles) splits into chunks of G.
 // Consecutive sibling groups are merged while their sum stays <= G: a parent of more than G packs
@@ -467,6 +586,7 @@ __global__ void fill_int_kernel(int* __restrict__ a, size_t m, int v) {
 __global__ void tree_reset_kernel(uint32_t* __restrict__ dsz, unsigned long long* __restrict__ counters) {
     int t = threadIdx.x;
     if (t == DSZ_NODES || t == DSZ_GROUPS || t == DSZ_OVERFLOW) dsz[t] = 0u;
+    if ((t >= DSZ_LVL && t < DSZ_LOFF) || (t >= DSZ_LCUR && t < DSZ_COUNT)) dsz[t] = 0u;
     if (t < 4) counters[t] = 0ull;
 }
 
@@ -1214,15 +1334,29 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     hipLaunchKernelGGL(children_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->key, box->dsz, cap, L, nleaf,
                        box->noff, box->nd_info, box->nd_child, box->nd_parent);
     HIPCHK(ctx, hipGetLastError());
+    // nodes by level (the level order lives in goff until the group scan reuses it)
+    uint32_t* order = box->goff;
+    hipLaunchKernelGGL(level_count_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, box->nd_info);
+    hipLaunchKernelGGL(level_offsets_kernel, dim3(1), dim3(64), 0, st, box->dsz);
+    hipLaunchKernelGGL(level_scatter_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, box->nd_info, order);
+    HIPCHK(ctx, hipGetLastError());
+    const unsigned lblocks = (unsigned)std::min<size_t>(LEVEL_BLOCKS, blocks_for(cap, 256));
     for (int l = LAGPM_KEY_BITS; l >= L; l--) {
         if (f64)
-            hipLaunchKernelGGL(moments_f64_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->x, box->y, box->z,
-                               1.0, box->dsz, cap, l, box->nd_info, box->nd_child, box->nd_d);
+            hipLaunchKernelGGL(moments_f64_kernel, dim3(lblocks), dim3(256), 0, st, box->x, box->y, box->z, 1.0, box->dsz, l,
+                               order, box->nd_info, box->nd_child, box->nd_d);
         else
-            hipLaunchKernelGGL(moments_f32_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->pp, box->dsz, cap, l,
+            hipLaunchKernelGGL(moments_f32_kernel, dim3(lblocks), dim3(256), 0, st, box->pp, box->dsz, l, order,
                                box->nd_info, box->nd_child, box->nd_com, box->nd_q0, box->nd_q1);
         HIPCHK(ctx, hipGetLastError());
     }
+    if (f64)
+        hipLaunchKernelGGL(rmax_f64_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->x, box->y, box->z, box->dsz,
+                           cap, box->nd_info, box->nd_parent, box->nd_d);
+    else
+        hipLaunchKernelGGL(rmax_f32_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->pp, box->dsz, cap,
+                           box->nd_info, box->nd_parent, box->nd_com, box->nd_q1);
+    HIPCHK(ctx, hipGetLastError());
     hipLaunchKernelGGL(group_count_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, L, G, box->nd_info,
                        box->nd_child, box->nd_parent, box->gflag);
     HIPCHK(ctx, hipGetLastError());
