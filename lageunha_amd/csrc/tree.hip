@@ -36,6 +36,12 @@ constexpr int STACK_MAX = 8192;  // per-wave walk stack entries
 constexpr int WALK_XCDS = LAGPM_WALK_XCDS;
 constexpr int LSTACK = 256;      // the first LSTACK entries of a wave's stack live in LDS, the rest in global memory
 constexpr int WALK_MAX_ITERS = 1 << 20;  // per-group bound on 64-node pop rounds
+// rmax: a leaf and an inner node of up to RMAX_EXACT_MAX particles get the exact largest distance
+// of their particles from the com (the reference's node size); larger nodes get the bound
+// max(|com_child - com| + rmax_child), which only opens such nodes somewhat earlier.
+#ifndef RMAX_EXACT_MAX
+#define RMAX_EXACT_MAX 128
+#endif
 constexpr float RAN_F = (float)(6.0 / 4096.0);
 constexpr int TAB_LDS = LAGPM_LOG_BINS > 4096 ? LAGPM_LOG_BINS : 4096;
 
@@ -279,7 +285,8 @@ __device__ inline void kahan(float& s, float& c, float v) {
 // FP32 moments with Kahan-compensated sums, one level per launch (deepest first).
 // Node moments, one node per thread: the com (Kahan sums relative to the first member), the
 // second moments about it (Kahan), and rmax. Leaves read their particles, inner nodes their
-// children, whose moments the previous level's launch wrote.
+// children, whose moments the previous level's launch wrote (and small inner nodes their
+// particles again for the exact rmax).
 __device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp, const int4* __restrict__ info,
                                         const int* __restrict__ child, float4* __restrict__ com, float4* __restrict__ q0,
                                         float4* __restrict__ q1) {
@@ -287,6 +294,7 @@ __device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp,
     float M = 0, cM = 0, Sx = 0, cx = 0, Sy = 0, cy = 0, Sz = 0, cz = 0;
     float Q[6] = {0, 0, 0, 0, 0, 0}, cQ[6] = {0, 0, 0, 0, 0, 0};
     float3 cm;
+    float r2max = 0.f, rbound = 0.f;
     if (nd.z == 0) {
         const float4 r = pp[nd.x];
         for (int p = nd.x; p < nd.x + nd.y; p++) {
@@ -306,6 +314,7 @@ __device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp,
             kahan(Q[3], cQ[3], a.w * dx * dy);
             kahan(Q[4], cQ[4], a.w * dx * dz);
             kahan(Q[5], cQ[5], a.w * dy * dz);
+            r2max = fmaxf(r2max, dx * dx + dy * dy + dz * dz);
         }
     } else {
         const float4 r = com[child[8 * v]];
@@ -327,11 +336,21 @@ __device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp,
             kahan(Q[3], cQ[3], b0.w + a.w * dx * dy);
             kahan(Q[4], cQ[4], b1.x + a.w * dx * dz);
             kahan(Q[5], cQ[5], b1.y + a.w * dy * dz);
+            rbound = fmaxf(rbound, sqrtf(dx * dx + dy * dy + dz * dz) + b1.w);
+        }
+        if (nd.y <= RMAX_EXACT_MAX) {
+            for (int p = nd.x; p < nd.x + nd.y; p++) {
+                const float4 a = pp[p];
+                const float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
+                r2max = fmaxf(r2max, dx * dx + dy * dy + dz * dz);
+            }
+        } else {
+            r2max = rbound * rbound;
         }
     }
     com[v] = make_float4(cm.x, cm.y, cm.z, M);
     q0[v] = make_float4(Q[0], Q[1], Q[2], Q[3]);
-    q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], 0.f);  // rmax: rmax_f32_kernel
+    q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], nd.z == 0 || nd.y <= RMAX_EXACT_MAX ? sqrtf(r2max) : rbound);
 }
 
 // Nodes grouped by level (level_count / level_offsets / level_scatter), so a level's moments
@@ -413,7 +432,7 @@ __device__ inline void moments_f64_node(size_t v, const double* __restrict__ x, 
                                         const double* __restrict__ z, double mass, const int4* __restrict__ info,
                                         const int* __restrict__ child, double* __restrict__ nd_d) {
     const int4 nd = info[v];
-    double M = 0, Sx = 0, Sy = 0, Sz = 0, Q[6] = {0, 0, 0, 0, 0, 0};
+    double M = 0, Sx = 0, Sy = 0, Sz = 0, Q[6] = {0, 0, 0, 0, 0, 0}, rmax = 0;
     double cxx, cyy, czz;
     if (nd.z == 0) {
         for (int p = nd.x; p < nd.x + nd.y; p++) { M += mass; Sx += mass * x[p]; Sy += mass * y[p]; Sz += mass * z[p]; }
@@ -422,6 +441,7 @@ __device__ inline void moments_f64_node(size_t v, const double* __restrict__ x, 
             double dx = x[p] - cxx, dy = y[p] - cyy, dz = z[p] - czz;
             Q[0] += mass * dx * dx; Q[1] += mass * dy * dy; Q[2] += mass * dz * dz;
             Q[3] += mass * dx * dy; Q[4] += mass * dx * dz; Q[5] += mass * dy * dz;
+            rmax = fmax(rmax, sqrt(dx * dx + dy * dy + dz * dz));
         }
     } else {
         for (int c = 0; c < nd.z; c++) {
@@ -434,13 +454,22 @@ __device__ inline void moments_f64_node(size_t v, const double* __restrict__ x, 
             double dx = a[0] - cxx, dy = a[1] - cyy, dz = a[2] - czz, m = a[3];
             Q[0] += a[4] + m * dx * dx; Q[1] += a[5] + m * dy * dy; Q[2] += a[6] + m * dz * dz;
             Q[3] += a[7] + m * dx * dy; Q[4] += a[8] + m * dx * dz; Q[5] += a[9] + m * dy * dz;
+            rmax = fmax(rmax, sqrt(dx * dx + dy * dy + dz * dz) + a[11]);
+        }
+        if (nd.y <= RMAX_EXACT_MAX) {
+            double r2 = 0;
+            for (int p = nd.x; p < nd.x + nd.y; p++) {
+                const double dx = x[p] - cxx, dy = y[p] - cyy, dz = z[p] - czz;
+                r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+            }
+            rmax = sqrt(r2);
         }
     }
     double* o = nd_d + 12 * v;
     o[0] = cxx; o[1] = cyy; o[2] = czz; o[3] = M;
     for (int k = 0; k < 6; k++) o[4 + k] = Q[k];
     o[10] = Q[0] + Q[1] + Q[2];
-    o[11] = 0.0;  // rmax: rmax_f64_kernel
+    o[11] = rmax;
 }
 __global__ void moments_f64_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
                                    double mass, const uint32_t* __restrict__ dsz, int level, const uint32_t* __restrict__ order,
@@ -448,49 +477,6 @@ __global__ void moments_f64_kernel(const double* __restrict__ x, const double* _
     const uint32_t n = dsz[DSZ_LVL + level], o = dsz[DSZ_LOFF + level];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         moments_f64_node(order[o + i], x, y, z, mass, info, child, nd_d);
-}
-
-// Node sizes: rmax is the exact largest distance of a node's particles from its com, as the
-// reference's node size. One thread per leaf takes the largest distance of its particles from the
-// com of the leaf and of each ancestor and combines it with an atomic max (positive floats order
-// like their bit patterns), so no thread loops over a big node's particles serially. The moments
-// kernels leave rmax at 0; these run after every level's com is final.
-__global__ void rmax_f32_kernel(const float4* __restrict__ pp, const uint32_t* __restrict__ dsz, size_t cap,
-                                const int4* __restrict__ info, const int* __restrict__ parent,
-                                const float4* __restrict__ com, float4* __restrict__ q1) {
-    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
-    const int4 nd = info[v];
-    if (nd.z != 0) return;
-    for (int a = (int)v; a >= 0; a = parent[a]) {
-        const float4 c = com[a];
-        float r2 = 0.f;
-        for (int p = nd.x; p < nd.x + nd.y; p++) {
-            const float4 q = pp[p];
-            const float dx = q.x - c.x, dy = q.y - c.y, dz = q.z - c.z;
-            r2 = fmaxf(r2, dx * dx + dy * dy + dz * dz);
-        }
-        atomicMax(reinterpret_cast<unsigned int*>(&q1[a]) + 3, __float_as_uint(sqrtf(r2)));
-    }
-}
-__global__ void rmax_f64_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-                                const uint32_t* __restrict__ dsz, size_t cap, const int4* __restrict__ info,
-                                const int* __restrict__ parent, double* __restrict__ nd_d) {
-    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
-    const int4 nd = info[v];
-    if (nd.z != 0) return;
-    for (int a = (int)v; a >= 0; a = parent[a]) {
-        const double* c = nd_d + 12 * (size_t)a;
-        const double cx = c[0], cy = c[1], cz = c[2];
-        double r2 = 0.0;
-        for (int p = nd.x; p < nd.x + nd.y; p++) {
-            const double dx = x[p] - cx, dy = y[p] - cy, dz = z[p] - cz;
-            r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
-        }
-        atomicMax(reinterpret_cast<unsigned long long*>(nd_d + 12 * (size_t)a + 11),
-                  (unsigned long long)__double_as_longlong(sqrt(r2)));
-    }
 }
 
 // Target groups. A node of at most G particles whose parent holds more than G is a group, as is
@@ -1350,13 +1336,7 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
                                box->nd_info, box->nd_child, box->nd_com, box->nd_q0, box->nd_q1);
         HIPCHK(ctx, hipGetLastError());
     }
-    if (f64)
-        hipLaunchKernelGGL(rmax_f64_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->x, box->y, box->z, box->dsz,
-                           cap, box->nd_info, box->nd_parent, box->nd_d);
-    else
-        hipLaunchKernelGGL(rmax_f32_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->pp, box->dsz, cap,
-                           box->nd_info, box->nd_parent, box->nd_com, box->nd_q1);
-    HIPCHK(ctx, hipGetLastError());
+
     hipLaunchKernelGGL(group_count_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, L, G, box->nd_info,
                        box->nd_child, box->nd_parent, box->gflag);
     HIPCHK(ctx, hipGetLastError());
