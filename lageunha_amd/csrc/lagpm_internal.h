@@ -74,12 +74,13 @@ struct tree_tune {
 // Device-side sizes of a box's tree and deposit, written by the kernels that produce them so
 // that a step never waits for the host. Indices into lagpm_box::dsz.
 // device size words: node/group/deposit-chunk counts, the overflow flag, and for the tree build
-// the nodes per level (DSZ_LVL + level), their offsets in the level order and fill cursors
+// the nodes per level (DSZ_LVL + level) and their offsets in the level order (DSZ_LOFF + level)
 enum {
     DSZ_NODES = 0,
     DSZ_GROUPS = 1,
     DSZ_CHUNKS = 2,
     DSZ_OVERFLOW = 3,
+    DSZ_SCRATCH = 4,  // scan totals nobody reads
     DSZ_LVL = 8,
     DSZ_LOFF = DSZ_LVL + LAGPM_KEY_BITS + 1,
     DSZ_LCUR = DSZ_LOFF + LAGPM_KEY_BITS + 1,

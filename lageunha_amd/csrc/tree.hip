@@ -133,31 +133,6 @@ __device__ inline size_t lower_bound_pfx(const uint64_t* __restrict__ K, size_t 
     return lo;
 }
 
-// Galloping searches from a particle i whose own prefix is pfx: the run of equal prefixes around
-// i is usually short (a node's particles), so doubling steps outward and then bisecting the last
-// step costs about 2 log2(run) loads instead of log2(N) over the whole key array.
-// First index in [i, hi) whose prefix exceeds pfx (hi if none); K[i] >> sh == pfx.
-__device__ inline size_t upper_gallop(const uint64_t* __restrict__ K, size_t i, size_t hi, uint64_t pfx, int sh) {
-    size_t lo = i + 1, step = 1, probe = i + 1;
-    while (probe < hi && (K[probe] >> sh) == pfx) {
-        lo = probe + 1;
-        step <<= 1;
-        probe = i + step;
-    }
-    return upper_bound_pfx(K, lo, probe < hi ? probe : hi, pfx, sh);
-}
-// First index in [lo0, i] whose prefix equals pfx; K[i] >> sh == pfx and earlier prefixes are <= pfx.
-__device__ inline size_t lower_gallop(const uint64_t* __restrict__ K, size_t lo0, size_t i, uint64_t pfx, int sh) {
-    size_t hi = i, step = 1;
-    for (;;) {
-        if (i < lo0 + step) return lower_bound_pfx(K, lo0, hi, pfx, sh);
-        const size_t probe = i - step;
-        if ((K[probe] >> sh) < pfx) return lower_bound_pfx(K, probe + 1, hi, pfx, sh);
-        hi = probe;
-        step <<= 1;
-    }
-}
-
 __global__ void keys_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
                             size_t np, double s, uint64_t* __restrict__ key, uint32_t* __restrict__ perm) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,101 +153,136 @@ __global__ void pack_pp_kernel(const double* __restrict__ x, const double* __res
     if (i < np) pp[i] = make_float4((float)x[i], (float)y[i], (float)z[i], mass);
 }
 
-// Nodes beginning at particle i: levels l0(i) .. l0(i)+cnt-1 of the runs that start at i.
-__device__ inline int node_levels(const uint64_t* __restrict__ K, size_t N, size_t i, int L, int nleaf, int* l0_out,
-                                  uint32_t* ends /* may be null; up to 17 entries */) {
-    const uint64_t k = K[i];
-    int cpl = (i == 0) ? -1 : common_digits(K[i - 1], k);
-    int l0 = max(L, cpl + 1);
-    *l0_out = l0;
-    if (l0 > LAGPM_KEY_BITS) return 0;
-    size_t epar = N;
-    if (l0 > L) {
-        int sh = key_shift(l0 - 1);
-        uint64_t pfx = k >> sh;
-        size_t s = lower_gallop(K, 0, i, pfx, sh);
-        size_t e = upper_gallop(K, i, N, pfx, sh);
-        if (e - s <= (size_t)nleaf) return 0;  // parent is a leaf
-        epar = e;
-    }
-    int c = 0;
-    for (int l = l0; l <= LAGPM_KEY_BITS; l++) {
-        int sh = key_shift(l);
-        size_t e = upper_gallop(K, i, epar, k >> sh, sh);
-        if (ends) ends[c] = (uint32_t)e;
-        c++;
-        if (e - i <= (size_t)nleaf) break;
-        epar = e;
-    }
-    return c;
-}
-
-__global__ void node_count_kernel(const uint64_t* __restrict__ K, size_t N, int L, int nleaf, uint32_t* __restrict__ cnt) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    int l0;
-    cnt[i] = (uint32_t)node_levels(K, N, i, L, nleaf, &l0, nullptr);
-}
-
-__global__ void node_write_kernel(const uint64_t* __restrict__ K, size_t N, int L, int nleaf, const uint32_t* __restrict__ off,
-                                  int4* __restrict__ info, int* __restrict__ cellroot, int mcell, size_t cap,
-                                  uint32_t* __restrict__ dsz) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    uint32_t ends[LAGPM_KEY_BITS + 1];
-    int l0;
-    int c = node_levels(K, N, i, L, nleaf, &l0, ends);
-    uint32_t v0 = off[i];
-    // Every node index below min(total, cap) is written (each belongs to one (particle, t)),
-    // so the later kernels see a consistent partial tree when the arrays are too small; the
-    // flag makes the walk skip the box and the host grow the arrays and rebuild.
-    if ((size_t)v0 + c > cap) dsz[DSZ_OVERFLOW] = 1u;
-    for (int t = 0; t < c; t++) {
-        if ((size_t)v0 + t >= cap) break;
-        int l = l0 + t;
-        info[v0 + t] = make_int4((int)i, (int)(ends[t] - i), 0, l);
-        if (l == L) {
-            uint64_t cell = K[i] >> key_shift(L);
-            uint32_t cx = compact3(cell), cy = compact3(cell >> 1), cz = compact3(cell >> 2);
-            cellroot[cx + (size_t)mcell * (cy + (size_t)mcell * cz)] = (int)(v0 + t);
-        }
-    }
-}
-
-// Child lists: children of internal node (l, b) are the level-(l+1) runs inside [b, b+count).
 // node count on the device, clamped to the array capacity
 __device__ inline size_t dev_count(const uint32_t* __restrict__ dsz, int which, size_t cap) {
     size_t c = dsz[which];
     return c < cap ? c : cap;
 }
 
-__global__ void children_kernel(const uint64_t* __restrict__ K, const uint32_t* __restrict__ dsz, size_t cap, int L, int nleaf,
-                                const uint32_t* __restrict__ off, int4* __restrict__ info, int* __restrict__ child,
-                                int* __restrict__ parent) {
-    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t nnodes = dev_count(dsz, DSZ_NODES, cap);
-    if (v >= nnodes) return;
-    int4 nd = info[v];
-    if (nd.w == L) parent[v] = -1;
-    if (nd.y <= nleaf || nd.w >= LAGPM_KEY_BITS) return;  // leaf
-    const int lc = nd.w + 1;
-    const uint32_t e = (uint32_t)(nd.x + nd.y);
-    uint32_t b = (uint32_t)nd.x;
-    int nc = 0;
-    while (b < e && nc < 8) {
-        int cpl = (b == 0) ? -1 : common_digits(K[b - 1], K[b]);
-        int l0 = max(L, cpl + 1);
-        if (lc < l0) break;  // cannot happen for a well-formed key order
-        size_t cv = (size_t)off[b] + (size_t)(lc - l0);
-        if (cv >= nnodes) break;
-        int cnt = info[cv].y;
-        if (cnt <= 0) break;
-        child[8 * v + nc] = (int)cv;
-        parent[cv] = (int)v;
-        nc++;
-        b += (uint32_t)cnt;
+// ---------------------------------------------------------------------------------------
+// Top-down build, one level per launch. A node record is (first particle, count, level, children)
+// in the order it was made; the records of level l are contiguous (dsz[DSZ_LOFF + l], count
+// dsz[DSZ_LVL + l]), which is also the level order the moments use. Cells of the chaining level L
+// are the roots; a node of more than nleaf particles above level 16 splits into its non-empty
+// octants, found by 8 lanes bisecting the node's key range in parallel, one octant boundary each.
+// Afterwards every node gets its pre-order index (first particle, then level): nodes that begin at
+// particle f are a chain of consecutive levels from lmin(f) = max(L, 1 + digits f shares with f-1),
+// so index = (nodes beginning before f) + level - lmin(f), with the first term from a scan of
+// per-particle node counts. The index does not depend on the order in which records were made,
+// so the numbering (and everything downstream) is deterministic.
+
+// Append for the lanes whose flag is set: one atomic per wave, slots in lane order. Every lane of
+// the wave must call it.
+__device__ inline uint32_t wave_append(bool make, uint32_t* counter) {
+    const unsigned long long m = __ballot(make);
+    if (!m) return 0u;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0u;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+    return base + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+// the level-L cells that hold particles, one thread per cell (Morton index c = the key prefix)
+__global__ void cells_kernel(const uint64_t* __restrict__ K, size_t N, int L, size_t cap, uint32_t* __restrict__ dsz,
+                             int4* __restrict__ rec, uint32_t* __restrict__ ncount) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t ncell = (size_t)1 << (3 * L);
+    uint32_t a = 0u, cnt = 0u;
+    if (c < ncell) {
+        const int sh = key_shift(L);
+        a = (uint32_t)lower_bound_pfx(K, 0, N, (uint64_t)c, sh);
+        cnt = (uint32_t)(upper_bound_pfx(K, a, N, (uint64_t)c, sh) - a);
     }
-    info[v].z = nc;
+    const bool make = cnt > 0u;
+    const uint32_t r = wave_append(make, &dsz[DSZ_NODES]);
+    if (make) {
+        if (r < cap) {
+            rec[r] = make_int4((int)a, (int)cnt, L, 0);
+            ncount[a] += 1u;  // one node per first particle per level: no race
+        } else {
+            dsz[DSZ_OVERFLOW] = 1u;
+        }
+    }
+}
+// level l's records: from the previous level's end to the records made so far (clamped to cap)
+__global__ void level_mark_kernel(uint32_t* __restrict__ dsz, size_t cap, int l, int first_level) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t made = dsz[DSZ_NODES] < cap ? dsz[DSZ_NODES] : (uint32_t)cap;
+    const uint32_t s = l == first_level ? 0u : dsz[DSZ_LOFF + l - 1] + dsz[DSZ_LVL + l - 1];
+    dsz[DSZ_LOFF + l] = s;
+    dsz[DSZ_LVL + l] = made - s;
+}
+// split the level-l records: 8 lanes per record, lane j finds where octant j begins
+__global__ void split_kernel(const uint64_t* __restrict__ K, int l, int nleaf, size_t cap, uint32_t* __restrict__ dsz,
+                             int4* __restrict__ rec, int* __restrict__ rch0, int* __restrict__ rch1,
+                             uint32_t* __restrict__ ncount) {
+    const uint32_t n = dsz[DSZ_LVL + l], o = dsz[DSZ_LOFF + l];
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int sub = (int)(threadIdx.x & 7);
+    const size_t stride = (size_t)gridDim.x * blockDim.x / 8;
+    const size_t wave_first = (tid & ~(size_t)63) / 8;
+    // every lane of a wave runs the same passes (the appends below are wave-wide)
+    for (size_t pass = 0; wave_first + pass < n; pass += stride) {
+        const size_t r = tid / 8 + pass;
+        const bool valid = r < n;
+        const int4 nd = valid ? rec[o + r] : make_int4(0, 0, 0, 0);
+        const bool split = valid && nd.y > nleaf && l < LAGPM_KEY_BITS;
+        uint32_t b = 0u;
+        if (split) {
+            const uint64_t pfx = ((K[nd.x] >> key_shift(l)) << 3) | (uint64_t)sub;
+            b = (uint32_t)lower_bound_pfx(K, (size_t)nd.x, (size_t)nd.x + nd.y, pfx, key_shift(l + 1));
+        }
+        const uint32_t bn = (uint32_t)__shfl_down((int)b, 1, 64);
+        const uint32_t e = sub == 7 ? (uint32_t)(nd.x + nd.y) : bn;
+        const bool make = split && e > b;
+        const uint32_t rr = wave_append(make, &dsz[DSZ_NODES]);
+        if (make) {
+            if (rr < cap) {
+                rec[rr] = make_int4((int)b, (int)(e - b), l + 1, 0);
+                ncount[b] += 1u;
+            } else {
+                dsz[DSZ_OVERFLOW] = 1u;
+            }
+        }
+        // the record's children, in octant order: lane j's rank among its group's non-empty octants
+        const unsigned long long mm = __ballot(make);
+        const uint32_t grp = (uint32_t)(mm >> (threadIdx.x & 56)) & 0xffu;
+        const int ci = __popc(grp & ((1u << sub) - 1u));
+        const size_t R = (size_t)o + r;
+        if (make) {
+            if (ci < 4) rch0[4 * R + ci] = (int)b;
+            else rch1[4 * R + ci - 4] = (int)b;
+        }
+        if (valid && sub == 0) rec[R].w = __popc(grp);
+    }
+}
+__device__ inline int chain_first_level(const uint64_t* __restrict__ K, uint32_t f, int L) {
+    const int cpl = f == 0 ? -1 : common_digits(K[f - 1], K[f]);
+    return max(L, cpl + 1);
+}
+// pre-order numbering: info, child and parent links, cell roots and the level order
+__global__ void number_kernel(const uint64_t* __restrict__ K, int L, size_t cap, const uint32_t* __restrict__ dsz,
+                              const int4* __restrict__ rec, const int* __restrict__ rch0, const int* __restrict__ rch1,
+                              const uint32_t* __restrict__ noff, int4* __restrict__ info, int* __restrict__ child,
+                              int* __restrict__ parent, int* __restrict__ cellroot, int mcell, uint32_t* __restrict__ order) {
+    const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (dsz[DSZ_OVERFLOW] || r >= dev_count(dsz, DSZ_NODES, cap)) return;  // an overflowed build is redone
+    const int4 nd = rec[r];  // (first, count, level, children)
+    const uint32_t v = noff[nd.x] + (uint32_t)(nd.z - chain_first_level(K, (uint32_t)nd.x, L));
+    if ((size_t)v >= cap) return;  // cannot happen for sorted keys (the chain levels are consecutive)
+    order[r] = v;
+    info[v] = make_int4(nd.x, nd.y, nd.w, nd.z);
+    if (nd.z == L) {
+        parent[v] = -1;
+        const uint64_t cell = K[nd.x] >> key_shift(L);
+        cellroot[compact3(cell) + (size_t)mcell * (compact3(cell >> 1) + (size_t)mcell * compact3(cell >> 2))] = (int)v;
+    }
+    for (int c = 0; c < nd.w; c++) {
+        const uint32_t cf = (uint32_t)(c < 4 ? rch0[4 * r + c] : rch1[4 * r + c - 4]);
+        const uint32_t vc = noff[cf] + (uint32_t)(nd.z + 1 - chain_first_level(K, cf, L));
+        child[8 * (size_t)v + c] = (int)vc;
+        if ((size_t)vc < cap) parent[vc] = (int)v;
+    }
 }
 
 __device__ inline void kahan(float& s, float& c, float v) {
@@ -353,75 +363,12 @@ __device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp,
     q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], nd.z == 0 || nd.y <= RMAX_EXACT_MAX ? sqrtf(r2max) : rbound);
 }
 
-// Nodes grouped by level (level_count / level_offsets / level_scatter), so a level's moments
-// launch visits exactly its own nodes.
-// Reserve slots in the block's per-level counters h: one LDS atomic per distinct level in the
-// wave (a ballot per level, the lane's rank from mbcnt). Every lane of the wave calls it.
-__device__ inline uint32_t wave_level_reserve(int l, uint32_t* h) {
-    const int lane = threadIdx.x & 63;
-    bool pending = l >= 0;
-    uint32_t r = 0;
-    for (;;) {
-        const unsigned long long act = __ballot(pending);
-        if (!act) break;
-        const int leader = __ffsll((long long)act) - 1;
-        const int lead = __builtin_amdgcn_readlane(l, leader);
-        const bool mine = pending && l == lead;
-        const unsigned long long m = __ballot(mine);
-        uint32_t b0 = 0;
-        if (lane == leader) b0 = atomicAdd(&h[lead], (uint32_t)__popcll(m));
-        b0 = (uint32_t)__builtin_amdgcn_readlane((int)b0, leader);
-        if (mine) {
-            r = b0 + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-            pending = false;
-        }
-    }
-    return r;
-}
-__global__ void level_count_kernel(uint32_t* __restrict__ dsz, size_t cap, const int4* __restrict__ info) {
-    __shared__ uint32_t h[LAGPM_KEY_BITS + 1];
-    if (threadIdx.x <= LAGPM_KEY_BITS) h[threadIdx.x] = 0u;
-    __syncthreads();
-    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int l = -1;
-    if (v < dev_count(dsz, DSZ_NODES, cap)) {
-        l = info[v].w;
-        if (l < 0 || l > LAGPM_KEY_BITS) l = -1;
-    }
-    (void)wave_level_reserve(l, h);
-    __syncthreads();
-    if (threadIdx.x <= LAGPM_KEY_BITS && h[threadIdx.x]) atomicAdd(&dsz[DSZ_LVL + threadIdx.x], h[threadIdx.x]);
-}
-__global__ void level_offsets_kernel(uint32_t* __restrict__ dsz) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint32_t o = 0;
-    for (int l = 0; l <= LAGPM_KEY_BITS; l++) {
-        dsz[DSZ_LOFF + l] = o;
-        o += dsz[DSZ_LVL + l];
-    }
-}
-__global__ void level_scatter_kernel(uint32_t* __restrict__ dsz, size_t cap, const int4* __restrict__ info,
-                                     uint32_t* __restrict__ order) {
-    __shared__ uint32_t h[LAGPM_KEY_BITS + 1], base[LAGPM_KEY_BITS + 1];
-    if (threadIdx.x <= LAGPM_KEY_BITS) h[threadIdx.x] = 0u;
-    __syncthreads();
-    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int l = -1;
-    if (v < dev_count(dsz, DSZ_NODES, cap)) {
-        l = info[v].w;
-        if (l < 0 || l > LAGPM_KEY_BITS) l = -1;
-    }
-    const uint32_t r = wave_level_reserve(l, h);
-    __syncthreads();
-    if (threadIdx.x <= LAGPM_KEY_BITS && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&dsz[DSZ_LCUR + threadIdx.x], h[threadIdx.x]);
-    __syncthreads();
-    if (l >= 0) order[dsz[DSZ_LOFF + l] + base[l] + r] = (uint32_t)v;
-}
 constexpr int LEVEL_BLOCKS = 2048;  // grid of the per-level launches (grid-stride over the level)
 __global__ void moments_f32_kernel(const float4* __restrict__ pp, const uint32_t* __restrict__ dsz, int level,
                                    const uint32_t* __restrict__ order, const int4* __restrict__ info,
                                    const int* __restrict__ child, float4* __restrict__ com, float4* __restrict__ q0,
                                    float4* __restrict__ q1) {
+    if (dsz[DSZ_OVERFLOW]) return;
     const uint32_t n = dsz[DSZ_LVL + level], o = dsz[DSZ_LOFF + level];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         moments_f32_node(order[o + i], pp, info, child, com, q0, q1);
@@ -474,6 +421,7 @@ __device__ inline void moments_f64_node(size_t v, const double* __restrict__ x, 
 __global__ void moments_f64_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
                                    double mass, const uint32_t* __restrict__ dsz, int level, const uint32_t* __restrict__ order,
                                    const int4* __restrict__ info, const int* __restrict__ child, double* __restrict__ nd_d) {
+    if (dsz[DSZ_OVERFLOW]) return;
     const uint32_t n = dsz[DSZ_LVL + level], o = dsz[DSZ_LOFF + level];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         moments_f64_node(order[o + i], x, y, z, mass, info, child, nd_d);
@@ -525,7 +473,7 @@ __global__ void group_count_kernel(const uint32_t* __restrict__ dsz, size_t cap,
                                    uint32_t* __restrict__ gcnt) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= cap) return;
-    if (v >= dev_count(dsz, DSZ_NODES, cap)) {  // entries past the node count scan as zero
+    if (dsz[DSZ_OVERFLOW] || v >= dev_count(dsz, DSZ_NODES, cap)) {  // entries past the node count scan as zero
         gcnt[v] = 0u;
         return;
     }
@@ -537,7 +485,7 @@ __global__ void group_write_kernel(uint32_t* __restrict__ dsz, size_t cap, size_
                                    const uint32_t* __restrict__ goff,
                                    int2* __restrict__ groups) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= dev_count(dsz, DSZ_NODES, cap)) return;
+    if (dsz[DSZ_OVERFLOW] || v >= dev_count(dsz, DSZ_NODES, cap)) return;
     uint32_t c = gcnt[v];
     if (!c) return;
     if ((size_t)goff[v] + c > gcap) {
@@ -1308,25 +1256,36 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     const size_t ncell = (size_t)box->mcell * box->mcell * box->mcell;
     hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(ncell, 256)), dim3(256), 0, st, box->cellroot, ncell, -1);
     HIPCHK(ctx, hipGetLastError());
-    hipLaunchKernelGGL(node_count_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->key, np, L, nleaf,
+    // top-down build: records of level-L cells, then one split per level, then the numbering.
+    // The records and child lists live in the moment arrays until the moments overwrite them.
+    int4* rec;
+    int *rch0, *rch1;
+    if (f64) {
+        rec = reinterpret_cast<int4*>(box->nd_d);
+        rch0 = reinterpret_cast<int*>(box->nd_d) + 4 * cap;
+        rch1 = reinterpret_cast<int*>(box->nd_d) + 8 * cap;
+    } else {
+        rec = reinterpret_cast<int4*>(box->nd_com);
+        rch0 = reinterpret_cast<int*>(box->nd_q0);
+        rch1 = reinterpret_cast<int*>(box->nd_q1);
+    }
+    uint32_t* order = box->goff;  // level order (goff is free until the group scan)
+    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, reinterpret_cast<int*>(box->ncount), np, 0);
+    hipLaunchKernelGGL(cells_kernel, dim3(blocks_for(ncell, 256)), dim3(256), 0, st, box->key, np, L, cap, box->dsz, rec,
                        box->ncount);
-    HIPCHK(ctx, hipGetLastError());
-    RCHK(lagpm_exclusive_scan_u32(ctx, st, box->scan_tmp, box->ncount, box->noff, np, box->dsz + DSZ_NODES));
-    hipLaunchKernelGGL(node_write_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->key, np, L, nleaf,
-                       box->noff, box->nd_info, box->cellroot, box->mcell, cap, box->dsz);
-    HIPCHK(ctx, hipGetLastError());
-    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->nd_parent, cap, -1);
-    HIPCHK(ctx, hipGetLastError());
-    hipLaunchKernelGGL(children_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->key, box->dsz, cap, L, nleaf,
-                       box->noff, box->nd_info, box->nd_child, box->nd_parent);
-    HIPCHK(ctx, hipGetLastError());
-    // nodes by level (the level order lives in goff until the group scan reuses it)
-    uint32_t* order = box->goff;
-    hipLaunchKernelGGL(level_count_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, box->nd_info);
-    hipLaunchKernelGGL(level_offsets_kernel, dim3(1), dim3(64), 0, st, box->dsz);
-    hipLaunchKernelGGL(level_scatter_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, box->nd_info, order);
+    hipLaunchKernelGGL(level_mark_kernel, dim3(1), dim3(64), 0, st, box->dsz, cap, L, L);
     HIPCHK(ctx, hipGetLastError());
     const unsigned lblocks = (unsigned)std::min<size_t>(LEVEL_BLOCKS, blocks_for(cap, 256));
+    for (int l = L; l < LAGPM_KEY_BITS; l++) {
+        hipLaunchKernelGGL(split_kernel, dim3(lblocks), dim3(256), 0, st, box->key, l, nleaf, cap, box->dsz, rec, rch0,
+                           rch1, box->ncount);
+        hipLaunchKernelGGL(level_mark_kernel, dim3(1), dim3(64), 0, st, box->dsz, cap, l + 1, L);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    RCHK(lagpm_exclusive_scan_u32(ctx, st, box->scan_tmp, box->ncount, box->noff, np, box->dsz + DSZ_SCRATCH));
+    hipLaunchKernelGGL(number_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->key, L, cap, box->dsz, rec, rch0,
+                       rch1, box->noff, box->nd_info, box->nd_child, box->nd_parent, box->cellroot, box->mcell, order);
+    HIPCHK(ctx, hipGetLastError());
     for (int l = LAGPM_KEY_BITS; l >= L; l--) {
         if (f64)
             hipLaunchKernelGGL(moments_f64_kernel, dim3(lblocks), dim3(256), 0, st, box->x, box->y, box->z, 1.0, box->dsz, l,
