@@ -7,6 +7,7 @@
 #include <algorithm>
 
 #include "lagpm_internal.h"
+#include "wave_ops.h"
 
 namespace {
 
@@ -89,6 +90,10 @@ __global__ void __launch_bounds__(RS_THREADS) rs_hist_kernel(const uint64_t* __r
     ghist[(size_t)tid * nblocks + blockIdx.x] = h[tid];
 }
 
+// Scatter of one 4096-key tile: keys are ranked stably per wavefront (ballot digit matching),
+// placed digit by digit into LDS (tile order: digit, then wave, then position), and written out
+// from LDS so that consecutive threads write consecutive slots of one digit's output run
+// (coalesced stores; writing from the original positions scatters 64 lanes over 64 runs).
 __global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* __restrict__ key,
                                                                 const uint32_t* __restrict__ val, size_t n, int shift,
                                                                 const uint32_t* __restrict__ goff, uint32_t nblocks,
@@ -96,6 +101,10 @@ __global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* 
                                                                 uint32_t* __restrict__ val_out) {
     __shared__ uint32_t wcount[RS_WAVES][256];
     __shared__ uint32_t boff[256];
+    __shared__ uint32_t toff[256];
+    __shared__ uint32_t wsum[RS_WAVES];
+    __shared__ uint64_t skey[RS_TILE];
+    __shared__ uint32_t sval[RS_TILE];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     for (int d = tid; d < 256; d += RS_THREADS) {
         boff[d] = goff[(size_t)d * nblocks + blockIdx.x];
@@ -104,7 +113,8 @@ __global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* 
     }
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const size_t seg = (size_t)blockIdx.x * RS_TILE + (size_t)wid * (RS_ITEMS * 64);
+    const size_t tile0 = (size_t)blockIdx.x * RS_TILE;
+    const size_t seg = tile0 + (size_t)wid * (RS_ITEMS * 64);
     uint64_t k[RS_ITEMS];
     uint32_t v[RS_ITEMS];
     uint32_t rank[RS_ITEMS];
@@ -130,23 +140,42 @@ __global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* 
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    if (tid < 256) {
+    // per digit: the waves' exclusive prefix (in place) and the tile total, then the tile's
+    // exclusive scan over digits (RS_THREADS == 256: thread d owns digit d)
+    {
+        const int d = tid;
         uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < RS_WAVES; w++) {
-            uint32_t t = wcount[w][tid];
-            wcount[w][tid] = run;
+            uint32_t t = wcount[w][d];
+            wcount[w][d] = run;
             run += t;
         }
+        int wt;
+        const uint32_t ex = (uint32_t)lagpm_wave::wave_excl_scan((int)run, lane, wt);
+        if (lane == 0) wsum[wid] = (uint32_t)wt;
+        __syncthreads();
+        uint32_t wo = 0;
+        for (int w = 0; w < wid; w++) wo += wsum[w];
+        toff[d] = wo + ex;
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < RS_ITEMS; it++) {
         if (rank[it] == 0xffffffffu) continue;
-        uint32_t d = (uint32_t)(k[it] >> shift) & 255u;
-        size_t dst = (size_t)boff[d] + wcount[wid][d] + rank[it];
-        key_out[dst] = k[it];
-        val_out[dst] = v[it];
+        const uint32_t d = (uint32_t)(k[it] >> shift) & 255u;
+        const uint32_t lp = toff[d] + wcount[wid][d] + rank[it];
+        skey[lp] = k[it];
+        sval[lp] = v[it];
+    }
+    __syncthreads();
+    const uint32_t tn = (uint32_t)(n - tile0 < (size_t)RS_TILE ? n - tile0 : (size_t)RS_TILE);
+    for (uint32_t j = tid; j < tn; j += RS_THREADS) {
+        const uint64_t kk = skey[j];
+        const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+        const size_t dst = (size_t)boff[d] + (j - toff[d]);
+        key_out[dst] = kk;
+        val_out[dst] = sval[j];
     }
 }
 

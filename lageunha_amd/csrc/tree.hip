@@ -141,10 +141,24 @@ __global__ void keys_kernel(const double* __restrict__ x, const double* __restri
     perm[i] = (uint32_t)i;
 }
 
-template <typename T>
-__global__ void gather_kernel(const T* __restrict__ src, const uint32_t* __restrict__ perm, size_t np, T* __restrict__ dst) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < np) dst[i] = src[perm[i]];
+// all particle arrays into the new order in one pass (one permutation load, seven gathers in flight)
+struct gather_set {
+    const double* src[6];
+    double* dst[6];
+    const uint32_t* id;
+    uint32_t* id_dst;
+};
+__global__ void gather_all_kernel(gather_set g, const uint32_t* __restrict__ perm, size_t np) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const uint32_t p = perm[i];
+    double v[6];
+#pragma unroll
+    for (int a = 0; a < 6; a++) v[a] = g.src[a][p];
+    const uint32_t id = g.id[p];
+#pragma unroll
+    for (int a = 0; a < 6; a++) g.dst[a][i] = v[a];
+    g.id_dst[i] = id;
 }
 
 __global__ void pack_pp_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
@@ -1220,15 +1234,16 @@ int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box) {
         std::swap(box->perm, box->perm2);
     }
     double** arrs[6] = {&box->x, &box->y, &box->z, &box->vx, &box->vy, &box->vz};
+    gather_set g;
     for (int a = 0; a < 6; a++) {
-        hipLaunchKernelGGL(gather_kernel<double>, dim3(blocks_for(np, 256)), dim3(256), 0, st, *arrs[a], box->perm,
-                           np, box->alt[a]);
-        HIPCHK(ctx, hipGetLastError());
-        std::swap(*arrs[a], box->alt[a]);
+        g.src[a] = *arrs[a];
+        g.dst[a] = box->alt[a];
     }
-    hipLaunchKernelGGL(gather_kernel<uint32_t>, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->id, box->perm, np,
-                       box->id_alt);
+    g.id = box->id;
+    g.id_dst = box->id_alt;
+    hipLaunchKernelGGL(gather_all_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, g, box->perm, np);
     HIPCHK(ctx, hipGetLastError());
+    for (int a = 0; a < 6; a++) std::swap(*arrs[a], box->alt[a]);
     std::swap(box->id, box->id_alt);
     box->parity ^= 1;
     box->sorted = true;
