@@ -226,17 +226,23 @@ __global__ void level_mark_kernel(uint32_t* __restrict__ dsz, size_t cap, int l,
     dsz[DSZ_LOFF + l] = s;
     dsz[DSZ_LVL + l] = made - s;
 }
-// split the level-l records: 8 lanes per record, lane j finds where octant j begins
-__global__ void split_kernel(const uint64_t* __restrict__ K, int l, int nleaf, size_t cap, uint32_t* __restrict__ dsz,
-                             int4* __restrict__ rec, int* __restrict__ rch0, int* __restrict__ rch1,
-                             uint32_t* __restrict__ ncount) {
+// split the level-l records: 8 lanes per record, lane j finds where octant j begins. New
+// records are appended with one global atomic per workgroup and pass (a single counter that
+// every wave hit directly serialised ~10^5 atomics per level).
+constexpr int SPLIT_THREADS = 256;
+__global__ void __launch_bounds__(SPLIT_THREADS) split_kernel(const uint64_t* __restrict__ K, int l, int nleaf, size_t cap,
+                                                              uint32_t* __restrict__ dsz, int4* __restrict__ rec,
+                                                              int* __restrict__ rch0, int* __restrict__ rch1,
+                                                              uint32_t* __restrict__ ncount) {
+    __shared__ uint32_t wcnt[SPLIT_THREADS / 64];
+    __shared__ uint32_t bbase;
     const uint32_t n = dsz[DSZ_LVL + l], o = dsz[DSZ_LOFF + l];
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int sub = (int)(threadIdx.x & 7);
+    const int sub = (int)(threadIdx.x & 7), lane = (int)(threadIdx.x & 63), wid = (int)(threadIdx.x >> 6);
     const size_t stride = (size_t)gridDim.x * blockDim.x / 8;
-    const size_t wave_first = (tid & ~(size_t)63) / 8;
-    // every lane of a wave runs the same passes (the appends below are wave-wide)
-    for (size_t pass = 0; wave_first + pass < n; pass += stride) {
+    const size_t block_first = (size_t)blockIdx.x * blockDim.x / 8;
+    // every thread of the workgroup runs the same passes (the appends below are block-wide)
+    for (size_t pass = 0; block_first + pass < n; pass += stride) {
         const size_t r = tid / 8 + pass;
         const bool valid = r < n;
         const int4 nd = valid ? rec[o + r] : make_int4(0, 0, 0, 0);
@@ -249,7 +255,19 @@ __global__ void split_kernel(const uint64_t* __restrict__ K, int l, int nleaf, s
         const uint32_t bn = (uint32_t)__shfl_down((int)b, 1, 64);
         const uint32_t e = sub == 7 ? (uint32_t)(nd.x + nd.y) : bn;
         const bool make = split && e > b;
-        const uint32_t rr = wave_append(make, &dsz[DSZ_NODES]);
+        const unsigned long long wm = __ballot(make);
+        if (lane == 0) wcnt[wid] = (uint32_t)__popcll(wm);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int w = 0; w < SPLIT_THREADS / 64; w++) t += wcnt[w];
+            bbase = t ? atomicAdd(&dsz[DSZ_NODES], t) : 0u;
+        }
+        __syncthreads();
+        uint32_t rr = bbase;
+        for (int w = 0; w < wid; w++) rr += wcnt[w];
+        rr += __builtin_amdgcn_mbcnt_hi((unsigned)(wm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)wm, 0u));
+        __syncthreads();  // wcnt and bbase are rewritten by the next pass
         if (make) {
             if (rr < cap) {
                 rec[rr] = make_int4((int)b, (int)(e - b), l + 1, 0);
@@ -321,6 +339,7 @@ __device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp,
     float r2max = 0.f, rbound = 0.f;
     if (nd.z == 0) {
         const float4 r = pp[nd.x];
+#pragma unroll 4
         for (int p = nd.x; p < nd.x + nd.y; p++) {
             float4 a = pp[p];
             kahan(M, cM, a.w);
@@ -329,6 +348,7 @@ __device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp,
             kahan(Sz, cz, a.w * (a.z - r.z));
         }
         cm = make_float3(r.x + Sx / M, r.y + Sy / M, r.z + Sz / M);
+#pragma unroll 4
         for (int p = nd.x; p < nd.x + nd.y; p++) {
             float4 a = pp[p];
             float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
@@ -363,6 +383,7 @@ __device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp,
             rbound = fmaxf(rbound, sqrtf(dx * dx + dy * dy + dz * dz) + b1.w);
         }
         if (nd.y <= RMAX_EXACT_MAX) {
+#pragma unroll 4
             for (int p = nd.x; p < nd.x + nd.y; p++) {
                 const float4 a = pp[p];
                 const float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
@@ -1292,8 +1313,8 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     HIPCHK(ctx, hipGetLastError());
     const unsigned lblocks = (unsigned)std::min<size_t>(LEVEL_BLOCKS, blocks_for(cap, 256));
     for (int l = L; l < LAGPM_KEY_BITS; l++) {
-        hipLaunchKernelGGL(split_kernel, dim3(lblocks), dim3(256), 0, st, box->key, l, nleaf, cap, box->dsz, rec, rch0,
-                           rch1, box->ncount);
+        hipLaunchKernelGGL(split_kernel, dim3(lblocks), dim3(SPLIT_THREADS), 0, st, box->key, l, nleaf, cap, box->dsz, rec,
+                           rch0, rch1, box->ncount);
         hipLaunchKernelGGL(level_mark_kernel, dim3(1), dim3(64), 0, st, box->dsz, cap, l + 1, L);
         HIPCHK(ctx, hipGetLastError());
     }
