@@ -418,7 +418,7 @@ int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box
     if (!ctx || !out) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
     *out = nullptr;
     RCHK(check_params(ctx, p));
-    if (np >= (size_t)LAGPM_MAX_INDEX) return lagpm_fail(ctx, LAGPM_E_INVAL, "np must be below 2^27");
+    if (np > (size_t)LAGPM_MAX_INDEX) return lagpm_fail(ctx, LAGPM_E_INVAL, "np must be at most 2^27");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     lagpm_box* box = new lagpm_box();
     box->p = *p;

@@ -13,7 +13,8 @@
 
 #define LAGPM_NACCEPT 10     // nodes with fewer particles are never accepted (ost.h:4 MIN_CELL_PARTICLE_NUM)
 #define LAGPM_KEY_BITS 16    // Morton bits per axis; octree depth <= 16
-#define LAGPM_MAX_INDEX (1u << 27)  // walk lists pack (index << 5 | neighbour)
+#define LAGPM_MAX_INDEX (1u << 27)  // particles per box: walk lists pack (index << 5 | neighbour) in 32 bits
+#define LAGPM_MAX_NODES (1u << 26)  // nodes per box: walk stacks pack (node << 5 | neighbour) in a non-negative int
 
 struct fft_plans {
     rocfft_plan fwd = nullptr, inv = nullptr;

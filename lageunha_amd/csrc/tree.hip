@@ -1213,6 +1213,11 @@ size_t lagpm_tree_node_cap_guess(size_t np, int mcell) {
 
 int lagpm_tree_alloc(lagpm_ctx* ctx, lagpm_box* box, size_t cap) {
     const bool f64 = box->p.precision == LAGPM_PREC_FP64;
+    if (cap > (size_t)LAGPM_MAX_NODES) {
+        if (box->node_cap >= (size_t)LAGPM_MAX_NODES)
+            return lagpm_fail(ctx, LAGPM_E_INVAL, "tree needs more than 2^26 nodes (the walk's node index limit)");
+        cap = LAGPM_MAX_NODES;
+    }
     lagpm_box_graphs_release(box);  // captured steps refer to the old arrays
     const size_t gcap = cap + box->np / 64 + 1;
     RCHK(regrow(ctx, &box->nd_info, cap));
