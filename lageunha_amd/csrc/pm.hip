@@ -17,6 +17,7 @@
 // cells with one global 64-bit atomic each. Contributions are fixed point (x 2^S, round to
 // nearest even), so the sum is exact and independent of the order: deterministic.
 #include "lagpm_internal.h"
+#include "wave_ops.h"
 
 #include <cmath>
 
@@ -98,10 +99,12 @@ __global__ void __launch_bounds__(DEP_THREADS) deposit_kernel(const double* __re
     __shared__ unsigned long long box[DEP_LDS];
     __shared__ int red[6][DEP_THREADS / 64];
     // the grid is sized for the largest possible chunk count; the actual count is on the device
-    if (blockIdx.x >= *nchunk) return;
+    const unsigned nch = *nchunk;
+    if (blockIdx.x >= nch) return;
+    const unsigned chunk = lagpm_wave::xcd_block(blockIdx.x, nch);  // neighbouring chunks on one XCD's L2
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const size_t base = start[blockIdx.x];
-    const size_t np = start[blockIdx.x + 1];
+    const size_t base = start[chunk];
+    const size_t np = start[chunk + 1];
 
     int ix[DEP_PPT], iy[DEP_PPT], iz[DEP_PPT], sx[DEP_PPT], sy[DEP_PPT], sz[DEP_PPT];
     float wx[DEP_PPT][3], wy[DEP_PPT][3], wz[DEP_PPT][3];
@@ -261,7 +264,7 @@ __device__ inline double fda4_d(double m1, double p1, double m2, double p2) {
 template <typename T, typename T4>
 __global__ void gradient_kernel(const T* __restrict__ phi, int n, T4* __restrict__ grad) {
     const size_t m = (size_t)n * n * n;
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t q = (size_t)lagpm_wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t nn = (size_t)n * n;
     const size_t mx = 2 * (size_t)(n / 2 + 1), plane = mx * n;  // padded rows of the FFT mesh
@@ -306,7 +309,7 @@ __global__ void interp_kick_f32(const double* __restrict__ px, const double* __r
                                 double* __restrict__ vz, size_t np, int n, const float4* __restrict__ grad,
                                 const double* __restrict__ fac) {
     const float fact2 = (float)fac[1];
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t p = (size_t)lagpm_wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= np) return;
     int ix, iy, iz, sx, sy, sz;
     float wx[3], wy[3], wz[3];
@@ -344,7 +347,7 @@ __global__ void interp_kick_f64(const double* __restrict__ px, const double* __r
                                 double* __restrict__ vz, size_t np, int n, const double4* __restrict__ grad,
                                 const double* __restrict__ fac) {
     const double fact2 = fac[1];
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t p = (size_t)lagpm_wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= np) return;
     int ix, iy, iz, sx, sy, sz;
     double wx[3], wy[3], wz[3];

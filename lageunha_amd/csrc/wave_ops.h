@@ -65,4 +65,12 @@ __device__ __forceinline__ float wave_maxf(float v) {
     return fmaxf(fmaxf(a, b), fmaxf(c, d));
 }
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round robin (block b runs on XCD b % 8),
+// and each XCD has its own L2. xcd_block renumbers block b of nb so that every XCD takes one
+// contiguous range of logical blocks, and neighbouring work shares an L2. A bijection on [0, nb).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    const unsigned x = b % 8u, i = b / 8u;
+    return x * (nb / 8u) + (x < nb % 8u ? x : nb % 8u) + i;
+}
+
 }  // namespace lagpm_wave
