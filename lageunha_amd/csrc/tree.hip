@@ -663,12 +663,14 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
     int slot = (int)(blockIdx.x % WALK_XCDS), tries = 0;
 
     for (;;) {
-        // Group ticket. Every lane executes the atomic (lane 0 adds 1, the others 0) and the
-        // ticket is read from lane 0 into a scalar register: no lane-dependent branch sits at
-        // the loop head, so the compiler cannot thread the back edge differently per lane
-        // (which would let lanes 1..63 re-enter a group without a new ticket).
-        const unsigned long long ticket = atomicAdd(ticket_ctr + slot, lane == 0 ? 1ull : 0ull);
-        const long long t = (long long)__builtin_amdgcn_readlane((int)ticket, 0) |
+        // Group ticket: one atomic per group, from lane 0 (an atomic executed by all 64 lanes is
+        // 64 serialised operations on one address, which capped the group rate). The ticket is
+        // then read from lane 0 into a scalar register, so every branch of the loop below is
+        // wave-uniform: the compiler cannot thread the back edge differently per lane (which
+        // would let lanes 1..63 re-enter a group without a new ticket).
+        unsigned long long ticket = 0ull;
+        if (lane == 0) ticket = atomicAdd(ticket_ctr + slot, 1ull);
+        const long long t = (long long)(unsigned)__builtin_amdgcn_readlane((int)ticket, 0) |
                             ((long long)__builtin_amdgcn_readlane((int)(ticket >> 32), 0) << 32);
         const int lo = range_lo(slot), hi = range_lo(slot + 1);
         if (t >= (long long)(hi - lo)) {
@@ -1102,9 +1104,8 @@ struct walk64_args {
 };
 
 template <bool LOGT>
-__global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.np || A.dsz[DSZ_OVERFLOW]) return;
+// one target particle; returns false on a stack overflow (the kick is then not applied)
+__device__ inline bool walk_f64_target(const walk64_args& A, size_t i, unsigned long long& npp, unsigned long long& npc) {
     const double px0 = A.x[i], py0 = A.y[i], pz0 = A.z[i];
     const int g = A.pgroup[i];
     const double lo[3] = {A.gbox[6 * g], A.gbox[6 * g + 1], A.gbox[6 * g + 2]};
@@ -1121,7 +1122,6 @@ __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
         if (r >= 0) stack[sp++] = (r << 5) | nb;
     }
     double ax = 0, ay = 0, az = 0;
-    unsigned long long npp = 0, npc = 0;
     bool overflow = false;
     while (sp > 0) {
         int e = stack[--sp];
@@ -1177,15 +1177,36 @@ __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
             for (int c = nd.z - 1; c >= 0; c--) stack[sp++] = (A.child[8 * v + c] << 5) | nb;
         }
     }
-    if (overflow) {
-        atomicAdd(&A.counters[2], 1ull);
-        return;
-    }
+    if (overflow) return false;
     A.vx[i] += A.fact * ax;
     A.vy[i] += A.fact * ay;
     A.vz[i] += A.fact * az;
-    atomicAdd(&A.counters[0], npp);
-    atomicAdd(&A.counters[1], npc);
+    return true;
+}
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned lo = (unsigned)__shfl_xor((int)(unsigned)v, o, 64);
+        const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(v >> 32), o, 64);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
+// Thread per target. The interaction counters are summed over the wave first: one atomic per
+// wave and counter (an atomic per thread on one address serialises millions of operations).
+template <bool LOGT>
+__global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long npp = 0, npc = 0, ovf = 0;
+    if (i < A.np && !A.dsz[DSZ_OVERFLOW]) ovf = walk_f64_target<LOGT>(A, i, npp, npc) ? 0ull : 1ull;
+    npp = wave_sum_u64(npp);
+    npc = wave_sum_u64(npc);
+    ovf = wave_sum_u64(ovf);
+    if ((threadIdx.x & 63) == 0) {
+        if (npp) atomicAdd(&A.counters[0], npp);
+        if (npc) atomicAdd(&A.counters[1], npc);
+        if (ovf) atomicAdd(&A.counters[2], ovf);
+    }
 }
 
 unsigned blocks_for(size_t n, int t) { return (unsigned)std::max<size_t>(1, (n + t - 1) / t); }
