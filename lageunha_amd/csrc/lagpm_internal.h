@@ -212,6 +212,7 @@ int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact);
 // arrays and returns LAGPM_E_OVERFLOW with *retry set when they were too small.
 int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry);
 int lagpm_tree_alloc(lagpm_ctx* ctx, lagpm_box* box, size_t node_cap);
+double lagpm_tree_scale(const lagpm_box* box);  // FP32 tree coordinate scale (sqrt(lam) or 1)
 size_t lagpm_tree_node_cap_guess(size_t np, int mcell);
 int lagpm_choose_cell_level(int n, double rsphere);
 size_t lagpm_walk_stack_entries(int num_cu);

@@ -27,14 +27,23 @@
 namespace {
 using namespace lagpm_wave;
 
-constexpr int WALK_THREADS = 1024;  // one workgroup per CU (LDS: shared table + 16 wave lists)
+#ifndef WALK_THREADS_PER_CU
+#define WALK_THREADS_PER_CU 1024
+#endif
+constexpr int WALK_THREADS = WALK_THREADS_PER_CU;  // one workgroup per CU (LDS: shared table + 16 wave lists)
 constexpr int WALK_WAVES = WALK_THREADS / 64;
-constexpr int PP_TAKE = 4;          // particles a lane appends per round
+#ifndef WALK_PP_TAKE
+#define WALK_PP_TAKE 4
+#endif
+#ifndef WALK_LSTACK
+#define WALK_LSTACK 256
+#endif
+constexpr int PP_TAKE = WALK_PP_TAKE;  // particles a lane appends per round
 constexpr int PP_CAP = 64 * PP_TAKE + 64;  // one round + 63 left over
 constexpr int PC_CAP = 128;
 constexpr int STACK_MAX = 8192;  // per-wave walk stack entries
 constexpr int WALK_XCDS = LAGPM_WALK_XCDS;
-constexpr int LSTACK = 256;      // the first LSTACK entries of a wave's stack live in LDS, the rest in global memory
+constexpr int LSTACK = WALK_LSTACK;  // the first LSTACK entries of a wave's stack live in LDS, the rest in global memory
 constexpr int WALK_MAX_ITERS = 1 << 20;  // per-group bound on 64-node pop rounds
 // rmax: a leaf and an inner node of up to RMAX_EXACT_MAX particles get the exact largest distance
 // of their particles from the com (the reference's node size); larger nodes get the bound
@@ -161,10 +170,14 @@ __global__ void gather_all_kernel(gather_set g, const uint32_t* __restrict__ per
     g.id_dst[i] = id;
 }
 
+// FP32 copy of the positions for the tree, scaled by sc (rounded once from the FP64 product):
+// with the log-r^2 force tables sc = sqrt(lam), so a pair's r^2 in these units is lam r^2, the
+// table variable, and needs no multiply in the pair loop. Everything the FP32 tree derives from
+// pp (centres, second moments, node sizes, group boxes) is in the same units.
 __global__ void pack_pp_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-                               size_t np, float mass, float4* __restrict__ pp) {
+                               size_t np, float mass, double sc, float4* __restrict__ pp) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < np) pp[i] = make_float4((float)x[i], (float)y[i], (float)z[i], mass);
+    if (i < np) pp[i] = make_float4((float)(x[i] * sc), (float)(y[i] * sc), (float)(z[i] * sc), mass);
 }
 
 // node count on the device, clamped to the array capacity
@@ -625,12 +638,14 @@ template <bool LOGT>
 __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* __restrict__ B, const uint32_t* __restrict__ gprefix,
                                                                 int nbox, const float2* __restrict__ tab0,
                                                                 const float4* __restrict__ tab12, int* __restrict__ stacks,
-                                                                unsigned long long* __restrict__ ticket_ctr, float lam) {
+                                                                unsigned long long* __restrict__ ticket_ctr) {
+    // positions are in scaled units (pack_pp_kernel): for the log tables r^2 here is lam r^2
+    constexpr float lam = 1.0f;
     // One 1024-thread workgroup per CU: the 16 waves share one LDS copy of the particle force
     // table (32 KB) next to their private interaction lists.
     __shared__ float2 s_tab0[TAB_LDS];
     __shared__ uint32_t s_ppidx[WALK_WAVES][PP_CAP];
-    __shared__ int s_stack[WALK_WAVES][LSTACK];
+    __shared__ int s_stack[WALK_WAVES][LSTACK > 0 ? LSTACK : 1];
     __shared__ int s_pcidx[WALK_WAVES][PC_CAP];
     __shared__ float4 s_pp[WALK_WAVES][64];
     __shared__ float4 s_pc[WALK_WAVES][3][64];
@@ -1232,6 +1247,11 @@ size_t lagpm_tree_node_cap_guess(size_t np, int mcell) {
     return np / 3 + (size_t)mcell * mcell * mcell + 4096;
 }
 
+// scale of the FP32 tree's coordinates: sqrt(lam) for the log-r^2 tables, 1 for the r-table law
+double lagpm_tree_scale(const lagpm_box* box) {
+    return box->tab && box->tab->logr2 ? std::sqrt((double)box->tab->lam) : 1.0;
+}
+
 int lagpm_tree_alloc(lagpm_ctx* ctx, lagpm_box* box, size_t cap) {
     const bool f64 = box->p.precision == LAGPM_PREC_FP64;
     if (cap > (size_t)LAGPM_MAX_NODES) {
@@ -1312,6 +1332,7 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     if (!f64) {
         // the tree works in units of the particle mass; the walk multiplies by it with the kick
         hipLaunchKernelGGL(pack_pp_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->x, box->y, box->z, np, 1.0f,
+                           lagpm_tree_scale(box),
                            box->pp);
         HIPCHK(ctx, hipGetLastError());
     }
@@ -1395,11 +1416,13 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
         A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell);
         A.mcell = box->mcell;
         A.naccept = LAGPM_NACCEPT;
-        A.box = (float)box->p.n;
-        A.rc2 = (float)(box->p.rsphere * box->p.rsphere);
-        A.rcut = (float)box->p.rsphere;
+        // lengths in the units of pp (scaled by sc, see pack_pp_kernel)
+        const double sc = lagpm_tree_scale(box);
+        A.box = (float)(box->p.n * sc);
+        A.rc2 = (float)(box->p.rsphere * box->p.rsphere * sc * sc);
+        A.rcut = (float)(box->p.rsphere * sc);
         A.inv_theta2 = (float)(th > 0 ? 1.0 / (th * th) : 1e30);
-        A.fact = (float)(facts[b] * box->p.mass / ((double)box->p.n * (double)box->p.n));
+        A.fact = (float)(facts[b] * box->p.mass / ((double)box->p.n * (double)box->p.n) / sc);
     }
     // pageable source: the copy is staged before the call returns, so hb may go out of scope
     HIPCHK(ctx, hipMemcpyAsync(ctx->walk_boxes, hb.data(), hb.size() * sizeof(walk_box), hipMemcpyHostToDevice, ctx->stream));
@@ -1409,10 +1432,10 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
     const unsigned blocks = (unsigned)(ctx->walk_stack_entries / ((size_t)STACK_MAX * WALK_WAVES));
     if (tab->logr2)
         hipLaunchKernelGGL(walk_f32_kernel<true>, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes,
-                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket, tab->lam);
+                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
     else
         hipLaunchKernelGGL(walk_f32_kernel<false>, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes,
-                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket, tab->lam);
+                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }

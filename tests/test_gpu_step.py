@@ -110,7 +110,8 @@ def test_node_overflow_is_regrown_and_the_kick_is_exact(ctx, prec, tol):
     """Tight 17-particle clumps make about one node per particle, beyond the initial node
     capacity: the build must detect it, grow the arrays and still give the exact pair sum.
     The FP32 walk sees FP32 positions (clump members 2e-4 cells apart differ in the 3rd
-    digit at that rounding), so its exact sum is taken over the FP32-rounded positions."""
+    digit at that rounding), so its exact sum is taken over the positions it sees: with the
+    erfc law's log-r^2 tables the FP32 tree holds float32(x * sqrt(lam)), lam = float32(32 / rcut^2)."""
     n = 32
     g = np.random.default_rng(12)
     centres = g.random((400, 3)) * n
@@ -126,7 +127,11 @@ def test_node_overflow_is_regrown_and_the_kick_is_exact(ctx, prec, tol):
         assert np.array_equal(b.velocities(), 2 * a)
     print(f"clumps prec={prec}: N={len(pos)} nodes {st.n_nodes} groups {st.n_groups}", flush=True)
     assert st.n_nodes > len(pos) // 3
-    seen = pos.astype(np.float32).astype(np.float64) if prec == lagpm.PREC_FP32 else pos
+    if prec == lagpm.PREC_FP32:
+        sc = np.sqrt(np.float64(np.float32(32.0 / (p.rsphere * p.rsphere))))
+        seen = (pos * sc).astype(np.float32).astype(np.float64) / sc
+    else:
+        seen = pos
     ref = orc.pp_direct_analytic(1, n, p.epsilon, p.efold / 2, seen, p.rsphere)
     err = float(np.sqrt(np.sum((a - ref) ** 2) / np.sum(ref ** 2)))
     print(f"  rms rel {err:.3e}", flush=True)
