@@ -56,13 +56,22 @@ __global__ void scale3_kernel(double* __restrict__ a, double* __restrict__ b, do
 }
 // skip (optional): a box whose tree build overflowed got no tree kick yet, so it must not
 // drift until the host has redone the kick
+// Drift x += pfact v (wrapped into [0, L)). With tv, the tree kick of the step is first added to
+// the velocities (v += tv, after the PM kick, as the reference adds them).
 __global__ void drift_kernel(double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
-                             const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
-                             size_t np, const double* __restrict__ fac, double L, const uint32_t* __restrict__ skip) {
+                             double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+                             const double* __restrict__ tvx, const double* __restrict__ tvy,
+                             const double* __restrict__ tvz, size_t np, const double* __restrict__ fac, double L,
+                             const uint32_t* __restrict__ skip) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np || (skip && *skip)) return;
     const double pfact = fac[2];
-    double p[3] = {x[i] + pfact * vx[i], y[i] + pfact * vy[i], z[i] + pfact * vz[i]};
+    double u = vx[i], w = vy[i], q = vz[i];
+    if (tvx) {
+        u += tvx[i]; w += tvy[i]; q += tvz[i];
+        vx[i] = u; vy[i] = w; vz[i] = q;
+    }
+    double p[3] = {x[i] + pfact * u, y[i] + pfact * w, z[i] + pfact * q};
     for (int a = 0; a < 3; a++) {
         double v = fmod(p[a], L);
         if (v < 0) v += L;
@@ -72,6 +81,17 @@ __global__ void drift_kernel(double* __restrict__ x, double* __restrict__ y, dou
     x[i] = p[0];
     y[i] = p[1];
     z[i] = p[2];
+}
+
+// v += tv: a tree kick computed on its own (see the walk) applied to the velocities
+__global__ void add3_kernel(double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+                            const double* __restrict__ tvx, const double* __restrict__ tvy, const double* __restrict__ tvz,
+                            size_t np) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    vx[i] += tvx[i];
+    vy[i] += tvy[i];
+    vz[i] += tvz[i];
 }
 
 unsigned nblocks(size_t n, int t) { return (unsigned)std::max<size_t>(1, (n + t - 1) / t); }
@@ -114,6 +134,7 @@ void free_box(lagpm_box* b) {
     dfree(b->nd_com); dfree(b->nd_q0); dfree(b->nd_q1); dfree(b->nd_d); dfree(b->cellroot); dfree(b->gflag);
     dfree(b->goff); dfree(b->groups); dfree(b->gbox); dfree(b->pgroup); dfree(b->counters); dfree(b->dsz);
     dfree(b->scan_tmp);
+    dfree(b->dep_scan_tmp);
 }
 
 int check_params(lagpm_ctx* ctx, const lagpm_params* p) {
@@ -390,6 +411,7 @@ int lagpm_destroy(lagpm_ctx* ctx) {
     for (int i = 0; i < 4; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     for (hipStream_t s : ctx->pool) (void)hipStreamDestroy(s);
+    for (hipStream_t s : ctx->pool2) (void)hipStreamDestroy(s);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LAGPM_OK;
@@ -418,6 +440,7 @@ size_t lagpm_estimate_box_bytes(const lagpm_params* p, size_t np) {
     const size_t cap = lagpm_tree_node_cap_guess(np, (int)mcell), gcap = cap + np / 64 + 1;
     b += cap * (16 + 4 + 32 + 4 + 4 + (f64 ? 96 : 48)) + gcap * (8 + (f64 ? 48 : 0));
     b += std::max(lagpm_scan_tmp_bytes(cap), std::max(lagpm_scan_tmp_bytes(np), lagpm_sort_tmp_bytes(np)));
+    b += lagpm_scan_tmp_bytes(np);  // the deposit's own scan scratch
     return b;
 }
 
@@ -452,6 +475,7 @@ int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box
     box->spec = box->phi = box->rho;
     if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->grad, m * 4 * real);
     ALLOC(box->ncount, np); ALLOC(box->noff, np);
+    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->dep_scan_tmp, lagpm_scan_tmp_bytes(np));
     ALLOC(box->cellroot, (size_t)box->mcell * box->mcell * box->mcell);
     ALLOC(box->counters, 4);
     ALLOC(box->dsz, DSZ_COUNT);
@@ -477,7 +501,8 @@ int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box
         return rc;
     }
     box->st = ctx->stream;
-    for (int i = 0; i < 12; i++) (void)hipEventCreate(&box->ev[i]);
+    for (int i = 0; i < 15; i++) (void)hipEventCreate(&box->ev[i]);
+    (void)hipEventCreateWithFlags(&box->evfork, hipEventDisableTiming);
     (void)hipMemsetAsync(box->vx, 0, np * sizeof(double), ctx->stream);
     (void)hipMemsetAsync(box->vy, 0, np * sizeof(double), ctx->stream);
     (void)hipMemsetAsync(box->vz, 0, np * sizeof(double), ctx->stream);
@@ -490,8 +515,9 @@ int lagpm_box_destroy(lagpm_ctx* ctx, lagpm_box* box) {
     if (!box) return LAGPM_OK;
     if (ctx) (void)hipStreamSynchronize(box->st);
     if (ctx) (void)hipStreamSynchronize(ctx->stream);
-    for (int i = 0; i < 12; i++)
+    for (int i = 0; i < 15; i++)
         if (box->ev[i]) (void)hipEventDestroy(box->ev[i]);
+    if (box->evfork) (void)hipEventDestroy(box->evfork);
     free_box(box);
     delete box;
     return LAGPM_OK;
@@ -734,6 +760,7 @@ int lagpm_pmmain(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef, int p
     HIPCHK(ctx, hipEventRecord(ev[0], box->st));
     if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
     HIPCHK(ctx, hipEventRecord(ev[1], box->st));
+    HIPCHK(ctx, hipEventRecord(ev[12], box->st));
     RCHK(lagpm_pm_deposit(ctx, box));
     HIPCHK(ctx, hipEventRecord(ev[2], box->st));
     RCHK(lagpm_pm_solve(ctx, box, ev + 3));
@@ -785,7 +812,9 @@ int lagpm_force_spline(const lagpm_params* p, double* diff, double* slope) {
 
 // Queues sort (when needed), build and walk of one box. The FP32 walk runs on the context
 // stream (its stacks are shared), so it waits for the box stream and the box stream for it.
-static int tree_enqueue(lagpm_ctx* ctx, lagpm_box* box, double fact) {
+// The FP32 walk writes its kick to the box's scratch arrays; with combine it is added to the
+// velocities here, otherwise the caller adds it (the drift of a step) or drops it (autotuning).
+static int tree_enqueue(lagpm_ctx* ctx, lagpm_box* box, double fact, bool combine) {
     hipEvent_t* ev = box->ev + 8;
     HIPCHK(ctx, hipEventRecord(ev[0], box->st));
     if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
@@ -793,13 +822,20 @@ static int tree_enqueue(lagpm_ctx* ctx, lagpm_box* box, double fact) {
     RCHK(lagpm_tree_build(ctx, box));
     HIPCHK(ctx, hipEventRecord(ev[2], box->st));
     if (box->p.precision == LAGPM_PREC_FP64) {
+        HIPCHK(ctx, hipEventRecord(box->ev[14], box->st));
         RCHK(lagpm_tree_walk_f64(ctx, box, fact));
         HIPCHK(ctx, hipEventRecord(ev[3], box->st));
     } else {
         if (box->st != ctx->stream) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ev[2], 0));
+        HIPCHK(ctx, hipEventRecord(box->ev[14], ctx->stream));
         RCHK(lagpm_tree_walk_f32(ctx, &box, &fact, 1));
         HIPCHK(ctx, hipEventRecord(ev[3], ctx->stream));
         if (box->st != ctx->stream) HIPCHK(ctx, hipStreamWaitEvent(box->st, ev[3], 0));
+        if (combine) {
+            hipLaunchKernelGGL(add3_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, box->st, box->vx, box->vy, box->vz,
+                               (const double*)box->alt[0], (const double*)box->alt[1], (const double*)box->alt[2], box->np);
+            HIPCHK(ctx, hipGetLastError());
+        }
     }
     box->tree_timed = true;
     return LAGPM_OK;
@@ -811,7 +847,7 @@ static void tree_times(lagpm_box* box) {
     (void)hipEventSynchronize(ev[3]);
     (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
     (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
-    (void)hipEventElapsedTime(&t23, ev[2], ev[3]);
+    (void)hipEventElapsedTime(&t23, box->ev[14], ev[3]);  // walk start .. walk end
     box->stats.ms_build = t01 + t12;
     box->stats.ms_walk = t23;
 }
@@ -829,7 +865,7 @@ int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* e
     RCHK(tree_check(ctx, box));
     if (box->np == 0) return LAGPM_OK;
     for (int attempt = 0;; attempt++) {
-        RCHK(tree_enqueue(ctx, box, (double)ef->fact2));
+        RCHK(tree_enqueue(ctx, box, (double)ef->fact2, true));
         bool retry = false;
         int rc = lagpm_tree_finish(ctx, box, &retry);
         tree_times(box);
@@ -865,7 +901,7 @@ int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group) {
         RCHK(lagpm_box_set_tuning(box, c[0], c[1]));
         double t = 1e300;
         for (int rep = 0; rep < 2; rep++) {  // the first run of a candidate may grow the node arrays
-            RCHK(tree_enqueue(ctx, box, 0.0));
+            RCHK(tree_enqueue(ctx, box, 0.0, false));
             bool retry = false;
             int rc = lagpm_tree_finish(ctx, box, &retry);
             if (rc != LAGPM_OK && !retry) return rc;
@@ -903,7 +939,7 @@ int lagpm_box_timings(lagpm_ctx* ctx, lagpm_box* box, lagpm_timings* out) {
     };
     if (box->pm_timed) {
         out->ms_sort = el(0, 1);
-        out->ms_tsc = el(1, 2);
+        out->ms_tsc = el(12, 2);
         out->ms_fft_r2c = el(2, 3);
         out->ms_green = el(3, 4);
         out->ms_fft_c2r = el(4, 5);
@@ -949,7 +985,8 @@ int lagpm_drift(lagpm_ctx* ctx, lagpm_box* box, double pfact) {
     if (box->np == 0) return LAGPM_OK;
     hipLaunchKernelGGL(set_factors_kernel, dim3(1), dim3(64), 0, box->st, box->dfac, 0.0, 0.0, pfact, 4);
     hipLaunchKernelGGL(drift_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx,
-                       box->vy, box->vz, box->np, (const double*)box->dfac, (double)box->p.n, (const uint32_t*)nullptr);
+                       box->vy, box->vz, (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, box->np,
+                       (const double*)box->dfac, (double)box->p.n, (const uint32_t*)nullptr);
     HIPCHK(ctx, hipGetLastError());
     box->sorted = false;
     return LAGPM_OK;
@@ -961,35 +998,49 @@ namespace {
 
 int ensure_pool(lagpm_ctx* ctx, size_t want) {
     while (ctx->pool.size() < want) {
-        hipStream_t s = nullptr;
+        hipStream_t s = nullptr, s2 = nullptr;
         HIPCHK(ctx, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         ctx->pool.push_back(s);
+        HIPCHK(ctx, hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        ctx->pool2.push_back(s2);
     }
     return LAGPM_OK;
 }
 
-// The kernels of one box's step up to the tree build, queued on box->st with the factors
-// already in box->dfac. With ev: phase events (eager runs only; a capture records none).
+// One box's step before the walk, queued on box->st with the factors already in box->dfac:
+// Hubble drag and sort, then a fork: the tree build on box->st2 beside the PM kick (deposit,
+// FFTs, Green, gradient, interpolation) on box->st, joined on box->st. The build's small,
+// latency-bound kernels and the PM's memory-bound ones share the GPU. With ev: phase events
+// (eager runs only; a capture records none, but the fork and join events are part of it).
 int front_body(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     const size_t np = box->np;
-    hipLaunchKernelGGL(scale3_kernel, dim3(nblocks(np, 256)), dim3(256), 0, box->st, box->vx, box->vy, box->vz, np,
+    hipStream_t st = box->st, st2 = box->st2;
+    hipLaunchKernelGGL(scale3_kernel, dim3(nblocks(np, 256)), dim3(256), 0, st, box->vx, box->vy, box->vz, np,
                        (const double*)box->dfac);
     HIPCHK(ctx, hipGetLastError());
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[0], box->st));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
     RCHK(lagpm_sort_particles(ctx, box));
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[1], box->st));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[1], st));
+    HIPCHK(ctx, hipEventRecord(box->evfork, st));
+    HIPCHK(ctx, hipStreamWaitEvent(st2, box->evfork, 0));
+    if (ev) {
+        HIPCHK(ctx, hipEventRecord(ev[8], st2));
+        HIPCHK(ctx, hipEventRecord(ev[9], st2));  // sorted above: no tree sort
+    }
+    box->st = st2;
+    const int rb = lagpm_tree_build(ctx, box);
+    box->st = st;
+    RCHK(rb);
+    HIPCHK(ctx, hipEventRecord(box->ev[10], st2));  // build done: the join
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[12], st));
     RCHK(lagpm_pm_deposit(ctx, box));
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[2], box->st));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
     RCHK(lagpm_pm_solve(ctx, box, ev ? ev + 3 : nullptr));
     RCHK(lagpm_pm_gradient(ctx, box));
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[6], box->st));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
     RCHK(lagpm_pm_interp(ctx, box));
-    if (ev) {
-        HIPCHK(ctx, hipEventRecord(ev[7], box->st));
-        HIPCHK(ctx, hipEventRecord(ev[8], box->st));
-        HIPCHK(ctx, hipEventRecord(ev[9], box->st));  // sorted above: no tree sort
-    }
-    RCHK(lagpm_tree_build(ctx, box));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[7], st));
+    HIPCHK(ctx, hipStreamWaitEvent(st, box->ev[10], 0));
     return LAGPM_OK;
 }
 
@@ -1018,18 +1069,20 @@ void front_host_effects(lagpm_box* b) {
     b->sorted = true;
 }
 
-// Everything of one box's step up to the tree build, queued on box->st. With graphs on, the
-// kernels are captured once per particle-set parity and replayed; factors are written first.
+// Everything of one box's step before the walk, queued on box->st (and box->st2), then the event
+// the walk waits for (ev[13]). With graphs on, the front is captured once per particle-set parity
+// and replayed; factors are written first.
 int step_front(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact& ef) {
     hipLaunchKernelGGL(set_factors_kernel, dim3(1), dim3(64), 0, box->st, box->dfac, (double)ef.fact1, (double)ef.fact2,
                        (double)ef.pfact, 7);
     HIPCHK(ctx, hipGetLastError());
-    if (!ctx->use_graphs || box->graph_failed) {
+    auto eager = [&]() -> int {
         RCHK(front_body(ctx, box, box->ev));
         box->pm_timed = true;
-        HIPCHK(ctx, hipEventRecord(box->ev[10], box->st));
+        HIPCHK(ctx, hipEventRecord(box->ev[13], box->st));
         return LAGPM_OK;
-    }
+    };
+    if (!ctx->use_graphs || box->graph_failed) return eager();
     // everything a capture must not do (plan creation and its check synchronise) happens first
     RCHK(lagpm_pm_prepare(ctx, box));
     HIPCHK(ctx, hipEventRecord(box->ev[8], box->st));
@@ -1043,31 +1096,32 @@ int step_front(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact& ef) {
         if (ok) ok = hipStreamEndCapture(box->st, &g) == hipSuccess && rc == LAGPM_OK && g;
         if (ok) ok = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
         if (g) (void)hipGraphDestroy(g);
-        const box_ptrs after = save_ptrs(box);
         load_ptrs(box, before);
         (void)hipGetLastError();
         if (!ok) {  // run eagerly from now on for this box
             ex = nullptr;
             box->graph_failed = true;
-            RCHK(front_body(ctx, box, box->ev));
-            box->pm_timed = true;
-            HIPCHK(ctx, hipEventRecord(box->ev[10], box->st));
-            return LAGPM_OK;
+            return eager();
         }
-        (void)after;
     }
     HIPCHK(ctx, hipGraphLaunch(ex, box->st));
     front_host_effects(box);
     box->pm_timed = false;  // no phase events inside a graph
     HIPCHK(ctx, hipEventRecord(box->ev[10], box->st));
+    HIPCHK(ctx, hipEventRecord(box->ev[13], box->st));
     return LAGPM_OK;
 }
 
+// The drift of a step; an FP32 box's tree kick (written by the walk to the scratch arrays) is
+// added to the velocities first.
 int drift_enqueue(lagpm_ctx* ctx, lagpm_box* box, double pfact, bool guarded) {
     if (!guarded)  // a redone step: the factor of this step is already in dfac[2] otherwise
         hipLaunchKernelGGL(set_factors_kernel, dim3(1), dim3(64), 0, box->st, box->dfac, 0.0, 0.0, pfact, 4);
+    const bool tk = box->p.precision == LAGPM_PREC_FP32;
     hipLaunchKernelGGL(drift_kernel, dim3(nblocks(box->np, 256)), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx,
-                       box->vy, box->vz, box->np, (const double*)box->dfac, (double)box->p.n,
+                       box->vy, box->vz, tk ? (const double*)box->alt[0] : nullptr,
+                       tk ? (const double*)box->alt[1] : nullptr, tk ? (const double*)box->alt[2] : nullptr, box->np,
+                       (const double*)box->dfac, (double)box->p.n,
                        guarded ? (const uint32_t*)(box->dsz + DSZ_OVERFLOW) : (const uint32_t*)nullptr);
     HIPCHK(ctx, hipGetLastError());
     box->sorted = false;
@@ -1104,6 +1158,7 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
         lagpm_box* b = boxes[i];
         fact[(size_t)i] = ef[(size_t)i].fact2;
         if (!b->np) continue;
+        b->st2 = ctx->pool2[k % S];
         b->st = ctx->pool[k++ % S];
         rc = step_front(ctx, b, ef[(size_t)i]);
     }
@@ -1113,7 +1168,8 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
         lagpm_box* b = boxes[i];
         if (!b->np || done[(size_t)i]) continue;
         if (b->p.precision == LAGPM_PREC_FP64) {
-            rc = lagpm_tree_walk_f64(ctx, b, fact[(size_t)i]);
+            if (hipEventRecord(b->ev[14], b->st) != hipSuccess) rc = LAGPM_E_HIP;
+            if (rc == LAGPM_OK) rc = lagpm_tree_walk_f64(ctx, b, fact[(size_t)i]);
             if (rc == LAGPM_OK && hipEventRecord(b->ev[11], b->st) != hipSuccess) rc = LAGPM_E_HIP;
             done[(size_t)i] = 1;
             continue;
@@ -1123,11 +1179,13 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
         for (int j = i; j < nbox; j++) {
             lagpm_box* c = boxes[j];
             if (!c->np || done[(size_t)j] || c->p.precision != LAGPM_PREC_FP32 || c->tab != b->tab) continue;
-            if (hipStreamWaitEvent(ctx->stream, c->ev[10], 0) != hipSuccess) rc = LAGPM_E_HIP;
+            if (hipStreamWaitEvent(ctx->stream, c->ev[13], 0) != hipSuccess) rc = LAGPM_E_HIP;
             cls.push_back(c);
             cf.push_back(fact[(size_t)j]);
             done[(size_t)j] = 1;
         }
+        for (lagpm_box* c : cls)
+            if (rc == LAGPM_OK && hipEventRecord(c->ev[14], ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
         if (rc == LAGPM_OK) rc = lagpm_tree_walk_f32(ctx, cls.data(), cf.data(), (int)cls.size());
         if (rc == LAGPM_OK && hipEventRecord(ctx->ev[1], ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
         for (lagpm_box* c : cls) {
@@ -1154,7 +1212,7 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
             if (retry) {
                 b->st = ctx->stream;
                 HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-                r = tree_enqueue(ctx, b, fact[(size_t)i]);
+                r = tree_enqueue(ctx, b, fact[(size_t)i], false);
                 if (r == LAGPM_OK) r = lagpm_tree_finish(ctx, b, &retry);
                 if (r == LAGPM_OK) r = drift_enqueue(ctx, b, (double)ef[(size_t)i].pfact, false);
                 if (r != LAGPM_OK && retry) r = lagpm_fail(ctx, LAGPM_E_OVERFLOW, "tree node arrays overflowed");
@@ -1164,7 +1222,10 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
         }
         b->st = ctx->stream;
     }
-    for (size_t q = 0; q < S; q++) (void)hipStreamSynchronize(ctx->pool[q]);
+    for (size_t q = 0; q < S; q++) {
+        (void)hipStreamSynchronize(ctx->pool[q]);
+        (void)hipStreamSynchronize(ctx->pool2[q]);
+    }
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return first_err;
 }

@@ -51,6 +51,7 @@ struct lagpm_ctx {
     int num_cu = 0;
     hipStream_t stream = nullptr;            // default stream of the context
     std::vector<hipStream_t> pool;           // per-box streams of batched steps
+    std::vector<hipStream_t> pool2;          // their second streams (tree build beside the PM kick)
     std::string err;
     std::map<std::pair<int, int>, fft_plans> plans;  // (n, precision) -> plans
     std::map<std::tuple<int, double, double, double>, law_tables> tables;  // (law, epsilon, efold, rsphere)
@@ -96,6 +97,7 @@ struct lagpm_box {
     bool sorted = false;
     tree_tune tune;
     hipStream_t st = nullptr;  // stream this box's work is queued on (context stream or a pool stream)
+    hipStream_t st2 = nullptr;  // second stream of a step front (the tree build beside the PM kick)
 
     // particle state, kept in the current (sorted after a sort) order. A sort gathers into the
     // other set and swaps the two, so the pointers alternate with period two (a captured step
@@ -114,10 +116,12 @@ struct lagpm_box {
     rocfft_execution_info fft_info = nullptr;
     void* fft_work = nullptr;
     // captured front half of a step (drag .. tree build), one per particle-set parity
-    hipGraphExec_t gfront[2] = {nullptr, nullptr};
+    hipGraphExec_t gfront[2] = {nullptr, nullptr};  // captured step front per particle-set parity
+    hipEvent_t evfork = nullptr;  // fork of the step front: the tree build branches off after the sort
     bool graph_failed = false;
-    uint32_t* scan_tmp = nullptr;  // scan / radix-sort scratch of this box
+    uint32_t* scan_tmp = nullptr;  // scan / radix-sort scratch of this box (sort and tree build)
     size_t scan_tmp_bytes = 0;
+    uint32_t* dep_scan_tmp = nullptr;  // the deposit's own scan scratch: it may run beside the tree build
 
     // PM meshes
     int scale_bits = 32;
@@ -158,7 +162,7 @@ struct lagpm_box {
     lagpm_tree_stats stats{};
     // phase events: [0..7] pmmain (start, sort, tsc, r2c, green, c2r, grad, interp),
     //               [8..11] treecorrection (start, sort, build, walk)
-    hipEvent_t ev[12] = {};
+    hipEvent_t ev[15] = {};  // phase events; [12] deposit start, [13] step front joined, [14] walk start
     bool pm_timed = false, tree_timed = false;
 };
 

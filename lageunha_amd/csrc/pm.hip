@@ -94,17 +94,18 @@ __global__ void dep_start_kernel(const uint32_t* __restrict__ flag, const uint32
 template <bool F64>
 __global__ void __launch_bounds__(DEP_THREADS) deposit_kernel(const double* __restrict__ px, const double* __restrict__ py,
                                                               const double* __restrict__ pz, const uint32_t* __restrict__ start,
-                                                              const uint32_t* __restrict__ nchunk, int n, float mass,
-                                                              double scale, unsigned long long* __restrict__ acc) {
+                                                              const uint32_t* __restrict__ nchunk, size_t npart, int n,
+                                                              float mass, double scale, unsigned long long* __restrict__ acc) {
     __shared__ unsigned long long box[DEP_LDS];
     __shared__ int red[6][DEP_THREADS / 64];
     // the grid is sized for the largest possible chunk count; the actual count is on the device
-    const unsigned nch = *nchunk;
+    const unsigned nch = min(*nchunk, gridDim.x);  // the grid covers every possible chunk
     if (blockIdx.x >= nch) return;
     const unsigned chunk = lagpm_wave::xcd_block(blockIdx.x, nch);  // neighbouring chunks on one XCD's L2
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const size_t base = start[chunk];
-    const size_t np = start[chunk + 1];
+    // chunk bounds clamped to the particle count (they come from the chunk scan of this box)
+    const size_t np = min((size_t)start[chunk + 1], npart);
+    const size_t base = min((size_t)start[chunk], np);
 
     int ix[DEP_PPT], iy[DEP_PPT], iz[DEP_PPT], sx[DEP_PPT], sy[DEP_PPT], sz[DEP_PPT];
     float wx[DEP_PPT][3], wy[DEP_PPT][3], wz[DEP_PPT][3];
@@ -696,24 +697,27 @@ int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
     const double scale = std::ldexp(1.0, box->scale_bits);
     if (box->np) {
         // chunk list from the sorted keys (the caller sorted the particles); the chunk count
-        // stays on the device and the deposit grid covers its upper bound
+        // stays on the device and the deposit grid covers its upper bound. Its scratch is its own
+        // (perm2 and key2 hold nothing between sorts, dep_scan_tmp is the deposit's): a step runs
+        // the deposit beside the tree build, which uses ncount, noff and scan_tmp.
         const int sh = 3 * (LAGPM_KEY_BITS - deposit_block_level(n));
         const size_t np = box->np;
-        hipLaunchKernelGGL(dep_flag_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, st, box->key, np, sh,
-                           box->ncount);
+        uint32_t* flag = box->perm2;
+        uint32_t* off = reinterpret_cast<uint32_t*>(box->key2);
+        hipLaunchKernelGGL(dep_flag_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, st, box->key, np, sh, flag);
         HIPCHK(ctx, hipGetLastError());
         uint32_t* nchunk = box->dsz + DSZ_CHUNKS;
-        RCHK(lagpm_exclusive_scan_u32(ctx, st, box->scan_tmp, box->ncount, box->noff, np, nchunk));
-        hipLaunchKernelGGL(dep_start_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, st, box->ncount,
-                           box->noff, np, box->dep_start, nchunk);
+        RCHK(lagpm_exclusive_scan_u32(ctx, st, box->dep_scan_tmp, flag, off, np, nchunk));
+        hipLaunchKernelGGL(dep_start_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, st, flag, off, np,
+                           box->dep_start, nchunk);
         HIPCHK(ctx, hipGetLastError());
         const unsigned grid = (unsigned)(box->dep_cap - 1);
         if (f64)
             hipLaunchKernelGGL(deposit_kernel<true>, dim3(grid), dim3(DEP_THREADS), 0, st, box->x, box->y, box->z,
-                               box->dep_start, nchunk, n, (float)box->p.mass, scale, box->acc);
+                               box->dep_start, nchunk, np, n, (float)box->p.mass, scale, box->acc);
         else
             hipLaunchKernelGGL(deposit_kernel<false>, dim3(grid), dim3(DEP_THREADS), 0, st, box->x, box->y, box->z,
-                               box->dep_start, nchunk, n, (float)box->p.mass, scale, box->acc);
+                               box->dep_start, nchunk, np, n, (float)box->p.mass, scale, box->acc);
         HIPCHK(ctx, hipGetLastError());
     }
     const double inv_scale = std::ldexp(1.0, -box->scale_bits);

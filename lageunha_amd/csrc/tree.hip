@@ -306,7 +306,7 @@ __device__ inline int chain_first_level(const uint64_t* __restrict__ K, uint32_t
     return max(L, cpl + 1);
 }
 // pre-order numbering: info, child and parent links, cell roots and the level order
-__global__ void number_kernel(const uint64_t* __restrict__ K, int L, size_t cap, const uint32_t* __restrict__ dsz,
+__global__ void number_kernel(const uint64_t* __restrict__ K, int L, size_t cap, uint32_t* __restrict__ dsz,
                               const int4* __restrict__ rec, const int* __restrict__ rch0, const int* __restrict__ rch1,
                               const uint32_t* __restrict__ noff, int4* __restrict__ info, int* __restrict__ child,
                               int* __restrict__ parent, int* __restrict__ cellroot, int mcell, uint32_t* __restrict__ order) {
@@ -322,11 +322,16 @@ __global__ void number_kernel(const uint64_t* __restrict__ K, int L, size_t cap,
         const uint64_t cell = K[nd.x] >> key_shift(L);
         cellroot[compact3(cell) + (size_t)mcell * (compact3(cell >> 1) + (size_t)mcell * compact3(cell >> 2))] = (int)v;
     }
+    const uint32_t nodes = dsz[DSZ_NODES];
     for (int c = 0; c < nd.w; c++) {
         const uint32_t cf = (uint32_t)(c < 4 ? rch0[4 * r + c] : rch1[4 * r + c - 4]);
         const uint32_t vc = noff[cf] + (uint32_t)(nd.z + 1 - chain_first_level(K, cf, L));
+        if (vc >= nodes) {  // cannot happen for sorted keys; the build is then marked failed
+            dsz[DSZ_OVERFLOW] = 1u;
+            return;
+        }
         child[8 * (size_t)v + c] = (int)vc;
-        if ((size_t)vc < cap) parent[vc] = (int)v;
+        parent[vc] = (int)v;
     }
 }
 
@@ -604,7 +609,7 @@ struct walk_box {
     const int* cellroot;
     const int2* groups;
     const uint64_t* key;
-    double *vx, *vy, *vz;
+    double *tvx, *tvy, *tvz;       // the tree kick of each particle (written, not added: see below)
     unsigned long long* counters;  // [0] n_pp, [1] n_pc, [2] walk stack overflow
     const uint32_t* dsz;
     int cell_shift;
@@ -1081,12 +1086,14 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             ay += __shfl_xor(ay, o, 64);
             az += __shfl_xor(az, o, 64);
         }
-        if (overflow) {
-            if (lane == 0) atomicAdd(&A.counters[2], 1ull);
-        } else if (valid && rep == 0) {
-            A.vx[me] += (double)(A.fact * ax);
-            A.vy[me] += (double)(A.fact * ay);
-            A.vz[me] += (double)(A.fact * az);
+        // The kick is written to its own arrays rather than added to the velocities, so the walk
+        // can run while the PM kick of the same step updates them; the drift (or, for a tree
+        // kick alone, an add kernel) adds it afterwards, in the same order as v += pm; v += tree.
+        if (overflow && lane == 0) atomicAdd(&A.counters[2], 1ull);
+        if (valid && rep == 0) {
+            A.tvx[me] = overflow ? 0.0 : (double)(A.fact * ax);
+            A.tvy[me] = overflow ? 0.0 : (double)(A.fact * ay);
+            A.tvz[me] = overflow ? 0.0 : (double)(A.fact * az);
         }
         if (lane == 0) {
             atomicAdd(&A.counters[0], npp * (unsigned long long)cnt);
@@ -1412,7 +1419,8 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
         walk_box& A = hb[(size_t)b];
         A.pp = box->pp; A.info = box->nd_info; A.child = box->nd_child; A.com = box->nd_com; A.q0 = box->nd_q0;
         A.q1 = box->nd_q1; A.cellroot = box->cellroot; A.groups = box->groups; A.key = box->key;
-        A.vx = box->vx; A.vy = box->vy; A.vz = box->vz; A.counters = box->counters; A.dsz = box->dsz;
+        A.tvx = box->alt[0]; A.tvy = box->alt[1]; A.tvz = box->alt[2];  // free between sorts
+        A.counters = box->counters; A.dsz = box->dsz;
         A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell);
         A.mcell = box->mcell;
         A.naccept = LAGPM_NACCEPT;
