@@ -411,7 +411,6 @@ int lagpm_destroy(lagpm_ctx* ctx) {
     for (int i = 0; i < 4; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     for (hipStream_t s : ctx->pool) (void)hipStreamDestroy(s);
-    for (hipStream_t s : ctx->pool2) (void)hipStreamDestroy(s);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LAGPM_OK;
@@ -998,11 +997,9 @@ namespace {
 
 int ensure_pool(lagpm_ctx* ctx, size_t want) {
     while (ctx->pool.size() < want) {
-        hipStream_t s = nullptr, s2 = nullptr;
+        hipStream_t s = nullptr;
         HIPCHK(ctx, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         ctx->pool.push_back(s);
-        HIPCHK(ctx, hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-        ctx->pool2.push_back(s2);
     }
     return LAGPM_OK;
 }
@@ -1158,7 +1155,9 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
         lagpm_box* b = boxes[i];
         fact[(size_t)i] = ef[(size_t)i].fact2;
         if (!b->np) continue;
-        b->st2 = ctx->pool2[k % S];
+        // the build branch goes on the context stream: a second pool stream can share the box
+        // stream's hardware queue (4 per process), which would serialise the two branches
+        b->st2 = ctx->stream;
         b->st = ctx->pool[k++ % S];
         rc = step_front(ctx, b, ef[(size_t)i]);
     }
@@ -1222,10 +1221,7 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
         }
         b->st = ctx->stream;
     }
-    for (size_t q = 0; q < S; q++) {
-        (void)hipStreamSynchronize(ctx->pool[q]);
-        (void)hipStreamSynchronize(ctx->pool2[q]);
-    }
+    for (size_t q = 0; q < S; q++) (void)hipStreamSynchronize(ctx->pool[q]);
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return first_err;
 }

@@ -51,7 +51,6 @@ struct lagpm_ctx {
     int num_cu = 0;
     hipStream_t stream = nullptr;            // default stream of the context
     std::vector<hipStream_t> pool;           // per-box streams of batched steps
-    std::vector<hipStream_t> pool2;          // their second streams (tree build beside the PM kick)
     std::string err;
     std::map<std::pair<int, int>, fft_plans> plans;  // (n, precision) -> plans
     std::map<std::tuple<int, double, double, double>, law_tables> tables;  // (law, epsilon, efold, rsphere)
