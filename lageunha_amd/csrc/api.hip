@@ -1005,10 +1005,12 @@ int ensure_pool(lagpm_ctx* ctx, size_t want) {
 }
 
 // One box's step before the walk, queued on box->st with the factors already in box->dfac:
-// Hubble drag and sort, then a fork: the tree build on box->st2 beside the PM kick (deposit,
-// FFTs, Green, gradient, interpolation) on box->st, joined on box->st. The build's small,
-// latency-bound kernels and the PM's memory-bound ones share the GPU. With ev: phase events
-// (eager runs only; a capture records none, but the fork and join events are part of it).
+// Hubble drag and sort, then a fork: the PM kick (deposit, FFTs, Green, gradient, interpolation)
+// on box->st beside the tree build on box->st2, joined on box->st. The PM branch is queued first:
+// a graph launch and an eager run both submit in queue order, and the build's many small kernels
+// would otherwise hold back the start of the longer PM chain. The build's latency-bound kernels
+// and the PM's memory-bound ones share the GPU. With ev: phase events (eager runs only; a capture
+// records none, but the fork and join events are part of it).
 int front_body(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     const size_t np = box->np;
     hipStream_t st = box->st, st2 = box->st2;
@@ -1019,6 +1021,14 @@ int front_body(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     RCHK(lagpm_sort_particles(ctx, box));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[1], st));
     HIPCHK(ctx, hipEventRecord(box->evfork, st));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[12], st));
+    RCHK(lagpm_pm_deposit(ctx, box));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
+    RCHK(lagpm_pm_solve(ctx, box, ev ? ev + 3 : nullptr));
+    RCHK(lagpm_pm_gradient(ctx, box));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
+    RCHK(lagpm_pm_interp(ctx, box));
+    if (ev) HIPCHK(ctx, hipEventRecord(ev[7], st));
     HIPCHK(ctx, hipStreamWaitEvent(st2, box->evfork, 0));
     if (ev) {
         HIPCHK(ctx, hipEventRecord(ev[8], st2));
@@ -1029,14 +1039,6 @@ int front_body(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     box->st = st;
     RCHK(rb);
     HIPCHK(ctx, hipEventRecord(box->ev[10], st2));  // build done: the join
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[12], st));
-    RCHK(lagpm_pm_deposit(ctx, box));
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[2], st));
-    RCHK(lagpm_pm_solve(ctx, box, ev ? ev + 3 : nullptr));
-    RCHK(lagpm_pm_gradient(ctx, box));
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
-    RCHK(lagpm_pm_interp(ctx, box));
-    if (ev) HIPCHK(ctx, hipEventRecord(ev[7], st));
     HIPCHK(ctx, hipStreamWaitEvent(st, box->ev[10], 0));
     return LAGPM_OK;
 }
