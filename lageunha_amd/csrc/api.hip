@@ -1029,7 +1029,8 @@ int front_body(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
     RCHK(lagpm_pm_interp(ctx, box));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[7], st));
-    HIPCHK(ctx, hipStreamWaitEvent(st2, box->evfork, 0));
+    const bool fork = st2 != st;  // several boxes in a batch build on their own streams
+    if (fork) HIPCHK(ctx, hipStreamWaitEvent(st2, box->evfork, 0));
     if (ev) {
         HIPCHK(ctx, hipEventRecord(ev[8], st2));
         HIPCHK(ctx, hipEventRecord(ev[9], st2));  // sorted above: no tree sort
@@ -1039,7 +1040,7 @@ int front_body(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     box->st = st;
     RCHK(rb);
     HIPCHK(ctx, hipEventRecord(box->ev[10], st2));  // build done: the join
-    HIPCHK(ctx, hipStreamWaitEvent(st, box->ev[10], 0));
+    if (fork) HIPCHK(ctx, hipStreamWaitEvent(st, box->ev[10], 0));
     return LAGPM_OK;
 }
 
@@ -1157,10 +1158,12 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
         lagpm_box* b = boxes[i];
         fact[(size_t)i] = ef[(size_t)i].fact2;
         if (!b->np) continue;
-        // the build branch goes on the context stream: a second pool stream can share the box
-        // stream's hardware queue (4 per process), which would serialise the two branches
-        b->st2 = ctx->stream;
+        // A lone box forks its build onto the context stream (a second pool stream can share the
+        // box stream's hardware queue, 4 per process, which would serialise the two branches).
+        // Several boxes already run side by side on the pool streams: each keeps its build on its
+        // own stream, as one shared build stream would serialise the builds of all boxes.
         b->st = ctx->pool[k++ % S];
+        b->st2 = live.size() == 1 ? ctx->stream : b->st;
         rc = step_front(ctx, b, ef[(size_t)i]);
     }
     // FP32 walks: one persistent launch per force table on the context stream
