@@ -122,8 +122,21 @@ def ensemble(args, ctx, lagpm, ics):
             bx.upload_pos(pos)
             sets[k].append(bx)
         total += len(pos)
+    # autotuned tree tuning per box size (on a scratch box with the first box of that size's
+    # particles), applied to the boxes of that size in both sets
+    tuned = {}
+    if not args.no_autotune:
+        for b, (n, om) in enumerate(specs):
+            if n in tuned:
+                continue
+            with lagpm.Box(ctx, sets["plain"][b].params, sets["plain"][b].np) as tb:
+                tb.upload_pos(ics.perturbed_lattice(n, rms=0.3, seed=100 + b))
+                tuned[n] = tb.autotune()
+        for k in sets:
+            for bx, (n, _) in zip(sets[k], specs):
+                bx.set_tuning(*tuned[n])
     log(f"[bench] ensemble: {len(specs)} boxes, {total} particles, sizes {sorted(set(n for n, _ in specs))} "
-        f"(setup {time.perf_counter() - t0:.1f} s)")
+        f"(setup {time.perf_counter() - t0:.1f} s; tuning {tuned})")
     a0, da, steps = 1.0, 0.1, args.ens_steps
     # warm-up step (plans, tables) in both modes
     for bx in sets["plain"]:
@@ -165,7 +178,8 @@ def ensemble(args, ctx, lagpm, ics):
         for bx in sets[k]:
             bx.close()
     out = {"boxes": len(specs), "particles": total, "steps": steps,
-           "sizes": sorted(n for n, _ in specs), "identical_plain_vs_batched": bool(same)}
+           "sizes": sorted(n for n, _ in specs), "identical_plain_vs_batched": bool(same),
+           "tree_tuning": {str(n): {"nleaf": t[0], "group": t[1]} for n, t in sorted(tuned.items())}}
     for k in ("plain", "batched"):
         L = np.array(lat[k]) * 1e3
         out[k] = {"particle_steps_per_s": total * steps / tt[k], "seconds": tt[k],

@@ -16,6 +16,8 @@
  *   HubbleDragDM  Cosmos/globalT.c:102     void HubbleDragDM(SimParameters*, float)
  *   drift         Cosmos/stepevolv.c:38    void onestepforwardposition(SimParameters*)
  *   XofP decode   eunha.h:1228-1232        (GOTPM + XYZDBL 32-byte particle layout)
+ *   jread/jwrite  Cosmos/CosmosRW/kjhrw.c:51,155  void jread/jwrite(SimParameters*)
+ *   read_head     Params/header.c:72        void read_head(FILE*, SimParameters*)
  *
  * Error behaviour: the reference returns void and calls exit()/MPI_Finalize() on error
  * (Cosmos/treecorrection.c:240-247). Here every call returns 0 on success or a negative
@@ -41,6 +43,7 @@ extern "C" {
 #define LAGPM_E_FFT      -4   /* rocFFT error */
 #define LAGPM_E_STATE    -5   /* call out of order (e.g. no particles uploaded) */
 #define LAGPM_E_OVERFLOW -6   /* a bounded device structure overflowed (walk stack, node pool) */
+#define LAGPM_E_IO       -7   /* a particle file could not be opened, read, parsed or written */
 
 /* PM status of the kick, same values as PUSH/PULL/KICK of eunha.h:69-77
  * (selection in Cosmos/globalT.c:113-131 and Cosmos/fda4.c:49-63) */
@@ -142,6 +145,44 @@ int lagpm_box_download(lagpm_ctx* ctx, lagpm_box* box,
 int lagpm_box_upload_eunha(lagpm_ctx* ctx, lagpm_box* box, const void* dm_bp, size_t np, long nx);
 /* Download velocities back into the same 32-byte layout (positions untouched). */
 int lagpm_box_download_eunha_vel(lagpm_ctx* ctx, lagpm_box* box, void* dm_bp, size_t np);
+/* Download positions and velocities into the 32-byte layout: the index/flag word of each record
+ * is kept and names its lattice site; x,y,z become the displacement from that site (the periodic
+ * image nearest to it, in [-n/2, n/2)), so XofP of the record gives the position back. */
+int lagpm_box_download_eunha(lagpm_ctx* ctx, lagpm_box* box, void* dm_bp, size_t np);
+
+/* ---- GOTPM particle files and records (host only, no GPU needed) --------------------- */
+/* What the ASCII header of a jread/jwrite file holds (Params/params.h:49-139 keys). */
+typedef struct lagpm_snapshot_info {
+    long nx, ny, nz;                           /* "Nx" "Ny" "Nz" */
+    long np_sum, np_dm, np_sph, np_star, np_agn; /* "Np" "DNp" "BNp" "SNp" "ANp" */
+    int ptype;                                 /* "Particle Type": 0 PMTYPE (32 B), 1 TREETYPE (40 B) */
+    int nspace, stepcount, xyzshift;           /* "Nspace" "Stepcount" "XYZ_Shift_Flag" */
+    double amax, anow, astep;                  /* "Amax" "Anow" "Astep" */
+    double omep, omeplam, hubble, boxsize;     /* "OmegaMatter0" "OmegaLambda0" "Hubble" "Boxsize(Mpc/h)" */
+    double theta, epsilon, rsphere;            /* "Theta" "EPSILON" "Rsphere" */
+    size_t header_bytes;                       /* offset of the first particle record */
+    size_t record_bytes;                       /* 32 or 40 */
+    size_t file_bytes;
+} lagpm_snapshot_info;
+/* Parses the header up to "#End of Ascii Header" (read_head, Params/header.c:72-105). Keys not
+ * listed above are skipped; a listed key whose value does not parse is LAGPM_E_IO. ctx may be
+ * NULL (then no message is kept). */
+int lagpm_snapshot_read_header(lagpm_ctx* ctx, const char* path, lagpm_snapshot_info* info);
+/* Reads the file's DNp dark-matter records (kjhrw.c:98-116) into np = DNp 32-byte records;
+ * 40-byte tree records lose their leading list pointer. */
+int lagpm_snapshot_read_dm(lagpm_ctx* ctx, const char* path, void* dm_bp, size_t np);
+/* Writes dst_path as jwrite does (kjhrw.c:155-220) for a DM-only file: the header lines of
+ * src_path with "Anow" (written with 17 digits), "Stepcount"/"Stepnum" and "Particle Type" (0)
+ * rewritten, np 32-byte records, then src_path's bytes after its particle array (the
+ * SimParameters block) unchanged. np must equal src_path's DNp. */
+int lagpm_snapshot_write(lagpm_ctx* ctx, const char* src_path, const char* dst_path, const void* dm_bp, size_t np,
+                         double anow, int stepcount);
+/* XofP/YofP/ZofP (eunha.h:1212,1228-1232) of np 32-byte records into FP64 arrays (v* may be NULL). */
+int lagpm_eunha_decode(const void* dm_bp, size_t np, long nx, double* x, double* y, double* z, double* vx, double* vy,
+                       double* vz);
+/* The inverse, as lagpm_box_download_eunha writes it (any array may be NULL: that field is kept). */
+int lagpm_eunha_encode(void* dm_bp, size_t np, long nx, const double* x, const double* y, const double* z,
+                       const double* vx, const double* vy, const double* vz);
 
 /* ---- PM long-range force (pmmain / TSC / psolver / fda4) -------------------------- */
 /* Full PM kick: deposit -> r2c -> Green -> c2r -> FDA4 gradient -> TSC interpolation,

@@ -574,32 +574,15 @@ int lagpm_box_download(lagpm_ctx* ctx, lagpm_box* box, double* x, double* y, dou
 }
 
 // GOTPM + XYZDBL particle: u64 index|flag (flag in the top byte), f32 dx,dy,dz, f32 vx,vy,vz.
-struct eunha_dm {
-    uint64_t u4if;
-    float x, y, z, vx, vy, vz;
-};
-static_assert(sizeof(eunha_dm) == 32, "GOTPM particle is 32 bytes");
-
 int lagpm_box_upload_eunha(lagpm_ctx* ctx, lagpm_box* box, const void* dm_bp, size_t np, long nx) {
     if (!ctx || !box || (np && !dm_bp)) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
     if (np != box->np) return lagpm_fail(ctx, LAGPM_E_INVAL, "particle count differs from the box");
     if (nx != box->p.n) return lagpm_fail(ctx, LAGPM_E_INVAL, "nx differs from the box mesh size");
-    const eunha_dm* p = (const eunha_dm*)dm_bp;
     std::vector<double> a(6 * np);
-    const double L = (double)nx;
-    const uint64_t nxny = (uint64_t)nx * (uint64_t)nx;
-    for (size_t i = 0; i < np; i++) {
-        uint64_t idx = (p[i].u4if << 8) >> 8;  // PINDX, eunha.h:1212
-        double ii = (double)(idx % (uint64_t)nx), jj = (double)((idx % nxny) / (uint64_t)nx), kk = (double)(idx / nxny);
-        a[i] = std::fmod((double)p[i].x + ii + L, L);
-        a[np + i] = std::fmod((double)p[i].y + jj + L, L);
-        a[2 * np + i] = std::fmod((double)p[i].z + kk + L, L);
-        a[3 * np + i] = p[i].vx;
-        a[4 * np + i] = p[i].vy;
-        a[5 * np + i] = p[i].vz;
-    }
-    return lagpm_box_upload(ctx, box, a.data(), a.data() + np, a.data() + 2 * np, a.data() + 3 * np, a.data() + 4 * np,
-                            a.data() + 5 * np);
+    double* q = a.data();
+    if (lagpm_eunha_decode(dm_bp, np, nx, q, q + np, q + 2 * np, q + 3 * np, q + 4 * np, q + 5 * np) != LAGPM_OK)
+        return lagpm_fail(ctx, LAGPM_E_INVAL, "bad particle records");
+    return lagpm_box_upload(ctx, box, q, q + np, q + 2 * np, q + 3 * np, q + 4 * np, q + 5 * np);
 }
 
 int lagpm_box_download_eunha_vel(lagpm_ctx* ctx, lagpm_box* box, void* dm_bp, size_t np) {
@@ -607,13 +590,16 @@ int lagpm_box_download_eunha_vel(lagpm_ctx* ctx, lagpm_box* box, void* dm_bp, si
     if (np != box->np) return lagpm_fail(ctx, LAGPM_E_INVAL, "particle count differs from the box");
     std::vector<double> v(3 * np);
     RCHK(lagpm_box_download(ctx, box, nullptr, nullptr, nullptr, v.data(), v.data() + np, v.data() + 2 * np));
-    eunha_dm* p = (eunha_dm*)dm_bp;
-    for (size_t i = 0; i < np; i++) {
-        p[i].vx = (float)v[i];
-        p[i].vy = (float)v[np + i];
-        p[i].vz = (float)v[2 * np + i];
-    }
-    return LAGPM_OK;
+    return lagpm_eunha_encode(dm_bp, np, box->p.n, nullptr, nullptr, nullptr, v.data(), v.data() + np, v.data() + 2 * np);
+}
+
+int lagpm_box_download_eunha(lagpm_ctx* ctx, lagpm_box* box, void* dm_bp, size_t np) {
+    if (!ctx || !box || (np && !dm_bp)) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    if (np != box->np) return lagpm_fail(ctx, LAGPM_E_INVAL, "particle count differs from the box");
+    std::vector<double> a(6 * np);
+    double* q = a.data();
+    RCHK(lagpm_box_download(ctx, box, q, q + np, q + 2 * np, q + 3 * np, q + 4 * np, q + 5 * np));
+    return lagpm_eunha_encode(dm_bp, np, box->p.n, q, q + np, q + 2 * np, q + 3 * np, q + 4 * np, q + 5 * np);
 }
 
 static int pm_factor(lagpm_ctx* ctx, const lagpm_evolfact* ef, int pmstatus, double* f) {

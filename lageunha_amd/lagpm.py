@@ -28,7 +28,7 @@ NSPLINE = 4096
 
 _STATUS = {
     0: "OK", -1: "LAGPM_E_INVAL", -2: "LAGPM_E_NOMEM", -3: "LAGPM_E_HIP", -4: "LAGPM_E_FFT",
-    -5: "LAGPM_E_STATE", -6: "LAGPM_E_OVERFLOW",
+    -5: "LAGPM_E_STATE", -6: "LAGPM_E_OVERFLOW", -7: "LAGPM_E_IO",
 }
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblagpm.so")
@@ -58,6 +58,28 @@ class EvolFact(ctypes.Structure):
 class TreeStats(ctypes.Structure):
     _fields_ = [("n_pp", ctypes.c_uint64), ("n_pc", ctypes.c_uint64), ("n_nodes", ctypes.c_uint64),
                 ("n_groups", ctypes.c_uint64), ("ms_build", ctypes.c_double), ("ms_walk", ctypes.c_double)]
+
+
+class SnapshotInfo(ctypes.Structure):
+    """The ASCII header of a GOTPM particle file (lagpm_snapshot_info)."""
+    _fields_ = [("nx", ctypes.c_long), ("ny", ctypes.c_long), ("nz", ctypes.c_long),
+                ("np_sum", ctypes.c_long), ("np_dm", ctypes.c_long), ("np_sph", ctypes.c_long),
+                ("np_star", ctypes.c_long), ("np_agn", ctypes.c_long),
+                ("ptype", ctypes.c_int), ("nspace", ctypes.c_int), ("stepcount", ctypes.c_int),
+                ("xyzshift", ctypes.c_int),
+                ("amax", ctypes.c_double), ("anow", ctypes.c_double), ("astep", ctypes.c_double),
+                ("omep", ctypes.c_double), ("omeplam", ctypes.c_double), ("hubble", ctypes.c_double),
+                ("boxsize", ctypes.c_double), ("theta", ctypes.c_double), ("epsilon", ctypes.c_double),
+                ("rsphere", ctypes.c_double),
+                ("header_bytes", ctypes.c_size_t), ("record_bytes", ctypes.c_size_t), ("file_bytes", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# dmparticletype of a GOTPM + XYZDBL build (eunha.h:389-399): 32 bytes
+DM_RECORD = np.dtype([("u4if", "<u8"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"),
+                      ("vx", "<f4"), ("vy", "<f4"), ("vz", "<f4")])
 
 
 class Timings(ctypes.Structure):
@@ -90,6 +112,13 @@ SIGNATURES = {
     "lagpm_box_download": (ctypes.c_int, [_P, _P, _D, _D, _D, _D, _D, _D]),
     "lagpm_box_upload_eunha": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.c_long]),
     "lagpm_box_download_eunha_vel": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "lagpm_box_download_eunha": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "lagpm_snapshot_read_header": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(SnapshotInfo)]),
+    "lagpm_snapshot_read_dm": (ctypes.c_int, [_P, ctypes.c_char_p, _P, ctypes.c_size_t]),
+    "lagpm_snapshot_write": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_char_p, _P, ctypes.c_size_t,
+                                            ctypes.c_double, ctypes.c_int]),
+    "lagpm_eunha_decode": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.c_long, _D, _D, _D, _D, _D, _D]),
+    "lagpm_eunha_encode": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.c_long, _D, _D, _D, _D, _D, _D]),
     "lagpm_pmmain": (ctypes.c_int, [_P, _P, ctypes.POINTER(EvolFact), ctypes.c_int]),
     "lagpm_tsc": (ctypes.c_int, [_P, _P, _F]),
     "lagpm_tsc_fixed": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]),
@@ -256,6 +285,13 @@ class Box:
         self.ctx.check(self.lib.lagpm_box_download_eunha_vel(self.ctx.h, self.h, records.ctypes.data_as(_P),
                                                              records.shape[0]))
 
+    def download_eunha(self, records: np.ndarray):
+        """Positions (as displacements from each record's lattice site) and velocities, in place."""
+        if records.dtype != DM_RECORD or not records.flags.c_contiguous:
+            raise ValueError("records must be a contiguous DM_RECORD array")
+        self.ctx.check(self.lib.lagpm_box_download_eunha(self.ctx.h, self.h, records.ctypes.data_as(_P),
+                                                         records.shape[0]))
+
     # ---- the drop-in entry points ----
     def pmmain(self, ef: EvolFact, pmstatus: int = KICK):
         self.ctx.check(self.lib.lagpm_pmmain(self.ctx.h, self.h, ctypes.byref(ef), pmstatus))
@@ -397,3 +433,53 @@ def treepm_accel(ctx: Context, params: Params, pos: np.ndarray) -> ForceResult:
         b.treecorrection(evolfact(1.0))
         v_tree = b.velocities()
     return ForceResult(v_pm, v_tree)
+
+
+# ---- GOTPM particle files (host only) ----
+def _file_check(rc: int, what: str):
+    if rc != 0:
+        raise LagpmError(rc, what)
+
+
+def read_snapshot_header(path: str) -> SnapshotInfo:
+    """The header of a jread-format file (Params/header.c:72-105)."""
+    info = SnapshotInfo()
+    _file_check(load().lagpm_snapshot_read_header(None, os.fsencode(path), ctypes.byref(info)),
+                f"cannot read the header of {path}")
+    return info
+
+
+def read_snapshot(path: str):
+    """(header, DM records as a DM_RECORD array) of a jread-format file (kjhrw.c:51-120)."""
+    info = read_snapshot_header(path)
+    rec = np.zeros(info.np_dm, dtype=DM_RECORD)
+    _file_check(load().lagpm_snapshot_read_dm(None, os.fsencode(path), rec.ctypes.data_as(_P), rec.shape[0]),
+                f"cannot read the particles of {path}")
+    return info, rec
+
+
+def write_snapshot(template: str, path: str, records: np.ndarray, anow: float, stepcount: int):
+    """jwrite (kjhrw.c:155-220): template's header with new Anow/Stepcount, records, template's tail."""
+    records = np.ascontiguousarray(records, dtype=DM_RECORD)
+    _file_check(load().lagpm_snapshot_write(None, os.fsencode(template), os.fsencode(path),
+                                            records.ctypes.data_as(_P), records.shape[0], anow, stepcount),
+                f"cannot write {path} from {template}")
+
+
+def eunha_decode(records: np.ndarray, nx: int):
+    """XofP/YofP/ZofP and velocities of DM records: (pos (N,3), vel (N,3)) in FP64."""
+    records = np.ascontiguousarray(records, dtype=DM_RECORD)
+    q = [np.empty(records.shape[0]) for _ in range(6)]
+    _file_check(load().lagpm_eunha_decode(records.ctypes.data_as(_P), records.shape[0], nx, *[_dp(a) for a in q]),
+                "bad records")
+    return np.stack(q[:3], axis=1), np.stack(q[3:], axis=1)
+
+
+def eunha_encode(records: np.ndarray, nx: int, pos: np.ndarray, vel: np.ndarray):
+    """Write positions (as displacements from each record's lattice site) and velocities into records."""
+    if records.dtype != DM_RECORD or not records.flags.c_contiguous:
+        raise ValueError("records must be a contiguous DM_RECORD array")
+    q = [np.ascontiguousarray(pos[:, a], dtype=np.float64) for a in range(3)]
+    q += [np.ascontiguousarray(vel[:, a], dtype=np.float64) for a in range(3)]
+    _file_check(load().lagpm_eunha_encode(records.ctypes.data_as(_P), records.shape[0], nx, *[_dp(a) for a in q]),
+                "bad records")
