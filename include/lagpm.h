@@ -18,6 +18,7 @@
  *   XofP decode   eunha.h:1228-1232        (GOTPM + XYZDBL 32-byte particle layout)
  *   jread/jwrite  Cosmos/CosmosRW/kjhrw.c:51,155  void jread/jwrite(SimParameters*)
  *   read_head     Params/header.c:72        void read_head(FILE*, SimParameters*)
+ *   PreFoF        Cosmos/PreFoF.c:267       void PreFoF(SimParameters*)
  *
  * Error behaviour: the reference returns void and calls exit()/MPI_Finalize() on error
  * (Cosmos/treecorrection.c:240-247). Here every call returns 0 on success or a negative
@@ -149,6 +150,15 @@ int lagpm_box_download_eunha_vel(lagpm_ctx* ctx, lagpm_box* box, void* dm_bp, si
  * is kept and names its lattice site; x,y,z become the displacement from that site (the periodic
  * image nearest to it, in [-n/2, n/2)), so XofP of the record gives the position back. */
 int lagpm_box_download_eunha(lagpm_ctx* ctx, lagpm_box* box, void* dm_bp, size_t np);
+
+/* ---- PreFoF halo candidates --------------------------------------------------------- */
+/* PreFoF (Cosmos/PreFoF.c:267-386): friends-of-friends groups with linking length fof_link
+ * (mesh units) inside each cell of width 4 alone (OST/fof.c:220-293); a group is kept unless it
+ * lies inside [fof_link, 4 - fof_link) on every axis with fewer than minnum members (the
+ * reference fixes FOF_MINNUM = 20, PreFoF.c:273). flags (np bytes, caller order, may be NULL)
+ * gets 1 for the particles of kept groups (the reference's FoFflag); nflagged their count.
+ * Uses box->np * ~93 bytes of device scratch for the call. */
+int lagpm_prefof(lagpm_ctx* ctx, lagpm_box* box, float fof_link, int minnum, uint8_t* flags, size_t* nflagged);
 
 /* ---- GOTPM particle files and records (host only, no GPU needed) --------------------- */
 /* What the ASCII header of a jread/jwrite file holds (Params/params.h:49-139 keys). */

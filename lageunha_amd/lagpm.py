@@ -113,6 +113,8 @@ SIGNATURES = {
     "lagpm_box_upload_eunha": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.c_long]),
     "lagpm_box_download_eunha_vel": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "lagpm_box_download_eunha": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "lagpm_prefof": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_int, ctypes.POINTER(ctypes.c_uint8),
+                                    ctypes.POINTER(ctypes.c_size_t)]),
     "lagpm_snapshot_read_header": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(SnapshotInfo)]),
     "lagpm_snapshot_read_dm": (ctypes.c_int, [_P, ctypes.c_char_p, _P, ctypes.c_size_t]),
     "lagpm_snapshot_write": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_char_p, _P, ctypes.c_size_t,
@@ -291,6 +293,14 @@ class Box:
             raise ValueError("records must be a contiguous DM_RECORD array")
         self.ctx.check(self.lib.lagpm_box_download_eunha(self.ctx.h, self.h, records.ctypes.data_as(_P),
                                                          records.shape[0]))
+
+    def prefof(self, fof_link: float = 0.2, minnum: int = 20):
+        """PreFoF halo-candidate flags (uint8 per particle, caller order) and their count."""
+        flag = np.zeros(self.np, dtype=np.uint8)
+        cnt = ctypes.c_size_t()
+        self.ctx.check(self.lib.lagpm_prefof(self.ctx.h, self.h, fof_link, minnum,
+                                             flag.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(cnt)))
+        return flag, cnt.value
 
     # ---- the drop-in entry points ----
     def pmmain(self, ef: EvolFact, pmstatus: int = KICK):

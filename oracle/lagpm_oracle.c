@@ -1097,3 +1097,99 @@ void orc_power_spectrum(int n, const double* spec, double boxsize, double* pk, d
     *slope = (cor[3] > 0 && cor[2] > 0) ? (log(cor[3]) - log(cor[2])) / (log(4.0) - log(3.0)) : 0.0;
 }
 
+
+/* ---- PreFoF (Cosmos/PreFoF.c:207-386, OST/fof.c:190-293) ---- */
+static size_t uf_find(size_t* par, size_t a) {
+    while (par[a] != a) {
+        par[a] = par[par[a]];
+        a = par[a];
+    }
+    return a;
+}
+
+size_t orc_prefof(int n, size_t np, const double* x, const double* y, const double* z, float fof_link,
+                  int minnum, uint8_t* flag) {
+    /* cells of width CellWidth = 4.L over the single-rank domain [0, n): mx = ceil(n / 4)
+       (PreFoF.c:303-306); the cell of a particle is the truncation of X / 4 (PreFoF.c:326-328) */
+    const long double cw = 4.0L;
+    const size_t mx = (size_t)ceill((long double)n / cw);
+    const size_t ncell = mx * mx * mx;
+    size_t* cnt = calloc(ncell + 1, sizeof(size_t));
+    size_t* cell = malloc((np ? np : 1) * sizeof(size_t));
+    for (size_t i = 0; i < np; i++) {
+        size_t ix = (size_t)((long double)x[i] / cw), iy = (size_t)((long double)y[i] / cw),
+               iz = (size_t)((long double)z[i] / cw);
+        cell[i] = ix + mx * (iy + mx * iz);
+        cnt[cell[i] + 1]++;
+    }
+    for (size_t c = 0; c < ncell; c++) cnt[c + 1] += cnt[c];
+    size_t* members = malloc((np ? np : 1) * sizeof(size_t));
+    size_t* fill = malloc((ncell + 1) * sizeof(size_t));
+    memcpy(fill, cnt, (ncell + 1) * sizeof(size_t));
+    for (size_t i = 0; i < np; i++) members[fill[cell[i]]++] = i;
+    free(fill);
+    const double link = (double)fof_link; /* the float link compared in PosType (double) */
+    size_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+    for (size_t c = 0; c < ncell; c++) {
+        const size_t a0 = cnt[c], m = cnt[c + 1] - a0;
+        if (!m) continue;
+        const size_t ix = c % mx, iy = (c / mx) % mx, iz = c / (mx * mx);
+        /* Dump2FoFPtl, PreFoF.c:38-54: coordinates relative to the cell corner, FP64 */
+        const double cx = (double)(cw * ix), cy = (double)(cw * iy), cz = (double)(cw * iz);
+        double* r = malloc(3 * m * sizeof(double));
+        size_t* par = malloc(m * sizeof(size_t));
+        for (size_t k = 0; k < m; k++) {
+            const size_t i = members[a0 + k];
+            r[3 * k] = x[i] - cx;
+            r[3 * k + 1] = y[i] - cy;
+            r[3 * k + 2] = z[i] - cz;
+            par[k] = k;
+        }
+        /* the groups pnew_fof_link grows are the connected components of the link graph */
+        for (size_t a = 0; a < m; a++)
+            for (size_t b = a + 1; b < m; b++) {
+                const double tx = fabs(r[3 * a] - r[3 * b]), ty = fabs(r[3 * a + 1] - r[3 * b + 1]),
+                             tz = fabs(r[3 * a + 2] - r[3 * b + 2]);
+                const double d = sqrt(tx * tx + ty * ty + tz * tz);
+                if (d <= link) {
+                    size_t ra = uf_find(par, a), rb = uf_find(par, b);
+                    if (ra != rb) par[ra > rb ? ra : rb] = ra < rb ? ra : rb;
+                }
+            }
+        /* CountHaloCandidates: member count and bounds per group */
+        size_t* nm = calloc(m, sizeof(size_t));
+        double* lo = malloc(3 * m * sizeof(double));
+        double* hi = malloc(3 * m * sizeof(double));
+        for (size_t k = 0; k < 3 * m; k++) {
+            lo[k] = 1.E20;
+            hi[k] = -1.E20;
+        }
+        for (size_t k = 0; k < m; k++) {
+            const size_t g = uf_find(par, k);
+            nm[g]++;
+            for (int d = 0; d < 3; d++) {
+                if (r[3 * k + d] < lo[3 * g + d]) lo[3 * g + d] = r[3 * k + d];
+                if (r[3 * k + d] > hi[3 * g + d]) hi[3 * g + d] = r[3 * k + d];
+            }
+        }
+        const double xf = (double)cw; /* PosType xf = CellWidth */
+        for (size_t k = 0; k < m; k++) {
+            const size_t g = uf_find(par, k);
+            const int inside = lo[3 * g] > link && lo[3 * g + 1] > link && lo[3 * g + 2] > link &&
+                               hi[3 * g] < xf - link && hi[3 * g + 1] < xf - link && hi[3 * g + 2] < xf - link;
+            const int keep = inside ? nm[g] >= (size_t)minnum : 1;
+            flag[members[a0 + k]] = (uint8_t)keep;
+            total += (size_t)keep;
+        }
+        free(nm);
+        free(lo);
+        free(hi);
+        free(r);
+        free(par);
+    }
+    free(members);
+    free(cell);
+    free(cnt);
+    return total;
+}

@@ -98,4 +98,14 @@ void   orc_evolfactor(double omep, double omeplam, double amax, int n, double an
 void   orc_power_spectrum(int n, const double* spec, double boxsize, double* pk /*n+1*/, double* npk /*n+1*/,
                           double* cor /*n+1*/, double* sigma8, double* slope);
 
+/* PreFoF halo-candidate flags (Cosmos/PreFoF.c:267-386): particles are binned into cells of
+ * width 4 (Cosmos/BasicCell.h:1) over [0, n); friends-of-friends groups with linking length
+ * fof_link are found inside each cell alone (OST/fof.c:220-293: a pair links when
+ * sqrt(dx^2 + dy^2 + dz^2) <= fof_link in FP64, coordinates relative to the cell corner); a group
+ * is kept (its particles flagged) unless it lies inside [fof_link, 4 - fof_link) on every axis
+ * with fewer than minnum members (CountHaloCandidates, PreFoF.c:207-265). Returns the number of
+ * flagged particles; flag[i] is 0 or 1. Brute-force pairs per cell. */
+size_t orc_prefof(int n, size_t np, const double* x, const double* y, const double* z, float fof_link,
+                  int minnum, uint8_t* flag);
+
 #endif

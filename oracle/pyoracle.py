@@ -62,6 +62,7 @@ def lib():
         "orc_fact1": (d, [d, d, d, d, d]), "orc_fact2": (d, [d, d, d, i, d, d]),
         "orc_evolfactor": (None, [d, d, d, i, d, d, ctypes.POINTER(OrcEvolFact)]),
         "orc_power_spectrum": (None, [i, _D, d, _D, _D, _D, _D, _D]),
+        "orc_prefof": (sz, [i, sz, _D, _D, _D, f, i, ctypes.POINTER(ctypes.c_uint8)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -276,3 +277,12 @@ def power_spectrum(n, spec, boxsize):
     s8, sl = ctypes.c_double(), ctypes.c_double()
     lib().orc_power_spectrum(n, _dp(sp), boxsize, _dp(pk), _dp(npk), _dp(cor), ctypes.byref(s8), ctypes.byref(sl))
     return pk, npk, cor, s8.value, sl.value
+
+
+def prefof(n, pos, fof_link=0.2, minnum=20):
+    """PreFoF halo-candidate flags (uint8 per particle) and their count."""
+    x, y, z = split(pos)
+    flag = np.zeros(len(x), dtype=np.uint8)
+    cnt = lib().orc_prefof(n, len(x), _dp(x), _dp(y), _dp(z), fof_link, minnum,
+                           flag.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return flag, int(cnt)
