@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_file_step_file_round_trip(ctx, tmp_path):
-    n = 16
+    n = 24  # the smallest mesh for the default cutoff (n >= 4 rsphere)
     src, dst = str(tmp_path / "rv.00000"), str(tmp_path / "rv.00001")
     _write_file(src, n ** 3, n, seed=11, tail=b"SimParameters block" * 10)
     info, rec = lagpm.read_snapshot(src)
