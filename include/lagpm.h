@@ -270,6 +270,12 @@ int lagpm_batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const do
 /* Whole steps capture each box's step (up to the tree walk) in a hipGraph on first use and
  * replay it (default on); off: every kernel is launched each step. Results are identical. */
 int lagpm_set_step_graphs(lagpm_ctx* ctx, int enable);
+/* Batch steps of FP32 boxes sharing one force law run as `stages` groups of about equal particle
+ * count (default 3, reserve_cu 32; used when the batch has at least 2 * stages boxes): a group's
+ * fused walk leaves reserve_cu CUs free and runs beside the next group's front halves (PM and
+ * tree build). stages = 1 queues every front, then one walk. Results are identical either way.
+ * No counterpart in the reference. */
+int lagpm_set_batch_pipeline(lagpm_ctx* ctx, int stages, int reserve_cu);
 
 #ifdef __cplusplus
 }

@@ -812,8 +812,7 @@ static int tree_enqueue(lagpm_ctx* ctx, lagpm_box* box, double fact, bool combin
         HIPCHK(ctx, hipEventRecord(ev[3], box->st));
     } else {
         if (box->st != ctx->stream) HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ev[2], 0));
-        HIPCHK(ctx, hipEventRecord(box->ev[14], ctx->stream));
-        RCHK(lagpm_tree_walk_f32(ctx, &box, &fact, 1));
+        RCHK(lagpm_tree_walk_f32(ctx, &box, &fact, 1));  // records ev[14] before the walk launch
         HIPCHK(ctx, hipEventRecord(ev[3], ctx->stream));
         if (box->st != ctx->stream) HIPCHK(ctx, hipStreamWaitEvent(box->st, ev[3], 0));
         if (combine) {
@@ -1114,6 +1113,12 @@ int drift_enqueue(lagpm_ctx* ctx, lagpm_box* box, double pfact, bool guarded) {
     return LAGPM_OK;
 }
 
+// Queues step_front for each (box, factors) pair (the boxes already carry their streams).
+int queue_fronts(lagpm_ctx* ctx, const std::vector<lagpm_box*>& bs, const std::vector<const lagpm_evolfact*>& efs) {
+    for (size_t i = 0; i < bs.size(); i++) RCHK(step_front(ctx, bs[i], *efs[i]));
+    return LAGPM_OK;
+}
+
 int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* anow, const double* astep) {
     if (!ctx || (nbox > 0 && (!boxes || !anow || !astep)) || nbox < 0)
         return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument or negative box count");
@@ -1138,57 +1143,111 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     for (size_t k = 0; k < S; k++) HIPCHK(ctx, hipStreamWaitEvent(ctx->pool[k], ctx->ev[0], 0));
     std::vector<double> fact((size_t)nbox);
+    for (int i = 0; i < nbox; i++) fact[(size_t)i] = ef[(size_t)i].fact2;
+    // Pipelined batch: all boxes FP32 with one force table, enough of them for the stages. The
+    // boxes are cut into `stages` consecutive groups of about equal particle count; group g's walk
+    // (one fused launch, leaving ctx->stage_reserve_cu CUs free) runs while the fronts of group
+    // g+1 are on the pool streams, whose small kernels are latency bound and fill those CUs. Every
+    // drift is queued after all fronts, so no front waits behind a walk on its pool stream.
+    int stages = std::max(1, ctx->batch_stages);
+    for (lagpm_box* b : live)
+        if (b->p.precision != LAGPM_PREC_FP32 || b->tab != live[0]->tab) stages = 1;
+    if (live.size() < (size_t)(2 * stages)) stages = 1;
     int rc = LAGPM_OK;
-    size_t k = 0;
-    for (int i = 0; i < nbox && rc == LAGPM_OK; i++) {
-        lagpm_box* b = boxes[i];
-        fact[(size_t)i] = ef[(size_t)i].fact2;
-        if (!b->np) continue;
-        // A lone box forks its build onto the context stream (a second pool stream can share the
-        // box stream's hardware queue, 4 per process, which would serialise the two branches).
-        // Several boxes already run side by side on the pool streams: each keeps its build on its
-        // own stream, as one shared build stream would serialise the builds of all boxes.
-        b->st = ctx->pool[k++ % S];
-        b->st2 = live.size() == 1 ? ctx->stream : b->st;
-        rc = step_front(ctx, b, ef[(size_t)i]);
-    }
-    // FP32 walks: one persistent launch per force table on the context stream
-    std::vector<int> done((size_t)nbox, 0);
-    for (int i = 0; i < nbox && rc == LAGPM_OK; i++) {
-        lagpm_box* b = boxes[i];
-        if (!b->np || done[(size_t)i]) continue;
-        if (b->p.precision == LAGPM_PREC_FP64) {
-            if (hipEventRecord(b->ev[14], b->st) != hipSuccess) rc = LAGPM_E_HIP;
-            if (rc == LAGPM_OK) rc = lagpm_tree_walk_f64(ctx, b, fact[(size_t)i]);
-            if (rc == LAGPM_OK && hipEventRecord(b->ev[11], b->st) != hipSuccess) rc = LAGPM_E_HIP;
-            done[(size_t)i] = 1;
-            continue;
+    if (stages > 1) {
+        size_t ptot = 0;
+        for (lagpm_box* b : live) ptot += b->np;
+        size_t k = 0, acc = 0, li = 0;
+        int g = 0;
+        std::vector<lagpm_box*> grp;
+        std::vector<double> gf;
+        std::vector<const lagpm_evolfact*> gef;
+        for (int i = 0; i < nbox && rc == LAGPM_OK; i++) {
+            lagpm_box* b = boxes[i];
+            if (!b->np) continue;
+            b->st = ctx->pool[k++ % S];
+            b->st2 = b->st;
+            grp.push_back(b);
+            gf.push_back(fact[(size_t)i]);
+            gef.push_back(&ef[(size_t)i]);
+            acc += b->np;
+            li++;
+            const bool last = li == live.size();
+            // close the group at its share of the particles (the last group takes the rest)
+            if (last || (g < stages - 1 && acc * (size_t)stages >= ptot * (size_t)(g + 1))) {
+                rc = queue_fronts(ctx, grp, gef);
+                for (lagpm_box* c : grp)
+                    if (hipStreamWaitEvent(ctx->stream, c->ev[13], 0) != hipSuccess) rc = LAGPM_E_HIP;
+                if (rc == LAGPM_OK)
+                    rc = lagpm_tree_walk_f32(ctx, grp.data(), gf.data(), (int)grp.size(), last ? 0 : ctx->stage_reserve_cu);
+                for (lagpm_box* c : grp)
+                    if (rc == LAGPM_OK && hipEventRecord(c->ev[11], ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
+                grp.clear();
+                gf.clear();
+                gef.clear();
+                g++;
+            }
         }
-        std::vector<lagpm_box*> cls;
-        std::vector<double> cf;
-        for (int j = i; j < nbox; j++) {
-            lagpm_box* c = boxes[j];
-            if (!c->np || done[(size_t)j] || c->p.precision != LAGPM_PREC_FP32 || c->tab != b->tab) continue;
-            if (hipStreamWaitEvent(ctx->stream, c->ev[13], 0) != hipSuccess) rc = LAGPM_E_HIP;
-            cls.push_back(c);
-            cf.push_back(fact[(size_t)j]);
-            done[(size_t)j] = 1;
+        for (int i = 0; i < nbox && rc == LAGPM_OK; i++) {
+            lagpm_box* b = boxes[i];
+            if (!b->np) continue;
+            if (hipStreamWaitEvent(b->st, b->ev[11], 0) != hipSuccess) rc = LAGPM_E_HIP;
+            b->tree_timed = true;
+            if (rc == LAGPM_OK) rc = drift_enqueue(ctx, b, (double)ef[(size_t)i].pfact, true);
         }
-        for (lagpm_box* c : cls)
-            if (rc == LAGPM_OK && hipEventRecord(c->ev[14], ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
-        if (rc == LAGPM_OK) rc = lagpm_tree_walk_f32(ctx, cls.data(), cf.data(), (int)cls.size());
-        if (rc == LAGPM_OK && hipEventRecord(ctx->ev[1], ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
-        for (lagpm_box* c : cls) {
-            if (rc != LAGPM_OK) break;
-            if (hipEventRecord(c->ev[11], ctx->stream) != hipSuccess || hipStreamWaitEvent(c->st, ctx->ev[1], 0) != hipSuccess)
-                rc = LAGPM_E_HIP;
+    } else {
+        size_t k = 0;
+        std::vector<lagpm_box*> fb;
+        std::vector<const lagpm_evolfact*> fe;
+        for (int i = 0; i < nbox; i++) {
+            lagpm_box* b = boxes[i];
+            if (!b->np) continue;
+            // A lone box forks its build onto the context stream (a second pool stream can share the
+            // box stream's hardware queue, 4 per process, which would serialise the two branches).
+            // Several boxes already run side by side on the pool streams: each keeps its build on its
+            // own stream, as one shared build stream would serialise the builds of all boxes.
+            b->st = ctx->pool[k++ % S];
+            b->st2 = live.size() == 1 ? ctx->stream : b->st;
+            fb.push_back(b);
+            fe.push_back(&ef[(size_t)i]);
         }
-    }
-    for (int i = 0; i < nbox && rc == LAGPM_OK; i++) {
-        lagpm_box* b = boxes[i];
-        if (!b->np) continue;
-        b->tree_timed = true;
-        rc = drift_enqueue(ctx, b, (double)ef[(size_t)i].pfact, true);
+        rc = queue_fronts(ctx, fb, fe);
+        // FP32 walks: one persistent launch per force table on the context stream
+        std::vector<int> done((size_t)nbox, 0);
+        for (int i = 0; i < nbox && rc == LAGPM_OK; i++) {
+            lagpm_box* b = boxes[i];
+            if (!b->np || done[(size_t)i]) continue;
+            if (b->p.precision == LAGPM_PREC_FP64) {
+                if (hipEventRecord(b->ev[14], b->st) != hipSuccess) rc = LAGPM_E_HIP;
+                if (rc == LAGPM_OK) rc = lagpm_tree_walk_f64(ctx, b, fact[(size_t)i]);
+                if (rc == LAGPM_OK && hipEventRecord(b->ev[11], b->st) != hipSuccess) rc = LAGPM_E_HIP;
+                done[(size_t)i] = 1;
+                continue;
+            }
+            std::vector<lagpm_box*> cls;
+            std::vector<double> cf;
+            for (int j = i; j < nbox; j++) {
+                lagpm_box* c = boxes[j];
+                if (!c->np || done[(size_t)j] || c->p.precision != LAGPM_PREC_FP32 || c->tab != b->tab) continue;
+                if (hipStreamWaitEvent(ctx->stream, c->ev[13], 0) != hipSuccess) rc = LAGPM_E_HIP;
+                cls.push_back(c);
+                cf.push_back(fact[(size_t)j]);
+                done[(size_t)j] = 1;
+            }
+            if (rc == LAGPM_OK) rc = lagpm_tree_walk_f32(ctx, cls.data(), cf.data(), (int)cls.size());
+            if (rc == LAGPM_OK && hipEventRecord(ctx->ev[1], ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
+            for (lagpm_box* c : cls) {
+                if (rc != LAGPM_OK) break;
+                if (hipEventRecord(c->ev[11], ctx->stream) != hipSuccess || hipStreamWaitEvent(c->st, ctx->ev[1], 0) != hipSuccess)
+                    rc = LAGPM_E_HIP;
+            }
+        }
+        for (int i = 0; i < nbox && rc == LAGPM_OK; i++) {
+            lagpm_box* b = boxes[i];
+            if (!b->np) continue;
+            b->tree_timed = true;
+            rc = drift_enqueue(ctx, b, (double)ef[(size_t)i].pfact, true);
+        }
     }
     // completion: sizes and counters of every box; a box whose node arrays overflowed got
     // neither tree kick nor drift, so it is rebuilt with larger arrays and finished here
@@ -1236,6 +1295,15 @@ int lagpm_batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const do
 int lagpm_set_step_graphs(lagpm_ctx* ctx, int enable) {
     if (!ctx) return LAGPM_E_INVAL;
     ctx->use_graphs = enable != 0;
+    return LAGPM_OK;
+}
+
+int lagpm_set_batch_pipeline(lagpm_ctx* ctx, int stages, int reserve_cu) {
+    if (!ctx) return LAGPM_E_INVAL;
+    if (stages < 1 || stages > 16 || reserve_cu < 0 || reserve_cu >= ctx->num_cu)
+        return lagpm_fail(ctx, LAGPM_E_INVAL, "stages must be in [1, 16] and reserve_cu in [0, CUs)");
+    ctx->batch_stages = stages;
+    ctx->stage_reserve_cu = reserve_cu;
     return LAGPM_OK;
 }
 

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -62,6 +63,8 @@ struct lagpm_ctx {
     uint32_t* walk_prefix = nullptr;         // group prefix over the descriptors
     size_t walk_box_cap = 0;
     bool use_graphs = true;                  // capture box step fronts in hipGraphs
+    int batch_stages = 3;                    // groups of a pipelined batch step (1: no pipeline)
+    int stage_reserve_cu = 32;               // CUs a non-final group's walk leaves to the next fronts
     hipEvent_t ev[4] = {};
 };
 
@@ -209,7 +212,8 @@ int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box);
 int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box);
 // One walk launch on ctx->stream over all groups of the given boxes (FP32 boxes sharing one
 // law table); facts[i] is the kick factor of box i. Waits are the caller's business.
-int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* facts, int nbox);
+// reserve_cu: CUs left without a walk workgroup (for kernels of other streams running beside it)
+int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* facts, int nbox, int reserve_cu = 0);
 int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact);
 // Reads the device sizes and counters after the walk (synchronises box->st); grows the node
 // arrays and returns LAGPM_E_OVERFLOW with *retry set when they were too small.

@@ -1401,7 +1401,7 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     return LAGPM_OK;
 }
 
-int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* facts, int nbox) {
+int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* facts, int nbox, int reserve_cu) {
     if (nbox <= 0) return LAGPM_OK;
     if ((size_t)nbox > ctx->walk_box_cap) {
         size_t cap = std::max<size_t>(64, (size_t)nbox * 2);
@@ -1437,7 +1437,14 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
     hipLaunchKernelGGL(walk_prefix_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->walk_boxes, nbox, ctx->walk_prefix,
                        ctx->walk_ticket);
     HIPCHK(ctx, hipGetLastError());
-    const unsigned blocks = (unsigned)(ctx->walk_stack_entries / ((size_t)STACK_MAX * WALK_WAVES));
+    // each box's walk-start event, right before the launch (after the descriptor copy and the
+    // prefix, so the phase time is the walk kernel's own)
+    for (int b = 0; b < nbox; b++) HIPCHK(ctx, hipEventRecord(boxes[b]->ev[14], ctx->stream));
+    // one workgroup per CU fills it (registers and LDS); reserve_cu CUs are left free for other
+    // streams' kernels, in whole XCD rounds so every XCD keeps the same number of walk workgroups
+    const int full = (int)(ctx->walk_stack_entries / ((size_t)STACK_MAX * WALK_WAVES));
+    const int keep = full - ((std::max(0, reserve_cu) + WALK_XCDS - 1) / WALK_XCDS) * WALK_XCDS;
+    const unsigned blocks = (unsigned)(keep >= WALK_XCDS ? keep : std::min(full, WALK_XCDS));
     if (tab->logr2)
         hipLaunchKernelGGL(walk_f32_kernel<true>, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes,
                            ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);

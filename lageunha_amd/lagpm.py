@@ -144,6 +144,7 @@ SIGNATURES = {
     "lagpm_kdk_step": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double]),
     "lagpm_batch_step": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int, _D, _D]),
     "lagpm_set_step_graphs": (ctypes.c_int, [_P, ctypes.c_int]),
+    "lagpm_set_batch_pipeline": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
 }
 
 _lib = None
@@ -212,6 +213,9 @@ class Context:
 
     def set_step_graphs(self, enable: bool):
         self.check(self.lib.lagpm_set_step_graphs(self.h, 1 if enable else 0))
+
+    def set_batch_pipeline(self, stages: int, reserve_cu: int = 32):
+        self.check(self.lib.lagpm_set_batch_pipeline(self.h, stages, reserve_cu))
 
     def synchronize(self):
         self.check(self.lib.lagpm_synchronize(self.h))

@@ -105,6 +105,39 @@ def test_batch_equals_each_box_alone(ctx, prec):
             b.close()
 
 
+@pytest.mark.parametrize("stages", [1, 2, 3])
+def test_pipelined_batch_equals_each_box_alone(ctx, stages):
+    """A batch run as groups whose walks overlap the next group's fronts (lagpm_set_batch_pipeline)
+    gives every box its single-box result bit for bit."""
+    specs = [(32, 0.25, 1.0), (40, 0.30, 1.5), (24, 0.35, 3.0), (48, 0.28, 1.2), (32, 0.33, 2.0), (36, 0.27, 1.1),
+             (24, 0.31, 1.4)]
+    inputs, alone, boxes = [], [], []
+    for i, (n, om, a0) in enumerate(specs):
+        pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=max(4, n ** 3 // 2000), seed=30 + i)
+        vel = np.random.default_rng(40 + i).standard_normal(pos.shape) * 0.01
+        inputs.append((lagpm.make_params(n, omep=om, omeplam=1 - om), pos, vel, a0))
+    print(f"pipelined batch of {len(specs)} boxes, {stages} stages", flush=True)
+    for p, pos, vel, a0 in inputs:
+        (x, v), _ = run_kdk(ctx, p, pos, vel, a0, 0.1, 2)
+        alone.append((x, v))
+    try:
+        ctx.set_batch_pipeline(stages, 32)
+        for p, pos, vel, a0 in inputs:
+            b = lagpm.Box(ctx, p, len(pos))
+            b.upload_pos(pos, vel)
+            boxes.append(b)
+        anow = np.array([a0 for *_, a0 in inputs])
+        for s in range(2):
+            lagpm.batch_step(ctx, boxes, anow + 0.1 * s, 0.1)
+        for b, (x, v) in zip(boxes, alone):
+            xb, vb = b.download()
+            assert np.array_equal(xb, x) and np.array_equal(vb, v)
+    finally:
+        ctx.set_batch_pipeline(3, 32)
+        for b in boxes:
+            b.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(lagpm.PREC_FP32, 3e-5), (lagpm.PREC_FP64, 1e-5)])
 def test_node_overflow_is_regrown_and_the_kick_is_exact(ctx, prec, tol):
     """Tight 17-particle clumps make about one node per particle, beyond the initial node
