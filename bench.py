@@ -366,8 +366,14 @@ def main():
         log(f"[bench] warmup step {i + 1}/{args.warmup} done")
     ctx.synchronize()
     t0 = time.perf_counter()
+    walk_ms, walk_stats = [], []
     for i in range(args.steps):
         step()
+        # the walk runs outside the step graphs, between HIP events on the context stream (the
+        # stream it is launched on); the step has completed when kdk_step returns
+        walk_ms.append(box.timings().ms_walk)
+        s = box.tree_stats()
+        walk_stats.append((s.n_pp, s.n_pc))
     ctx.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
@@ -383,8 +389,10 @@ def main():
     ctx.set_step_graphs(True)
 
     ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
-    npp = float(np.mean([s[0] for s in stats]))
-    npc = float(np.mean([s[1] for s in stats]))
+    # the walk: its launches in the timed steps themselves, and their interaction counts
+    ph["ms_walk"] = float(np.mean(walk_ms))
+    npp = float(np.mean([s[0] for s in walk_stats]))
+    npc = float(np.mean([s[1] for s in walk_stats]))
     M = n ** 3
     K = (n // 2 + 1) * n * n
     # algorithmic bytes / flops per launch

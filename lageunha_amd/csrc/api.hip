@@ -933,7 +933,7 @@ int lagpm_box_timings(lagpm_ctx* ctx, lagpm_box* box, lagpm_timings* out) {
     if (box->tree_timed) {
         out->ms_tree_sort = el(8, 9);
         out->ms_build = el(9, 10);
-        out->ms_walk = el(10, 11);
+        out->ms_walk = el(14, 11);  // from right before the walk launch to its end
     }
     return LAGPM_OK;
 }
