@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define LAGPM_ABI_VERSION 2
+#define LAGPM_ABI_VERSION 3
 
 /* status codes */
 #define LAGPM_OK          0
@@ -96,6 +96,9 @@ typedef struct lagpm_tree_stats {
     uint64_t n_groups;  /* target groups walked */
     double   ms_build;  /* sort + build + moments, device time */
     double   ms_walk;   /* walk kernel, device time */
+    int32_t  deepest;   /* deepest tree level with nodes (levels of the 16-bit-per-axis keys) */
+    int32_t  depth_limit; /* deepest level the next build splits to (a deeper tree is detected
+                             and rebuilt at full depth) */
 } lagpm_tree_stats;
 
 /* Device time (HIP events on the context stream) of the phases of the last pmmain and

@@ -854,14 +854,18 @@ int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* e
         int rc = lagpm_tree_finish(ctx, box, &retry);
         tree_times(box);
         if (rc == LAGPM_OK) return LAGPM_OK;
-        // a build that overflowed its node arrays applied no kick: grow once and redo it
-        if (!retry || attempt > 0) return rc == LAGPM_E_OVERFLOW ? lagpm_fail(ctx, rc, "tree node arrays overflowed") : rc;
+        // a build that overflowed its node arrays or met its depth limit applied no kick: grow
+        // or deepen and redo it (a depth redo may still need larger node arrays)
+        if (!retry || attempt > 1) return rc == LAGPM_E_OVERFLOW ? lagpm_fail(ctx, rc, "tree node arrays overflowed") : rc;
     }
 }
 
 int lagpm_box_set_tuning(lagpm_box* box, int nleaf, int group) {
     if (!box || nleaf < 4 || nleaf > 256 || group < 8 || group > 64) return LAGPM_E_INVAL;
-    if (box->tune.nleaf != nleaf || box->tune.group != group) lagpm_box_graphs_release(box);
+    if (box->tune.nleaf != nleaf || box->tune.group != group) {
+        lagpm_box_graphs_release(box);
+        box->depth = LAGPM_KEY_BITS;  // another leaf size: find the tree depth again
+    }
     box->tune.nleaf = nleaf;
     box->tune.group = group;
     return LAGPM_OK;
@@ -909,6 +913,7 @@ int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group) {
 int lagpm_tree_stats_get(const lagpm_box* box, lagpm_tree_stats* out) {
     if (!box || !out) return LAGPM_E_INVAL;
     *out = box->stats;
+    out->depth_limit = box->depth;
     return LAGPM_OK;
 }
 
@@ -1258,14 +1263,16 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
         if (first_err == LAGPM_OK) {
             bool retry = false;
             int r = lagpm_tree_finish(ctx, b, &retry);
-            if (retry) {
+            // redo (at most twice: a depth redo may still need larger node arrays)
+            for (int attempt = 0; retry && attempt < 2; attempt++) {
+                retry = false;
                 b->st = ctx->stream;
                 HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
                 r = tree_enqueue(ctx, b, fact[(size_t)i], false);
                 if (r == LAGPM_OK) r = lagpm_tree_finish(ctx, b, &retry);
                 if (r == LAGPM_OK) r = drift_enqueue(ctx, b, (double)ef[(size_t)i].pfact, false);
-                if (r != LAGPM_OK && retry) r = lagpm_fail(ctx, LAGPM_E_OVERFLOW, "tree node arrays overflowed");
             }
+            if (r != LAGPM_OK && retry) r = lagpm_fail(ctx, LAGPM_E_OVERFLOW, "tree node arrays overflowed");
             if (r == LAGPM_OK) tree_times(b);
             if (r != LAGPM_OK) first_err = r;
         }

@@ -85,6 +85,7 @@ enum {
     DSZ_CHUNKS = 2,
     DSZ_OVERFLOW = 3,
     DSZ_SCRATCH = 4,  // scan totals nobody reads
+    DSZ_DEEPER = 5,   // the build met a node to split at its depth limit (the tree is redone)
     DSZ_LVL = 8,
     DSZ_LOFF = DSZ_LVL + LAGPM_KEY_BITS + 1,
     DSZ_LCUR = DSZ_LOFF + LAGPM_KEY_BITS + 1,
@@ -98,6 +99,11 @@ struct lagpm_box {
     bool uploaded = false;
     bool sorted = false;
     tree_tune tune;
+    // deepest level the build splits to: the full key depth until one build has run, then two
+    // levels below the deepest level that build used; a build that meets a node still to split
+    // at this limit flags it, and the tree is redone at full depth (levels below the tree's
+    // depth cost one empty launch each per kernel)
+    int depth = LAGPM_KEY_BITS;
     hipStream_t st = nullptr;  // stream this box's work is queued on (context stream or a pool stream)
     hipStream_t st2 = nullptr;  // second stream of a step front (the tree build beside the PM kick)
 

@@ -243,8 +243,9 @@ __global__ void level_mark_kernel(uint32_t* __restrict__ dsz, size_t cap, int l,
 // records are appended with one global atomic per workgroup and pass (a single counter that
 // every wave hit directly serialised ~10^5 atomics per level).
 constexpr int SPLIT_THREADS = 256;
-__global__ void __launch_bounds__(SPLIT_THREADS) split_kernel(const uint64_t* __restrict__ K, int l, int nleaf, size_t cap,
-                                                              uint32_t* __restrict__ dsz, int4* __restrict__ rec,
+// At the depth limit (l == lmax) nothing is split: a record that would be is flagged instead.
+__global__ void __launch_bounds__(SPLIT_THREADS) split_kernel(const uint64_t* __restrict__ K, int l, int lmax, int nleaf,
+                                                              size_t cap, uint32_t* __restrict__ dsz, int4* __restrict__ rec,
                                                               int* __restrict__ rch0, int* __restrict__ rch1,
                                                               uint32_t* __restrict__ ncount) {
     __shared__ uint32_t wcnt[SPLIT_THREADS / 64];
@@ -259,7 +260,12 @@ __global__ void __launch_bounds__(SPLIT_THREADS) split_kernel(const uint64_t* __
         const size_t r = tid / 8 + pass;
         const bool valid = r < n;
         const int4 nd = valid ? rec[o + r] : make_int4(0, 0, 0, 0);
-        const bool split = valid && nd.y > nleaf && l < LAGPM_KEY_BITS;
+        const bool need = valid && nd.y > nleaf && l < LAGPM_KEY_BITS;
+        if (need && l >= lmax) {
+            dsz[DSZ_DEEPER] = 1u;
+            dsz[DSZ_OVERFLOW] = 1u;  // the walk and the drift skip this box until it is redone
+        }
+        const bool split = need && l < lmax;
         uint32_t b = 0u;
         if (split) {
             const uint64_t pfx = ((K[nd.x] >> key_shift(l)) << 3) | (uint64_t)sub;
@@ -572,7 +578,7 @@ __global__ void fill_int_kernel(int* __restrict__ a, size_t m, int v) {
 // memset, so the step can be captured in a graph)
 __global__ void tree_reset_kernel(uint32_t* __restrict__ dsz, unsigned long long* __restrict__ counters) {
     int t = threadIdx.x;
-    if (t == DSZ_NODES || t == DSZ_GROUPS || t == DSZ_OVERFLOW) dsz[t] = 0u;
+    if (t == DSZ_NODES || t == DSZ_GROUPS || t == DSZ_OVERFLOW || t == DSZ_DEEPER) dsz[t] = 0u;
     if ((t >= DSZ_LVL && t < DSZ_LOFF) || (t >= DSZ_LCUR && t < DSZ_COUNT)) dsz[t] = 0u;
     if (t < 4) counters[t] = 0ull;
 }
@@ -1366,17 +1372,23 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     hipLaunchKernelGGL(level_mark_kernel, dim3(1), dim3(64), 0, st, box->dsz, cap, L, L);
     HIPCHK(ctx, hipGetLastError());
     const unsigned lblocks = (unsigned)std::min<size_t>(LEVEL_BLOCKS, blocks_for(cap, 256));
-    for (int l = L; l < LAGPM_KEY_BITS; l++) {
-        hipLaunchKernelGGL(split_kernel, dim3(lblocks), dim3(SPLIT_THREADS), 0, st, box->key, l, nleaf, cap, box->dsz, rec,
-                           rch0, rch1, box->ncount);
+    const int D = std::min(LAGPM_KEY_BITS, std::max(L, box->depth));
+    for (int l = L; l < D; l++) {
+        hipLaunchKernelGGL(split_kernel, dim3(lblocks), dim3(SPLIT_THREADS), 0, st, box->key, l, D, nleaf, cap, box->dsz,
+                           rec, rch0, rch1, box->ncount);
         hipLaunchKernelGGL(level_mark_kernel, dim3(1), dim3(64), 0, st, box->dsz, cap, l + 1, L);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    if (D < LAGPM_KEY_BITS) {  // the limit level: flag a record that still needs splitting
+        hipLaunchKernelGGL(split_kernel, dim3(lblocks), dim3(SPLIT_THREADS), 0, st, box->key, D, D, nleaf, cap, box->dsz,
+                           rec, rch0, rch1, box->ncount);
         HIPCHK(ctx, hipGetLastError());
     }
     RCHK(lagpm_exclusive_scan_u32(ctx, st, box->scan_tmp, box->ncount, box->noff, np, box->dsz + DSZ_SCRATCH));
     hipLaunchKernelGGL(number_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->key, L, cap, box->dsz, rec, rch0,
                        rch1, box->noff, box->nd_info, box->nd_child, box->nd_parent, box->cellroot, box->mcell, order);
     HIPCHK(ctx, hipGetLastError());
-    for (int l = LAGPM_KEY_BITS; l >= L; l--) {
+    for (int l = D; l >= L; l--) {  // levels below D hold no records
         if (f64)
             hipLaunchKernelGGL(moments_f64_kernel, dim3(lblocks), dim3(256), 0, st, box->x, box->y, box->z, 1.0, box->dsz, l,
                                order, box->nd_info, box->nd_child, box->nd_d);
@@ -1487,12 +1499,34 @@ int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry) {
     box->stats.n_pc = c[1];
     box->stats.n_nodes = d[DSZ_NODES];
     box->stats.n_groups = d[DSZ_GROUPS];
+    if (d[DSZ_DEEPER]) {
+        // the tree needs levels below the depth limit: redo it at full depth, with room for the
+        // nodes the truncated build could not count
+        box->depth = LAGPM_KEY_BITS;
+        lagpm_box_graphs_release(box);  // captured steps hold the old limit
+        const size_t need = std::max<size_t>((size_t)d[DSZ_NODES], box->node_cap);
+        if (d[DSZ_NODES] >= box->node_cap) RCHK(lagpm_tree_alloc(ctx, box, need + need / 2 + 4096));
+        *retry = true;
+        return LAGPM_E_OVERFLOW;
+    }
     if (d[DSZ_OVERFLOW]) {
         // grow to the node count this input needs (groups follow from nodes) and rebuild
         size_t need = std::max<size_t>((size_t)d[DSZ_NODES], box->node_cap);
         RCHK(lagpm_tree_alloc(ctx, box, need + need / 4 + 4096));
         *retry = true;
         return LAGPM_E_OVERFLOW;
+    }
+    int deepest = box->level_cell;
+    for (int l = box->level_cell; l <= LAGPM_KEY_BITS; l++)
+        if (d[DSZ_LVL + l]) deepest = l;
+    box->stats.deepest = deepest;
+    if (box->depth >= LAGPM_KEY_BITS) {
+        // first complete build at full depth: limit later builds to two levels below its deepest
+        const int lim = std::min(LAGPM_KEY_BITS, deepest + 2);
+        if (lim < box->depth) {
+            box->depth = lim;
+            lagpm_box_graphs_release(box);  // captured steps hold the old limit
+        }
     }
     if (c[2]) return lagpm_fail(ctx, LAGPM_E_OVERFLOW, "tree walk stack overflow in " + std::to_string(c[2]) + " groups");
     return LAGPM_OK;

@@ -57,7 +57,8 @@ class EvolFact(ctypes.Structure):
 
 class TreeStats(ctypes.Structure):
     _fields_ = [("n_pp", ctypes.c_uint64), ("n_pc", ctypes.c_uint64), ("n_nodes", ctypes.c_uint64),
-                ("n_groups", ctypes.c_uint64), ("ms_build", ctypes.c_double), ("ms_walk", ctypes.c_double)]
+                ("n_groups", ctypes.c_uint64), ("ms_build", ctypes.c_double), ("ms_walk", ctypes.c_double),
+                ("deepest", ctypes.c_int32), ("depth_limit", ctypes.c_int32)]
 
 
 class SnapshotInfo(ctypes.Structure):

@@ -138,6 +138,38 @@ def test_pipelined_batch_equals_each_box_alone(ctx, stages):
             b.close()
 
 
+@pytest.mark.parametrize("prec", [lagpm.PREC_FP32, lagpm.PREC_FP64])
+def test_depth_limit_meets_a_deeper_tree_and_redoes_it(ctx, prec):
+    """After a step on a perturbed lattice the build is limited to two levels below that tree's
+    deepest level. New particles in tight clumps need the full key depth: the limited build
+    flags it, the step redoes the tree at full depth, and the state equals a fresh box's step on
+    the same input, bit for bit."""
+    n = 32
+    lat = lattice_box(n, 3)
+    g = np.random.default_rng(21)
+    # clumps of 40 (more than a leaf holds) narrower than the finest key cell: full key depth
+    centres = g.random((len(lat) // 40, 3)) * n
+    tight = np.repeat(centres, 40, axis=0) + g.standard_normal((len(centres) * 40, 3)) * 1e-4
+    clumps = np.mod(np.concatenate([tight, g.random((len(lat) - len(tight), 3)) * n]), n)
+    vel = g.standard_normal(clumps.shape) * 0.01
+    p = lagpm.make_params(n, precision=prec)
+    print(f"depth limit: N={len(lat)} prec={prec}", flush=True)
+    with lagpm.Box(ctx, p, len(lat)) as a, lagpm.Box(ctx, p, len(lat)) as b:
+        a.upload_pos(lat)
+        a.kdk_step(1.0, 0.1)
+        st = a.tree_stats()
+        assert st.depth_limit == min(16, st.deepest + 2) < 16
+        a.upload_pos(clumps, vel)
+        a.kdk_step(1.1, 0.1)
+        st2 = a.tree_stats()
+        assert st2.deepest > st.depth_limit and st2.depth_limit == 16
+        b.upload_pos(clumps, vel)
+        b.kdk_step(1.1, 0.1)
+        xa, va = a.download()
+        xb, vb = b.download()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+
+
 @pytest.mark.parametrize("prec,tol", [(lagpm.PREC_FP32, 3e-5), (lagpm.PREC_FP64, 1e-5)])
 def test_node_overflow_is_regrown_and_the_kick_is_exact(ctx, prec, tol):
     """Tight 17-particle clumps make about one node per particle, beyond the initial node
