@@ -118,7 +118,7 @@ def test_pipelined_batch_equals_each_box_alone(ctx, stages):
         inputs.append((lagpm.make_params(n, omep=om, omeplam=1 - om), pos, vel, a0))
     print(f"pipelined batch of {len(specs)} boxes, {stages} stages", flush=True)
     for p, pos, vel, a0 in inputs:
-        (x, v), _ = run_kdk(ctx, p, pos, vel, a0, 0.1, 2)
+        (x, v), _ = run_kdk(ctx, p, pos, vel, a0, 0.1, 3)
         alone.append((x, v))
     try:
         ctx.set_batch_pipeline(stages, 32)
@@ -127,7 +127,7 @@ def test_pipelined_batch_equals_each_box_alone(ctx, stages):
             b.upload_pos(pos, vel)
             boxes.append(b)
         anow = np.array([a0 for *_, a0 in inputs])
-        for s in range(2):
+        for s in range(3):  # the third step replays the first step's captured graphs
             lagpm.batch_step(ctx, boxes, anow + 0.1 * s, 0.1)
         for b, (x, v) in zip(boxes, alone):
             xb, vb = b.download()
