@@ -136,6 +136,10 @@ __global__ void fof_link_kernel(const uint64_t* __restrict__ key, const double* 
     const int sx = (int)(sub % (uint32_t)k), sy = (int)((sub / (uint32_t)k) % (uint32_t)k), sz = (int)(sub / (uint32_t)(k * k));
     const uint32_t cs = start[cell], ce = start[cell + 1];
     const double px = r[3 * (size_t)p], py = r[3 * (size_t)p + 1], pz = r[3 * (size_t)p + 2];
+    // an ancestor of p (ancestors stay ancestors): a partner whose parent is it, or p itself, is
+    // already in p's set, which saves the two root searches of a union (dense halo cells link
+    // most of their pairs)
+    uint32_t rp = uf_root(par, p);
     for (int dz = -1; dz <= 1; dz++) {
         const int tz = sz + dz;
         if (tz < 0 || tz >= k) continue;
@@ -155,7 +159,13 @@ __global__ void fof_link_kernel(const uint64_t* __restrict__ key, const double* 
                     const double ex = fabs(px - r[3 * (size_t)q]), ey = fabs(py - r[3 * (size_t)q + 1]),
                                  ez = fabs(pz - r[3 * (size_t)q + 2]);
                     const double d = sqrt(ex * ex + ey * ey + ez * ez);  // OST/fof.c:272-276
-                    if (d <= link) uf_unite(par, p, q);
+                    if (d <= link) {
+                        const uint32_t pq = par_load(par + q);
+                        if (pq != rp && pq != p) {
+                            uf_unite(par, p, q);
+                            rp = uf_root(par, p);
+                        }
+                    }
                 }
             }
         }
@@ -165,8 +175,37 @@ __global__ void fof_link_kernel(const uint64_t* __restrict__ key, const double* 
 __global__ void fof_stats_kernel(const double* __restrict__ r, uint32_t np, uint32_t* __restrict__ par,
                                  uint32_t* __restrict__ cnt, uint64_t* __restrict__ lo, uint64_t* __restrict__ hi) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    const uint32_t g = uf_root(par, p);
+    const bool valid = p < np;
+    const uint32_t g = valid ? uf_root(par, p) : 0xffffffffu;
+    // A wave whose particles all belong to one group (the sorted order puts a halo core's
+    // particles together) reduces its count and bounds first and does one set of atomics: the
+    // cores' roots otherwise take ~10^5 atomics each on one address.
+    const uint32_t g0 = (uint32_t)__shfl((int)g, 0, 64);
+    if (g0 != 0xffffffffu && __all(!valid || g == g0)) {
+        const unsigned long long vm = __ballot(valid);
+        double mn[3], mx[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const double v = valid ? r[3 * (size_t)p + a] : 0.0;
+            mn[a] = valid ? v : __longlong_as_double(0x7ff0000000000000ll);  // +inf
+            mx[a] = valid ? v : -1.0;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                mn[a] = fmin(mn[a], __shfl_xor(mn[a], o, 64));
+                mx[a] = fmax(mx[a], __shfl_xor(mx[a], o, 64));
+            }
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(cnt + g0, (uint32_t)__popcll(vm));
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                atomicMin((unsigned long long*)(lo + 3 * (size_t)g0 + a), (unsigned long long)__double_as_longlong(mn[a]));
+                atomicMax((unsigned long long*)(hi + 3 * (size_t)g0 + a), (unsigned long long)__double_as_longlong(mx[a]));
+            }
+        }
+        return;
+    }
+    if (!valid) return;
     atomicAdd(cnt + g, 1u);
 #pragma unroll
     for (int a = 0; a < 3; a++) {

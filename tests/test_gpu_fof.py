@@ -46,3 +46,19 @@ def test_prefof_known_configuration(ctx):
         with pytest.raises(lagpm.LagpmError):
             b.prefof(0.2, 0)
     assert cnt == 28 and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("link", [0.5, 1.3, 2.5, 4.5])
+def test_prefof_long_links_and_cell_faces(ctx, link):
+    """Links from 0.5 to beyond the cell width (one sub-cell per cell), with particles placed
+    exactly on cell faces (x = 4k) and corners: the flags equal the oracle's."""
+    n = 24
+    g = np.random.default_rng(int(link * 10))
+    pos = g.random((3000, 3)) * n
+    faces = np.stack(np.meshgrid(np.arange(0, n, 4.0), np.arange(0, n, 4.0), [2.0, 6.5]), -1).reshape(-1, 3)
+    pos = np.concatenate([pos, faces, faces + [0.0, 0.0, 0.3]])
+    want, wcnt = orc.prefof(n, pos, link, 20)
+    with lagpm.Box(ctx, lagpm.make_params(n), len(pos)) as b:
+        b.upload_pos(pos)
+        got, cnt = b.prefof(link, 20)
+    assert cnt == wcnt and np.array_equal(got, want)
