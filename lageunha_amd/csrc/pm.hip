@@ -305,39 +305,99 @@ __device__ inline void fda4_axis(double xp, int& near, int& s, float w[3]) {
     w[0] = w1; w[1] = w2; w[2] = w3;
 }
 
-__global__ void interp_kick_f32(const double* __restrict__ px, const double* __restrict__ py,
-                                const double* __restrict__ pz, double* __restrict__ vx, double* __restrict__ vy,
-                                double* __restrict__ vz, size_t np, int n, const float4* __restrict__ grad,
-                                const double* __restrict__ fac) {
-    const float fact2 = (float)fac[1];
-    size_t p = (size_t)lagpm_wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    int ix, iy, iz, sx, sy, sz;
-    float wx[3], wy[3], wz[3];
-    fda4_axis(px[p], ix, sx, wx);
-    fda4_axis(py[p], iy, sy, wy);
-    fda4_axis(pz[p], iz, sz, wz);
-    int xi[3] = {wrapi(ix, n), wrapi(ix + sx, n), wrapi(ix - sx, n)};
-    int yi[3] = {wrapi(iy, n), wrapi(iy + sy, n), wrapi(iy - sy, n)};
-    int zi[3] = {wrapi(iz, n), wrapi(iz + sz, n), wrapi(iz - sz, n)};
-    float fx = 0.f, fy = 0.f, fz = 0.f;
+// TSC interpolation of the FDA4 gradient mesh and the kick (fda4.c:31-153). The 256 particles of
+// a block are consecutive in Morton order, so their 27-cell stencils usually cover a small box of
+// cells: that box is staged in LDS once (coalesced rows, periodic indices) and the 27 reads per
+// particle come from LDS instead of the caches. A block whose box does not fit (its particles
+// straddle a periodic face or two distant Morton cells) reads the mesh directly. Same cells, same
+// weights and the same order of additions either way, so the kick is the same bit for bit.
+constexpr int INTERP_THREADS = 256;
+constexpr int INTERP_LDS = 1536;  // float4 cells of one block's box (24 KB)
+
+__device__ inline void interp_sum(const float wx[3], const float wy[3], const float wz[3], const int xi[3],
+                                  const int yi[3], const int zi[3], size_t sy, size_t sz, const float4* __restrict__ g,
+                                  float& fx, float& fy, float& fz) {
 #pragma unroll
     for (int c = 0; c < 3; c++)
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-            size_t row = (size_t)n * ((size_t)yi[b] + (size_t)n * zi[c]);
+            const size_t row = sy * (size_t)yi[b] + sz * (size_t)zi[c];
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 float wxy = wx[a] * wy[b];
                 float wt = wxy * wz[c];
-                float4 g = grad[row + xi[a]];
+                float4 v = g[row + xi[a]];
                 if (a == 0 && b == 0 && c == 0) {
-                    fx = wt * g.x; fy = wt * g.y; fz = wt * g.z;
+                    fx = wt * v.x; fy = wt * v.y; fz = wt * v.z;
                 } else {
-                    fx = fx + wt * g.x; fy = fy + wt * g.y; fz = fz + wt * g.z;
+                    fx = fx + wt * v.x; fy = fy + wt * v.y; fz = fz + wt * v.z;
                 }
             }
         }
+}
+
+__global__ void __launch_bounds__(INTERP_THREADS) interp_kick_f32(const double* __restrict__ px, const double* __restrict__ py,
+                                const double* __restrict__ pz, double* __restrict__ vx, double* __restrict__ vy,
+                                double* __restrict__ vz, size_t np, int n, const float4* __restrict__ grad,
+                                const double* __restrict__ fac) {
+    __shared__ float4 sg[INTERP_LDS];
+    __shared__ int wred[6][INTERP_THREADS / 64];
+    __shared__ int box[6];
+    const float fact2 = (float)fac[1];
+    const size_t p = (size_t)lagpm_wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const bool valid = p < np;
+    int near[3] = {0, 0, 0}, sgn[3] = {1, 1, 1};
+    float wx[3] = {0.f, 0.f, 0.f}, wy[3] = {0.f, 0.f, 0.f}, wz[3] = {0.f, 0.f, 0.f};
+    if (valid) {
+        fda4_axis(px[p], near[0], sgn[0], wx);
+        fda4_axis(py[p], near[1], sgn[1], wy);
+        fda4_axis(pz[p], near[2], sgn[2], wz);
+    }
+    // the block's box of stencil cells in unwrapped indices (near - 1 .. near + 1 per axis)
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int lo = valid ? near[a] - 1 : 0x7fffffff, hi = valid ? near[a] + 1 : -0x7fffffff;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            lo = min(lo, __shfl_xor(lo, o, 64));
+            hi = max(hi, __shfl_xor(hi, o, 64));
+        }
+        if (lane == 0) {
+            wred[a][wid] = lo;
+            wred[3 + a][wid] = hi;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int v = wred[threadIdx.x][0];
+        for (int w = 1; w < INTERP_THREADS / 64; w++)
+            v = threadIdx.x < 3 ? min(v, wred[threadIdx.x][w]) : max(v, wred[threadIdx.x][w]);
+        box[threadIdx.x] = v;
+    }
+    __syncthreads();
+    const int bx = box[0], by = box[1], bz = box[2];
+    const long long ex = (long long)box[3] - bx + 1, ey = (long long)box[4] - by + 1, ez = (long long)box[5] - bz + 1;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (ex * ey * ez <= INTERP_LDS) {  // block-uniform
+        const int vol = (int)(ex * ey * ez);
+        for (int t = threadIdx.x; t < vol; t += INTERP_THREADS) {
+            const int a = t % (int)ex, b = (t / (int)ex) % (int)ey, c = t / (int)(ex * ey);
+            sg[t] = grad[(size_t)n * ((size_t)wrapi(by + b, n) + (size_t)n * wrapi(bz + c, n)) + wrapi(bx + a, n)];
+        }
+        __syncthreads();
+        if (!valid) return;
+        const int xi[3] = {near[0] - bx, near[0] + sgn[0] - bx, near[0] - sgn[0] - bx};
+        const int yi[3] = {near[1] - by, near[1] + sgn[1] - by, near[1] - sgn[1] - by};
+        const int zi[3] = {near[2] - bz, near[2] + sgn[2] - bz, near[2] - sgn[2] - bz};
+        interp_sum(wx, wy, wz, xi, yi, zi, (size_t)ex, (size_t)(ex * ey), sg, fx, fy, fz);
+    } else {
+        if (!valid) return;
+        const int xi[3] = {wrapi(near[0], n), wrapi(near[0] + sgn[0], n), wrapi(near[0] - sgn[0], n)};
+        const int yi[3] = {wrapi(near[1], n), wrapi(near[1] + sgn[1], n), wrapi(near[1] - sgn[1], n)};
+        const int zi[3] = {wrapi(near[2], n), wrapi(near[2] + sgn[2], n), wrapi(near[2] - sgn[2], n)};
+        interp_sum(wx, wy, wz, xi, yi, zi, (size_t)n, (size_t)n * n, grad, fx, fy, fz);
+    }
     vx[p] += (double)(fact2 * fx);
     vy[p] += (double)(fact2 * fy);
     vz[p] += (double)(fact2 * fz);
@@ -785,7 +845,7 @@ int lagpm_pm_interp(lagpm_ctx* ctx, lagpm_box* box) {
         hipLaunchKernelGGL(interp_kick_f64, dim3(nblk), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx, box->vy,
                            box->vz, box->np, box->p.n, (const double4*)box->grad, (const double*)box->dfac);
     else
-        hipLaunchKernelGGL(interp_kick_f32, dim3(nblk), dim3(256), 0, box->st, box->x, box->y, box->z, box->vx, box->vy,
+        hipLaunchKernelGGL(interp_kick_f32, dim3(nblk), dim3(INTERP_THREADS), 0, box->st, box->x, box->y, box->z, box->vx, box->vy,
                            box->vz, box->np, box->p.n, (const float4*)box->grad, (const double*)box->dfac);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
