@@ -313,7 +313,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
-        print(json.dumps({"error": "this build runs on one GPU: the TreePM step of one box is not sharded; "
+        print(json.dumps({"error": "this library runs on one GPU: the TreePM step of one box is not sharded; "
                                    "run with --gpus 1 (sharding one box over several GPUs is not built)"}))
         sys.exit(2)
 

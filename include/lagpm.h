@@ -69,7 +69,7 @@ typedef struct lagpm_evolfact {
 } lagpm_evolfact;
 
 /* Per-box gravity parameters (the gravity keys of SimParameters: NX, GRV_EFOLD,
- * GRV_EPSILON, RSPHERE, THETA, eunha.h:1033-1090, plus this build's switches). */
+ * GRV_EPSILON, RSPHERE, THETA, eunha.h:1033-1090, plus this library's switches). */
 typedef struct lagpm_params {
     int32_t n;          /* mesh cells per side; the box side is n cells (NX) */
     int32_t law;        /* LAGPM_LAW_* */

@@ -273,7 +273,7 @@ void orc_c2r(int n, const double* in, double* out) {
 /* ------------------------------------------------------------------------------------
  * Green's function, Cosmos/psolver.c:13-86: w = i^2 + wy^2 + wz^2 (integer modes,
  * wrapped above n/2), factor = (-1/pi)/w * exp(-w (pi efold/n)^2), times 1/n^3, DC = 0.
- * deconvolve (this build's option): divide by the TSC window squared per axis,
+ * deconvolve (an option of the GPU library, not of the reference): divide by the TSC window squared per axis,
  * W(k) = sinc^3(k/2), k = 2 pi m / n.
  * ------------------------------------------------------------------------------------ */
 static double tsc_w2(int n, double m) {
