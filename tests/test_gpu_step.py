@@ -138,6 +138,40 @@ def test_pipelined_batch_equals_each_box_alone(ctx, stages):
             b.close()
 
 
+def test_mixed_precision_batch_is_not_pipelined_and_equals_each_box_alone(ctx):
+    """Seven boxes, FP32 and FP64 mixed: the default pipeline needs one FP32 force table for the
+    whole batch, so this batch runs unpipelined (FP64 walks per box, one fused FP32 walk); every box
+    still gets its single-box result bit for bit. Bad pipeline settings are refused."""
+    with pytest.raises(lagpm.LagpmError):
+        ctx.set_batch_pipeline(0, 32)
+    with pytest.raises(lagpm.LagpmError):
+        ctx.set_batch_pipeline(3, 100000)
+    specs = [(24, lagpm.PREC_FP32), (32, lagpm.PREC_FP64), (24, lagpm.PREC_FP64), (36, lagpm.PREC_FP32),
+             (24, lagpm.PREC_FP32), (32, lagpm.PREC_FP32), (24, lagpm.PREC_FP64)]
+    inputs, alone, boxes = [], [], []
+    for i, (n, prec) in enumerate(specs):
+        pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=4, seed=50 + i)
+        vel = np.random.default_rng(60 + i).standard_normal(pos.shape) * 0.01
+        inputs.append((lagpm.make_params(n, precision=prec), pos, vel))
+    print(f"mixed batch of {len(specs)} boxes", flush=True)
+    for p, pos, vel in inputs:
+        (x, v), _ = run_kdk(ctx, p, pos, vel, 1.0, 0.1, 2)
+        alone.append((x, v))
+    try:
+        for p, pos, vel in inputs:
+            b = lagpm.Box(ctx, p, len(pos))
+            b.upload_pos(pos, vel)
+            boxes.append(b)
+        for s in range(2):
+            lagpm.batch_step(ctx, boxes, 1.0 + 0.1 * s, 0.1)
+        for b, (x, v) in zip(boxes, alone):
+            xb, vb = b.download()
+            assert np.array_equal(xb, x) and np.array_equal(vb, v)
+    finally:
+        for b in boxes:
+            b.close()
+
+
 @pytest.mark.parametrize("prec", [lagpm.PREC_FP32, lagpm.PREC_FP64])
 def test_depth_limit_meets_a_deeper_tree_and_redoes_it(ctx, prec):
     """After a step on a perturbed lattice the build is limited to two levels below that tree's
