@@ -1002,13 +1002,10 @@ int ensure_pool(lagpm_ctx* ctx, size_t want) {
 // and the PM's memory-bound ones share the GPU. With ev: phase events (eager runs only; a capture
 // records none, but the fork and join events are part of it).
 int front_body(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
-    const size_t np = box->np;
     hipStream_t st = box->st, st2 = box->st2;
-    hipLaunchKernelGGL(scale3_kernel, dim3(nblocks(np, 256)), dim3(256), 0, st, box->vx, box->vy, box->vz, np,
-                       (const double*)box->dfac);
-    HIPCHK(ctx, hipGetLastError());
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
-    RCHK(lagpm_sort_particles(ctx, box));
+    // the Hubble drag (v *= dfac[0]) rides in the sort's key pass
+    RCHK(lagpm_sort_particles(ctx, box, (const double*)box->dfac));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[1], st));
     HIPCHK(ctx, hipEventRecord(box->evfork, st));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[12], st));

@@ -213,7 +213,7 @@ int lagpm_pm_power(lagpm_ctx* ctx, lagpm_box* box, double* pk_host, double* npk_
 void lagpm_box_graphs_release(lagpm_box* box);
 
 // ---- tree (tree.hip) ----
-int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box);
+int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box, const double* drag = nullptr);
 // Queues the octree build on box->st; sizes stay on the device (dsz).
 int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box);
 // One walk launch on ctx->stream over all groups of the given boxes (FP32 boxes sharing one

@@ -142,12 +142,22 @@ __device__ inline size_t lower_bound_pfx(const uint64_t* __restrict__ K, size_t 
     return lo;
 }
 
+// Morton keys of the positions and the identity permutation; with drag, also the Hubble drag of
+// the step's front (v *= drag[0], Cosmos/globalT.c:102-110) in the same pass over the particles.
 __global__ void keys_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-                            size_t np, double s, uint64_t* __restrict__ key, uint32_t* __restrict__ perm) {
+                            size_t np, double s, uint64_t* __restrict__ key, uint32_t* __restrict__ perm,
+                            double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+                            const double* __restrict__ drag) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     key[i] = spread3(quant(x[i], s)) | (spread3(quant(y[i], s)) << 1) | (spread3(quant(z[i], s)) << 2);
     perm[i] = (uint32_t)i;
+    if (drag) {
+        const double f = drag[0];
+        vx[i] *= f;
+        vy[i] *= f;
+        vz[i] *= f;
+    }
 }
 
 // all particle arrays into the new order in one pass (one permutation load, seven gathers in flight)
@@ -1298,13 +1308,13 @@ int lagpm_tree_alloc(lagpm_ctx* ctx, lagpm_box* box, size_t cap) {
     return LAGPM_OK;
 }
 
-int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box) {
+int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box, const double* drag) {
     const size_t np = box->np;
     if (np == 0) return LAGPM_OK;
     hipStream_t st = box->st;
     const double s = 65536.0 / (double)box->p.n;
     hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->x, box->y, box->z, np, s,
-                       box->key, box->perm);
+                       box->key, box->perm, box->vx, box->vy, box->vz, drag);
     HIPCHK(ctx, hipGetLastError());
     bool alt = false;
     RCHK(lagpm_radix_sort_pairs(ctx, st, box->scan_tmp, box->key, box->key2, box->perm, box->perm2, np, 3 * LAGPM_KEY_BITS,
