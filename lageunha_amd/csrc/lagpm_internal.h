@@ -197,7 +197,8 @@ size_t lagpm_sort_tmp_bytes(size_t n);
 int lagpm_exclusive_scan_u32(lagpm_ctx* ctx, hipStream_t s, uint32_t* tmp, const uint32_t* in, uint32_t* out, size_t n,
                              uint32_t* total_dev);
 int lagpm_radix_sort_pairs(lagpm_ctx* ctx, hipStream_t s, uint32_t* tmp, uint64_t* key, uint64_t* key_alt, uint32_t* val,
-                           uint32_t* val_alt, size_t n, int bits, bool* result_in_alt);
+                           uint32_t* val_alt, size_t n, int bits, bool* result_in_alt,
+                           int lo_bit = 0);
 
 // ---- PM (pm.hip) ----
 int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box);          // fills acc and rho

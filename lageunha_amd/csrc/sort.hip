@@ -226,9 +226,9 @@ int lagpm_exclusive_scan_u32(lagpm_ctx* ctx, hipStream_t st, uint32_t* tmp, cons
     return LAGPM_OK;
 }
 
-// Stable sort of (key, val) pairs on the low `bits` bits of key (rounded up to 8).
+// Stable sort of (key, val) pairs on `bits` bits of key from bit lo_bit up (rounded up to 8).
 int lagpm_radix_sort_pairs(lagpm_ctx* ctx, hipStream_t st, uint32_t* base, uint64_t* key, uint64_t* key_alt, uint32_t* val,
-                           uint32_t* val_alt, size_t n, int bits, bool* result_in_alt) {
+                           uint32_t* val_alt, size_t n, int bits, bool* result_in_alt, int lo_bit) {
     *result_in_alt = false;
     if (n <= 1) return LAGPM_OK;
     uint32_t nblocks = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
@@ -241,7 +241,7 @@ int lagpm_radix_sort_pairs(lagpm_ctx* ctx, hipStream_t st, uint32_t* base, uint6
     uint32_t *vi = val, *vo = val_alt;
     int passes = (bits + 7) / 8;
     for (int p = 0; p < passes; p++) {
-        int shift = 8 * p;
+        int shift = lo_bit + 8 * p;
         hipLaunchKernelGGL(rs_hist_kernel, dim3(nblocks), dim3(RS_THREADS), 0, st, ki, n, shift, ghist, nblocks);
         HIPCHK(ctx, hipGetLastError());
         RCHK(scan_rec(ctx, st, ghist, goff, hist_elems, base + 4));

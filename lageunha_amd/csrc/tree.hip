@@ -1316,13 +1316,17 @@ int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box, const double* drag) {
     hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->x, box->y, box->z, np, s,
                        box->key, box->perm, box->vx, box->vy, box->vz, drag);
     HIPCHK(ctx, hipGetLastError());
+    // The tree splits nodes down to box->depth at most (deeper trees are detected and redone after
+    // a full sort), so only the top 3 * depth key bits must be in order: a box whose tree is
+    // shallower needs fewer passes. Below them the order is the previous one (the sort is stable).
+    // An even number of passes leaves the result in key/perm, where captured steps and the walk's
+    // descriptors expect it.
+    const int D = std::min(LAGPM_KEY_BITS, std::max(box->level_cell, box->depth));
+    const int passes = (((3 * D + 7) / 8) + 1) & ~1;
     bool alt = false;
-    RCHK(lagpm_radix_sort_pairs(ctx, st, box->scan_tmp, box->key, box->key2, box->perm, box->perm2, np, 3 * LAGPM_KEY_BITS,
-                                &alt));
-    if (alt) {
-        std::swap(box->key, box->key2);
-        std::swap(box->perm, box->perm2);
-    }
+    RCHK(lagpm_radix_sort_pairs(ctx, st, box->scan_tmp, box->key, box->key2, box->perm, box->perm2, np, 8 * passes, &alt,
+                                3 * LAGPM_KEY_BITS - 8 * passes));
+    if (alt) return lagpm_fail(ctx, LAGPM_E_STATE, "radix sort ended in the scratch buffers");
     double** arrs[6] = {&box->x, &box->y, &box->z, &box->vx, &box->vy, &box->vz};
     gather_set g;
     for (int a = 0; a < 6; a++) {
@@ -1513,6 +1517,7 @@ int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry) {
         // the tree needs levels below the depth limit: redo it at full depth, with room for the
         // nodes the truncated build could not count
         box->depth = LAGPM_KEY_BITS;
+        box->sorted = false;  // the keys are in order down to the old limit only: sort again
         lagpm_box_graphs_release(box);  // captured steps hold the old limit
         const size_t need = std::max<size_t>((size_t)d[DSZ_NODES], box->node_cap);
         if (d[DSZ_NODES] >= box->node_cap) RCHK(lagpm_tree_alloc(ctx, box, need + need / 2 + 4096));
