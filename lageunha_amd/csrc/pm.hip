@@ -79,10 +79,14 @@ __global__ void zero_u64_kernel(unsigned long long* __restrict__ a, size_t m) {
 // inside one Morton block of <= 13 mesh cells per side, so every chunk's TSC footprint fits
 // the 4096-cell LDS box. A chunk starts where the block prefix changes or at a multiple of
 // DEP_CHUNK.
-__global__ void dep_flag_kernel(const uint64_t* __restrict__ K, size_t N, int sh, uint32_t* __restrict__ flag) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    flag[i] = (i == 0 || (i % DEP_CHUNK) == 0 || (K[i] >> sh) != (K[i - 1] >> sh)) ? 1u : 0u;
+// chunk starts (flag) of the N particles, and the accumulator cleared (m cells), in one pass
+__global__ void dep_flag_kernel(const uint64_t* __restrict__ K, size_t N, int sh, uint32_t* __restrict__ flag,
+                                unsigned long long* __restrict__ acc, size_t m) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m || i < N; i += stride) {
+        if (i < m) acc[i] = 0ull;
+        if (i < N) flag[i] = (i == 0 || (i % DEP_CHUNK) == 0 || (K[i] >> sh) != (K[i - 1] >> sh)) ? 1u : 0u;
+    }
 }
 __global__ void dep_start_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off, size_t N,
                                  uint32_t* __restrict__ start, const uint32_t* __restrict__ nchunk) {
@@ -752,10 +756,11 @@ int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
     const size_t m = (size_t)n * n * n;
     const bool f64 = box->p.precision == LAGPM_PREC_FP64;
     hipStream_t st = box->st;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(m, 256)), dim3(256), 0, st, box->acc, m);
-    HIPCHK(ctx, hipGetLastError());
     const double scale = std::ldexp(1.0, box->scale_bits);
-    if (box->np) {
+    if (!box->np) {
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(m, 256)), dim3(256), 0, st, box->acc, m);
+        HIPCHK(ctx, hipGetLastError());
+    } else {
         // chunk list from the sorted keys (the caller sorted the particles); the chunk count
         // stays on the device and the deposit grid covers its upper bound. Its scratch is its own
         // (perm2 and key2 hold nothing between sorts, dep_scan_tmp is the deposit's): a step runs
@@ -764,7 +769,8 @@ int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
         const size_t np = box->np;
         uint32_t* flag = box->perm2;
         uint32_t* off = reinterpret_cast<uint32_t*>(box->key2);
-        hipLaunchKernelGGL(dep_flag_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, st, box->key, np, sh, flag);
+        hipLaunchKernelGGL(dep_flag_kernel, dim3(grid_for(std::max(np, m), 256)), dim3(256), 0, st, box->key, np, sh, flag,
+                           box->acc, m);
         HIPCHK(ctx, hipGetLastError());
         uint32_t* nchunk = box->dsz + DSZ_CHUNKS;
         RCHK(lagpm_exclusive_scan_u32(ctx, st, box->dep_scan_tmp, flag, off, np, nchunk));

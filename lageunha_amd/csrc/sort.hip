@@ -29,10 +29,12 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v, int lane) {
     return v;
 }
 
-// Block-level exclusive scan of SCAN_TILE elements; writes per-block totals.
+// Block-level exclusive scan of SCAN_TILE elements; writes per-block totals, and the grand total
+// when this launch is a single tile (the top of the recursion) and `total` is given.
 __global__ void __launch_bounds__(SCAN_THREADS) scan_tile_kernel(const uint32_t* __restrict__ in,
                                                                  uint32_t* __restrict__ out, size_t n,
-                                                                 uint32_t* __restrict__ block_sums) {
+                                                                 uint32_t* __restrict__ block_sums,
+                                                                 uint32_t* __restrict__ total) {
     __shared__ uint32_t wsum[SCAN_THREADS / 64];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)tid * SCAN_ITEMS;
@@ -61,6 +63,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_tile_kernel(const uint32_t*
         run += v[k];
     }
     if (tid == 0 && block_sums) block_sums[blockIdx.x] = btot;
+    if (tid == 0 && total && gridDim.x == 1) *total = btot;
 }
 
 __global__ void scan_add_kernel(uint32_t* __restrict__ out, size_t n, const uint32_t* __restrict__ block_off) {
@@ -179,17 +182,19 @@ __global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* 
     }
 }
 
-// recursive exclusive scan using tmp (enough room for all levels)
-int scan_rec(lagpm_ctx* ctx, hipStream_t st, const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp) {
+// recursive exclusive scan using tmp (enough room for all levels); the grand total into *total
+// (when given) from the single-tile scan at the top of the recursion
+int scan_rec(lagpm_ctx* ctx, hipStream_t st, const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp,
+             uint32_t* total = nullptr) {
     size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb == 0) return LAGPM_OK;
     uint32_t* sums = tmp;
     uint32_t* sums_scan = tmp + nb;
     hipLaunchKernelGGL(scan_tile_kernel, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, in, out, n,
-                       nb > 1 ? sums : (uint32_t*)nullptr);
+                       nb > 1 ? sums : (uint32_t*)nullptr, nb > 1 ? (uint32_t*)nullptr : total);
     HIPCHK(ctx, hipGetLastError());
     if (nb > 1) {
-        RCHK(scan_rec(ctx, st, sums, sums_scan, nb, tmp + 2 * nb));
+        RCHK(scan_rec(ctx, st, sums, sums_scan, nb, tmp + 2 * nb, total));
         hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n,
                            sums_scan);
         HIPCHK(ctx, hipGetLastError());
@@ -218,8 +223,8 @@ size_t lagpm_sort_tmp_bytes(size_t n) {
 
 int lagpm_exclusive_scan_u32(lagpm_ctx* ctx, hipStream_t st, uint32_t* tmp, const uint32_t* in, uint32_t* out, size_t n,
                              uint32_t* total_dev) {
-    if (n > 0) RCHK(scan_rec(ctx, st, in, out, n, tmp + 4));
-    if (total_dev) {
+    if (n > 0) RCHK(scan_rec(ctx, st, in, out, n, tmp + 4, total_dev));
+    if (total_dev && n == 0) {
         hipLaunchKernelGGL(scan_total_kernel, dim3(1), dim3(64), 0, st, in, out, n, total_dev);
         HIPCHK(ctx, hipGetLastError());
     }
