@@ -180,14 +180,30 @@ __global__ void gather_all_kernel(gather_set g, const uint32_t* __restrict__ per
     g.id_dst[i] = id;
 }
 
-// FP32 copy of the positions for the tree, scaled by sc (rounded once from the FP64 product):
-// with the log-r^2 force tables sc = sqrt(lam), so a pair's r^2 in these units is lam r^2, the
-// table variable, and needs no multiply in the pair loop. Everything the FP32 tree derives from
-// pp (centres, second moments, node sizes, group boxes) is in the same units.
-__global__ void pack_pp_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-                               size_t np, float mass, double sc, float4* __restrict__ pp) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < np) pp[i] = make_float4((float)(x[i] * sc), (float)(y[i] * sc), (float)(z[i] * sc), mass);
+__device__ inline void tree_reset(int t, uint32_t* __restrict__ dsz, unsigned long long* __restrict__ counters) {
+    if (t == DSZ_NODES || t == DSZ_GROUPS || t == DSZ_OVERFLOW || t == DSZ_DEEPER) dsz[t] = 0u;
+    if ((t >= DSZ_LVL && t < DSZ_LOFF) || (t >= DSZ_LCUR && t < DSZ_COUNT)) dsz[t] = 0u;
+    if (t < 4) counters[t] = 0ull;
+}
+
+// Everything the build needs cleared or packed before the level-L cells, in one pass: the build's
+// sizes and counters, per-particle node counts, empty cell roots, and (FP32 tree, pp non-null)
+// the pp copy of the positions in units of the particle mass.
+// pp is the FP32 copy of the positions for the tree, scaled by sc (rounded once from the FP64
+// product): with the log-r^2 force tables sc = sqrt(lam), so a pair's r^2 in these units is lam r^2,
+// the table variable, and needs no multiply in the pair loop. Everything the FP32 tree derives
+// from pp (centres, second moments, node sizes, group boxes) is in the same units.
+__global__ void tree_prep_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ z, size_t np, double sc, float4* __restrict__ pp,
+                                 uint32_t* __restrict__ ncount, int* __restrict__ cellroot, size_t ncell,
+                                 uint32_t* __restrict__ dsz, unsigned long long* __restrict__ counters) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < 64) tree_reset(threadIdx.x, dsz, counters);
+    if (i < np) {
+        if (pp) pp[i] = make_float4((float)(x[i] * sc), (float)(y[i] * sc), (float)(z[i] * sc), 1.0f);
+        ncount[i] = 0u;
+    }
+    if (i < ncell) cellroot[i] = -1;
 }
 
 // node count on the device, clamped to the array capacity
@@ -580,17 +596,10 @@ __global__ void group_box_kernel(const double* __restrict__ x, const double* __r
     for (int a = 0; a < 3; a++) { gbox[6 * g + a] = lo[a]; gbox[6 * g + 3 + a] = hi[a]; }
 }
 
-__global__ void fill_int_kernel(int* __restrict__ a, size_t m, int v) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) a[i] = v;
-}
 // start of a tree build: clear the size words and the interaction counters (a kernel, not a
 // memset, so the step can be captured in a graph)
 __global__ void tree_reset_kernel(uint32_t* __restrict__ dsz, unsigned long long* __restrict__ counters) {
-    int t = threadIdx.x;
-    if (t == DSZ_NODES || t == DSZ_GROUPS || t == DSZ_OVERFLOW || t == DSZ_DEEPER) dsz[t] = 0u;
-    if ((t >= DSZ_LVL && t < DSZ_LOFF) || (t >= DSZ_LCUR && t < DSZ_COUNT)) dsz[t] = 0u;
-    if (t < 4) counters[t] = 0ull;
+    tree_reset(threadIdx.x, dsz, counters);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -660,7 +669,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                                                                 int nbox, const float2* __restrict__ tab0,
                                                                 const float4* __restrict__ tab12, int* __restrict__ stacks,
                                                                 unsigned long long* __restrict__ ticket_ctr) {
-    // positions are in scaled units (pack_pp_kernel): for the log tables r^2 here is lam r^2
+    // positions are in scaled units (tree_prep_kernel): for the log tables r^2 here is lam r^2
     constexpr float lam = 1.0f;
     // One 1024-thread workgroup per CU: the 16 waves share one LDS copy of the particle force
     // table (32 KB) next to their private interaction lists.
@@ -1352,19 +1361,18 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     const bool f64 = box->p.precision == LAGPM_PREC_FP64;
     hipStream_t st = box->st;
     const size_t cap = box->node_cap, gcap = box->group_cap;
-    hipLaunchKernelGGL(tree_reset_kernel, dim3(1), dim3(64), 0, st, box->dsz, box->counters);
-    HIPCHK(ctx, hipGetLastError());
-    if (np == 0) return LAGPM_OK;
-    if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
-    if (!f64) {
-        // the tree works in units of the particle mass; the walk multiplies by it with the kick
-        hipLaunchKernelGGL(pack_pp_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->x, box->y, box->z, np, 1.0f,
-                           lagpm_tree_scale(box),
-                           box->pp);
+    if (np == 0) {
+        hipLaunchKernelGGL(tree_reset_kernel, dim3(1), dim3(64), 0, st, box->dsz, box->counters);
         HIPCHK(ctx, hipGetLastError());
+        return LAGPM_OK;
     }
+    if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
+    // sizes, counters, node counts and cell roots cleared; for the FP32 tree the pp copy in units
+    // of the particle mass (the walk multiplies by it with the kick)
     const size_t ncell = (size_t)box->mcell * box->mcell * box->mcell;
-    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(ncell, 256)), dim3(256), 0, st, box->cellroot, ncell, -1);
+    hipLaunchKernelGGL(tree_prep_kernel, dim3(blocks_for(std::max(np, ncell), 256)), dim3(256), 0, st, box->x, box->y,
+                       box->z, np, f64 ? 1.0 : lagpm_tree_scale(box), f64 ? (float4*)nullptr : box->pp, box->ncount,
+                       box->cellroot, ncell, box->dsz, box->counters);
     HIPCHK(ctx, hipGetLastError());
     // top-down build: records of level-L cells, then one split per level, then the numbering.
     // The records and child lists live in the moment arrays until the moments overwrite them.
@@ -1380,7 +1388,6 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
         rch1 = reinterpret_cast<int*>(box->nd_q1);
     }
     uint32_t* order = box->goff;  // level order (goff is free until the group scan)
-    hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, reinterpret_cast<int*>(box->ncount), np, 0);
     hipLaunchKernelGGL(cells_kernel, dim3(blocks_for(ncell, 256)), dim3(256), 0, st, box->key, np, L, cap, box->dsz, rec,
                        box->ncount);
     hipLaunchKernelGGL(level_mark_kernel, dim3(1), dim3(64), 0, st, box->dsz, cap, L, L);
@@ -1450,7 +1457,7 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
         A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell);
         A.mcell = box->mcell;
         A.naccept = LAGPM_NACCEPT;
-        // lengths in the units of pp (scaled by sc, see pack_pp_kernel)
+        // lengths in the units of pp (scaled by sc, see tree_prep_kernel)
         const double sc = lagpm_tree_scale(box);
         A.box = (float)(box->p.n * sc);
         A.rc2 = (float)(box->p.rsphere * box->p.rsphere * sc * sc);
