@@ -48,6 +48,26 @@ def test_tsc_deposit_bit_exact(ctx, case, prec):
     assert abs(acc.sum() * 2.0 ** -bits - len(pos)) < 1e-3
 
 
+@pytest.mark.parametrize("n,npart", [(16, 50000), (64, 500)])
+def test_tsc_deposit_repeated_in_one_box(ctx, n, npart):
+    """Two deposits of different particles into one box: the second equals the oracle bit for bit,
+    so the accumulator is fully cleared between calls, with more particles than cells (16^3 mesh,
+    50000 particles) and far fewer (64^3 mesh, 500 particles)."""
+    g = np.random.default_rng(n)
+    print(f"repeated deposit n={n} N={npart}", flush=True)
+    with lagpm.Box(ctx, lagpm.make_params(n), npart) as b:
+        for rep in range(2):
+            pos = g.random((npart, 3)) * n
+            if rep == 0:
+                pos[: npart // 2] = n / 3 + g.standard_normal((npart // 2, 3))  # a clump the second set lacks
+                pos %= n
+            b.upload_pos(pos)
+            acc, bits = b.tsc_fixed()
+            oacc, obits = orc.tsc_fixed(n, pos, mass=1.0)
+            assert bits == obits
+            assert np.array_equal(acc, oacc), f"deposit {rep}"
+
+
 def test_tsc_deposit_dense_halo_falls_back_and_stays_exact(ctx):
     n = 32
     g = np.random.default_rng(9)
