@@ -21,6 +21,14 @@
  *   reference walk    theta -> 0 equals the direct pair sum; quadrupole error O((s/d)^3)
  *   Ewald             independence of the splitting parameter alpha; lattice symmetry
  *   kick factors      Einstein-de Sitter closed forms; scipy quadrature for LCDM
+ *   power spectrum    bin weights per mode; single Fourier mode amplitude; sigma8 and xi(r)
+ *                     against real-space measurements of a Gaussian field
+ *   PreFoF            scipy kd-tree pair search + graph components per cell; hand-built groups
+ *                     (19/20 interior members, face groups, pairs just beyond the link)
+ *
+ * Parity unpinned (no run of the reference is possible here): the exact topology and last bits
+ * of the reference tree at theta > 0, the float-Simpson kick factors at the last bits, and the
+ * OpenMP-order-dependent last bits of the reference deposit.
  */
 #include "lagpm_oracle.h"
 
