@@ -109,14 +109,17 @@ __global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* 
     __shared__ uint64_t skey[RS_TILE];
     __shared__ uint32_t sval[RS_TILE];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    // the tile this workgroup scatters: neighbouring tiles write the adjacent ends of each digit's
+    // runs, so one XCD takes a contiguous range of tiles and writes those lines whole
+    const uint32_t tile = xcd_contiguous_block(blockIdx.x, gridDim.x);
     for (int d = tid; d < 256; d += RS_THREADS) {
-        boff[d] = goff[(size_t)d * nblocks + blockIdx.x];
+        boff[d] = goff[(size_t)d * nblocks + tile];
 #pragma unroll
         for (int w = 0; w < RS_WAVES; w++) wcount[w][d] = 0;
     }
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const size_t tile0 = (size_t)blockIdx.x * RS_TILE;
+    const size_t tile0 = (size_t)tile * RS_TILE;
     const size_t seg = tile0 + (size_t)wid * (RS_ITEMS * 64);
     uint64_t k[RS_ITEMS];
     uint32_t v[RS_ITEMS];

@@ -167,15 +167,9 @@ struct gather_set {
     const uint32_t* id;
     uint32_t* id_dst;
 };
-// Workgroups go to the XCDs in turn (workgroup b to XCD b mod 8), and each XCD has its own L2. The
-// sources of neighbouring outputs share cache lines (the order changes little between steps), so
-// the gather renumbers its workgroups to give each XCD one contiguous range of outputs: the lines
+// The sources of neighbouring outputs share cache lines (the order changes little between steps),
+// so the gather gives each XCD one contiguous range of outputs (xcd_contiguous_block): the lines
 // those outputs read are fetched into one L2 instead of into several.
-__device__ inline unsigned xcd_contiguous_block(unsigned b, unsigned nb) {
-    const unsigned x = b % LAGPM_WALK_XCDS, q = b / LAGPM_WALK_XCDS;
-    const unsigned base = nb / LAGPM_WALK_XCDS, rem = nb % LAGPM_WALK_XCDS;
-    return x * base + min(x, rem) + q;  // XCD x's range starts after the ranges of XCDs 0..x-1
-}
 __global__ void gather_all_kernel(gather_set g, const uint32_t* __restrict__ perm, size_t np) {
     const size_t i = (size_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= np) return;
