@@ -47,16 +47,6 @@ struct walk_box;  // device view of one box for the walk (tree.hip)
 // XCDs of one MI355X in SPX mode: the walk splits its groups into this many ranges.
 constexpr int LAGPM_WALK_XCDS = 8;
 
-// Workgroups go to the XCDs in turn (workgroup b to XCD b mod 8), and each XCD has its own L2.
-// Renumbered by this bijection of [0, nb), each XCD's workgroups cover one contiguous range of
-// logical blocks, so lines shared by neighbouring blocks are fetched into (or written back from)
-// one L2 instead of several.
-__device__ inline unsigned xcd_contiguous_block(unsigned b, unsigned nb) {
-    const unsigned x = b % LAGPM_WALK_XCDS, q = b / LAGPM_WALK_XCDS;
-    const unsigned base = nb / LAGPM_WALK_XCDS, rem = nb % LAGPM_WALK_XCDS;
-    return x * base + min(x, rem) + q;  // XCD x's range starts after the ranges of XCDs 0..x-1
-}
-
 struct lagpm_ctx {
     int device = 0;
     int num_cu = 0;

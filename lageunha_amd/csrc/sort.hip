@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     // the tile this workgroup scatters: neighbouring tiles write the adjacent ends of each digit's
     // runs, so one XCD takes a contiguous range of tiles and writes those lines whole
-    const uint32_t tile = xcd_contiguous_block(blockIdx.x, gridDim.x);
+    const uint32_t tile = lagpm_wave::xcd_block(blockIdx.x, gridDim.x);
     for (int d = tid; d < 256; d += RS_THREADS) {
         boff[d] = goff[(size_t)d * nblocks + tile];
 #pragma unroll

@@ -168,10 +168,10 @@ struct gather_set {
     uint32_t* id_dst;
 };
 // The sources of neighbouring outputs share cache lines (the order changes little between steps),
-// so the gather gives each XCD one contiguous range of outputs (xcd_contiguous_block): the lines
+// so the gather gives each XCD one contiguous range of outputs (lagpm_wave::xcd_block): the lines
 // those outputs read are fetched into one L2 instead of into several.
 __global__ void gather_all_kernel(gather_set g, const uint32_t* __restrict__ perm, size_t np) {
-    const size_t i = (size_t)xcd_contiguous_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const size_t i = (size_t)lagpm_wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const uint32_t p = perm[i];
     double v[6];
