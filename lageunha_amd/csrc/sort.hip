@@ -76,6 +76,47 @@ __global__ void scan_total_kernel(const uint32_t* __restrict__ in, const uint32_
     if (threadIdx.x == 0 && blockIdx.x == 0) *total = (n ? out[n - 1] + in[n - 1] : 0u);
 }
 
+// Radix pass offsets, step 1: one workgroup per digit d scans that digit's column of per-tile
+// counts (hist[d * nblocks + t], contiguous in t) into tile offsets within the digit's run, and
+// writes the column total to coltot[d]. The scatter adds the digit's base (the totals of the
+// smaller digits), so a pass needs one scan launch over its 256 x nblocks counts.
+__global__ void __launch_bounds__(SCAN_THREADS) rs_colscan_kernel(const uint32_t* __restrict__ hist,
+                                                                  uint32_t* __restrict__ off, uint32_t nblocks,
+                                                                  uint32_t* __restrict__ coltot) {
+    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const size_t col = (size_t)blockIdx.x * nblocks;
+    uint32_t carry = 0;
+    for (size_t c0 = 0; c0 < nblocks; c0 += SCAN_TILE) {
+        const size_t first = c0 + (size_t)tid * SCAN_ITEMS;
+        uint32_t v[SCAN_ITEMS];
+        uint32_t s = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            v[k] = first + k < nblocks ? hist[col + first + k] : 0u;
+            s += v[k];
+        }
+        const uint32_t incl = wave_incl_scan(s, lane);
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        uint32_t woff = 0, ctot = 0;
+#pragma unroll
+        for (int w = 0; w < SCAN_THREADS / 64; w++) {
+            if (w < wid) woff += wsum[w];
+            ctot += wsum[w];
+        }
+        uint32_t run = carry + woff + incl - s;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            if (first + k < nblocks) off[col + first + k] = run;
+            run += v[k];
+        }
+        carry += ctot;
+        __syncthreads();  // wsum is written again by the next chunk
+    }
+    if (tid == 0) coltot[blockIdx.x] = carry;
+}
+
 // ---- radix sort ----
 __global__ void __launch_bounds__(RS_THREADS) rs_hist_kernel(const uint64_t* __restrict__ key, size_t n, int shift,
                                                              uint32_t* __restrict__ ghist, uint32_t nblocks) {
@@ -100,8 +141,11 @@ __global__ void __launch_bounds__(RS_THREADS) rs_hist_kernel(const uint64_t* __r
 __global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* __restrict__ key,
                                                                 const uint32_t* __restrict__ val, size_t n, int shift,
                                                                 const uint32_t* __restrict__ goff, uint32_t nblocks,
+                                                                const uint32_t* __restrict__ coltot,
                                                                 uint64_t* __restrict__ key_out,
                                                                 uint32_t* __restrict__ val_out) {
+    static_assert(RS_THREADS == 256, "one thread per digit in the offset set-up");
+    __shared__ uint32_t dsum[RS_WAVES];
     __shared__ uint32_t wcount[RS_WAVES][256];
     __shared__ uint32_t boff[256];
     __shared__ uint32_t toff[256];
@@ -112,10 +156,20 @@ __global__ void __launch_bounds__(RS_THREADS) rs_scatter_kernel(const uint64_t* 
     // the tile this workgroup scatters: neighbouring tiles write the adjacent ends of each digit's
     // runs, so one XCD takes a contiguous range of tiles and writes those lines whole
     const uint32_t tile = lagpm_wave::xcd_block(blockIdx.x, gridDim.x);
-    for (int d = tid; d < 256; d += RS_THREADS) {
-        boff[d] = goff[(size_t)d * nblocks + tile];
+    {
+        // this tile's start in digit tid's output run: the digit's base (exclusive prefix of the
+        // column totals over the smaller digits) plus the tile's offset within the column
+        const uint32_t t = coltot[tid];
+        const uint32_t incl = wave_incl_scan(t, lane);
+        if (lane == 63) dsum[wid] = incl;
+        __syncthreads();
+        uint32_t dbase = incl - t;
 #pragma unroll
-        for (int w = 0; w < RS_WAVES; w++) wcount[w][d] = 0;
+        for (int w = 0; w < RS_WAVES; w++)
+            if (w < wid) dbase += dsum[w];
+        boff[tid] = dbase + goff[(size_t)tid * nblocks + tile];
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) wcount[w][tid] = 0;
     }
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -221,7 +275,7 @@ size_t lagpm_scan_tmp_bytes(size_t n) { return (scan_tmp_elems(n) + 4) * sizeof(
 size_t lagpm_sort_tmp_bytes(size_t n) {
     size_t nblocks = (n + RS_TILE - 1) / RS_TILE;
     size_t hist_elems = (size_t)256 * std::max<size_t>(nblocks, 1);
-    return (scan_tmp_elems(hist_elems) + 4 + 2 * hist_elems) * sizeof(uint32_t);
+    return (2 * hist_elems + 256) * sizeof(uint32_t);  // counts, offsets, column totals
 }
 
 int lagpm_exclusive_scan_u32(lagpm_ctx* ctx, hipStream_t st, uint32_t* tmp, const uint32_t* in, uint32_t* out, size_t n,
@@ -241,10 +295,9 @@ int lagpm_radix_sort_pairs(lagpm_ctx* ctx, hipStream_t st, uint32_t* base, uint6
     if (n <= 1) return LAGPM_OK;
     uint32_t nblocks = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
     size_t hist_elems = (size_t)256 * nblocks;
-    // histogram + its scan live after the scan temp area
-    size_t scan_elems = scan_tmp_elems(hist_elems) + 4;
-    uint32_t* ghist = base + scan_elems;
-    uint32_t* goff = ghist + hist_elems;
+    uint32_t* ghist = base;               // per-tile digit counts, digit-major
+    uint32_t* goff = ghist + hist_elems;  // tile offsets within each digit's run
+    uint32_t* coltot = goff + hist_elems;  // per-digit totals
     uint64_t *ki = key, *ko = key_alt;
     uint32_t *vi = val, *vo = val_alt;
     int passes = (bits + 7) / 8;
@@ -252,9 +305,10 @@ int lagpm_radix_sort_pairs(lagpm_ctx* ctx, hipStream_t st, uint32_t* base, uint6
         int shift = lo_bit + 8 * p;
         hipLaunchKernelGGL(rs_hist_kernel, dim3(nblocks), dim3(RS_THREADS), 0, st, ki, n, shift, ghist, nblocks);
         HIPCHK(ctx, hipGetLastError());
-        RCHK(scan_rec(ctx, st, ghist, goff, hist_elems, base + 4));
+        hipLaunchKernelGGL(rs_colscan_kernel, dim3(256), dim3(SCAN_THREADS), 0, st, ghist, goff, nblocks, coltot);
+        HIPCHK(ctx, hipGetLastError());
         hipLaunchKernelGGL(rs_scatter_kernel, dim3(nblocks), dim3(RS_THREADS), 0, st, ki, vi, n, shift, goff,
-                           nblocks, ko, vo);
+                           nblocks, coltot, ko, vo);
         HIPCHK(ctx, hipGetLastError());
         std::swap(ki, ko);
         std::swap(vi, vo);
