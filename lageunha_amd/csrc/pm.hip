@@ -266,34 +266,37 @@ __device__ inline double fda4_d(double m1, double p1, double m2, double p2) {
 }
 
 // FDA4 gradient mesh: one thread per cell, 12 neighbour reads (cached), one 16/32-byte store.
+// A workgroup takes one segment of `segs` of one mesh row (fixed j, k), so the row's j and k are
+// workgroup-uniform and every index is an add; neighbours within +-2 wrap with one compare. Rows
+// go to the XCDs in contiguous ranges (xcd_block): the +-2 planes and rows a row reads are then
+// mostly in the same L2.
+__device__ inline int wrap2(int a, int n) {  // a in [-n, 2n)
+    return a < 0 ? a + n : (a >= n ? a - n : a);
+}
 template <typename T, typename T4>
-__global__ void gradient_kernel(const T* __restrict__ phi, int n, T4* __restrict__ grad) {
-    const size_t m = (size_t)n * n * n;
-    size_t q = (size_t)lagpm_wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t nn = (size_t)n * n;
+__global__ void gradient_kernel(const T* __restrict__ phi, int n, int segs, T4* __restrict__ grad) {
+    const unsigned lb = lagpm_wave::xcd_block(blockIdx.x, gridDim.x);
+    const unsigned row = lb / (unsigned)segs;  // row = j + n k
+    const int i = (int)(lb - row * (unsigned)segs) * (int)blockDim.x + (int)threadIdx.x;
+    if (i >= n) return;
+    const int j = (int)(row % (unsigned)n), k = (int)(row / (unsigned)n);
     const size_t mx = 2 * (size_t)(n / 2 + 1), plane = mx * n;  // padded rows of the FFT mesh
-    for (; q < m; q += stride) {
-        int i = (int)(q % n);
-        int j = (int)((q / n) % n);
-        int k = (int)(q / nn);
-        size_t rowj = mx * j + plane * k;
-        auto X = [&](int a) { return phi[(size_t)wrapi(a, n) + rowj]; };
-        auto Y = [&](int b) { return phi[(size_t)i + mx * wrapi(b, n) + plane * k]; };
-        auto Z = [&](int c) { return phi[(size_t)i + mx * j + plane * wrapi(c, n)]; };
-        T4 g;
-        if constexpr (sizeof(T) == 4) {
-            g.x = fda4_f(X(i - 1), X(i + 1), X(i - 2), X(i + 2));
-            g.y = fda4_f(Y(j - 1), Y(j + 1), Y(j - 2), Y(j + 2));
-            g.z = fda4_f(Z(k - 1), Z(k + 1), Z(k - 2), Z(k + 2));
-        } else {
-            g.x = fda4_d(X(i - 1), X(i + 1), X(i - 2), X(i + 2));
-            g.y = fda4_d(Y(j - 1), Y(j + 1), Y(j - 2), Y(j + 2));
-            g.z = fda4_d(Z(k - 1), Z(k + 1), Z(k - 2), Z(k + 2));
-        }
-        g.w = phi[(size_t)i + rowj];
-        grad[q] = g;
+    const size_t rowj = mx * j + plane * k;
+    auto X = [&](int a) { return phi[(size_t)wrap2(a, n) + rowj]; };
+    auto Y = [&](int b) { return phi[(size_t)i + mx * wrap2(b, n) + plane * k]; };
+    auto Z = [&](int c) { return phi[(size_t)i + mx * j + plane * wrap2(c, n)]; };
+    T4 g;
+    if constexpr (sizeof(T) == 4) {
+        g.x = fda4_f(X(i - 1), X(i + 1), X(i - 2), X(i + 2));
+        g.y = fda4_f(Y(j - 1), Y(j + 1), Y(j - 2), Y(j + 2));
+        g.z = fda4_f(Z(k - 1), Z(k + 1), Z(k - 2), Z(k + 2));
+    } else {
+        g.x = fda4_d(X(i - 1), X(i + 1), X(i - 2), X(i + 2));
+        g.y = fda4_d(Y(j - 1), Y(j + 1), Y(j - 2), Y(j + 2));
+        g.z = fda4_d(Z(k - 1), Z(k + 1), Z(k - 2), Z(k + 2));
     }
+    g.w = phi[(size_t)i + rowj];
+    grad[(size_t)i + (size_t)n * row] = g;
 }
 
 // Interpolation weights of FDA4_HEAD: xmin is a float there (fda4.h:40-66).
@@ -833,13 +836,16 @@ int lagpm_pm_solve(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
 
 int lagpm_pm_gradient(lagpm_ctx* ctx, lagpm_box* box) {
     const int n = box->p.n;
-    const size_t m = (size_t)n * n * n;
+    // workgroups of whole wavefronts covering a row (256 threads, or the row rounded up to 64)
+    const int bs = n >= 256 ? 256 : (n + 63) / 64 * 64;
+    const int segs = (n + bs - 1) / bs;
+    const dim3 grid((unsigned)((size_t)n * n * segs));
     if (box->p.precision == LAGPM_PREC_FP64)
-        hipLaunchKernelGGL((gradient_kernel<double, double4>), dim3(grid_for(m, 256)), dim3(256), 0, box->st,
-                           (const double*)box->phi, n, (double4*)box->grad);
+        hipLaunchKernelGGL((gradient_kernel<double, double4>), grid, dim3(bs), 0, box->st, (const double*)box->phi, n, segs,
+                           (double4*)box->grad);
     else
-        hipLaunchKernelGGL((gradient_kernel<float, float4>), dim3(grid_for(m, 256)), dim3(256), 0, box->st,
-                           (const float*)box->phi, n, (float4*)box->grad);
+        hipLaunchKernelGGL((gradient_kernel<float, float4>), grid, dim3(bs), 0, box->st, (const float*)box->phi, n, segs,
+                           (float4*)box->grad);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }

@@ -90,8 +90,8 @@ def test_green_multiply_bit_exact(ctx):
         assert np.array_equal(out, ref), (efold, dec)
 
 
-def test_fda4_gradient_and_kick_bit_exact(ctx):
-    n = 32
+@pytest.mark.parametrize("n", [32, 36, 300])  # a row in one workgroup, a partial one, two segments
+def test_fda4_gradient_and_kick_bit_exact(ctx, n):
     g = np.random.default_rng(23)
     phi = np.fft.irfftn(np.fft.rfftn(g.standard_normal((n, n, n))) * np.exp(-np.arange(n // 2 + 1) / 4.0)).astype(np.float32)
     pos = edge_positions(n)
