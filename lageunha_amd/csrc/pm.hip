@@ -225,33 +225,31 @@ __device__ inline double tsc_w2(int n, double m) {
 
 // Green multiply over the half-complex spectrum, layout (i + h*(j + n*k)), psolver.c:49-76.
 template <typename T>
-__global__ void green_kernel(T* __restrict__ spec, int n, double sfact, double irngc, int deconvolve) {
+// One workgroup per segment of `segs` of one half-complex row (fixed j, k): the row's indices and
+// its y and z wave numbers are workgroup-uniform, so no cell divides its linear index.
+__global__ void green_kernel(T* __restrict__ spec, int n, int segs, double sfact, double irngc, int deconvolve) {
     const int h = n / 2 + 1;
-    const size_t K = (size_t)h * n * n;
-    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
+    const unsigned row = blockIdx.x / (unsigned)segs;  // row = j + n k
+    const int i = (int)(blockIdx.x - row * (unsigned)segs) * (int)blockDim.x + (int)threadIdx.x;
+    if (i >= h) return;
+    const int j = (int)(row % (unsigned)n), k = (int)(row / (unsigned)n);
+    const size_t q = (size_t)i + (size_t)h * row;
     const double rn = n;
-    for (; q < K; q += stride) {
-        int i = (int)(q % h);
-        size_t r = q / h;
-        int j = (int)(r % n);
-        int k = (int)(r / n);
-        float wy = (float)j;
-        if (wy > rn / 2) wy = wy - (float)rn;
-        float wz = (float)k;
-        if (wz > rn / 2) wz = wz - (float)rn;
-        double wave = (double)((float)(i * i) + wy * wy + wz * wz);
-        if (wave == 0.0) {
-            spec[2 * q] = (T)0;
-            spec[2 * q + 1] = (T)0;
-            continue;
-        }
-        double factr = (-1.0 / M_PI) / wave;
-        double factor = factr * exp(wave * sfact);
-        if (deconvolve) factor /= tsc_w2(n, (double)i) * tsc_w2(n, (double)wy) * tsc_w2(n, (double)wz);
-        spec[2 * q] = (T)(factor * (double)spec[2 * q] * irngc);
-        spec[2 * q + 1] = (T)(factor * (double)spec[2 * q + 1] * irngc);
+    float wy = (float)j;
+    if (wy > rn / 2) wy = wy - (float)rn;
+    float wz = (float)k;
+    if (wz > rn / 2) wz = wz - (float)rn;
+    double wave = (double)((float)(i * i) + wy * wy + wz * wz);
+    if (wave == 0.0) {
+        spec[2 * q] = (T)0;
+        spec[2 * q + 1] = (T)0;
+        return;
     }
+    double factr = (-1.0 / M_PI) / wave;
+    double factor = factr * exp(wave * sfact);
+    if (deconvolve) factor /= tsc_w2(n, (double)i) * tsc_w2(n, (double)wy) * tsc_w2(n, (double)wz);
+    spec[2 * q] = (T)(factor * (double)spec[2 * q] * irngc);
+    spec[2 * q + 1] = (T)(factor * (double)spec[2 * q + 1] * irngc);
 }
 
 __device__ inline float fda4_f(float m1, float p1, float m2, float p2) {
@@ -805,13 +803,15 @@ int lagpm_green_device(lagpm_ctx* ctx, hipStream_t st, int n, int prec, double e
     const double rn = n;
     const double sfact = -std::pow(M_PI * efold / rn, 2);
     const double irngc = 1.0 / (rn * rn * rn);
-    const size_t K = (size_t)(n / 2 + 1) * n * n;
+    // workgroups of whole wavefronts covering a half-complex row of n / 2 + 1 cells
+    const int h = n / 2 + 1;
+    const int bs = h >= 256 ? 256 : (h + 63) / 64 * 64;
+    const int segs = (h + bs - 1) / bs;
+    const dim3 grid((unsigned)((size_t)n * n * segs));
     if (prec == LAGPM_PREC_FP64)
-        hipLaunchKernelGGL(green_kernel<double>, dim3(grid_for(K, 256)), dim3(256), 0, st, (double*)spec, n, sfact,
-                           irngc, deconvolve);
+        hipLaunchKernelGGL(green_kernel<double>, grid, dim3(bs), 0, st, (double*)spec, n, segs, sfact, irngc, deconvolve);
     else
-        hipLaunchKernelGGL(green_kernel<float>, dim3(grid_for(K, 256)), dim3(256), 0, st, (float*)spec, n, sfact,
-                           irngc, deconvolve);
+        hipLaunchKernelGGL(green_kernel<float>, grid, dim3(bs), 0, st, (float*)spec, n, segs, sfact, irngc, deconvolve);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }

@@ -80,8 +80,8 @@ def test_tsc_deposit_dense_halo_falls_back_and_stays_exact(ctx):
     assert np.array_equal(acc, oacc)
 
 
-def test_green_multiply_bit_exact(ctx):
-    n = 32
+@pytest.mark.parametrize("n", [32, 300, 514])  # rows of 17, 151 and 258 (two segments) cells
+def test_green_multiply_bit_exact(ctx, n):
     g = np.random.default_rng(22)
     spec = (g.standard_normal((n, n, n // 2 + 1)) + 1j * g.standard_normal((n, n, n // 2 + 1))).astype(np.complex64)
     for efold, dec in ((0.9, 0), (2.5, 1)):
