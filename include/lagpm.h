@@ -55,6 +55,9 @@ extern "C" {
 /* short-range force law of the tree correction */
 #define LAGPM_LAW_PMFIT 0  /* reference: Plummer Newton minus fitted PM force, OST/Model4TreeForce.c:92-116 */
 #define LAGPM_LAW_ERFC  1  /* Plummer Newton times the erfc complement of a Gaussian filter exp(-k^2 rs^2) */
+#define LAGPM_LAW_TREEFORCE 2 /* the fitted law as ForceLaw/treeforce.c:134-186 tabulates it: softening fixed at
+                                 EPSILON = 0.1 (OST/ost.h:6) whatever `epsilon` says, derivatives with a step of
+                                 0.2 bin, entry 0 zero (the reference law uses 0.1 bin and entry 0 = entry 1) */
 
 /* arithmetic of the force evaluation */
 #define LAGPM_PREC_FP32 0  /* FP32 mesh/FFT/pairs, FP64 positions and velocities (fast path) */
@@ -231,13 +234,27 @@ int lagpm_power_spectrum(lagpm_ctx* ctx, lagpm_box* box, double boxsize, double*
 
 /* ---- short-range tree correction --------------------------------------------------- */
 /* Builds FP64 force tables (NSPLINE=4096 entries over r in [0,6)): diff[i][3], slope[i][3]
- * exactly as OST/Model4TreeForce.c:48-91 (law PMFIT) or for the erfc law. Host only. */
+ * exactly as OST/Model4TreeForce.c:48-91 (law PMFIT), as ForceLaw/treeforce.c:150-186
+ * (law TREEFORCE) or for the erfc law. Host only. */
 #define LAGPM_NSPLINE 4096
 int lagpm_force_spline(const lagpm_params* p, double* diff /*[4096*3]*/, double* slope /*[4096*3]*/);
 /* Tree correction kick v += fact2 * a_short (GOTPM: the tree uses EvolFactArr[0].fact2,
  * Cosmos/treecorrection.c:227). Sorts, builds the octree, walks. */
 int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef);
 int lagpm_tree_stats_get(const lagpm_box* box, lagpm_tree_stats* out);
+/* The octree of the box's last tree build (Make_Tree3 / divide_node3, OST/ost.ver2.c:36-144), for
+ * tests and inspection; it describes the positions the box had at that build. Nodes are in
+ * pre-order, *nnodes of them (LAGPM_E_INVAL when cap is smaller; *nnodes is still set).
+ *   info[4v..4v+3]: first particle, particle count, number of children, level. A node's particles
+ *                   are a run of the box's current particle order; order[i] (np entries) is the
+ *                   caller index of the particle at position i of that order.
+ *   com[4v..4v+3]:  centre of mass (cells; positions wrapped into [0, n)) and mass.
+ *   quad[8v..8v+7]: Qxx, Qyy, Qzz, Qxy, Qxz, Qyz about the centre of mass (mass x cells^2), their
+ *                   trace, and rmax (cells): the largest particle distance from the centre of mass
+ *                   for leaves and nodes of up to 128 particles, an upper bound of it above.
+ * Any of info, com, quad, order may be NULL. */
+int lagpm_tree_nodes_get(lagpm_ctx* ctx, lagpm_box* box, size_t cap, size_t* nnodes, int32_t* info, double* com,
+                         double* quad, uint32_t* order);
 /* Tree tuning of a box: nleaf (a node with more particles is split, 4..256) and the target
  * group size (8..64, one wavefront walks one group). Defaults 16 and 64. */
 int lagpm_box_set_tuning(lagpm_box* box, int nleaf, int group);

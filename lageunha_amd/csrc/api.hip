@@ -6,7 +6,6 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
-#include <unistd.h>
 #include <cstdlib>
 #include <vector>
 
@@ -140,7 +139,8 @@ void free_box(lagpm_box* b) {
 int check_params(lagpm_ctx* ctx, const lagpm_params* p) {
     if (!p) return lagpm_fail(ctx, LAGPM_E_INVAL, "params is NULL");
     if (p->n < 8 || p->n > 4096) return lagpm_fail(ctx, LAGPM_E_INVAL, "n must be in [8, 4096]");
-    if (p->law != LAGPM_LAW_PMFIT && p->law != LAGPM_LAW_ERFC) return lagpm_fail(ctx, LAGPM_E_INVAL, "unknown force law");
+    if (p->law != LAGPM_LAW_PMFIT && p->law != LAGPM_LAW_ERFC && p->law != LAGPM_LAW_TREEFORCE)
+        return lagpm_fail(ctx, LAGPM_E_INVAL, "unknown force law");
     if (p->precision != LAGPM_PREC_FP32 && p->precision != LAGPM_PREC_FP64)
         return lagpm_fail(ctx, LAGPM_E_INVAL, "unknown precision");
     if (!(p->mass > 0)) return lagpm_fail(ctx, LAGPM_E_INVAL, "mass must be positive");
@@ -205,11 +205,22 @@ double law_erfc(int n, double eps, double rs, double r) {
 
 }  // namespace
 
+// LAW_PMFIT: i_force_spline of OST/Model4TreeForce.c:48-91 (softening GRV_EPSILON, derivative
+// step 0.1 bin, entry 0 a copy of entry 1). LAW_TREEFORCE: i_force_spline(nx, Epsilon) of
+// ForceLaw/treeforce.c:150-186, the same fitted law with the softening fixed at the EPSILON macro
+// (0.1, OST/ost.h:6; its pmfit ignores the Epsilon argument, :134-147), derivative step 0.2 bin
+// (:159), and entry 0 left zero with the slope up to entry 1 (:153,170-174). LAW_ERFC: the erfc
+// complement of the Gaussian filter, built like LAW_PMFIT.
 void lagpm_build_tables(const lagpm_params* p, double* diff, double* slope) {
     const int NS = LAGPM_NSPLINE;
-    const double ran = 6.0 / (double)NS, dr = 0.1 * ran;
+    const bool tf = p->law == LAGPM_LAW_TREEFORCE;
+    const double ran = 6.0 / (double)NS, dr = (tf ? 0.2 : 0.1) * ran;
     const double rs = 0.5 * p->efold;
-    auto law = [&](double r) { return p->law == LAGPM_LAW_PMFIT ? law_pmfit(p->n, p->epsilon, r) : law_erfc(p->n, p->epsilon, rs, r); };
+    auto law = [&](double r) {
+        if (p->law == LAGPM_LAW_PMFIT) return law_pmfit(p->n, p->epsilon, r);
+        if (tf) return law_pmfit(p->n, 0.1, r);
+        return law_erfc(p->n, p->epsilon, rs, r);
+    };
     std::memset(diff, 0, 3 * NS * sizeof(double));
     std::memset(slope, 0, 3 * NS * sizeof(double));
     for (int i = 1; i < NS; i++) {
@@ -221,11 +232,13 @@ void lagpm_build_tables(const lagpm_params* p, double* diff, double* slope) {
         diff[3 * i + 1] = -(0.5 * (fp * r - fr) / r / r / r);
         diff[3 * i + 2] = -(0.5 * (fpp * r * r - 3 * fp * r + 3 * fr) / r / r / r / r / r);
     }
-    for (int i = 1; i < NS - 1; i++)
+    for (int i = tf ? 0 : 1; i < NS - 1; i++)
         for (int c = 0; c < 3; c++) slope[3 * i + c] = (diff[3 * (i + 1) + c] - diff[3 * i + c]) / ran;
     for (int c = 0; c < 3; c++) {
-        slope[c] = slope[3 + c];
-        diff[c] = diff[3 + c];
+        if (!tf) {
+            slope[c] = slope[3 + c];
+            diff[c] = diff[3 + c];
+        }
         diff[3 * (NS - 1) + c] = 0;
         slope[3 * (NS - 1) + c] = 0;
     }
@@ -347,9 +360,18 @@ int lagpm_get_tables(lagpm_ctx* ctx, const lagpm_params* p, const law_tables** o
     return LAGPM_OK;
 }
 
+int lagpm_d2h(lagpm_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t s) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    const hipError_t w = hipStreamSynchronize(s);
+    if (e != hipSuccess) return lagpm_fail(ctx, LAGPM_E_HIP, std::string("device-to-host copy: ") + hipGetErrorString(e));
+    if (w != hipSuccess) return lagpm_fail(ctx, LAGPM_E_HIP, std::string("device-to-host copy (wait): ") + hipGetErrorString(w));
+    return LAGPM_OK;
+}
+
 extern "C" {
 
 int lagpm_abi_version(void) { return LAGPM_ABI_VERSION; }
+
 
 int lagpm_create(lagpm_ctx** out, int device) {
     if (!out) return LAGPM_E_INVAL;
@@ -378,18 +400,11 @@ int lagpm_create(lagpm_ctx** out, int device) {
         lagpm_destroy(ctx);
         return LAGPM_E_NOMEM;
     }
-    std::call_once(g_rocfft_once, []() {
-        // rocFFT compiles some kernels at run time and caches them; give it a writable cache
-        // file when the caller did not choose one (the default lives under $HOME, which may
-        // not be writable for service accounts).
-        if (!std::getenv("ROCFFT_RTC_CACHE_PATH")) {
-            const char* tmp = std::getenv("TMPDIR");
-            std::string path = std::string(tmp && *tmp ? tmp : "/tmp") + "/lagpm_rocfft_" +
-                               std::to_string((long)getuid()) + ".db";
-            setenv("ROCFFT_RTC_CACHE_PATH", path.c_str(), 0);
-        }
-        rocfft_setup();
-    });
+    // The library never writes the process environment: setenv() is not thread-safe, and by
+    // now the HIP runtime has threads of its own (a caller that wants rocFFT's run-time kernel
+    // cache in a chosen file sets ROCFFT_RTC_CACHE_PATH before the first context; the Python
+    // binding does so before it loads this library).
+    std::call_once(g_rocfft_once, []() { rocfft_setup(); });
     *out = ctx;
     return LAGPM_OK;
 }
@@ -555,6 +570,7 @@ int lagpm_box_upload(lagpm_ctx* ctx, lagpm_box* box, const double* x, const doub
     HIPCHK(ctx, hipStreamSynchronize(box->st));
     box->uploaded = true;
     box->sorted = false;
+    box->tree_built = false;
     return LAGPM_OK;
 }
 
@@ -567,8 +583,7 @@ int lagpm_box_download(lagpm_ctx* ctx, lagpm_box* box, double* x, double* y, dou
         if (!dst[a] || np == 0) continue;
         hipLaunchKernelGGL(scatter_kernel, dim3(nblocks(np, 256)), dim3(256), 0, box->st, src[a], box->id, np, box->alt[0]);
         HIPCHK(ctx, hipGetLastError());
-        HIPCHK(ctx, hipMemcpyAsync(dst[a], box->alt[0], np * sizeof(double), hipMemcpyDeviceToHost, box->st));
-        HIPCHK(ctx, hipStreamSynchronize(box->st));
+        RCHK(lagpm_d2h(ctx, dst[a], box->alt[0], np * sizeof(double), box->st));
     }
     return LAGPM_OK;
 }
@@ -657,7 +672,7 @@ int lagpm_tsc_fixed(lagpm_ctx* ctx, lagpm_box* box, int64_t* acc, int* scale_bit
     if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
     RCHK(lagpm_pm_deposit(ctx, box));
     const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
-    if (acc) HIPCHK(ctx, hipMemcpyAsync(acc, box->acc, m * 8, hipMemcpyDeviceToHost, box->st));
+    if (acc) RCHK(lagpm_d2h(ctx, acc, box->acc, m * 8, box->st));
     if (scale_bits) *scale_bits = box->scale_bits;
     HIPCHK(ctx, hipStreamSynchronize(box->st));
     return LAGPM_OK;
@@ -727,12 +742,10 @@ int lagpm_get_gradient(lagpm_ctx* ctx, lagpm_box* box, float* grad4) {
     const size_t m = (size_t)box->p.n * box->p.n * box->p.n;
     if (box->p.precision == LAGPM_PREC_FP64) {
         std::vector<double> t(4 * m);
-        HIPCHK(ctx, hipMemcpyAsync(t.data(), box->grad, 4 * m * 8, hipMemcpyDeviceToHost, box->st));
-        HIPCHK(ctx, hipStreamSynchronize(box->st));
+        RCHK(lagpm_d2h(ctx, t.data(), box->grad, 4 * m * 8, box->st));
         for (size_t i = 0; i < 4 * m; i++) grad4[i] = (float)t[i];
     } else {
-        HIPCHK(ctx, hipMemcpyAsync(grad4, box->grad, 4 * m * 4, hipMemcpyDeviceToHost, box->st));
-        HIPCHK(ctx, hipStreamSynchronize(box->st));
+        RCHK(lagpm_d2h(ctx, grad4, box->grad, 4 * m * 4, box->st));
     }
     return LAGPM_OK;
 }

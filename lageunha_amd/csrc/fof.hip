@@ -319,8 +319,7 @@ extern "C" int lagpm_prefof(lagpm_ctx* ctx, lagpm_box* box, float fof_link, int 
                        lo, hi, link, (uint32_t)minnum, dflag);
     HIPCHK(ctx, hipGetLastError());
     std::vector<uint8_t> h(np);
-    HIPCHK(ctx, hipMemcpyAsync(h.data(), dflag, np, hipMemcpyDeviceToHost, st));
-    HIPCHK(ctx, hipStreamSynchronize(st));
+    RCHK(lagpm_d2h(ctx, h.data(), dflag, np, st));
     size_t c = 0;
     for (size_t i = 0; i < np; i++) c += h[i];
     if (flags) std::copy(h.begin(), h.end(), flags);

@@ -98,6 +98,7 @@ struct lagpm_box {
     size_t bytes = 0;
     bool uploaded = false;
     bool sorted = false;
+    bool tree_built = false;  // a tree build has been queued (node arrays hold a tree)
     tree_tune tune;
     // deepest level the build splits to: the full key depth until one build has run, then two
     // levels below the deepest level that build used; a build that meets a node still to split
@@ -187,6 +188,11 @@ int lagpm_fail(lagpm_ctx* ctx, int code, const std::string& msg);
         int _r = (expr);      \
         if (_r != LAGPM_OK) return _r; \
     } while (0)
+
+// Device-to-host copy of `bytes` on stream s that has completed when the call returns, whatever
+// the outcome: the stream is synchronised even when queuing the copy fails, so no copy can still
+// be writing into host memory the caller is about to free or leave (stack buffers included).
+int lagpm_d2h(lagpm_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // ---- scans / sort (sort.hip) ----
 // Scratch bytes a box needs for scans of n entries and for a radix sort of n keys.

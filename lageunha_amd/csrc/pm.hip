@@ -592,8 +592,7 @@ int check_plans_t(lagpm_ctx* ctx, int n, fft_plans* fp, hipStream_t st, double t
     }
     if (rc == LAGPM_OK) {
         hipLaunchKernelGGL(fftchk_cmp<T>, dim3(g), dim3(256), 0, st, b, ref, 2 * K, 1.0 / (double)m, (size_t)n, mx, err + 2);
-        if (hipMemcpyAsync(e, err, sizeof(e), hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
-            rc = lagpm_fail(ctx, LAGPM_E_HIP, "FFT check readback");
+        rc = lagpm_d2h(ctx, e, err, sizeof(e), st);
     }
     (void)hipFree(a); (void)hipFree(b); (void)hipFree(ref); (void)hipFree(err);
     if (info) rocfft_execution_info_destroy(info);
@@ -711,13 +710,12 @@ int lagpm_pm_power(lagpm_ctx* ctx, lagpm_box* box, double* pk_host, double* npk_
                                (const float*)box->spec, n, d, d + n + 1);
         if (hipGetLastError() != hipSuccess) rc = LAGPM_E_HIP;
     }
-    if (rc == LAGPM_OK && (hipMemcpyAsync(pk_host, d, (n + 1) * sizeof(double), hipMemcpyDeviceToHost, box->st) != hipSuccess ||
-                           hipMemcpyAsync(npk_host, d + n + 1, (n + 1) * sizeof(double), hipMemcpyDeviceToHost, box->st) !=
-                               hipSuccess ||
-                           hipStreamSynchronize(box->st) != hipSuccess))
-        rc = LAGPM_E_HIP;
+    if (rc == LAGPM_OK) rc = lagpm_d2h(ctx, pk_host, d, (n + 1) * sizeof(double), box->st);
+    if (rc == LAGPM_OK) rc = lagpm_d2h(ctx, npk_host, d + n + 1, (n + 1) * sizeof(double), box->st);
+    (void)hipStreamSynchronize(box->st);
     (void)hipFree(d);
-    if (rc != LAGPM_OK) return lagpm_fail(ctx, rc, "power spectrum kernel failed");
+    if (rc == LAGPM_E_HIP) return lagpm_fail(ctx, rc, "power spectrum kernel failed: " + ctx->err);
+    if (rc != LAGPM_OK) return rc;
     return LAGPM_OK;
 }
 

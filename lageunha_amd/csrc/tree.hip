@@ -1434,6 +1434,7 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
                            box->dsz, gcap, box->groups, box->gbox, box->pgroup);
         HIPCHK(ctx, hipGetLastError());
     }
+    box->tree_built = true;
     return LAGPM_OK;
 }
 
@@ -1516,9 +1517,8 @@ int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry) {
     *retry = false;
     uint32_t d[DSZ_COUNT];
     unsigned long long c[4];
-    HIPCHK(ctx, hipMemcpyAsync(d, box->dsz, sizeof(d), hipMemcpyDeviceToHost, box->st));
-    HIPCHK(ctx, hipMemcpyAsync(c, box->counters, sizeof(c), hipMemcpyDeviceToHost, box->st));
-    HIPCHK(ctx, hipStreamSynchronize(box->st));
+    RCHK(lagpm_d2h(ctx, d, box->dsz, sizeof(d), box->st));
+    RCHK(lagpm_d2h(ctx, c, box->counters, sizeof(c), box->st));
     box->stats.n_pp = c[0];
     box->stats.n_pc = c[1];
     box->stats.n_nodes = d[DSZ_NODES];
@@ -1560,4 +1560,55 @@ int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry) {
 size_t lagpm_walk_stack_entries(int num_cu) {
     size_t blocks = (size_t)std::max(1, num_cu);  // one 16-wave workgroup per CU
     return blocks * WALK_WAVES * STACK_MAX;
+}
+
+// The octree of the last build in the box's own units: FP32 trees keep positions scaled by the
+// tree scale s (see tree_prep_kernel) and masses in particle masses; the FP64 tree keeps cells.
+extern "C" int lagpm_tree_nodes_get(lagpm_ctx* ctx, lagpm_box* box, size_t cap, size_t* nnodes, int32_t* info,
+                                    double* com, double* quad, uint32_t* order) {
+    if (!ctx || !box || !nnodes) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL argument");
+    *nnodes = 0;
+    if (box->np == 0) return LAGPM_OK;
+    if (!box->tree_built) return lagpm_fail(ctx, LAGPM_E_STATE, "no tree built: call lagpm_treecorrection first");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    uint32_t d[DSZ_COUNT];
+    RCHK(lagpm_d2h(ctx, d, box->dsz, sizeof(d), box->st));
+    if (d[DSZ_OVERFLOW]) return lagpm_fail(ctx, LAGPM_E_STATE, "the last build overflowed its node arrays");
+    const size_t nn = std::min<size_t>(d[DSZ_NODES], box->node_cap);
+    *nnodes = nn;
+    if (cap < nn) return lagpm_fail(ctx, LAGPM_E_INVAL, "the arrays hold fewer nodes than the tree has (see *nnodes)");
+    if (info) RCHK(lagpm_d2h(ctx, info, box->nd_info, nn * sizeof(int4), box->st));
+    if (order) RCHK(lagpm_d2h(ctx, order, box->id, box->np * sizeof(uint32_t), box->st));
+    if (!com && !quad) return LAGPM_OK;
+    const double mass = box->p.mass;
+    if (box->p.precision == LAGPM_PREC_FP64) {
+        std::vector<double> t(12 * nn);
+        RCHK(lagpm_d2h(ctx, t.data(), box->nd_d, t.size() * sizeof(double), box->st));
+        for (size_t v = 0; v < nn; v++) {
+            const double* a = &t[12 * v];
+            if (com) { com[4 * v] = a[0]; com[4 * v + 1] = a[1]; com[4 * v + 2] = a[2]; com[4 * v + 3] = a[3] * mass; }
+            if (quad) {
+                for (int k = 0; k < 7; k++) quad[8 * v + k] = a[4 + k] * mass;
+                quad[8 * v + 7] = a[11];
+            }
+        }
+        return LAGPM_OK;
+    }
+    std::vector<float4> c(nn), q0(nn), q1(nn);
+    RCHK(lagpm_d2h(ctx, c.data(), box->nd_com, nn * sizeof(float4), box->st));
+    RCHK(lagpm_d2h(ctx, q0.data(), box->nd_q0, nn * sizeof(float4), box->st));
+    RCHK(lagpm_d2h(ctx, q1.data(), box->nd_q1, nn * sizeof(float4), box->st));
+    const double s = lagpm_tree_scale(box), s2 = s * s;
+    for (size_t v = 0; v < nn; v++) {
+        if (com) {
+            com[4 * v] = c[v].x / s; com[4 * v + 1] = c[v].y / s; com[4 * v + 2] = c[v].z / s;
+            com[4 * v + 3] = c[v].w * mass;
+        }
+        if (quad) {
+            const double q[8] = {q0[v].x, q0[v].y, q0[v].z, q0[v].w, q1[v].x, q1[v].y, q1[v].z, q1[v].w};
+            for (int k = 0; k < 7; k++) quad[8 * v + k] = q[k] * mass / s2;
+            quad[8 * v + 7] = q[7] / s;
+        }
+    }
+    return LAGPM_OK;
 }

@@ -12,6 +12,7 @@ clustered_box       the clustered mix: a fraction of seeded-shuffled lattice sit
 from __future__ import annotations
 
 import numpy as np
+from scipy import fft as sfft
 
 
 def rng(seed: int) -> np.random.Generator:
@@ -22,7 +23,7 @@ def zeldovich_displacement(n: int, rms: float, seed: int, index: float = -1.0) -
     """Displacement field (3, n, n, n) [z][y][x] with rms |psi| = rms."""
     g = rng(seed)
     white = g.standard_normal((n, n, n))
-    wk = np.fft.rfftn(white)
+    wk = sfft.rfftn(white, workers=-1)
     kz = np.fft.fftfreq(n) * 2 * np.pi
     ky = np.fft.fftfreq(n) * 2 * np.pi
     kx = np.fft.rfftfreq(n) * 2 * np.pi
@@ -34,7 +35,7 @@ def zeldovich_displacement(n: int, rms: float, seed: int, index: float = -1.0) -
     delta = wk * amp
     psi = np.empty((3, n, n, n))
     for a, K in enumerate((KX, KY, KZ)):
-        psi[a] = np.fft.irfftn(1j * K * delta / k2, s=(n, n, n))
+        psi[a] = sfft.irfftn(1j * K * delta / k2, s=(n, n, n), axes=(0, 1, 2), workers=-1)
     cur = np.sqrt(np.mean(np.sum(psi ** 2, axis=0)))
     return psi * (rms / cur)
 

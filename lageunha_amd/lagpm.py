@@ -22,7 +22,7 @@ from dataclasses import dataclass
 import numpy as np
 
 PUSH, PULL, KICK = 0, 1, 2  # eunha.h:69-77
-LAW_PMFIT, LAW_ERFC = 0, 1
+LAW_PMFIT, LAW_ERFC, LAW_TREEFORCE = 0, 1, 2
 PREC_FP32, PREC_FP64 = 0, 1
 NSPLINE = 4096
 
@@ -134,6 +134,9 @@ SIGNATURES = {
     "lagpm_power_spectrum": (ctypes.c_int, [_P, _P, ctypes.c_double, _D, _D, _D, _D, _D]),
     "lagpm_treecorrection": (ctypes.c_int, [_P, _P, ctypes.POINTER(EvolFact)]),
     "lagpm_tree_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(TreeStats)]),
+    "lagpm_tree_nodes_get": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                            ctypes.POINTER(ctypes.c_int32), _D, _D,
+                                            ctypes.POINTER(ctypes.c_uint32)]),
     "lagpm_box_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "lagpm_box_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "lagpm_box_autotune": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
@@ -159,6 +162,12 @@ def load(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise LagpmError(-5, f"{path} is missing: build it with __graft_entry__.build() "
                              "(make -C lageunha_amd/csrc)")
+    # rocFFT compiles some kernels at run time and caches them in a file; its default lives under
+    # $HOME, which service accounts may not be able to write. Choose a per-user file in the
+    # temporary directory unless the caller chose one. Done here, before the library (and with it
+    # the HIP runtime's threads) is loaded: the library itself never writes the environment.
+    tmp = os.environ.get("TMPDIR") or "/tmp"
+    os.environ.setdefault("ROCFFT_RTC_CACHE_PATH", os.path.join(tmp, f"lagpm_rocfft_{os.getuid()}.db"))
     lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
@@ -347,6 +356,24 @@ class Box:
         s = TreeStats()
         self.lib.lagpm_tree_stats_get(self.h, ctypes.byref(s))
         return s
+
+    def tree_nodes(self):
+        """The octree of the last tree build: dict of info (nodes, 4: first, count, children, level),
+        com (nodes, 4: x, y, z, mass), quad (nodes, 8: Qxx Qyy Qzz Qxy Qxz Qyz trQ rmax) and order
+        (caller index of each particle of the box's current order)."""
+        nn = ctypes.c_size_t()
+        rc = self.lib.lagpm_tree_nodes_get(self.ctx.h, self.h, 0, ctypes.byref(nn), None, None, None, None)
+        if rc not in (0, -1) or (rc == -1 and nn.value == 0):
+            self.ctx.check(rc)
+        k = nn.value
+        info = np.empty((k, 4), dtype=np.int32)
+        com = np.empty((k, 4))
+        quad = np.empty((k, 8))
+        order = np.empty(self.np, dtype=np.uint32)
+        self.ctx.check(self.lib.lagpm_tree_nodes_get(
+            self.ctx.h, self.h, k, ctypes.byref(nn), info.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _dp(com),
+            _dp(quad), order.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return {"info": info, "com": com, "quad": quad, "order": order}
 
     def timings(self) -> Timings:
         t = Timings()

@@ -511,12 +511,21 @@ double orc_erfc_fr(int n, double eps, double rs, double r) {
     return (newton - flong) / ((double)n * (double)n);
 }
 
+/* law 0: OST/Model4TreeForce.c:48-91; law 2: ForceLaw/treeforce.c:150-186 (pmfit with the
+ * softening fixed at EPSILON = 0.1 whatever eps says, :134-147; derivative step 0.2 bin, :159;
+ * entry 0 zero with the slope up to entry 1, :153,170-174); law 1: the erfc law like law 0. */
 void orc_force_spline(int law, int n, double eps, double rs, double* diff, double* slope) {
+    orc_force_spline_dr(law, n, eps, rs, law == 2 ? 0.2 : 0.1, diff, slope);
+}
+
+void orc_force_spline_dr(int law, int n, double eps, double rs, double dr_bins, double* diff, double* slope) {
     const double ran = 6.0 / (double)ORC_NSPLINE;
-    const double dr = ran * 0.1;
+    const double dr = ran * dr_bins;
+    const int tf = law == 2;
+    if (tf) eps = 0.1;
     memset(diff, 0, 3 * ORC_NSPLINE * sizeof(double));
     memset(slope, 0, 3 * ORC_NSPLINE * sizeof(double));
-#define LAW(rr) (law == 0 ? orc_pmcf(n, eps, (rr)) : orc_erfc_fr(n, eps, rs, (rr)))
+#define LAW(rr) (law != 1 ? orc_pmcf(n, eps, (rr)) : orc_erfc_fr(n, eps, rs, (rr)))
     for (int i = 1; i < ORC_NSPLINE; i++) {
         double r = (double)i * ran;
         double rm1 = r - dr, rm2 = r - 2 * dr, rp1 = r + dr, rp2 = r + 2 * dr;
@@ -528,11 +537,13 @@ void orc_force_spline(int law, int n, double eps, double rs, double* diff, doubl
         diff[3 * i + 2] = -(0.5 * (fpp * r * r - 3 * fp * r + 3 * fr) / r / r / r / r / r);
     }
 #undef LAW
-    for (int i = 1; i < ORC_NSPLINE - 1; i++)
+    for (int i = tf ? 0 : 1; i < ORC_NSPLINE - 1; i++)
         for (int c = 0; c < 3; c++) slope[3 * i + c] = (diff[3 * (i + 1) + c] - diff[3 * i + c]) / ran;
     for (int c = 0; c < 3; c++) {
-        slope[c] = slope[3 + c];
-        diff[c] = diff[3 + c];
+        if (!tf) {
+            slope[c] = slope[3 + c];
+            diff[c] = diff[3 + c];
+        }
         diff[3 * (ORC_NSPLINE - 1) + c] = 0;
         slope[3 * (ORC_NSPLINE - 1) + c] = 0;
     }
@@ -804,10 +815,10 @@ static void tree_walk3(const rtree* t, double px, double py, double pz, const do
     }
 }
 
-int orc_treecorrection_ref(int n, size_t np, const double* x, const double* y,
-                           const double* z, float mass, const double* diff,
-                           const double* slope, double rsphere, double theta,
-                           double cellw, float fact2, int quirks, float* dv) {
+static int treecorrection_ref_impl(int n, size_t np, const double* x, const double* y,
+                                   const double* z, float mass, const double* diff,
+                                   const double* slope, double rsphere, double theta,
+                                   double cellw, float fact2, int quirks, const uint8_t* sel, float* dv) {
     const double L = n;
     const double width = cellw; /* ghost width = cell width >= rsphere */
     if (cellw < rsphere) return -1;
@@ -870,6 +881,7 @@ int orc_treecorrection_ref(int n, size_t np, const double* x, const double* y,
             int npos = 0;
             for (long q = head[ii]; q >= 0; q = next[q]) {
                 if (gid[q] < 0) continue;
+                if (sel && !sel[gid[q]]) continue;
                 tid[npos] = gid[q];
                 tpos[3 * npos] = gx[q] - cx; tpos[3 * npos + 1] = gy[q] - cy; tpos[3 * npos + 2] = gz[q] - cz;
                 npos++;
@@ -919,6 +931,24 @@ int orc_treecorrection_ref(int n, size_t np, const double* x, const double* y,
     return status;
 }
 
+int orc_treecorrection_ref(int n, size_t np, const double* x, const double* y,
+                           const double* z, float mass, const double* diff,
+                           const double* slope, double rsphere, double theta,
+                           double cellw, float fact2, int quirks, float* dv) {
+    return treecorrection_ref_impl(n, np, x, y, z, mass, diff, slope, rsphere, theta, cellw, fact2, quirks, NULL, dv);
+}
+
+/* The same for the targets with sel[p] != 0 only (cells without such targets build no tree;
+ * the kicks of the other particles stay zero). The DirectSummation branch (fewer than 16
+ * sources) keeps quirk D3 off: it adds into the listed target itself. */
+int orc_treecorrection_ref_sel(int n, size_t np, const double* x, const double* y,
+                               const double* z, float mass, const double* diff,
+                               const double* slope, double rsphere, double theta,
+                               double cellw, float fact2, int quirks, const uint8_t* sel, float* dv) {
+    return treecorrection_ref_impl(n, np, x, y, z, mass, diff, slope, rsphere, theta, cellw, fact2,
+                                   quirks & ~ORC_QUIRK_D3, sel, dv);
+}
+
 /* The reference quadrupole node interaction (AddNode) for a node built from the given
  * sources (monopole + quad about the COM as divide_node3 computes them). */
 void orc_node_accel(const double* diff, const double* slope, double rcut,
@@ -957,12 +987,20 @@ void orc_node_accel(const double* diff, const double* slope, double rcut,
  * Ewald sum: a_i = -G sum_j m_j [ sum_n r (erfc(a r) + 2 a r/sqrt(pi) e^{-a^2 r^2}) / r^3
  *            + (4 pi / L^3) sum_{k != 0} k / k^2 exp(-k^2 / 4a^2) sin(k . r_ij) ],
  * r = x_i - x_j + n L. Independent of alpha when converged.
+ * The reciprocal sum uses the structure factor S(k) = sum_j m_j exp(i k.x_j):
+ * sum_j m_j sin(k.(x_i - x_j)) = sin(k.x_i) Re S - cos(k.x_i) Im S, and the modes k and -k
+ * contribute equally, so half the modes are summed twice. Real-space terms with a r > 6
+ * (erfc(6) = 2e-17) are skipped.
+ * With eps > 0 the minimum-image pair force is Plummer-softened instead of Newtonian (the law
+ * the TreePM split reproduces at short range): -G m_j [1/(r^2+eps^2)^{3/2} - 1/r^3] r is added
+ * for every minimum-image pair closer than 80 eps.
  * ------------------------------------------------------------------------------------ */
-void orc_ewald_accel(size_t np, const double* x, const double* y, const double* z,
-                     const double* m, double L, double G, double alpha, int nreal, int kmax,
-                     double* acc) {
-    double sqpi = sqrt(M_PI);
-#pragma omp parallel for schedule(dynamic, 4)
+void orc_ewald_accel_soft(size_t np, const double* x, const double* y, const double* z,
+                          const double* m, double L, double G, double alpha, int nreal, int kmax,
+                          double eps, double* acc) {
+    const double sqpi = sqrt(M_PI);
+    const double rmax = 6.0 / alpha;
+#pragma omp parallel for schedule(dynamic, 16)
     for (long i = 0; i < (long)np; i++) {
         double ax = 0, ay = 0, az = 0;
         for (size_t j = 0; j < np; j++) {
@@ -973,26 +1011,67 @@ void orc_ewald_accel(size_t np, const double* x, const double* y, const double* 
                         if ((size_t)i == j && a == 0 && b == 0 && c == 0) continue;
                         double dx = dx0 + a * L, dy = dy0 + b * L, dz = dz0 + c * L;
                         double r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 > rmax * rmax) continue;
                         double r = sqrt(r2);
                         double f = (erfc(alpha * r) + 2 * alpha * r / sqpi * exp(-alpha * alpha * r2)) / (r2 * r);
                         ax -= G * m[j] * f * dx; ay -= G * m[j] * f * dy; az -= G * m[j] * f * dz;
                     }
+            if (eps > 0 && (size_t)i != j) {
+                double r2 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+                /* beyond 80 eps the correction is below 2.4e-4 of the pair's Newtonian force */
+                if (r2 > 6400.0 * eps * eps) continue;
+                double r = sqrt(r2);
+                double fs = 1.0 / pow(r2 + eps * eps, 1.5) - 1.0 / (r2 * r);
+                ax -= G * m[j] * fs * dx0; ay -= G * m[j] * fs * dy0; az -= G * m[j] * fs * dz0;
+            }
         }
-        double kf = 2 * M_PI / L;
-        for (int a = -kmax; a <= kmax; a++)
-            for (int b = -kmax; b <= kmax; b++)
-                for (int c = -kmax; c <= kmax; c++) {
-                    if (a == 0 && b == 0 && c == 0) continue;
-                    if (a * a + b * b + c * c > kmax * kmax) continue;
-                    double kx = kf * a, ky = kf * b, kz = kf * c;
-                    double k2 = kx * kx + ky * ky + kz * kz;
-                    double w = 4 * M_PI / (L * L * L) * exp(-k2 / (4 * alpha * alpha)) / k2;
-                    double s = 0;
-                    for (size_t j = 0; j < np; j++) s += m[j] * sin(kx * (x[i] - x[j]) + ky * (y[i] - y[j]) + kz * (z[i] - z[j]));
-                    ax -= G * w * kx * s; ay -= G * w * ky * s; az -= G * w * kz * s;
-                }
         acc[3 * i] = ax; acc[3 * i + 1] = ay; acc[3 * i + 2] = az;
     }
+    /* half space of modes: c > 0, or c == 0 and b > 0, or c == b == 0 and a > 0 */
+    int nk = 0;
+    int* ks = (int*)malloc(3 * (size_t)(2 * kmax + 1) * (2 * kmax + 1) * (2 * kmax + 1) * sizeof(int));
+    for (int c = 0; c <= kmax; c++)
+        for (int b = -kmax; b <= kmax; b++)
+            for (int a = -kmax; a <= kmax; a++) {
+                if (c == 0 && (b < 0 || (b == 0 && a <= 0))) continue;
+                if (a * a + b * b + c * c > kmax * kmax) continue;
+                ks[3 * nk] = a; ks[3 * nk + 1] = b; ks[3 * nk + 2] = c;
+                nk++;
+            }
+    double* sre = (double*)calloc((size_t)nk, sizeof(double));
+    double* sim = (double*)calloc((size_t)nk, sizeof(double));
+    const double kf = 2 * M_PI / L;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int q = 0; q < nk; q++) {
+        double kx = kf * ks[3 * q], ky = kf * ks[3 * q + 1], kz = kf * ks[3 * q + 2];
+        double cr = 0, ci = 0;
+        for (size_t j = 0; j < np; j++) {
+            double ph = kx * x[j] + ky * y[j] + kz * z[j];
+            cr += m[j] * cos(ph);
+            ci += m[j] * sin(ph);
+        }
+        sre[q] = cr; sim[q] = ci;
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < (long)np; i++) {
+        double ax = 0, ay = 0, az = 0;
+        for (int q = 0; q < nk; q++) {
+            double kx = kf * ks[3 * q], ky = kf * ks[3 * q + 1], kz = kf * ks[3 * q + 2];
+            double k2 = kx * kx + ky * ky + kz * kz;
+            double w = 2.0 * 4 * M_PI / (L * L * L) * exp(-k2 / (4 * alpha * alpha)) / k2;
+            double ph = kx * x[i] + ky * y[i] + kz * z[i];
+            double s = sin(ph) * sre[q] - cos(ph) * sim[q];
+            ax -= G * w * kx * s; ay -= G * w * ky * s; az -= G * w * kz * s;
+        }
+        acc[3 * i] += ax; acc[3 * i + 1] += ay; acc[3 * i + 2] += az;
+    }
+    free(ks); free(sre); free(sim);
+}
+
+void orc_ewald_accel(size_t np, const double* x, const double* y, const double* z,
+                     const double* m, double L, double G, double alpha, int nreal, int kmax,
+                     double* acc) {
+    orc_ewald_accel_soft(np, x, y, z, m, L, G, alpha, nreal, kmax, 0.0, acc);
 }
 
 /* ------------------------------------------------------------------------------------

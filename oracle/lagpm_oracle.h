@@ -50,8 +50,11 @@ void   orc_fda4_kick_f64(int n, const double* phi, size_t np, const double* x, c
 void   orc_pm_accel(int n, size_t np, const double* x, const double* y, const double* z,
                     double mass, double efold, int deconvolve, int prec, double* acc /*3*np*/);
 
-/* Force tables, OST/Model4TreeForce.c:48-116 (law 0) or erfc law (law 1). */
+/* Force tables, OST/Model4TreeForce.c:48-116 (law 0), erfc law (law 1), or
+ * ForceLaw/treeforce.c:134-186 (law 2: eps fixed at 0.1, step 0.2 bin, entry 0 zero). */
 void   orc_force_spline(int law, int n, double eps, double rs, double* diff, double* slope);
+/* The same with the derivative step dr_bins (in bins) chosen by the caller. */
+void   orc_force_spline_dr(int law, int n, double eps, double rs, double dr_bins, double* diff, double* slope);
 double orc_pmcf(int n, double eps, double r);
 double orc_erfc_fr(int n, double eps, double rs, double r);
 float  orc_point_grav(const double* diff, const double* slope, float dist);
@@ -73,6 +76,13 @@ int    orc_treecorrection_ref(int n, size_t np, const double* x, const double* y
                               const double* slope, double rsphere, double theta,
                               double cellw, float fact2, int quirks, float* dv /* 3*np */);
 
+/* The same kicks for the particles with sel[p] != 0 only (others get 0). */
+int    orc_treecorrection_ref_sel(int n, size_t np, const double* x, const double* y,
+                                  const double* z, float mass, const double* diff,
+                                  const double* slope, double rsphere, double theta,
+                                  double cellw, float fact2, int quirks, const uint8_t* sel,
+                                  float* dv /* 3*np */);
+
 /* Quadrupole node interaction of the reference (AddNode without D1), for pinning. */
 void   orc_node_accel(const double* diff, const double* slope, double rcut,
                       size_t ns, const double* sx, const double* sy, const double* sz,
@@ -82,6 +92,11 @@ void   orc_node_accel(const double* diff, const double* slope, double rcut,
 void   orc_ewald_accel(size_t np, const double* x, const double* y, const double* z,
                        const double* m, double L, double G, double alpha, int nreal, int kmax,
                        double* acc);
+/* The same with the minimum-image pair force Plummer-softened (eps > 0): the periodic force of
+ * the TreePM split's target law, r / (r^2 + eps^2)^{3/2} at short range. */
+void   orc_ewald_accel_soft(size_t np, const double* x, const double* y, const double* z,
+                            const double* m, double L, double G, double alpha, int nreal, int kmax,
+                            double eps, double* acc);
 
 /* Kick/drift factors, Cosmos/CosmosEvolFactor.c:230-256, FP64 quadrature. */
 typedef struct orc_evolfact {

@@ -51,13 +51,17 @@ def lib():
         "orc_fda4_kick_f64": (None, [i, _D, sz, _D, _D, _D, d, _D]),
         "orc_pm_accel": (None, [i, sz, _D, _D, _D, d, d, i, i, _D]),
         "orc_force_spline": (None, [i, i, d, d, _D, _D]),
+        "orc_force_spline_dr": (None, [i, i, d, d, d, _D, _D]),
         "orc_pmcf": (d, [i, d, d]), "orc_erfc_fr": (d, [i, d, d, d]),
         "orc_point_grav": (f, [_D, _D, f]),
         "orc_pp_direct": (None, [i, sz, _D, _D, _D, f, _D, _D, d, sz, _I64, _D]),
         "orc_pp_direct_analytic": (None, [i, i, d, d, sz, _D, _D, _D, d, d, sz, _I64, _D]),
         "orc_treecorrection_ref": (i, [i, sz, _D, _D, _D, f, _D, _D, d, d, d, f, i, _F]),
+        "orc_treecorrection_ref_sel": (i, [i, sz, _D, _D, _D, f, _D, _D, d, d, d, f, i,
+                                           ctypes.POINTER(ctypes.c_uint8), _F]),
         "orc_node_accel": (None, [_D, _D, d, sz, _D, _D, _D, _F, _D, i, _F]),
         "orc_ewald_accel": (None, [sz, _D, _D, _D, _D, d, d, d, i, i, _D]),
+        "orc_ewald_accel_soft": (None, [sz, _D, _D, _D, _D, d, d, d, i, i, d, _D]),
         "orc_hofez": (d, [d, d, d, d]), "orc_ap": (d, [d, d, d, d]),
         "orc_fact1": (d, [d, d, d, d, d]), "orc_fact2": (d, [d, d, d, i, d, d]),
         "orc_evolfactor": (None, [d, d, d, i, d, d, ctypes.POINTER(OrcEvolFact)]),
@@ -177,10 +181,15 @@ def pm_accel(n, pos, mass=1.0, efold=0.9, deconvolve=0, prec=0):
     return acc.reshape(-1, 3)
 
 
-def force_spline(law, n, eps, rs):
+def force_spline(law, n, eps, rs, dr_bins=None):
+    """Force tables (4096, 3) diff and slope; law 0 reference, 1 erfc, 2 ForceLaw/treeforce.c.
+    dr_bins overrides the derivative step (in bins)."""
     diff = np.empty((4096, 3))
     slope = np.empty((4096, 3))
-    lib().orc_force_spline(law, n, eps, rs, _dp(diff), _dp(slope))
+    if dr_bins is None:
+        lib().orc_force_spline(law, n, eps, rs, _dp(diff), _dp(slope))
+    else:
+        lib().orc_force_spline_dr(law, n, eps, rs, dr_bins, _dp(diff), _dp(slope))
     return diff, slope
 
 
@@ -221,16 +230,28 @@ def pp_direct_analytic(law, n, eps, rs, pos, rcut, mass=1.0, targets=None):
     return acc.reshape(-1, 3)
 
 
-def treecorrection_ref(n, pos, diff, slope, rsphere, theta, cellw=None, mass=1.0, fact2=1.0, quirks=0):
+def treecorrection_ref(n, pos, diff, slope, rsphere, theta, cellw=None, mass=1.0, fact2=1.0, quirks=0,
+                       targets=None):
+    """Reference tree correction kicks (N, 3); with `targets` (indices), the kicks of those
+    particles only, shape (len(targets), 3), computed without the trees of other cells."""
     x, y, z = split(pos)
     diff, slope = _d(diff), _d(slope)
     cellw = max(4.0, rsphere) if cellw is None else cellw
     dv = np.empty(3 * len(x), dtype=np.float32)
-    rc = lib().orc_treecorrection_ref(n, len(x), _dp(x), _dp(y), _dp(z), mass, _dp(diff), _dp(slope),
-                                      rsphere, theta, cellw, fact2, quirks, dv.ctypes.data_as(_F))
+    if targets is None:
+        rc = lib().orc_treecorrection_ref(n, len(x), _dp(x), _dp(y), _dp(z), mass, _dp(diff), _dp(slope),
+                                          rsphere, theta, cellw, fact2, quirks, dv.ctypes.data_as(_F))
+    else:
+        sel = np.zeros(len(x), dtype=np.uint8)
+        sel[np.asarray(targets)] = 1
+        rc = lib().orc_treecorrection_ref_sel(n, len(x), _dp(x), _dp(y), _dp(z), mass, _dp(diff), _dp(slope),
+                                              rsphere, theta, cellw, fact2, quirks,
+                                              sel.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                              dv.ctypes.data_as(_F))
     if rc != 0:
         raise RuntimeError("orc_treecorrection_ref failed")
-    return dv.reshape(-1, 3).astype(np.float64)
+    out = dv.reshape(-1, 3).astype(np.float64)
+    return out if targets is None else out[np.asarray(targets)]
 
 
 def node_accel(diff, slope, rcut, src, masses, target, quirks=0):
@@ -243,12 +264,23 @@ def node_accel(diff, slope, rcut, src, masses, target, quirks=0):
     return out.astype(np.float64)
 
 
-def ewald_accel(pos, masses, L, G, alpha, nreal=2, kmax=10):
+def ewald_accel(pos, masses, L, G, alpha, nreal=2, kmax=10, eps=0.0):
+    """Periodic force by Ewald summation; eps > 0 softens the minimum-image pair force (Plummer)."""
     x, y, z = split(pos)
     m = _d(masses)
     acc = np.empty(3 * len(x))
-    lib().orc_ewald_accel(len(x), _dp(x), _dp(y), _dp(z), _dp(m), L, G, alpha, nreal, kmax, _dp(acc))
+    lib().orc_ewald_accel_soft(len(x), _dp(x), _dp(y), _dp(z), _dp(m), L, G, alpha, nreal, kmax, eps, _dp(acc))
     return acc.reshape(-1, 3)
+
+
+def ewald_converged(pos, n, eps=0.1, tol=1e-9):
+    """Plummer-softened periodic force in code units (box side n cells, G = 1/n^2, unit masses),
+    with the Ewald parameters chosen so both sums are converged to about tol: real space on the
+    minimum image only (alpha n/2 > 6), reciprocal modes to exp(-k^2/4 alpha^2) < tol."""
+    L = float(n)
+    alpha = 12.5 / L
+    kmax = int(np.ceil(np.sqrt(-4 * alpha ** 2 * np.log(tol)) * L / (2 * np.pi)))
+    return ewald_accel(pos, np.ones(len(pos)), L, 1.0 / (L * L), alpha, nreal=0, kmax=kmax, eps=eps)
 
 
 def evolfactor(omep, omeplam, amax, n, anow, astep):

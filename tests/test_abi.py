@@ -35,7 +35,9 @@ def test_abi_version():
 
 
 def test_force_spline_matches_oracle_tables():
-    for law, p in ((0, lagpm.reference_params(256)), (1, lagpm.make_params(256, efold=2.5))):
+    tf = lagpm.reference_params(256)
+    tf.law, tf.epsilon = lagpm.LAW_TREEFORCE, 0.25  # the variant fixes its softening at 0.1
+    for law, p in ((0, lagpm.reference_params(256)), (1, lagpm.make_params(256, efold=2.5)), (2, tf)):
         diff, slope = lagpm.i_force_spline(p)
         od, os_ = orc.force_spline(law, 256, p.epsilon, p.efold / 2)
         assert np.allclose(diff, od, rtol=1e-13, atol=0)

@@ -66,6 +66,22 @@ def test_tree_reference_law_theta_zero(ctx):
     assert rel_rms(a, ref) < 3e-5
 
 
+def test_tree_treeforce_variant_law_theta_zero(ctx):
+    """The ForceLaw/treeforce.c table (softening 0.1, derivative step 0.2 bin, entry 0 zero) through
+    the walk: theta -> 0 gives the pair sum with the oracle's table of that law."""
+    n = 32
+    pos = workload(n, "clustered", seed=6)
+    p = lagpm.reference_params(n, theta=1e-6)
+    p.law = lagpm.LAW_TREEFORCE
+    a, _ = tree_accel(ctx, p, pos)
+    diff, slope = orc.force_spline(2, n, 0.1, 0.45)
+    ref = orc.pp_direct(n, pos, diff, slope, p.rsphere)
+    assert rel_rms(a, ref) < 3e-5
+    # a different law from the reference one: the tables differ at the 1e-6 level near r = 0
+    d0, s0 = orc.force_spline(0, n, 0.1, 0.45)
+    assert not np.array_equal(diff, d0)
+
+
 @pytest.mark.parametrize("kind", ["lattice", "clustered"])
 def test_tree_theta_half_fp32_vs_fp64_and_exact(ctx, kind):
     n = 48
@@ -83,7 +99,10 @@ def test_tree_theta_half_fp32_vs_fp64_and_exact(ctx, kind):
     print(f"theta=0.5 {kind}: fp32 vs fp64 {e_prec:.2e}; gpu tree vs exact {e_tree:.2e}; "
           f"reference walk vs exact {e_ref:.2e}; pp {st32.n_pp} pc {st32.n_pc}", flush=True)
     assert e_prec < 1e-4
-    assert e_tree < max(2 * e_ref, 1e-3)
+    # group-conservative opening: each target sees a stricter opening test than the reference's
+    # per-target test, so the GPU tree is at least as accurate as the reference walk
+    assert e_tree <= 1.02 * e_ref
+    assert rel_rms(a32, exact) <= 1.02 * e_ref
     assert st32.n_pc > 0
 
 
@@ -95,7 +114,9 @@ def test_tree_reference_mode_theta_03_close_to_reference_walk(ctx):
     diff, slope = orc.force_spline(0, n, p.epsilon, 0.45)
     exact = orc.pp_direct(n, pos, diff, slope, p.rsphere)
     ref_walk = orc.treecorrection_ref(n, pos, diff, slope, p.rsphere, 0.3)
-    assert rel_rms(a, exact) < max(2 * rel_rms(ref_walk, exact), 1e-4)
+    e_gpu, e_ref = rel_rms(a, exact), rel_rms(ref_walk, exact)
+    print(f"reference mode theta=0.3: gpu tree vs exact {e_gpu:.2e}; reference walk vs exact {e_ref:.2e}", flush=True)
+    assert e_gpu <= 1.02 * e_ref
     assert rel_rms(a, ref_walk) < 5e-3
 
 
@@ -181,3 +202,69 @@ def test_tree_large_box_sampled_targets_are_the_direct_sum(ctx):
     # ~6.7e3 FP32 table pairs per target at this density -> ~4e-5; one pair missing from one of the
     # 256 targets would show at ~8e-4, and particles out of order break the tree at O(1)
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("prec", [lagpm.PREC_FP32, lagpm.PREC_FP64])
+def test_node_moments_match_brute_force_and_leaves_partition_the_particles(ctx, prec):
+    """Make_Tree3 / divide_node3 (OST/ost.ver2.c:36-144) pinned node by node: every node's mass,
+    centre of mass, quadrupole about it (Qxx..Qyz, trQ) and rmax against FP64 sums over its own
+    particles; the leaves partition the particles; a node's particles lie in its octree cell and are
+    exactly its children's particles. The FP32 tree sums with Kahan compensation, so its moments
+    stay within a few float ulps of the box scale."""
+    n = 32
+    pos = np.mod(workload(n, "clustered", seed=21), n)
+    p = lagpm.make_params(n, theta=0.5, precision=prec)
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.upload_pos(pos)
+        b.treecorrection(lagpm.evolfact(0.0))
+        t = b.tree_nodes()
+    info, com, quad, order = t["info"], t["com"], t["quad"], t["order"].astype(np.int64)
+    assert np.array_equal(np.sort(order), np.arange(len(pos)))
+    sp = pos[order]  # positions in the box's order
+    first, count, nchild, level = info[:, 0], info[:, 1], info[:, 2], info[:, 3]
+    print(f"nodes {len(info)} (leaves {(nchild == 0).sum()}), levels {level.min()}..{level.max()}", flush=True)
+    # leaves partition [0, N)
+    leaf = nchild == 0
+    cover = np.zeros(len(pos), dtype=np.int64)
+    np.add.at(cover, np.concatenate([np.arange(f, f + c) for f, c in zip(first[leaf], count[leaf])]), 1)
+    assert np.all(cover == 1)
+    # cell roots partition [0, N) as well
+    roots = level == level.min()
+    assert count[roots].sum() == len(pos)
+    tol_x = 2e-5 if prec == lagpm.PREC_FP32 else 1e-10
+    width = n / 2.0 ** level
+    for v in range(len(info)):
+        f, c = first[v], count[v]
+        x = sp[f:f + c]
+        # the node is an octree cell: one cell index per axis at its level
+        cell = np.floor(x / width[v])
+        assert np.all(cell == cell[0]), v
+        cm = x.mean(axis=0)
+        d = x - cm
+        q = np.array([(d[:, 0] ** 2).sum(), (d[:, 1] ** 2).sum(), (d[:, 2] ** 2).sum(), (d[:, 0] * d[:, 1]).sum(),
+                      (d[:, 0] * d[:, 2]).sum(), (d[:, 1] * d[:, 2]).sum()])
+        rmax = np.sqrt((d ** 2).sum(axis=1).max())
+        assert com[v, 3] == c * p.mass
+        assert np.max(np.abs(com[v, :3] - cm)) <= tol_x, (v, com[v], cm)
+        qscale = c * (width[v] ** 2)
+        assert np.max(np.abs(quad[v, :6] - q)) <= (1e-5 if prec == lagpm.PREC_FP32 else 1e-12) * qscale, (v, quad[v], q)
+        assert abs(quad[v, 6] - q[:3].sum()) <= (1e-5 if prec == lagpm.PREC_FP32 else 1e-12) * qscale
+        if nchild[v] == 0 or c <= 128:
+            assert abs(quad[v, 7] - rmax) <= tol_x, (v, quad[v, 7], rmax)
+        else:
+            assert quad[v, 7] >= rmax - tol_x  # an upper bound of the largest distance
+    # a parent's particles are exactly its children's: every node below the cell level lies inside
+    # one node of the level above, and per parent the children's counts add up to its count
+    kids = np.zeros(len(info), dtype=np.int64)
+    kid_particles = np.zeros(len(info), dtype=np.int64)
+    for lv in range(level.min() + 1, level.max() + 1):
+        up = np.nonzero(level == lv - 1)[0]
+        up = up[np.argsort(first[up])]
+        for u in np.nonzero(level == lv)[0]:
+            k = up[np.searchsorted(first[up], first[u], side="right") - 1]
+            assert first[k] <= first[u] and first[u] + count[u] <= first[k] + count[k], u
+            kids[k] += 1
+            kid_particles[k] += count[u]
+    assert np.array_equal(kids, nchild)
+    inner = nchild > 0
+    assert np.array_equal(kid_particles[inner], count[inner])

@@ -27,6 +27,40 @@ def test_table_nodes_equal_law_over_r(law):
     assert np.all(diff[0] == diff[1]) and np.all(slope[0] == slope[1])
 
 
+def test_treeforce_variant_table():
+    """ForceLaw/treeforce.c:134-186: the fitted law with the softening fixed at EPSILON = 0.1 (the
+    Epsilon argument is not used by its pmfit), derivatives by central differences of step 0.2 bin,
+    entry 0 zero with the slope up to entry 1."""
+    n = 64
+    ran = 6.0 / 4096
+    d2, s2 = orc.force_spline(2, n, 0.1, 0.45)
+    d2b, s2b = orc.force_spline(2, n, 0.37, 0.45)
+    assert np.array_equal(d2, d2b) and np.array_equal(s2, s2b)  # eps is not an input of this law
+    assert np.all(d2[0] == 0) and np.allclose(s2[0], d2[1] / ran, rtol=1e-15)
+    assert np.all(d2[4095] == 0) and np.all(s2[4095] == 0)
+    for i in (1, 50, 700, 2600):
+        r = i * ran
+        assert np.isclose(d2[i, 0], -orc.pmcf(n, 0.1, r) / r, rtol=1e-14)
+    # the derivative step: a central difference of step h misses f' by h^2 f'''(r) / 6, so the
+    # step-0.2 table's f' error is four times the step-0.1 table's (f' recovered from diff1 and f)
+    d0, _ = orc.force_spline(0, n, 0.1, 0.45)
+
+    def fprime_from_table(d, i):
+        r = i * ran
+        fr = orc.pmcf(n, 0.1, r)
+        return (fr - 2 * r ** 3 * d[i, 1]) / r
+
+    for i in (200, 400, 800):
+        r, h = i * ran, 1e-3
+        f = lambda x: orc.pmcf(n, 0.1, x)  # noqa: E731
+        exact = (8 * (f(r + h) - f(r - h)) - (f(r + 2 * h) - f(r - 2 * h))) / (12 * h)  # O(h^4)
+        ratio = (fprime_from_table(d2, i) - exact) / (fprime_from_table(d0, i) - exact)
+        assert abs(ratio - 4.0) < 0.05, (i, ratio)
+    # the step parameter with the reference law's entry rules gives the reference table
+    d0b, _ = orc.force_spline(0, n, 0.1, 0.45, dr_bins=0.1)
+    assert np.array_equal(d0b, d0)
+
+
 def test_erfc_law_is_newton_times_split_complement():
     n, rs = 64, 1.25
     for r in (0.5, 1.0, 2.0, 4.0):
