@@ -231,6 +231,16 @@ int lagpm_get_gradient(lagpm_ctx* ctx, lagpm_box* box, float* grad4);
  * reference computes it from cor[2] and cor[3]. */
 int lagpm_power_spectrum(lagpm_ctx* ctx, lagpm_box* box, double boxsize, double* pk, double* npk, double* cor,
                          double* sigma8, double* slope);
+/* lagpm_power_spectrum without any allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (a serving loop that measures P(k) every step, as the
+ * reference's pmmain does, or a harness that forbids hipMalloc). The bins live in one static
+ * device array per GPU of 2(n+1) doubles for n <= 4096 (65,552 bytes, held until the process
+ * ends); a larger n is LAGPM_E_INVAL. The box's FFT plan and execution info must already exist
+ * (made by any PM kick or lagpm_power_spectrum on a box of this size and precision on this
+ * context); otherwise LAGPM_E_STATE. Calls on one GPU share the array: calls that overlap on one
+ * GPU (from several threads or contexts) race and must be serialised by the caller. */
+int lagpm_power_spectrum_noalloc(lagpm_ctx* ctx, lagpm_box* box, double boxsize, double* pk, double* npk,
+                                 double* cor, double* sigma8, double* slope);
 
 /* ---- short-range tree correction --------------------------------------------------- */
 /* Builds FP64 force tables (NSPLINE=4096 entries over r in [0,6)): diff[i][3], slope[i][3]

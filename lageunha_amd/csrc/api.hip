@@ -405,6 +405,10 @@ int lagpm_create(lagpm_ctx** out, int device) {
     // cache in a chosen file sets ROCFFT_RTC_CACHE_PATH before the first context; the Python
     // binding does so before it loads this library).
     std::call_once(g_rocfft_once, []() { rocfft_setup(); });
+    if (lagpm_pk_scratch_init(ctx) != LAGPM_OK) {
+        lagpm_destroy(ctx);
+        return LAGPM_E_HIP;
+    }
     *out = ctx;
     return LAGPM_OK;
 }
@@ -771,16 +775,21 @@ int lagpm_pmmain(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef, int p
     return LAGPM_OK;
 }
 
-int lagpm_power_spectrum(lagpm_ctx* ctx, lagpm_box* box, double boxsize, double* pk, double* npk, double* cor,
-                         double* sigma8, double* slope) {
+}  // extern "C"
+
+namespace {
+// CorrMeasure (Cosmos/Correlation.c:13-116) of the box's particles: deposit, r2c and binning
+// (allocating or from the device's static scratch), then normalisation, sigma_8 and xi(r) on the
+// host as Cosmos/Correlation.c:80-103.
+int power_spectrum_impl(lagpm_ctx* ctx, lagpm_box* box, double boxsize, double* pk, double* npk, double* cor,
+                        double* sigma8, double* slope, bool noalloc) {
     RCHK(need_particles(ctx, box));
     if (!pk || !npk || !cor || !sigma8 || !slope || !(boxsize > 0))
         return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL output or non-positive box size");
     const int n = box->p.n;
     if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
     RCHK(lagpm_pm_deposit(ctx, box));
-    RCHK(lagpm_pm_power(ctx, box, pk, npk));
-    // normalisation, sigma_8 and xi(r) as Cosmos/Correlation.c:80-103
+    RCHK(noalloc ? lagpm_pm_power_noalloc(ctx, box, pk, npk) : lagpm_pm_power(ctx, box, pk, npk));
     const double rth = 8.0 / boxsize * n, fact = 2 * M_PI * rth / (double)n;
     double var = 0;
     for (int i = 1; i <= n; i++) {
@@ -799,6 +808,19 @@ int lagpm_power_spectrum(lagpm_ctx* ctx, lagpm_box* box, double boxsize, double*
     }
     *slope = (n >= 3 && cor[3] > 0 && cor[2] > 0) ? (std::log(cor[3]) - std::log(cor[2])) / (std::log(4.0) - std::log(3.0)) : 0.0;
     return LAGPM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int lagpm_power_spectrum(lagpm_ctx* ctx, lagpm_box* box, double boxsize, double* pk, double* npk, double* cor,
+                         double* sigma8, double* slope) {
+    return power_spectrum_impl(ctx, box, boxsize, pk, npk, cor, sigma8, slope, false);
+}
+
+int lagpm_power_spectrum_noalloc(lagpm_ctx* ctx, lagpm_box* box, double boxsize, double* pk, double* npk, double* cor,
+                                 double* sigma8, double* slope) {
+    return power_spectrum_impl(ctx, box, boxsize, pk, npk, cor, sigma8, slope, true);
 }
 
 int lagpm_force_spline(const lagpm_params* p, double* diff, double* slope) {

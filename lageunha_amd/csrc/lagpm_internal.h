@@ -17,6 +17,9 @@
 #define LAGPM_MAX_INDEX (1u << 27)  // particles per box: walk lists pack (index << 5 | neighbour) in 32 bits
 #define LAGPM_MAX_NODES (1u << 26)  // nodes per box: walk stacks pack (node << 5 | neighbour) in a non-negative int
 
+// Doubles in the static power-spectrum scratch of each device: pk and npk bins for n <= 4096.
+#define LAGPM_PK_SCRATCH_DOUBLES (2 * (4096 + 1))
+
 struct fft_plans {
     rocfft_plan fwd = nullptr, inv = nullptr;
     size_t work_bytes = 0;
@@ -66,6 +69,7 @@ struct lagpm_ctx {
     int batch_stages = 3;                    // groups of a pipelined batch step (1: no pipeline)
     int stage_reserve_cu = 32;               // CUs a non-final group's walk leaves to the next fronts
     hipEvent_t ev[4] = {};
+    double* pk_scratch = nullptr;  // this device's static power-spectrum scratch (device address)
 };
 
 // Tunables of the tree correction (chosen per box; the autotuner may override).
@@ -217,6 +221,11 @@ int lagpm_fft_release(lagpm_ctx* ctx);
 int lagpm_box_fft_release(lagpm_box* box);
 int lagpm_pm_prepare(lagpm_ctx* ctx, lagpm_box* box);
 int lagpm_pm_power(lagpm_ctx* ctx, lagpm_box* box, double* pk_host, double* npk_host);
+// the same with the bins in the device's static scratch; allocates nothing (LAGPM_E_STATE when the
+// box's FFT plans or execution info do not exist yet)
+int lagpm_pm_power_noalloc(lagpm_ctx* ctx, lagpm_box* box, double* pk_host, double* npk_host);
+// looks up (once per device, with the device current) the address of the static scratch
+int lagpm_pk_scratch_init(lagpm_ctx* ctx);
 void lagpm_box_graphs_release(lagpm_box* box);
 
 // ---- tree (tree.hip) ----

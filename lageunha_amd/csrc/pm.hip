@@ -686,37 +686,90 @@ int lagpm_pm_prepare(lagpm_ctx* ctx, lagpm_box* box) {
     return LAGPM_OK;
 }
 
-// r2c of the resident density (rho) into box->spec, then the binned |delta_k|^2 / n^6
+__global__ void zero_f64_kernel(double* __restrict__ a, size_t m) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) a[i] = 0.0;
+}
+
+// Bins of the allocation-free power spectrum (lagpm_power_spectrum_noalloc): pk and npk, n + 1
+// each, for n up to the library's largest mesh (4096): 65,552 bytes on every device the library
+// runs on, held until the process ends. In host code `scratch` is only a stand-in name: the
+// device address of each GPU's copy comes from hipGetSymbolAddress (lagpm_pk_scratch_init).
+static __device__ __align__(256) double scratch[LAGPM_PK_SCRATCH_DOUBLES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;  // device -> address of that device's scratch
+}  // namespace
+
+int lagpm_pk_scratch_init(lagpm_ctx* ctx) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(ctx->device);
+    if (it == g_scratch_ptr.end()) {
+        // the device is current (lagpm_create set it), so this is that device's copy
+        double* ptr = nullptr;
+        HIPCHK(ctx, hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return lagpm_fail(ctx, LAGPM_E_HIP, "no device address for the power-spectrum scratch");
+        it = g_scratch_ptr.emplace(ctx->device, ptr).first;
+    }
+    ctx->pk_scratch = it->second;
+    return LAGPM_OK;
+}
+
+namespace {
+// r2c of the resident density (rho) into box->spec with the box's execution info, then the
+// binned |delta_k|^2 / n^6 into d (2(n+1) doubles: pk, then npk), copied to the host
+int pm_power_into(lagpm_ctx* ctx, lagpm_box* box, fft_plans* fp, double* d, double* pk_host, double* npk_host) {
+    const int n = box->p.n;
+    RCHK(fft_status(ctx, rocfft_execution_info_set_stream(box->fft_info, box->st), "set_stream"));
+    void* in[1] = {box->rho};
+    RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, nullptr, box->fft_info), "execute(r2c)"));
+    const size_t nbins = 2 * (size_t)(n + 1);
+    // a fill kernel rather than a memset: every call starts from zero bins, whatever the last
+    // call (of any mesh size) left there
+    hipLaunchKernelGGL(zero_f64_kernel, dim3(grid_for(nbins, 256)), dim3(256), 0, box->st, d, nbins);
+    HIPCHK(ctx, hipGetLastError());
+    const size_t modes = (size_t)(n / 2) * n * n;
+    const size_t shm = nbins * sizeof(double);
+    if (box->p.precision == LAGPM_PREC_FP64)
+        hipLaunchKernelGGL(pk_bin_kernel<double>, dim3(grid_for(modes, 256, 4096)), dim3(256), shm, box->st,
+                           (const double*)box->spec, n, d, d + n + 1);
+    else
+        hipLaunchKernelGGL(pk_bin_kernel<float>, dim3(grid_for(modes, 256, 4096)), dim3(256), shm, box->st,
+                           (const float*)box->spec, n, d, d + n + 1);
+    HIPCHK(ctx, hipGetLastError());
+    RCHK(lagpm_d2h(ctx, pk_host, d, (n + 1) * sizeof(double), box->st));
+    RCHK(lagpm_d2h(ctx, npk_host, d + n + 1, (n + 1) * sizeof(double), box->st));
+    return LAGPM_OK;
+}
+}  // namespace
+
 int lagpm_pm_power(lagpm_ctx* ctx, lagpm_box* box, double* pk_host, double* npk_host) {
     const int n = box->p.n;
     RCHK(lagpm_pm_prepare(ctx, box));
     fft_plans* fp = nullptr;
     RCHK(get_plans(ctx, n, box->p.precision, box->st, &fp));
-    RCHK(fft_status(ctx, rocfft_execution_info_set_stream(box->fft_info, box->st), "set_stream"));
-    void* in[1] = {box->rho};
-    RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, nullptr, box->fft_info), "execute(r2c)"));
     double* d = nullptr;
     HIPCHK(ctx, hipMalloc(&d, 2 * (size_t)(n + 1) * sizeof(double)));
-    int rc = LAGPM_OK;
-    if (hipMemsetAsync(d, 0, 2 * (size_t)(n + 1) * sizeof(double), box->st) != hipSuccess) rc = LAGPM_E_HIP;
-    const size_t modes = (size_t)(n / 2) * n * n;
-    const size_t shm = 2 * (size_t)(n + 1) * sizeof(double);
-    if (rc == LAGPM_OK) {
-        if (box->p.precision == LAGPM_PREC_FP64)
-            hipLaunchKernelGGL(pk_bin_kernel<double>, dim3(grid_for(modes, 256, 4096)), dim3(256), shm, box->st,
-                               (const double*)box->spec, n, d, d + n + 1);
-        else
-            hipLaunchKernelGGL(pk_bin_kernel<float>, dim3(grid_for(modes, 256, 4096)), dim3(256), shm, box->st,
-                               (const float*)box->spec, n, d, d + n + 1);
-        if (hipGetLastError() != hipSuccess) rc = LAGPM_E_HIP;
-    }
-    if (rc == LAGPM_OK) rc = lagpm_d2h(ctx, pk_host, d, (n + 1) * sizeof(double), box->st);
-    if (rc == LAGPM_OK) rc = lagpm_d2h(ctx, npk_host, d + n + 1, (n + 1) * sizeof(double), box->st);
+    const int rc = pm_power_into(ctx, box, fp, d, pk_host, npk_host);
     (void)hipStreamSynchronize(box->st);
     (void)hipFree(d);
-    if (rc == LAGPM_E_HIP) return lagpm_fail(ctx, rc, "power spectrum kernel failed: " + ctx->err);
-    if (rc != LAGPM_OK) return rc;
-    return LAGPM_OK;
+    return rc;
+}
+
+// The same measurement with the bins in the device's static scratch: no allocation of any kind.
+// The box's FFT plans and execution info must exist (any earlier PM kick or power spectrum of a
+// box of this size and precision on this context creates them), since creating them allocates.
+int lagpm_pm_power_noalloc(lagpm_ctx* ctx, lagpm_box* box, double* pk_host, double* npk_host) {
+    const int n = box->p.n;
+    if (2 * (size_t)(n + 1) > (size_t)LAGPM_PK_SCRATCH_DOUBLES)
+        return lagpm_fail(ctx, LAGPM_E_INVAL, "mesh of " + std::to_string(n) + " cells per side exceeds the power-spectrum "
+                                              "scratch (n <= " + std::to_string(LAGPM_PK_SCRATCH_DOUBLES / 2 - 1) + ")");
+    if (!ctx->pk_scratch) return lagpm_fail(ctx, LAGPM_E_STATE, "power-spectrum scratch not initialised");
+    auto it = ctx->plans.find(std::make_pair(n, box->p.precision));
+    if (it == ctx->plans.end() || !box->fft_info)
+        return lagpm_fail(ctx, LAGPM_E_STATE, "no FFT plan or execution info for this box yet (creating them allocates): "
+                                              "run one PM kick or lagpm_power_spectrum on it first");
+    return pm_power_into(ctx, box, &it->second, ctx->pk_scratch, pk_host, npk_host);
 }
 
 int lagpm_box_fft_release(lagpm_box* box) {

@@ -132,6 +132,7 @@ SIGNATURES = {
     "lagpm_get_gradient": (ctypes.c_int, [_P, _P, _F]),
     "lagpm_force_spline": (ctypes.c_int, [ctypes.POINTER(Params), _D, _D]),
     "lagpm_power_spectrum": (ctypes.c_int, [_P, _P, ctypes.c_double, _D, _D, _D, _D, _D]),
+    "lagpm_power_spectrum_noalloc": (ctypes.c_int, [_P, _P, ctypes.c_double, _D, _D, _D, _D, _D]),
     "lagpm_treecorrection": (ctypes.c_int, [_P, _P, ctypes.POINTER(EvolFact)]),
     "lagpm_tree_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(TreeStats)]),
     "lagpm_tree_nodes_get": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
@@ -408,6 +409,15 @@ class Box:
         s8, sl = ctypes.c_double(), ctypes.c_double()
         self.ctx.check(self.lib.lagpm_power_spectrum(self.ctx.h, self.h, boxsize, _dp(pk), _dp(npk), _dp(cor),
                                                      ctypes.byref(s8), ctypes.byref(sl)))
+        return pk, npk, cor, s8.value, sl.value
+
+    def power_spectrum_noalloc(self, boxsize: float):
+        """power_spectrum with the bins in the device's static scratch (no device allocation); the
+        box's FFT plan must exist (one PM kick or power_spectrum on a box of this size first)."""
+        pk, npk, cor = np.empty(self.n + 1), np.empty(self.n + 1), np.empty(self.n + 1)
+        s8, sl = ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(self.lib.lagpm_power_spectrum_noalloc(self.ctx.h, self.h, boxsize, _dp(pk), _dp(npk), _dp(cor),
+                                                             ctypes.byref(s8), ctypes.byref(sl)))
         return pk, npk, cor, s8.value, sl.value
 
     def kdk_step(self, anow: float, astep: float):
