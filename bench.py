@@ -38,6 +38,35 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+class ProfilerRegion:
+    """roctxProfilerResume/Pause around the timed steps: under `rocprofv3 --selected-regions` only
+    the timed steps are traced, so a kernel summary covers exactly the launches that `value`
+    times. Outside a profiler (or without the roctx library) both calls do nothing."""
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1"):
+            try:
+                lib = ctypes.CDLL(name)
+                lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
+                lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
+                self.lib = lib
+                break
+            except (OSError, AttributeError):
+                continue
+
+    def __enter__(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+        return self
+
+    def __exit__(self, *exc):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+        return False
+
+
 def rel_errors(a, ref):
     d = np.linalg.norm(a - ref, axis=1)
     r = np.linalg.norm(ref, axis=1)
@@ -60,31 +89,122 @@ def force_at(ctx, lagpm, params, pos, tuning):
     return v, dt
 
 
-def cpu_baseline(args):
-    """The CPU oracle (reference algorithm: TSC, FFT, psolver, FDA4 + per-cell reference tree
-    walk) on the same recipe at a bounded size, timed on this host's cores."""
-    from lageunha_amd import ics
-    from oracle import pyoracle as orc
-    if args.cpu_threads > 0:
-        orc.set_threads(args.cpu_threads)
-    threads = orc.num_threads()
-    n = args.cpu_n
-    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=max(4, int(round(2000 * (n / 256) ** 3))), seed=args.seed)
+def cpu_info():
+    """CPU model and core counts of this host (lscpu's sources: /proc/cpuinfo)."""
+    model, cores = "unknown", set()
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model == "unknown":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None:
+                cores.add((phys, core))
+                phys = core = None
+    except OSError:
+        pass
+    return model, len(cores) or None, os.cpu_count()
+
+
+PHASES = ("TSC", "F-FFT", "psolv", "B-FFT", "4FDA")  # the reference's timers, Cosmos/pmmain.c:447-522
+
+
+def cpu_force_eval(orc, n, pos, prec, theta, targets=None):
+    """One TreePM force evaluation by the CPU oracle (the reference algorithm: TSC, FFT, psolver,
+    FDA4, then the per-cell reference tree walk of the tree correction) with per-phase wall times.
+    prec 1: FP64 mesh chain; 0: the reference's float mesh. The tree walk is the reference's (FP64
+    positions, FP32 pair terms). With `targets`, the tree runs for those particles only."""
     diff, slope = orc.force_spline(1, n, 0.1, 1.25)
+    pm, ph = orc.pm_accel_timed(n, pos, mass=1.0, efold=2.5, deconvolve=1, prec=prec)
     t0 = time.perf_counter()
-    orc.pm_accel(n, pos, mass=1.0, efold=2.5, deconvolve=1, prec=0)
-    t1 = time.perf_counter()
-    orc.treecorrection_ref(n, pos, diff, slope, 5.625, args.theta, cellw=5.625)
-    t2 = time.perf_counter()
-    N = len(pos)
-    return {
-        "value": N / (t2 - t0), "unit": "forces/s", "cores": threads, "kind": "port",
-        "sample": f"oracle restatement of the reference path (TSC, FFT, psolver, FDA4, per-cell reference "
-                  f"tree walk theta={args.theta}) on the same clustered recipe at {n}^3 ({N} particles, "
-                  f"{max(4, int(round(2000 * (n / 256) ** 3)))} halos), one force evaluation; "
-                  f"PM {t1 - t0:.2f} s, tree {t2 - t1:.2f} s, {threads} OpenMP threads",
-        "seconds": t2 - t0,
-    }
+    tr = orc.treecorrection_ref(n, pos, diff, slope, 5.625, theta, cellw=5.625, targets=targets)
+    ph["Tree"] = time.perf_counter() - t0
+    return pm, tr, ph
+
+
+def cpu_baseline(args, n_cfg, pos_cfg, gpu_force):
+    """The CPU leg: the oracle's restatement of the reference's single-rank path, timed on this
+    host's cores, and the reference CPU TreePM forces as an accuracy yardstick.
+
+    value: the 64^3 configuration (N = 262,144 on a perturbed lattice, P(k) ~ k^-1, rms 0.3 cell,
+    seed 1; FP64 mesh chain; erfc split r_s = 1.25, cutoff 5.625, theta 0.5), one force evaluation
+    on every core of this host's share, and again on one thread. cfg1: the bench's own 256^3
+    clustered box with the reference's float mesh: the whole PM chain, and the tree for a 1 %
+    random subsample of targets (its time scaled to all targets). The same run gives the reference
+    CPU TreePM force on that subsample, against which the GPU force of the same particles is
+    compared (accuracy.vs_reference_cpu); and a 32^3 clustered box is compared with an FP64 Ewald
+    sum (accuracy.vs_ewald)."""
+    from lageunha_amd import ics, lagpm
+    from oracle import pyoracle as orc
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    model, phys, logical = cpu_info()
+    out = {"unit": "forces/s", "kind": "port", "cores": threads, "cpu_model": model,
+           "host_physical_cores": phys, "host_logical_cpus": logical}
+    # cfg0: 64^3, FP64, all cores then one thread
+    n0 = 64
+    pos0 = ics.perturbed_lattice(n0, rms=0.3, seed=1)
+    rates = {}
+    for nt in (threads, 1):
+        orc.set_threads(nt)
+        t0 = time.perf_counter()
+        _, _, ph = cpu_force_eval(orc, n0, pos0, 1, args.theta)
+        dt = time.perf_counter() - t0
+        rates[nt] = {"forces_per_s": len(pos0) / dt, "seconds": dt, "phases_s": ph}
+        log(f"[bench] cpu baseline 64^3 FP64 on {nt} threads: {dt:.2f} s ({ph})")
+    out["value"] = rates[threads]["forces_per_s"]
+    out["seconds"] = rates[threads]["seconds"]
+    out["phases_s"] = rates[threads]["phases_s"]
+    out["one_thread"] = rates[1]
+    out["sample"] = (f"64^3 perturbed lattice (262,144 particles, P(k)~k^-1, rms 0.3 cell, seed 1), 64^3 mesh, "
+                     f"FP64 mesh chain, erfc split r_s 1.25, cutoff 5.625, theta {args.theta}: one force "
+                     f"evaluation of the oracle's restatement of the reference path on {threads} OpenMP threads "
+                     f"({model}); phases named as the reference's timers (Cosmos/pmmain.c:447-522, Tree: "
+                     f"Cosmos/Treemain.c:662)")
+    orc.set_threads(threads)
+    # cfg1: the 256^3 bench box, float mesh (as the reference), tree on a 1 % subsample
+    N = len(pos_cfg)
+    sel = np.sort(np.random.default_rng(args.seed + 23).choice(N, max(1, N // 100), replace=False))
+    pm, tr, ph = cpu_force_eval(orc, n_cfg, pos_cfg, 0, args.theta, targets=sel)
+    t_full = sum(v for k, v in ph.items() if k != "Tree") + ph["Tree"] * N / len(sel)
+    out["cfg1"] = {"n": n_cfg, "particles": N, "tree_targets": int(len(sel)), "phases_s": ph,
+                   "forces_per_s": N / t_full,
+                   "note": "PM chain over all particles; tree walk for the subsample only, its time scaled "
+                           "by N / targets in forces_per_s"}
+    log(f"[bench] cpu baseline {n_cfg}^3 (tree on {len(sel)} targets): {ph}")
+    acc = {}
+    if gpu_force is not None:
+        ref = pm[sel] + tr
+        rms, p99, med = rel_errors(gpu_force[sel], ref)
+        acc["vs_reference_cpu"] = {
+            "rms_rel": rms, "p99_rel": p99, "median_rel": med, "targets": int(len(sel)),
+            "reference": "oracle restatement of the reference CPU TreePM on the same particles: float TSC, FFT, "
+                         "psolver (efold 2.5, TSC^2 deconvolution), FDA4 (Cosmos/pmmain.c:390-527) plus the "
+                         "per-cell octree walk of the tree correction (Cosmos/treecorrection.c:225-475) with the "
+                         f"same erfc law, cutoff and theta {args.theta}"}
+    # small N against physics: 32^3 clustered, FP64 Ewald of Plummer-softened particles
+    n3 = 32
+    pos3 = ics.clustered_box(n3, halo_fraction=0.3, nhalo=16, seed=3)
+    ewald = orc.ewald_converged(pos3, n3, eps=0.1)
+    ctx = lagpm.Context(0)
+    try:
+        ve = {"particles": len(pos3), "reference": "FP64 Ewald sum of Plummer-softened (eps 0.1 cell) periodic "
+                                                   "point masses, 32^3 clustered (30 % in 16 halos, seed 3)"}
+        for th in (args.theta, 1e-6):
+            p = lagpm.make_params(n3, theta=th)
+            a = lagpm.treepm_accel(ctx, p, pos3).total
+            pmr, trr, _ = cpu_force_eval(orc, n3, pos3, 0, th)
+            e_gpu, e_ref = rel_errors(a, ewald), rel_errors(pmr + trr, ewald)
+            ve[f"theta_{th:g}"] = {"gpu_rms_rel": e_gpu[0], "gpu_p99_rel": e_gpu[1],
+                                   "reference_cpu_rms_rel": e_ref[0], "reference_cpu_p99_rel": e_ref[1]}
+        acc["vs_ewald"] = ve
+    finally:
+        ctx.close()
+    return out, acc
 
 
 SMOOTH_SIZES = (32, 36, 48, 54, 64, 72, 96, 108, 128, 144, 162, 192)
@@ -179,7 +299,8 @@ def ensemble(args, ctx, lagpm, ics):
             bx.close()
     out = {"boxes": len(specs), "particles": total, "steps": steps,
            "sizes": sorted(n for n, _ in specs), "identical_plain_vs_batched": bool(same),
-           "tree_tuning": {str(n): {"nleaf": t[0], "group": t[1]} for n, t in sorted(tuned.items())},
+           "tree_tuning": {str(n): {"nleaf": t[0], "group": t[1], "group_mode": "runs" if t[2] else "tree"}
+                           for n, t in sorted(tuned.items())},
            "batch_pipeline": {"stages": 3, "reserve_cu": 32, "note": "library default (lagpm_set_batch_pipeline)"}}
     for k in ("plain", "batched"):
         L = np.array(lat[k]) * 1e3
@@ -280,15 +401,14 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--halo-fraction", type=float, default=0.0, help="default 0.3 (0.4 at n >= 512)")
     ap.add_argument("--nhalo", type=int, default=0, help="default 2000 scaled with volume beyond 256^3")
     ap.add_argument("--acc-sample", type=float, default=1.0, help="fraction of particles in the accuracy statistics")
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -296,6 +416,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--nleaf", type=int, default=0)
     ap.add_argument("--group", type=int, default=0)
+    ap.add_argument("--group-mode", choices=["tree", "runs"], default=None)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-boxes", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=10)
@@ -333,12 +454,13 @@ def main():
     ctx = lagpm.Context(0)
     box = lagpm.Box(ctx, params, N)
     box.upload_pos(pos)
-    if args.nleaf or args.group:
-        box.set_tuning(args.nleaf or 16, args.group or 64)
+    if args.nleaf or args.group or args.group_mode:
+        box.set_tuning(args.nleaf or 16, args.group or 64,
+                       lagpm.GROUPS_RUNS if args.group_mode == "runs" else lagpm.GROUPS_TREE)
     elif not args.no_autotune:
         t_at = time.perf_counter()
-        nl, gs = box.autotune()
-        log(f"[bench] autotune: nleaf {nl}, group {gs} ({time.perf_counter() - t_at:.2f} s)")
+        nl, gs, gm = box.autotune()
+        log(f"[bench] autotune: nleaf {nl}, group {gs}, group mode {gm} ({time.perf_counter() - t_at:.2f} s)")
     tuning = box.tuning()
 
     phases, stats = [], []
@@ -365,17 +487,18 @@ def main():
         step()
         log(f"[bench] warmup step {i + 1}/{args.warmup} done")
     ctx.synchronize()
-    t0 = time.perf_counter()
     walk_ms, walk_stats = [], []
-    for i in range(args.steps):
-        step()
-        # the walk runs outside the step graphs, between HIP events on the context stream (the
-        # stream it is launched on); the step has completed when kdk_step returns
-        walk_ms.append(box.timings().ms_walk)
-        s = box.tree_stats()
-        walk_stats.append((s.n_pp, s.n_pc))
-    ctx.synchronize()
-    t1 = time.perf_counter()
+    with ProfilerRegion():
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step()
+            # the walk runs outside the step graphs, between HIP events on the context stream (the
+            # stream it is launched on); the step has completed when kdk_step returns
+            walk_ms.append(box.timings().ms_walk)
+            s = box.tree_stats()
+            walk_stats.append((s.n_pp, s.n_pc))
+        ctx.synchronize()
+        t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     log(f"[bench] timed {args.steps} steps: {ms:.2f} ms/step")
     # per-phase device times: two more steps with the kernels launched eagerly (phase events are
@@ -387,6 +510,16 @@ def main():
         s = box.tree_stats()
         stats.append((s.n_pp, s.n_pc, s.n_nodes, s.n_groups))
     ctx.set_step_graphs(True)
+    # one more step with the walk counting the evaluated pairs within the cutoff (slower, untimed)
+    ctx.set_walk_census(True)
+    step()
+    s = box.tree_stats()
+    ctx.set_walk_census(False)
+    census = {"n_pp": s.n_pp, "n_pp_within_cutoff": s.n_pp_within,
+              "useful_pair_fraction": s.n_pp_within / max(1, s.n_pp), "n_groups": s.n_groups,
+              "targets_per_group": N / max(1, s.n_groups),
+              "note": "the step after the timed and phase steps, counted by the FP32 walk kernel itself"}
+    log(f"[bench] census: {census}")
 
     ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
     # the walk: its launches in the timed steps themselves, and their interaction counts
@@ -417,12 +550,15 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}
     else:
         achieved = d["work"] / (d["ms"] * 1e-3) / 1e12
-        roof = {"kernel": "walk_f32_kernel", "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                "engine": "FP32 vector ALU (the chip's FP32 peak, 157.3 TFLOP/s, is the same for vector and "
-                          "matrix FP32; the pair sums are not GEMM-shaped)",
+        roof = {"kernel": "walk_f32_kernel", "bound": "fp32-valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                "engine": "FP32 vector ALU, no MFMA (the chip's FP32 peak, 157.3 TFLOP/s, counts packed FP32, "
+                          "which this build does not use: the unpacked FMA rate is 78.6 TFLOP/s; the pair sums "
+                          "are not GEMM-shaped)",
                 "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                "work": "21 flops per pair-particle + 69 per particle-node interaction evaluated",
-                "n_pp": npp, "n_pc": npc}
+                "work": "21 flops per pair-particle + 69 per particle-node interaction evaluated, "
+                        "mean over the timed steps",
+                "n_pp": npp, "n_pc": npc, "census": census,
+                "per_step": [{"walk_ms": w, "n_pp": a, "n_pc": b} for w, (a, b) in zip(walk_ms, walk_stats)]}
     phase_report = {}
     for k, v in kern.items():
         if v["bound"] == "hbm" and v["ms"] > 0:
@@ -444,15 +580,18 @@ def main():
                                f"{n}^3 mesh, Rsplit 1.25, rcut 5.625, theta {args.theta}, one GOTPM step "
                                f"(drag + PM kick + tree kick + drift) per step",
                    "n": n, "particles": N, "parallelism": "single GPU",
-                   "tree_tuning": {"nleaf": tuning[0], "group": tuning[1]}},
+                   "tree_tuning": {"nleaf": tuning[0], "group": tuning[1],
+                                   "group_mode": "runs" if tuning[2] == lagpm.GROUPS_RUNS else "tree"}},
         "roofline": roof,
         "phases": phase_report,
     }
     box.close()
 
+    a32_full = None
     if not args.no_accuracy:
         # forces at the initial positions: FP32 fast path vs the plain FP64 kernels
         a32, t32 = force_at(ctx, lagpm, params, pos, tuning)
+        a32_full = a32
         acc = {"vs": "plain FP64 kernels, same tree/groups/criteria", "fp32_seconds": t32}
         if not args.no_fp64:
             p64 = lagpm.make_params(n, law=lagpm.LAW_ERFC, precision=lagpm.PREC_FP64, deconvolve=1, efold=2.5,
@@ -481,11 +620,12 @@ def main():
         log(f"[bench] serving: {sv['requests']} requests, {sv['particle_steps_per_s']:.3e} particle-steps/s, "
             f"step p50 {sv.get('step_latency_p50_ms', 0):.1f} ms p99 {sv.get('step_latency_p99_ms', 0):.1f} ms")
 
-    if not args.no_cpu_baseline:
-        log(f"[bench] cpu baseline at {args.cpu_n}^3 ...")
-        out["cpu_baseline"] = cpu_baseline(args)
-        log(f"[bench] cpu baseline: {out['cpu_baseline']['value']:.3e} forces/s")
     ctx.close()
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"], cpu_acc = cpu_baseline(args, n, pos, a32_full)
+        out.setdefault("accuracy", {}).update(cpu_acc)
+        log(f"[bench] cpu baseline: {out['cpu_baseline']['value']:.3e} forces/s; accuracy {cpu_acc}")
     print(json.dumps(out), flush=True)
 
 

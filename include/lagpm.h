@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define LAGPM_ABI_VERSION 3
+#define LAGPM_ABI_VERSION 4
 
 /* status codes */
 #define LAGPM_OK          0
@@ -102,6 +102,8 @@ typedef struct lagpm_tree_stats {
     int32_t  deepest;   /* deepest tree level with nodes (levels of the 16-bit-per-axis keys) */
     int32_t  depth_limit; /* deepest level the next build splits to (a deeper tree is detected
                              and rebuilt at full depth) */
+    uint64_t n_pp_within; /* of n_pp, the pairs within the cutoff (FP32 walk with the census on,
+                             lagpm_set_walk_census; 0 otherwise) */
 } lagpm_tree_stats;
 
 /* Device time (HIP events on the context stream) of the phases of the last pmmain and
@@ -133,6 +135,10 @@ int         lagpm_synchronize(lagpm_ctx* ctx);
 int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box** out);
 int lagpm_box_destroy(lagpm_ctx* ctx, lagpm_box* box);
 int lagpm_box_params(const lagpm_box* box, lagpm_params* out);
+/* Changes the box's cosmology (AMAX, OMEP, OMEPLAM), which only sets the kick factors of later
+ * steps: a box and its device memory can be reused for another box of the same size and force
+ * settings (a serving loop's pool). No counterpart in the reference. */
+int lagpm_box_set_cosmology(lagpm_box* box, double amax, double omep, double omeplam);
 size_t lagpm_box_np(const lagpm_box* box);
 /* Bytes of device memory the box holds (used for HBM-budget admission). */
 size_t lagpm_box_device_bytes(const lagpm_box* box);
@@ -189,8 +195,9 @@ int lagpm_snapshot_read_header(lagpm_ctx* ctx, const char* path, lagpm_snapshot_
 int lagpm_snapshot_read_dm(lagpm_ctx* ctx, const char* path, void* dm_bp, size_t np);
 /* Writes dst_path as jwrite does (kjhrw.c:155-220) for a DM-only file: the header lines of
  * src_path with "Anow" (written with 17 digits), "Stepcount"/"Stepnum" and "Particle Type" (0)
- * rewritten, np 32-byte records, then src_path's bytes after its particle array (the
- * SimParameters block) unchanged. np must equal src_path's DNp. */
+ * rewritten, then np 32-byte records, and no trailing SimParameters block: jread freads that
+ * block over the parsed header (kjhrw.c:118), so the template's block would restore the
+ * template's time and particle type; without it the header stands. np must equal src_path's DNp. */
 int lagpm_snapshot_write(lagpm_ctx* ctx, const char* src_path, const char* dst_path, const void* dm_bp, size_t np,
                          double anow, int stepcount);
 /* XofP/YofP/ZofP (eunha.h:1212,1228-1232) of np 32-byte records into FP64 arrays (v* may be NULL). */
@@ -252,6 +259,14 @@ int lagpm_force_spline(const lagpm_params* p, double* diff /*[4096*3]*/, double*
  * Cosmos/treecorrection.c:227). Sorts, builds the octree, walks. */
 int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* ef);
 int lagpm_tree_stats_get(const lagpm_box* box, lagpm_tree_stats* out);
+/* Short-range acceleration of selected particles only. No counterpart in the reference: it states
+ * force accuracy where every particle's exact pair sum is out of reach (halo cores of 10^5..10^6
+ * particles). Builds the box's octree as lagpm_treecorrection does, then walks each listed
+ * particle as a group of one, so the opening test is the reference's per-particle test
+ * (OST/ost.ver2.c:296-306), and writes its acceleration per unit kick factor to acc[3t..3t+2].
+ * ids: caller indices (the upload order), distinct, in [0, np). Velocities are not changed.
+ * Allocates scratch for the call. */
+int lagpm_tree_accel_sample(lagpm_ctx* ctx, lagpm_box* box, const int64_t* ids, size_t count, double* acc);
 /* The octree of the box's last tree build (Make_Tree3 / divide_node3, OST/ost.ver2.c:36-144), for
  * tests and inspection; it describes the positions the box had at that build. Nodes are in
  * pre-order, *nnodes of them (LAGPM_E_INVAL when cap is smaller; *nnodes is still set).
@@ -269,10 +284,19 @@ int lagpm_tree_nodes_get(lagpm_ctx* ctx, lagpm_box* box, size_t cap, size_t* nno
  * group size (8..64, one wavefront walks one group). Defaults 16 and 64. */
 int lagpm_box_set_tuning(lagpm_box* box, int nleaf, int group);
 int lagpm_box_get_tuning(const lagpm_box* box, int* nleaf, int* group);
-/* Times the tree build + walk of the box's current particles for a set of (nleaf, group)
- * candidates (the walk adds nothing to the velocities: factor 0), keeps the fastest, and
- * returns it. Forces at theta > 0 depend slightly on the tree, so tune once per box, before
- * stepping, and keep the choice. */
+/* How target groups are formed. TREE (default): a tree node of at most `group` particles whose
+ * parent holds more, consecutive small siblings merged up to `group`. RUNS: each chaining cell's
+ * particles, in Morton order, cut into runs of `group` (the last shorter), so all groups but one
+ * per cell fill a wavefront. Either way a group stays inside one chaining cell and the opening
+ * test is taken from the group's bounding box; at theta > 0 the forces differ slightly. */
+#define LAGPM_GROUPS_TREE 0
+#define LAGPM_GROUPS_RUNS 1
+int lagpm_box_set_group_mode(lagpm_box* box, int mode);
+int lagpm_box_get_group_mode(const lagpm_box* box);
+/* Times the tree build + walk of the box's current particles for a set of (nleaf, group, group
+ * mode) candidates (the walk adds nothing to the velocities: factor 0), keeps the fastest, and
+ * returns nleaf and group (the mode: lagpm_box_get_group_mode). Forces at theta > 0 depend
+ * slightly on the tree, so tune once per box, before stepping, and keep the choice. */
 int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group);
 /* Waits for the box's last recorded phase and fills the timings. */
 int lagpm_box_timings(lagpm_ctx* ctx, lagpm_box* box, lagpm_timings* out);
@@ -290,7 +314,11 @@ int lagpm_drift(lagpm_ctx* ctx, lagpm_box* box, double pfact);
  * Cosmos/globalT.c:221-306: PM_PULL_AND_PUSH :114-134, then TreeAndPreFoFmain :38-95, then
  * onestepforwardposition): v *= fact1; v += fact2 * (F_PM + F_tree); x += pfact * v, with the
  * factors of GetEvolFactor(anow, astep) (Cosmos/CosmosEvolFactor.c:230-256) from the box's own
- * cosmology. Returns when the step is complete; tree statistics and phase timings are filled. */
+ * cosmology. Returns when the step is complete; tree statistics and phase timings are filled.
+ * Node arrays too small for the tree are grown and the tree kick and drift redone inside the call.
+ * A tree walk whose bounded stack overflows returns LAGPM_E_OVERFLOW and leaves the box after the
+ * drag and PM kick: no tree kick (FP32; the FP64 walk has kicked every other particle) and no
+ * drift, so the positions are those the step started from. */
 int lagpm_kdk_step(lagpm_ctx* ctx, lagpm_box* box, double anow, double astep);
 /* The same step for nbox independent boxes (box i at anow[i], astep[i]); no counterpart in
  * the reference, which evolves one box per MPI job. The boxes' PM phases and tree builds run
@@ -300,6 +328,10 @@ int lagpm_batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const do
 /* Whole steps capture each box's step (up to the tree walk) in a hipGraph on first use and
  * replay it (default on); off: every kernel is launched each step. Results are identical. */
 int lagpm_set_step_graphs(lagpm_ctx* ctx, int enable);
+/* FP32 walks also count the evaluated particle pairs that are within the cutoff
+ * (lagpm_tree_stats.n_pp_within): the useful fraction of the interaction lists. Costs a compare
+ * and an add per pair; off by default. No counterpart in the reference. */
+int lagpm_set_walk_census(lagpm_ctx* ctx, int enable);
 /* Batch steps of FP32 boxes sharing one force law run as `stages` groups of about equal particle
  * count (default 3, reserve_cu 32; used when the batch has at least 2 * stages boxes): a group's
  * fused walk leaves reserve_cu CUs free and runs beside the next group's front halves (PM and

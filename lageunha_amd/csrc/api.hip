@@ -61,9 +61,11 @@ __global__ void drift_kernel(double* __restrict__ x, double* __restrict__ y, dou
                              double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
                              const double* __restrict__ tvx, const double* __restrict__ tvy,
                              const double* __restrict__ tvz, size_t np, const double* __restrict__ fac, double L,
-                             const uint32_t* __restrict__ skip) {
+                             const uint32_t* __restrict__ dsz) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np || (skip && *skip)) return;
+    // guarded drift of a step: no drift when the tree build overflowed (the step is redone) or a
+    // walk stack overflowed (the step returns the error)
+    if (i >= np || (dsz && (dsz[DSZ_OVERFLOW] | dsz[DSZ_WALKOVF]))) return;
     const double pfact = fac[2];
     double u = vx[i], w = vy[i], q = vz[i];
     if (tvx) {
@@ -546,6 +548,14 @@ int lagpm_box_params(const lagpm_box* box, lagpm_params* out) {
     *out = box->p;
     return LAGPM_OK;
 }
+int lagpm_box_set_cosmology(lagpm_box* box, double amax, double omep, double omeplam) {
+    if (!box || !(amax > 0) || !(omep > 0) || !(omeplam >= 0)) return LAGPM_E_INVAL;
+    // the kick factors come from these at every step; nothing on the device depends on them
+    box->p.amax = amax;
+    box->p.omep = omep;
+    box->p.omeplam = omeplam;
+    return LAGPM_OK;
+}
 size_t lagpm_box_np(const lagpm_box* box) { return box ? box->np : 0; }
 size_t lagpm_box_device_bytes(const lagpm_box* box) { return box ? box->bytes : 0; }
 
@@ -575,6 +585,13 @@ int lagpm_box_upload(lagpm_ctx* ctx, lagpm_box* box, const double* x, const doub
     box->uploaded = true;
     box->sorted = false;
     box->tree_built = false;
+    // new particles: the box starts over as a new box would, with a full-depth first tree (the
+    // depth limit also sets how many key bits the sort orders, and so the particle order that
+    // the sums follow), so a box reused for other particles steps exactly as a fresh one
+    if (box->depth != LAGPM_KEY_BITS) {
+        box->depth = LAGPM_KEY_BITS;
+        lagpm_box_graphs_release(box);  // captured steps hold the old limit
+    }
     return LAGPM_OK;
 }
 
@@ -708,6 +725,7 @@ int lagpm_green_apply(lagpm_ctx* ctx, int n, double efold, int deconvolve, float
     HIPCHK(ctx, hipMalloc(&d, K * 2 * sizeof(float)));
     int rc = LAGPM_OK;
     if (hipMemcpyAsync(d, spec_host, K * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
+    if (rc == LAGPM_OK) rc = lagpm_green_prepare(ctx, ctx->stream, n, efold, deconvolve);
     if (rc == LAGPM_OK) rc = lagpm_green_device(ctx, ctx->stream, n, LAGPM_PREC_FP32, efold, deconvolve, d);
     if (rc == LAGPM_OK && hipMemcpyAsync(spec_host, d, K * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) rc = LAGPM_E_HIP;
     (void)hipStreamSynchronize(ctx->stream);
@@ -895,6 +913,54 @@ int lagpm_treecorrection(lagpm_ctx* ctx, lagpm_box* box, const lagpm_evolfact* e
     }
 }
 
+int lagpm_tree_accel_sample(lagpm_ctx* ctx, lagpm_box* box, const int64_t* ids, size_t count, double* acc) {
+    RCHK(need_particles(ctx, box));
+    RCHK(tree_check(ctx, box));
+    if (count == 0) return LAGPM_OK;
+    if (!ids || !acc) return lagpm_fail(ctx, LAGPM_E_INVAL, "NULL ids or acc");
+    if (count > box->np) return lagpm_fail(ctx, LAGPM_E_INVAL, "more sampled particles than the box holds");
+    {
+        std::vector<uint8_t> seen(box->np, 0);
+        for (size_t t = 0; t < count; t++) {
+            if (ids[t] < 0 || (size_t)ids[t] >= box->np)
+                return lagpm_fail(ctx, LAGPM_E_INVAL, "sampled index " + std::to_string(ids[t]) + " out of range");
+            if (seen[(size_t)ids[t]]++) return lagpm_fail(ctx, LAGPM_E_INVAL, "sampled index " + std::to_string(ids[t]) + " repeated");
+        }
+    }
+    // the box's tree, built as the tree correction builds it (grown or deepened as needed)
+    for (int attempt = 0;; attempt++) {
+        HIPCHK(ctx, hipEventRecord(box->ev[8], box->st));
+        if (!box->sorted) RCHK(lagpm_sort_particles(ctx, box));
+        RCHK(lagpm_tree_build(ctx, box));
+        bool retry = false;
+        const int rc = lagpm_tree_finish(ctx, box, &retry);
+        if (rc == LAGPM_OK) break;
+        if (!retry || attempt > 1) return rc == LAGPM_E_OVERFLOW ? lagpm_fail(ctx, rc, "tree node arrays overflowed") : rc;
+    }
+    // scratch of this call (a diagnostic entry point: it allocates)
+    int64_t* d_ids = nullptr;
+    int* slot = nullptr;
+    int2* groups = nullptr;
+    uint32_t* sdsz = nullptr;
+    double* d_acc = nullptr;
+    int rc = lagpm_dalloc_bytes(ctx, (void**)&d_ids, count * sizeof(int64_t));
+    if (rc == LAGPM_OK) rc = lagpm_dalloc_bytes(ctx, (void**)&slot, box->np * sizeof(int));
+    if (rc == LAGPM_OK) rc = lagpm_dalloc_bytes(ctx, (void**)&groups, count * sizeof(int2));
+    if (rc == LAGPM_OK) rc = lagpm_dalloc_bytes(ctx, (void**)&sdsz, DSZ_COUNT * sizeof(uint32_t));
+    if (rc == LAGPM_OK) rc = lagpm_dalloc_bytes(ctx, (void**)&d_acc, 3 * count * sizeof(double));
+    if (rc == LAGPM_OK && hipMemcpyAsync(d_ids, ids, count * sizeof(int64_t), hipMemcpyHostToDevice, box->st) != hipSuccess)
+        rc = lagpm_fail(ctx, LAGPM_E_HIP, "upload of the sampled indices");
+    if (rc == LAGPM_OK) rc = lagpm_tree_walk_sample(ctx, box, d_ids, count, slot, groups, sdsz, d_acc);
+    unsigned long long c[4] = {0, 0, 0, 0};
+    if (rc == LAGPM_OK) rc = lagpm_d2h(ctx, acc, d_acc, 3 * count * sizeof(double), box->st);
+    if (rc == LAGPM_OK) rc = lagpm_d2h(ctx, c, box->counters, sizeof(c), box->st);
+    (void)hipStreamSynchronize(box->st);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_ids); (void)hipFree(slot); (void)hipFree(groups); (void)hipFree(sdsz); (void)hipFree(d_acc);
+    if (rc == LAGPM_OK && c[2]) rc = lagpm_fail(ctx, LAGPM_E_OVERFLOW, "tree walk stack overflow in " + std::to_string(c[2]) + " sampled particles");
+    return rc;
+}
+
 int lagpm_box_set_tuning(lagpm_box* box, int nleaf, int group) {
     if (!box || nleaf < 4 || nleaf > 256 || group < 8 || group > 64) return LAGPM_E_INVAL;
     if (box->tune.nleaf != nleaf || box->tune.group != group) {
@@ -913,15 +979,30 @@ int lagpm_box_get_tuning(const lagpm_box* box, int* nleaf, int* group) {
     return LAGPM_OK;
 }
 
+int lagpm_box_set_group_mode(lagpm_box* box, int mode) {
+    if (!box || (mode != LAGPM_GROUPS_TREE && mode != LAGPM_GROUPS_RUNS)) return LAGPM_E_INVAL;
+    const int runs = mode == LAGPM_GROUPS_RUNS ? 1 : 0;
+    if (box->tune.runs != runs) lagpm_box_graphs_release(box);  // captured steps hold the mode
+    box->tune.runs = runs;
+    return LAGPM_OK;
+}
+
+int lagpm_box_get_group_mode(const lagpm_box* box) {
+    return box ? (box->tune.runs ? LAGPM_GROUPS_RUNS : LAGPM_GROUPS_TREE) : LAGPM_E_INVAL;
+}
+
 int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group) {
     RCHK(need_particles(ctx, box));
     RCHK(tree_check(ctx, box));
     if (box->np == 0) return LAGPM_OK;
-    static const int cand[][2] = {{16, 64}, {32, 64}, {8, 64}, {16, 32}, {32, 32}, {8, 32}};
+    // (nleaf, group, group mode)
+    static const int cand[][3] = {{16, 64, 1}, {32, 64, 1}, {8, 64, 1}, {16, 32, 1}, {32, 32, 1}, {8, 32, 1},
+                                  {16, 64, 0}, {32, 64, 0}, {8, 64, 0}, {16, 32, 0}, {32, 32, 0}, {8, 32, 0}};
     double best = 1e300;
-    int bl = box->tune.nleaf, bg = box->tune.group;
+    int bl = box->tune.nleaf, bg = box->tune.group, bm = lagpm_box_get_group_mode(box);
     for (const auto& c : cand) {
         RCHK(lagpm_box_set_tuning(box, c[0], c[1]));
+        RCHK(lagpm_box_set_group_mode(box, c[2]));
         double t = 1e300;
         for (int rep = 0; rep < 2; rep++) {  // the first run of a candidate may grow the node arrays
             RCHK(tree_enqueue(ctx, box, 0.0, false));
@@ -937,9 +1018,11 @@ int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group) {
             best = t;
             bl = c[0];
             bg = c[1];
+            bm = c[2];
         }
     }
     RCHK(lagpm_box_set_tuning(box, bl, bg));
+    RCHK(lagpm_box_set_group_mode(box, bm));
     if (nleaf) *nleaf = bl;
     if (group) *group = bg;
     return LAGPM_OK;
@@ -1144,7 +1227,7 @@ int drift_enqueue(lagpm_ctx* ctx, lagpm_box* box, double pfact, bool guarded) {
                        box->vy, box->vz, tk ? (const double*)box->alt[0] : nullptr,
                        tk ? (const double*)box->alt[1] : nullptr, tk ? (const double*)box->alt[2] : nullptr, box->np,
                        (const double*)box->dfac, (double)box->p.n,
-                       guarded ? (const uint32_t*)(box->dsz + DSZ_OVERFLOW) : (const uint32_t*)nullptr);
+                       guarded ? (const uint32_t*)box->dsz : (const uint32_t*)nullptr);
     HIPCHK(ctx, hipGetLastError());
     box->sorted = false;
     return LAGPM_OK;
@@ -1334,6 +1417,12 @@ int lagpm_batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const do
 int lagpm_set_step_graphs(lagpm_ctx* ctx, int enable) {
     if (!ctx) return LAGPM_E_INVAL;
     ctx->use_graphs = enable != 0;
+    return LAGPM_OK;
+}
+
+int lagpm_set_walk_census(lagpm_ctx* ctx, int enable) {
+    if (!ctx) return LAGPM_E_INVAL;
+    ctx->walk_census = enable != 0;
     return LAGPM_OK;
 }
 

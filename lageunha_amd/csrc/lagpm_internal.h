@@ -66,16 +66,20 @@ struct lagpm_ctx {
     uint32_t* walk_prefix = nullptr;         // group prefix over the descriptors
     size_t walk_box_cap = 0;
     bool use_graphs = true;                  // capture box step fronts in hipGraphs
+    bool walk_census = false;                // FP32 walks count the pairs within the cutoff
     int batch_stages = 3;                    // groups of a pipelined batch step (1: no pipeline)
     int stage_reserve_cu = 32;               // CUs a non-final group's walk leaves to the next fronts
     hipEvent_t ev[4] = {};
     double* pk_scratch = nullptr;  // this device's static power-spectrum scratch (device address)
+    // Green factor tables, (n, efold, deconvolve) -> (n/2+1)^3 doubles (pm.hip)
+    std::map<std::tuple<int, double, int>, double*> green_tabs;
 };
 
 // Tunables of the tree correction (chosen per box; the autotuner may override).
 struct tree_tune {
     int nleaf = 16;        // a node with more particles than this is split
     int group = 64;        // maximal target group (one wavefront)
+    int runs = 0;          // 1: groups are runs of `group` particles of a chaining cell; 0: tree groups
     int walk_waves = 0;    // persistent walk waves (0: auto)
 };
 
@@ -90,6 +94,7 @@ enum {
     DSZ_OVERFLOW = 3,
     DSZ_SCRATCH = 4,  // scan totals nobody reads
     DSZ_DEEPER = 5,   // the build met a node to split at its depth limit (the tree is redone)
+    DSZ_WALKOVF = 6,  // a walk stack overflowed: that step's drift is skipped (the error is returned)
     DSZ_LVL = 8,
     DSZ_LOFF = DSZ_LVL + LAGPM_KEY_BITS + 1,
     DSZ_LCUR = DSZ_LOFF + LAGPM_KEY_BITS + 1,
@@ -216,6 +221,8 @@ int lagpm_pm_solve(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev = nullptr);  /
 int lagpm_pm_gradient(lagpm_ctx* ctx, lagpm_box* box);         // phi -> grad
 int lagpm_pm_interp(lagpm_ctx* ctx, lagpm_box* box);  // v += dfac[1] * F(grad)
 int lagpm_green_device(lagpm_ctx* ctx, hipStream_t s, int n, int prec, double efold, int deconvolve, void* spec);
+// the context's Green factor table for (n, efold, deconvolve); allocates (call before capture)
+int lagpm_green_prepare(lagpm_ctx* ctx, hipStream_t s, int n, double efold, int deconvolve);
 size_t lagpm_deposit_chunk_cap(int n, size_t np);
 int lagpm_fft_release(lagpm_ctx* ctx);
 int lagpm_box_fft_release(lagpm_box* box);
@@ -235,8 +242,21 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box);
 // One walk launch on ctx->stream over all groups of the given boxes (FP32 boxes sharing one
 // law table); facts[i] is the kick factor of box i. Waits are the caller's business.
 // reserve_cu: CUs left without a walk workgroup (for kernels of other streams running beside it)
-int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* facts, int nbox, int reserve_cu = 0);
-int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact);
+// A sampled walk: listed particles only, each as a group of one; groups and dsz are the sample's
+// own (device), out its accelerations (FP64 walk: written by the walk; FP32: gathered after it).
+struct walk_sample {
+    const int2* groups;
+    uint32_t* dsz;
+    double* out;
+    size_t count;
+};
+int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* facts, int nbox, int reserve_cu = 0,
+                        const walk_sample* sample = nullptr);
+int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact, const walk_sample* sample = nullptr);
+// the tree acceleration of the particles with caller indices ids[0..count) (device arrays; slot:
+// np ints, groups: count, sdsz: DSZ_COUNT words) into acc (device, 3 per particle)
+int lagpm_tree_walk_sample(lagpm_ctx* ctx, lagpm_box* box, const int64_t* ids, size_t count, int* slot, int2* groups,
+                           uint32_t* sdsz, double* acc);
 // Reads the device sizes and counters after the walk (synchronises box->st); grows the node
 // arrays and returns LAGPM_E_OVERFLOW with *retry set when they were too small.
 int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry);

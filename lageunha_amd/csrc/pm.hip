@@ -20,6 +20,7 @@
 #include "wave_ops.h"
 
 #include <cmath>
+#include <type_traits>
 
 namespace {
 
@@ -95,11 +96,29 @@ __global__ void dep_start_kernel(const uint32_t* __restrict__ flag, const uint32
     if (i == 0) start[*nchunk] = (uint32_t)N;
 }
 
+// Per-axis TSC weights in cell order (left, centre, right) = nodes near-1, near, near+1. The
+// reference's weights go to near, near+s and near-s (w1, w2, w3 with s the side of the nearest
+// node, pmmain.h:11-40); a cell's product of weights is the same, whichever order the nodes are
+// visited in.
+template <typename W>
+__device__ inline void cell_order(int s, const W w[3], W o[3]) {
+    o[0] = s > 0 ? w[2] : w[1];
+    o[1] = w[0];
+    o[2] = s > 0 ? w[1] : w[2];
+}
+
+// One block per chunk. Thread t takes DEP_PPT consecutive particles of the chunk (Morton order):
+// neighbouring particles often share the nearest mesh cell and with it the same 27 cells, so a
+// thread sums the fixed-point contributions of each run of particles with one nearest cell in
+// registers and issues the run's 27 atomics once (a halo core holds thousands of particles per
+// cell, whose atomics on one LDS address would otherwise serialise across the wave). The sums
+// are integer, so the result is the same in any grouping.
 template <bool F64>
 __global__ void __launch_bounds__(DEP_THREADS) deposit_kernel(const double* __restrict__ px, const double* __restrict__ py,
                                                               const double* __restrict__ pz, const uint32_t* __restrict__ start,
                                                               const uint32_t* __restrict__ nchunk, size_t npart, int n,
                                                               float mass, double scale, unsigned long long* __restrict__ acc) {
+    using W = typename std::conditional<F64, double, float>::type;
     __shared__ unsigned long long box[DEP_LDS];
     __shared__ int red[6][DEP_THREADS / 64];
     // the grid is sized for the largest possible chunk count; the actual count is on the device
@@ -111,24 +130,25 @@ __global__ void __launch_bounds__(DEP_THREADS) deposit_kernel(const double* __re
     const size_t np = min((size_t)start[chunk + 1], npart);
     const size_t base = min((size_t)start[chunk], np);
 
-    int ix[DEP_PPT], iy[DEP_PPT], iz[DEP_PPT], sx[DEP_PPT], sy[DEP_PPT], sz[DEP_PPT];
-    float wx[DEP_PPT][3], wy[DEP_PPT][3], wz[DEP_PPT][3];
-    double dwx[F64 ? DEP_PPT : 1][3], dwy[F64 ? DEP_PPT : 1][3], dwz[F64 ? DEP_PPT : 1][3];
-    bool ok[DEP_PPT];
+    int ix[DEP_PPT] = {}, iy[DEP_PPT] = {}, iz[DEP_PPT] = {};
+    W wx[DEP_PPT][3], wy[DEP_PPT][3], wz[DEP_PPT][3];  // in cell order (left, centre, right)
+    int cnt = 0;  // this thread's particles: base + tid * DEP_PPT + [0, cnt)
     int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
     for (int k = 0; k < DEP_PPT; k++) {
-        size_t i = base + (size_t)k * DEP_THREADS + tid;
-        ok[k] = i < np;
-        if (!ok[k]) continue;
+        const size_t i = base + (size_t)tid * DEP_PPT + k;
+        if (i >= np) break;
+        cnt = k + 1;
+        int sx, sy, sz;
+        W w[3];
         if constexpr (F64) {
-            tsc_axis_d(px[i], ix[k], sx[k], dwx[k]);
-            tsc_axis_d(py[i], iy[k], sy[k], dwy[k]);
-            tsc_axis_d(pz[i], iz[k], sz[k], dwz[k]);
+            tsc_axis_d(px[i], ix[k], sx, w); cell_order(sx, w, wx[k]);
+            tsc_axis_d(py[i], iy[k], sy, w); cell_order(sy, w, wy[k]);
+            tsc_axis_d(pz[i], iz[k], sz, w); cell_order(sz, w, wz[k]);
         } else {
-            tsc_axis_mass(px[i], mass, ix[k], sx[k], wx[k]);
-            tsc_axis_unit(py[i], iy[k], sy[k], wy[k]);
-            tsc_axis_unit(pz[i], iz[k], sz[k], wz[k]);
+            tsc_axis_mass(px[i], mass, ix[k], sx, w); cell_order(sx, w, wx[k]);
+            tsc_axis_unit(py[i], iy[k], sy, w); cell_order(sy, w, wy[k]);
+            tsc_axis_unit(pz[i], iz[k], sz, w); cell_order(sz, w, wz[k]);
         }
         lo[0] = min(lo[0], ix[k]); hi[0] = max(hi[0], ix[k]);
         lo[1] = min(lo[1], iy[k]); hi[1] = max(hi[1], iy[k]);
@@ -158,36 +178,58 @@ __global__ void __launch_bounds__(DEP_THREADS) deposit_kernel(const double* __re
         for (int c = tid; c < vol; c += DEP_THREADS) box[c] = 0ull;
         __syncthreads();
     }
-#pragma unroll
-    for (int k = 0; k < DEP_PPT; k++) {
-        if (!ok[k]) continue;
-        int xi[3] = {ix[k], ix[k] + sx[k], ix[k] - sx[k]};
-        int yi[3] = {iy[k], iy[k] + sy[k], iy[k] - sy[k]};
-        int zi[3] = {iz[k], iz[k] + sz[k], iz[k] - sz[k]};
+    // the 27 cells of nearest cell (cx, cy, cz) get the run's sums q[c][b][a]
+    auto flush = [&](int cx, int cy, int cz, const unsigned long long (&q)[27]) {
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
             for (int b = 0; b < 3; b++)
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    double cv;
-                    if constexpr (F64) {
-                        cv = (double)mass * dwx[k][a] * dwy[k][b] * dwz[k][c];
-                    } else {
-                        float wxy = wx[k][a] * wy[k][b];
-                        float cf = wxy * wz[k][c];
-                        cv = (double)cf;
-                    }
-                    unsigned long long q = (unsigned long long)__double2ll_rn(cv * scale);
+                    const unsigned long long v = q[9 * c + 3 * b + a];
+                    const int xa = cx + a - 1, yb = cy + b - 1, zc = cz + c - 1;
                     if (use_lds) {
-                        int li = (xi[a] - blo[0]) + bdim[0] * ((yi[b] - blo[1]) + bdim[1] * (zi[c] - blo[2]));
-                        atomicAdd(&box[li], q);
+                        atomicAdd(&box[(xa - blo[0]) + bdim[0] * ((yb - blo[1]) + bdim[1] * (zc - blo[2]))], v);
                     } else {
-                        size_t g = (size_t)wrapi(xi[a], n) + (size_t)n * ((size_t)wrapi(yi[b], n) + (size_t)n * wrapi(zi[c], n));
-                        atomicAdd(&acc[g], q);
+                        const size_t g = (size_t)wrapi(xa, n) + (size_t)n * ((size_t)wrapi(yb, n) + (size_t)n * wrapi(zc, n));
+                        atomicAdd(&acc[g], v);
                     }
                 }
+    };
+    // runs of equal nearest cells among this thread's particles (the loop is unrolled, so every
+    // register index is a constant)
+    unsigned long long q[27];
+#pragma unroll
+    for (int e = 0; e < 27; e++) q[e] = 0ull;
+    int cx = ix[0], cy = iy[0], cz = iz[0];
+#pragma unroll
+    for (int k = 0; k < DEP_PPT; k++) {
+        if (k < cnt) {
+            if (k > 0 && (ix[k] != cx || iy[k] != cy || iz[k] != cz)) {
+                flush(cx, cy, cz, q);
+#pragma unroll
+                for (int e = 0; e < 27; e++) q[e] = 0ull;
+                cx = ix[k]; cy = iy[k]; cz = iz[k];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int b = 0; b < 3; b++)
+#pragma unroll
+                    for (int a = 0; a < 3; a++) {
+                        double cv;
+                        if constexpr (F64) {
+                            cv = (double)mass * wx[k][a] * wy[k][b] * wz[k][c];
+                        } else {
+                            const float wxy = wx[k][a] * wy[k][b];
+                            const float cf = wxy * wz[k][c];
+                            cv = (double)cf;
+                        }
+                        q[9 * c + 3 * b + a] += (unsigned long long)__double2ll_rn(cv * scale);
+                    }
+        }
     }
+    if (cnt > 0) flush(cx, cy, cz, q);
     if (use_lds) {
         __syncthreads();
         for (int c = tid; c < vol; c += DEP_THREADS) {
@@ -223,10 +265,19 @@ __device__ inline double tsc_w2(int n, double m) {
     return s3 * s3;
 }
 
+// Green factor of psolver.c:59-72 for wave numbers (i, wy, wz), in the reference's own operations:
+// the squared wave number in float (exact for these integers), the factor in double.
+__device__ inline double green_factor(int n, int i, float wy, float wz, double sfact, int deconvolve) {
+    const double wave = (double)((float)(i * i) + wy * wy + wz * wz);
+    double factor = ((-1.0 / M_PI) / wave) * exp(wave * sfact);
+    if (deconvolve) factor /= tsc_w2(n, (double)i) * tsc_w2(n, (double)wy) * tsc_w2(n, (double)wz);
+    return factor;
+}
+
 // Green multiply over the half-complex spectrum, layout (i + h*(j + n*k)), psolver.c:49-76.
-template <typename T>
 // One workgroup per segment of `segs` of one half-complex row (fixed j, k): the row's indices and
 // its y and z wave numbers are workgroup-uniform, so no cell divides its linear index.
+template <typename T>
 __global__ void green_kernel(T* __restrict__ spec, int n, int segs, double sfact, double irngc, int deconvolve) {
     const int h = n / 2 + 1;
     const unsigned row = blockIdx.x / (unsigned)segs;  // row = j + n k
@@ -239,15 +290,45 @@ __global__ void green_kernel(T* __restrict__ spec, int n, int segs, double sfact
     if (wy > rn / 2) wy = wy - (float)rn;
     float wz = (float)k;
     if (wz > rn / 2) wz = wz - (float)rn;
-    double wave = (double)((float)(i * i) + wy * wy + wz * wz);
-    if (wave == 0.0) {
+    if (i == 0 && j == 0 && k == 0) {
         spec[2 * q] = (T)0;
         spec[2 * q + 1] = (T)0;
         return;
     }
-    double factr = (-1.0 / M_PI) / wave;
-    double factor = factr * exp(wave * sfact);
-    if (deconvolve) factor /= tsc_w2(n, (double)i) * tsc_w2(n, (double)wy) * tsc_w2(n, (double)wz);
+    const double factor = green_factor(n, i, wy, wz, sfact, deconvolve);
+    spec[2 * q] = (T)(factor * (double)spec[2 * q] * irngc);
+    spec[2 * q + 1] = (T)(factor * (double)spec[2 * q + 1] * irngc);
+}
+
+// The factor depends on i, |wy| and |wz| only (wave numbers enter squared, and the TSC window is
+// even: sin(-x)/(-x) = sin(x)/x in floating point, as sin is odd), so a context keeps it once per
+// (n, efold, deconvolution) in a table of (n/2+1)^3 doubles, index i + h (|wy| + h |wz|), and
+// the step's multiply streams the spectrum against it: the same double factor, the same products.
+__global__ void green_table_kernel(double* __restrict__ tab, int n, double sfact, int deconvolve) {
+    const int h = n / 2 + 1;
+    const size_t cnt = (size_t)h * h * h;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % h);
+        const size_t r = q / h;
+        const int ay = (int)(r % h), az = (int)(r / h);
+        tab[q] = (i | ay | az) ? green_factor(n, i, (float)ay, (float)az, sfact, deconvolve) : 0.0;
+    }
+}
+template <typename T>
+__global__ void green_tab_kernel(T* __restrict__ spec, const double* __restrict__ tab, int n, int segs, double irngc) {
+    const int h = n / 2 + 1;
+    const unsigned row = blockIdx.x / (unsigned)segs;  // row = j + n k
+    const int i = (int)(blockIdx.x - row * (unsigned)segs) * (int)blockDim.x + (int)threadIdx.x;
+    if (i >= h) return;
+    const int j = (int)(row % (unsigned)n), k = (int)(row / (unsigned)n);
+    const int ay = j <= n / 2 ? j : n - j, az = k <= n / 2 ? k : n - k;
+    const size_t q = (size_t)i + (size_t)h * row;
+    if (i == 0 && j == 0 && k == 0) {
+        spec[2 * q] = (T)0;
+        spec[2 * q + 1] = (T)0;
+        return;
+    }
+    const double factor = tab[(size_t)i + (size_t)h * ((size_t)ay + (size_t)h * az)];
     spec[2 * q] = (T)(factor * (double)spec[2 * q] * irngc);
     spec[2 * q + 1] = (T)(factor * (double)spec[2 * q + 1] * irngc);
 }
@@ -683,6 +764,7 @@ int lagpm_pm_prepare(lagpm_ctx* ctx, lagpm_box* box) {
     fft_plans* fp = nullptr;
     RCHK(get_plans(ctx, box->p.n, box->p.precision, box->st, &fp));
     if (!box->fft_info) RCHK(make_exec(ctx, fp, box->st, &box->fft_info, &box->fft_work));
+    RCHK(lagpm_green_prepare(ctx, box->st, box->p.n, box->p.efold, box->p.deconvolve));
     return LAGPM_OK;
 }
 
@@ -787,6 +869,8 @@ int lagpm_fft_release(lagpm_ctx* ctx) {
         if (fp.inv) rocfft_plan_destroy(fp.inv);
     }
     ctx->plans.clear();
+    for (auto& kv : ctx->green_tabs) (void)hipFree(kv.second);
+    ctx->green_tabs.clear();
     return LAGPM_OK;
 }
 
@@ -850,6 +934,34 @@ int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
     return LAGPM_OK;
 }
 
+// Largest Green factor table a context keeps (n <= 1024: 513^3 doubles, 1.08 GB); larger meshes
+// compute the factor per cell.
+constexpr size_t GREEN_TAB_MAX_BYTES = (size_t)1100 << 20;
+
+static double* green_table_find(lagpm_ctx* ctx, int n, double efold, int deconvolve) {
+    auto it = ctx->green_tabs.find(std::make_tuple(n, efold, deconvolve));
+    return it == ctx->green_tabs.end() ? nullptr : it->second;
+}
+
+// Makes the context's Green factor table for (n, efold, deconvolve) if the mesh is small enough
+// for one; allocates, so it runs before any captured step (lagpm_pm_prepare) or standalone call.
+int lagpm_green_prepare(lagpm_ctx* ctx, hipStream_t st, int n, double efold, int deconvolve) {
+    const size_t h = (size_t)n / 2 + 1, bytes = h * h * h * sizeof(double);
+    if (bytes > GREEN_TAB_MAX_BYTES || green_table_find(ctx, n, efold, deconvolve)) return LAGPM_OK;
+    double* tab = nullptr;
+    RCHK(lagpm_dalloc_bytes(ctx, (void**)&tab, bytes));
+    const double rn = n;
+    hipLaunchKernelGGL(green_table_kernel, dim3(grid_for(h * h * h, 256)), dim3(256), 0, st, tab, n,
+                       -std::pow(M_PI * efold / rn, 2), deconvolve);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        (void)hipFree(tab);
+        return lagpm_fail(ctx, LAGPM_E_HIP, std::string("green table: ") + hipGetErrorString(e));
+    }
+    ctx->green_tabs.emplace(std::make_tuple(n, efold, deconvolve), tab);
+    return LAGPM_OK;
+}
+
 int lagpm_green_device(lagpm_ctx* ctx, hipStream_t st, int n, int prec, double efold, int deconvolve, void* spec) {
     const double rn = n;
     const double sfact = -std::pow(M_PI * efold / rn, 2);
@@ -859,10 +971,16 @@ int lagpm_green_device(lagpm_ctx* ctx, hipStream_t st, int n, int prec, double e
     const int bs = h >= 256 ? 256 : (h + 63) / 64 * 64;
     const int segs = (h + bs - 1) / bs;
     const dim3 grid((unsigned)((size_t)n * n * segs));
-    if (prec == LAGPM_PREC_FP64)
+    if (const double* tab = green_table_find(ctx, n, efold, deconvolve)) {
+        if (prec == LAGPM_PREC_FP64)
+            hipLaunchKernelGGL(green_tab_kernel<double>, grid, dim3(bs), 0, st, (double*)spec, tab, n, segs, irngc);
+        else
+            hipLaunchKernelGGL(green_tab_kernel<float>, grid, dim3(bs), 0, st, (float*)spec, tab, n, segs, irngc);
+    } else if (prec == LAGPM_PREC_FP64) {
         hipLaunchKernelGGL(green_kernel<double>, grid, dim3(bs), 0, st, (double*)spec, n, segs, sfact, irngc, deconvolve);
-    else
+    } else {
         hipLaunchKernelGGL(green_kernel<float>, grid, dim3(bs), 0, st, (float*)spec, n, segs, sfact, irngc, deconvolve);
+    }
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }
@@ -877,6 +995,8 @@ int lagpm_pm_solve(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     void* in[1] = {box->rho};  // rho, spec and phi are one padded mesh
     RCHK(fft_status(ctx, rocfft_execute(fp->fwd, in, nullptr, info), "execute(r2c)"));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], box->st));
+    // (the table exists already when this runs inside a captured step: lagpm_pm_prepare made it)
+    RCHK(lagpm_green_prepare(ctx, box->st, box->p.n, box->p.efold, box->p.deconvolve));
     RCHK(lagpm_green_device(ctx, box->st, box->p.n, box->p.precision, box->p.efold, box->p.deconvolve, box->spec));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[1], box->st));
     void* in2[1] = {box->spec};

@@ -211,13 +211,10 @@ int lagpm_snapshot_write(lagpm_ctx* ctx, const char* src_path, const char* dst_p
         return fail(ctx, LAGPM_E_INVAL, "template holds SPH/star/AGN particles: only DM-only (GOTPM) files are written");
     if (np != (size_t)s.np_dm)
         return fail(ctx, LAGPM_E_INVAL, "np " + std::to_string(np) + " differs from the template's DNp " + std::to_string(s.np_dm));
-    FILE* in = fopen(src_path, "rb");
-    if (!in) return fail(ctx, LAGPM_E_IO, std::string("cannot open ") + src_path);
-    file_closer fi{in};
-    const size_t tail_at = s.header_bytes + (size_t)s.np_dm * s.record_bytes;
-    std::vector<char> tail(s.file_bytes - tail_at);
-    if (fseek(in, (long)tail_at, SEEK_SET) != 0 || (tail.size() && fread(tail.data(), 1, tail.size(), in) != tail.size()))
-        return fail(ctx, LAGPM_E_IO, "cannot read the template's trailing block");
+    // No trailing SimParameters block is written: jread freads one over the parsed header after
+    // the particles (Cosmos/CosmosRW/kjhrw.c:118), so a block carried over from the template would
+    // put the template's Anow, Stepcount and particle type back; without it that fread reads
+    // nothing and the header written here stands.
     FILE* out = fopen(dst_path, "wb");
     if (!out) return fail(ctx, LAGPM_E_IO, std::string("cannot create ") + dst_path + ": " + strerror(errno));
     file_closer fo{out};
@@ -248,7 +245,6 @@ int lagpm_snapshot_write(lagpm_ctx* ctx, const char* src_path, const char* dst_p
         ok = ok && fwrite(ln.data(), 1, ln.size(), out) == ln.size();
     }
     if (np) ok = ok && fwrite(dm_bp, sizeof(dm32), np, out) == np;
-    if (tail.size()) ok = ok && fwrite(tail.data(), 1, tail.size(), out) == tail.size();
     if (!ok) return fail(ctx, LAGPM_E_IO, std::string("write to ") + dst_path + " failed");
     return LAGPM_OK;
 }

@@ -184,7 +184,7 @@ __global__ void gather_all_kernel(gather_set g, const uint32_t* __restrict__ per
 }
 
 __device__ inline void tree_reset(int t, uint32_t* __restrict__ dsz, unsigned long long* __restrict__ counters) {
-    if (t == DSZ_NODES || t == DSZ_GROUPS || t == DSZ_OVERFLOW || t == DSZ_DEEPER) dsz[t] = 0u;
+    if (t == DSZ_NODES || t == DSZ_GROUPS || t == DSZ_OVERFLOW || t == DSZ_DEEPER || t == DSZ_WALKOVF) dsz[t] = 0u;
     if ((t >= DSZ_LVL && t < DSZ_LOFF) || (t >= DSZ_LCUR && t < DSZ_COUNT)) dsz[t] = 0u;
     if (t < 4) counters[t] = 0ull;
 }
@@ -522,10 +522,22 @@ __global__ void moments_f64_kernel(const double* __restrict__ x, const double* _
 // lattice or a halo are uneven (many of a few particles each), so merging cuts the number of
 // groups, and with it of tree walks, to about half while keeping each group inside its parent.
 // The parent (or the big leaf, or the small cell root) owns its groups; group_pack visits them.
+// Run groups (runs != 0): a chaining cell's particles, in their Morton order, are cut into
+// consecutive runs of G (the last one shorter), whatever the tree below the cell: every group but
+// the last of a cell fills a wavefront.
 template <typename F>
-__device__ inline uint32_t group_pack(const int4& nd, int L, int G, const int4* __restrict__ info,
+__device__ inline uint32_t group_pack(const int4& nd, int L, int G, int runs, const int4* __restrict__ info,
                                       const int* __restrict__ child, const int* __restrict__ parent, size_t v,
                                       F&& emit) {
+    if (runs) {
+        if (nd.w != L) return 0u;
+        const uint32_t c = (uint32_t)((nd.y + G - 1) / G);
+        for (uint32_t t = 0; t < c; t++) {
+            const int b = nd.x + (int)t * G;
+            emit(b, min(nd.x + nd.y, b + G) - b);
+        }
+        return c;
+    }
     if (nd.y <= G) {
         // a small node below the cell level belongs to its parent's runs (a node the child table
         // does not link, parent -1, is a group of its own)
@@ -556,7 +568,8 @@ __device__ inline uint32_t group_pack(const int4& nd, int L, int G, const int4* 
     if (rn) { emit(rb, rn); c++; }
     return c;
 }
-__global__ void group_count_kernel(const uint32_t* __restrict__ dsz, size_t cap, int L, int G, const int4* __restrict__ info,
+__global__ void group_count_kernel(const uint32_t* __restrict__ dsz, size_t cap, int L, int G, int runs,
+                                   const int4* __restrict__ info,
                                    const int* __restrict__ child, const int* __restrict__ parent,
                                    uint32_t* __restrict__ gcnt) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -565,9 +578,9 @@ __global__ void group_count_kernel(const uint32_t* __restrict__ dsz, size_t cap,
         gcnt[v] = 0u;
         return;
     }
-    gcnt[v] = group_pack(info[v], L, G, info, child, parent, v, [](int, int) {});
+    gcnt[v] = group_pack(info[v], L, G, runs, info, child, parent, v, [](int, int) {});
 }
-__global__ void group_write_kernel(uint32_t* __restrict__ dsz, size_t cap, size_t gcap, int L, int G,
+__global__ void group_write_kernel(uint32_t* __restrict__ dsz, size_t cap, size_t gcap, int L, int G, int runs,
                                    const int4* __restrict__ info, const int* __restrict__ child,
                                    const int* __restrict__ parent, const uint32_t* __restrict__ gcnt,
                                    const uint32_t* __restrict__ goff,
@@ -581,7 +594,7 @@ __global__ void group_write_kernel(uint32_t* __restrict__ dsz, size_t cap, size_
         return;
     }
     uint32_t g = goff[v];
-    group_pack(info[v], L, G, info, child, parent, v, [&](int b, int m) { groups[g++] = make_int2(b, m); });
+    group_pack(info[v], L, G, runs, info, child, parent, v, [&](int b, int m) { groups[g++] = make_int2(b, m); });
 }
 __global__ void group_box_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
                                  const uint32_t* __restrict__ dsz, size_t gcap, const int2* __restrict__ groups,
@@ -639,7 +652,7 @@ struct walk_box {
     const uint64_t* key;
     double *tvx, *tvy, *tvz;       // the tree kick of each particle (written, not added: see below)
     unsigned long long* counters;  // [0] n_pp, [1] n_pc, [2] walk stack overflow
-    const uint32_t* dsz;
+    uint32_t* dsz;                 // sizes; the walk sets DSZ_WALKOVF on a stack overflow
     int cell_shift;
     int mcell;
     int naccept;
@@ -667,7 +680,9 @@ __global__ void walk_prefix_kernel(const walk_box* __restrict__ B, int nbox, uin
     for (int x = 0; x < WALK_XCDS; x++) ticket[x] = 0ull;
 }
 
-template <bool LOGT>
+// CENSUS: also count the evaluated particle pairs that lie within the cutoff (counters[3]), a
+// measurement of the lists' useful fraction; it adds a compare and an add per pair.
+template <bool LOGT, bool CENSUS>
 __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* __restrict__ B, const uint32_t* __restrict__ gprefix,
                                                                 int nbox, const float2* __restrict__ tab0,
                                                                 const float4* __restrict__ tab12, int* __restrict__ stacks,
@@ -753,6 +768,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         const int cx = (int)compact3(cellkey), cy = (int)compact3(cellkey >> 1), cz = (int)compact3(cellkey >> 2);
         float ax = 0.f, ay = 0.f, az = 0.f;
         unsigned long long npp = 0, npc = 0;
+        unsigned nin = 0;  // CENSUS: this lane's pairs within the cutoff
         int sp = 0;
         bool overflow = false;
         {
@@ -799,6 +815,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         auto pp_one = [&](float4 s) {
             float dx = xt.x - s.x, dy = xt.y - s.y, dz = xt.z - s.z;
             float r2 = dx * dx + dy * dy + dz * dz;
+            if constexpr (CENSUS) nin += (valid && r2 <= A.rc2) ? 1u : 0u;
             float t;
             const int nt = table_bin<LOGT>(r2, t, lam);
             float2 tb = s_tab0[nt];
@@ -823,6 +840,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                         dx[u] = xt.x - s[u].x; dy[u] = xt.y - s[u].y; dz[u] = xt.z - s[u].z;
                         r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
                         if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
+                        if constexpr (CENSUS) nin += (valid && r2[u] <= A.rc2) ? 1u : 0u;
                     }
                     if constexpr (!LOGT)
                         if (!__any(any)) continue;  // wave-uniform skip of four far sources
@@ -935,6 +953,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                         dx[u] = xt.x - c[u].x; dy[u] = xt.y - c[u].y; dz[u] = xt.z - c[u].z;
                         r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
                         if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
+                        if constexpr (CENSUS) nin += (valid && r2[u] <= A.rc2) ? 1u : 0u;
                     }
                     if constexpr (!LOGT)
                         if (!__any(any)) continue;  // wave-uniform skip of two far nodes
@@ -1117,7 +1136,10 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         // The kick is written to its own arrays rather than added to the velocities, so the walk
         // can run while the PM kick of the same step updates them; the drift (or, for a tree
         // kick alone, an add kernel) adds it afterwards, in the same order as v += pm; v += tree.
-        if (overflow && lane == 0) atomicAdd(&A.counters[2], 1ull);
+        if (overflow && lane == 0) {
+            atomicAdd(&A.counters[2], 1ull);
+            atomicOr(&A.dsz[DSZ_WALKOVF], 1u);  // the step's drift reads it and leaves the box unmoved
+        }
         if (valid && rep == 0) {
             A.tvx[me] = overflow ? 0.0 : (double)(A.fact * ax);
             A.tvy[me] = overflow ? 0.0 : (double)(A.fact * ay);
@@ -1126,6 +1148,10 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         if (lane == 0) {
             atomicAdd(&A.counters[0], npp * (unsigned long long)cnt);
             atomicAdd(&A.counters[1], npc * (unsigned long long)cnt);
+        }
+        if constexpr (CENSUS) {
+            for (int o = 32; o > 0; o >>= 1) nin += (unsigned)__shfl_xor((int)nin, o, 64);
+            if (lane == 0) atomicAdd(&A.counters[3], (unsigned long long)nin);
         }
     }
 }
@@ -1146,20 +1172,31 @@ struct walk64_args {
     float lam;
     double *vx, *vy, *vz;
     unsigned long long* counters;
-    const uint32_t* dsz;
+    uint32_t* dsz;
     size_t np;
     int mcell;
     double box, cellw, rc2, rcut, inv_theta2, fact;
     int naccept;
+    // sampled walk: target t is the particle at sample[t].x, opened as a group of its own (the
+    // per-particle test), its acceleration written to out[3t..3t+2] (velocities untouched)
+    const int2* sample;
+    double* out;
+    size_t nsample;
 };
 
 template <bool LOGT>
 // one target particle; returns false on a stack overflow (the kick is then not applied)
-__device__ inline bool walk_f64_target(const walk64_args& A, size_t i, unsigned long long& npp, unsigned long long& npc) {
+__device__ inline bool walk_f64_target(const walk64_args& A, size_t i, unsigned long long& npp, unsigned long long& npc,
+                                       double* out) {
     const double px0 = A.x[i], py0 = A.y[i], pz0 = A.z[i];
-    const int g = A.pgroup[i];
-    const double lo[3] = {A.gbox[6 * g], A.gbox[6 * g + 1], A.gbox[6 * g + 2]};
-    const double hi[3] = {A.gbox[6 * g + 3], A.gbox[6 * g + 4], A.gbox[6 * g + 5]};
+    double lo[3] = {px0, py0, pz0}, hi[3] = {px0, py0, pz0};  // a group of one (sampled walk)
+    if (!out) {
+        const int g = A.pgroup[i];
+        for (int a = 0; a < 3; a++) {
+            lo[a] = A.gbox[6 * g + a];
+            hi[a] = A.gbox[6 * g + 3 + a];
+        }
+    }
     // all group members share a chaining cell: the level-L prefix of their Morton key
     const uint64_t cellkey = A.key[i] >> A.cell_shift;
     const int cx = (int)compact3(cellkey), cy = (int)compact3(cellkey >> 1), cz = (int)compact3(cellkey >> 2);
@@ -1228,6 +1265,12 @@ __device__ inline bool walk_f64_target(const walk64_args& A, size_t i, unsigned 
         }
     }
     if (overflow) return false;
+    if (out) {
+        out[0] = A.fact * ax;
+        out[1] = A.fact * ay;
+        out[2] = A.fact * az;
+        return true;
+    }
     A.vx[i] += A.fact * ax;
     A.vy[i] += A.fact * ay;
     A.vz[i] += A.fact * az;
@@ -1246,16 +1289,26 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
 // wave and counter (an atomic per thread on one address serialises millions of operations).
 template <bool LOGT>
 __global__ void __launch_bounds__(128) walk_f64_kernel(walk64_args A) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long npp = 0, npc = 0, ovf = 0;
-    if (i < A.np && !A.dsz[DSZ_OVERFLOW]) ovf = walk_f64_target<LOGT>(A, i, npp, npc) ? 0ull : 1ull;
+    if (!A.dsz[DSZ_OVERFLOW]) {
+        if (A.sample) {
+            if (t < A.nsample)
+                ovf = walk_f64_target<LOGT>(A, (size_t)A.sample[t].x, npp, npc, A.out + 3 * t) ? 0ull : 1ull;
+        } else if (t < A.np) {
+            ovf = walk_f64_target<LOGT>(A, t, npp, npc, nullptr) ? 0ull : 1ull;
+        }
+    }
     npp = wave_sum_u64(npp);
     npc = wave_sum_u64(npc);
     ovf = wave_sum_u64(ovf);
     if ((threadIdx.x & 63) == 0) {
         if (npp) atomicAdd(&A.counters[0], npp);
         if (npc) atomicAdd(&A.counters[1], npc);
-        if (ovf) atomicAdd(&A.counters[2], ovf);
+        if (ovf) {
+            atomicAdd(&A.counters[2], ovf);
+            atomicOr(&A.dsz[DSZ_WALKOVF], 1u);
+        }
     }
 }
 
@@ -1422,11 +1475,12 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
         HIPCHK(ctx, hipGetLastError());
     }
 
-    hipLaunchKernelGGL(group_count_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, L, G, box->nd_info,
-                       box->nd_child, box->nd_parent, box->gflag);
+    const int runs = box->tune.runs;
+    hipLaunchKernelGGL(group_count_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, L, G, runs,
+                       box->nd_info, box->nd_child, box->nd_parent, box->gflag);
     HIPCHK(ctx, hipGetLastError());
     RCHK(lagpm_exclusive_scan_u32(ctx, st, box->scan_tmp, box->gflag, box->goff, cap, box->dsz + DSZ_GROUPS));
-    hipLaunchKernelGGL(group_write_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, gcap, L, G,
+    hipLaunchKernelGGL(group_write_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->dsz, cap, gcap, L, G, runs,
                        box->nd_info, box->nd_child, box->nd_parent, box->gflag, box->goff, box->groups);
     HIPCHK(ctx, hipGetLastError());
     if (f64) {
@@ -1438,8 +1492,10 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     return LAGPM_OK;
 }
 
-int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* facts, int nbox, int reserve_cu) {
+int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* facts, int nbox, int reserve_cu,
+                        const walk_sample* sample) {
     if (nbox <= 0) return LAGPM_OK;
+    if (sample && nbox != 1) return lagpm_fail(ctx, LAGPM_E_INVAL, "a sampled walk takes one box");
     if ((size_t)nbox > ctx->walk_box_cap) {
         size_t cap = std::max<size_t>(64, (size_t)nbox * 2);
         RCHK(regrow(ctx, &ctx->walk_boxes, cap));
@@ -1458,6 +1514,10 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
         A.q1 = box->nd_q1; A.cellroot = box->cellroot; A.groups = box->groups; A.key = box->key;
         A.tvx = box->alt[0]; A.tvy = box->alt[1]; A.tvz = box->alt[2];  // free between sorts
         A.counters = box->counters; A.dsz = box->dsz;
+        if (sample) {  // the sampled particles as groups of one (sizes: the sample's own words)
+            A.groups = sample->groups;
+            A.dsz = sample->dsz;
+        }
         A.cell_shift = 3 * (LAGPM_KEY_BITS - box->level_cell);
         A.mcell = box->mcell;
         A.naccept = LAGPM_NACCEPT;
@@ -1482,17 +1542,15 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
     const int full = (int)(ctx->walk_stack_entries / ((size_t)STACK_MAX * WALK_WAVES));
     const int keep = full - ((std::max(0, reserve_cu) + WALK_XCDS - 1) / WALK_XCDS) * WALK_XCDS;
     const unsigned blocks = (unsigned)(keep >= WALK_XCDS ? keep : std::min(full, WALK_XCDS));
-    if (tab->logr2)
-        hipLaunchKernelGGL(walk_f32_kernel<true>, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes,
-                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
-    else
-        hipLaunchKernelGGL(walk_f32_kernel<false>, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes,
-                           ctx->walk_prefix, nbox, tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
+    auto kern = tab->logr2 ? (ctx->walk_census ? walk_f32_kernel<true, true> : walk_f32_kernel<true, false>)
+                           : (ctx->walk_census ? walk_f32_kernel<false, true> : walk_f32_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(WALK_THREADS), 0, ctx->stream, ctx->walk_boxes, ctx->walk_prefix, nbox,
+                       tab->tab0, tab->tab12, ctx->walk_stack, ctx->walk_ticket);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }
 
-int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact) {
+int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact, const walk_sample* sample) {
     const double th = box->p.theta;
     const double cellw = (double)box->p.n / (double)box->mcell;
     walk64_args A;
@@ -1504,11 +1562,15 @@ int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact) {
     A.rcut = box->p.rsphere; A.inv_theta2 = th > 0 ? 1.0 / (th * th) : 1e30;
     A.fact = fact * box->p.mass / ((double)box->p.n * (double)box->p.n);
     A.naccept = LAGPM_NACCEPT;
+    A.sample = sample ? sample->groups : nullptr;
+    A.out = sample ? sample->out : nullptr;
+    A.nsample = sample ? sample->count : 0;
     if (box->np == 0) return LAGPM_OK;
+    const size_t nthreads = sample ? sample->count : box->np;
     if (box->tab->logr2)
-        hipLaunchKernelGGL(walk_f64_kernel<true>, dim3(blocks_for(box->np, 128)), dim3(128), 0, box->st, A);
+        hipLaunchKernelGGL(walk_f64_kernel<true>, dim3(blocks_for(nthreads, 128)), dim3(128), 0, box->st, A);
     else
-        hipLaunchKernelGGL(walk_f64_kernel<false>, dim3(blocks_for(box->np, 128)), dim3(128), 0, box->st, A);
+        hipLaunchKernelGGL(walk_f64_kernel<false>, dim3(blocks_for(nthreads, 128)), dim3(128), 0, box->st, A);
     HIPCHK(ctx, hipGetLastError());
     return LAGPM_OK;
 }
@@ -1521,6 +1583,7 @@ int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry) {
     RCHK(lagpm_d2h(ctx, c, box->counters, sizeof(c), box->st));
     box->stats.n_pp = c[0];
     box->stats.n_pc = c[1];
+    box->stats.n_pp_within = c[3];
     box->stats.n_nodes = d[DSZ_NODES];
     box->stats.n_groups = d[DSZ_GROUPS];
     if (d[DSZ_DEEPER]) {
@@ -1554,6 +1617,77 @@ int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry) {
         }
     }
     if (c[2]) return lagpm_fail(ctx, LAGPM_E_OVERFLOW, "tree walk stack overflow in " + std::to_string(c[2]) + " groups");
+    return LAGPM_OK;
+}
+
+namespace {
+// slot[ids[t]] = t: the sample index of each sampled caller index (slot starts at -1)
+__global__ void sample_slot_kernel(const int64_t* __restrict__ ids, size_t count, int* __restrict__ slot) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) slot[ids[t]] = (int)t;
+}
+__global__ void fill_i32_kernel(int* __restrict__ a, size_t m, int v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+// every sample group starts as particle 0 alone (a valid group of one), then gets its particle
+__global__ void sample_init_kernel(int2* __restrict__ groups, size_t count) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) groups[t] = make_int2(0, 1);
+}
+// sample t is the particle now at position p (the box's current order): a group of one at p
+__global__ void sample_groups_kernel(const uint32_t* __restrict__ id, size_t np, const int* __restrict__ slot,
+                                     int2* __restrict__ groups) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int t = slot[id[p]];
+    if (t >= 0) groups[t] = make_int2((int)p, 1);
+}
+// the sample's size words: the box's (its overflow flag) with the group count replaced
+__global__ void sample_dsz_kernel(const uint32_t* __restrict__ dsz, uint32_t* __restrict__ sdsz, uint32_t count) {
+    for (int i = threadIdx.x; i < DSZ_COUNT; i += blockDim.x) sdsz[i] = i == DSZ_GROUPS ? count : dsz[i];
+}
+__global__ void sample_gather_kernel(const int2* __restrict__ groups, size_t count, const double* __restrict__ tx,
+                                     const double* __restrict__ ty, const double* __restrict__ tz, double* __restrict__ out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int p = groups[t].x;
+    out[3 * t] = tx[p];
+    out[3 * t + 1] = ty[p];
+    out[3 * t + 2] = tz[p];
+}
+}  // namespace
+
+// Walks the sampled particles of a built tree, each alone; acc (device, 3 per sample) gets their
+// accelerations. ids (device) are caller indices, distinct and below np (checked by the caller).
+int lagpm_tree_walk_sample(lagpm_ctx* ctx, lagpm_box* box, const int64_t* ids, size_t count, int* slot, int2* groups,
+                           uint32_t* sdsz, double* acc) {
+    const size_t np = box->np;
+    hipStream_t st = box->st;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, slot, np, -1);
+    hipLaunchKernelGGL(sample_slot_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, st, ids, count, slot);
+    hipLaunchKernelGGL(sample_init_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, st, groups, count);
+    hipLaunchKernelGGL(sample_groups_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->id, np, slot, groups);
+    hipLaunchKernelGGL(sample_dsz_kernel, dim3(1), dim3(64), 0, st, box->dsz, sdsz, (uint32_t)count);
+    HIPCHK(ctx, hipGetLastError());
+    walk_sample smp{groups, sdsz, acc, count};
+    if (box->p.precision == LAGPM_PREC_FP64) {
+        RCHK(lagpm_tree_walk_f64(ctx, box, 1.0, &smp));
+    } else {
+        // the walk runs on the context stream and writes its kicks at the particles' positions
+        if (st != ctx->stream) {
+            HIPCHK(ctx, hipEventRecord(box->ev[10], st));
+            HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, box->ev[10], 0));
+        }
+        const double one = 1.0;
+        RCHK(lagpm_tree_walk_f32(ctx, &box, &one, 1, 0, &smp));
+        hipLaunchKernelGGL(sample_gather_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, ctx->stream, groups, count,
+                           (const double*)box->alt[0], (const double*)box->alt[1], (const double*)box->alt[2], acc);
+        HIPCHK(ctx, hipGetLastError());
+        if (st != ctx->stream) {
+            HIPCHK(ctx, hipEventRecord(box->ev[11], ctx->stream));
+            HIPCHK(ctx, hipStreamWaitEvent(st, box->ev[11], 0));
+        }
+    }
     return LAGPM_OK;
 }
 

@@ -24,6 +24,7 @@ import numpy as np
 PUSH, PULL, KICK = 0, 1, 2  # eunha.h:69-77
 LAW_PMFIT, LAW_ERFC, LAW_TREEFORCE = 0, 1, 2
 PREC_FP32, PREC_FP64 = 0, 1
+GROUPS_TREE, GROUPS_RUNS = 0, 1
 NSPLINE = 4096
 
 _STATUS = {
@@ -58,7 +59,7 @@ class EvolFact(ctypes.Structure):
 class TreeStats(ctypes.Structure):
     _fields_ = [("n_pp", ctypes.c_uint64), ("n_pc", ctypes.c_uint64), ("n_nodes", ctypes.c_uint64),
                 ("n_groups", ctypes.c_uint64), ("ms_build", ctypes.c_double), ("ms_walk", ctypes.c_double),
-                ("deepest", ctypes.c_int32), ("depth_limit", ctypes.c_int32)]
+                ("deepest", ctypes.c_int32), ("depth_limit", ctypes.c_int32), ("n_pp_within", ctypes.c_uint64)]
 
 
 class SnapshotInfo(ctypes.Structure):
@@ -107,6 +108,7 @@ SIGNATURES = {
     "lagpm_box_destroy": (ctypes.c_int, [_P, _P]),
     "lagpm_box_params": (ctypes.c_int, [_P, ctypes.POINTER(Params)]),
     "lagpm_box_np": (ctypes.c_size_t, [_P]),
+    "lagpm_box_set_cosmology": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "lagpm_box_device_bytes": (ctypes.c_size_t, [_P]),
     "lagpm_estimate_box_bytes": (ctypes.c_size_t, [ctypes.POINTER(Params), ctypes.c_size_t]),
     "lagpm_box_upload": (ctypes.c_int, [_P, _P, _D, _D, _D, _D, _D, _D]),
@@ -135,11 +137,14 @@ SIGNATURES = {
     "lagpm_power_spectrum_noalloc": (ctypes.c_int, [_P, _P, ctypes.c_double, _D, _D, _D, _D, _D]),
     "lagpm_treecorrection": (ctypes.c_int, [_P, _P, ctypes.POINTER(EvolFact)]),
     "lagpm_tree_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(TreeStats)]),
+    "lagpm_tree_accel_sample": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int64), ctypes.c_size_t, _D]),
     "lagpm_tree_nodes_get": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
                                             ctypes.POINTER(ctypes.c_int32), _D, _D,
                                             ctypes.POINTER(ctypes.c_uint32)]),
     "lagpm_box_set_tuning": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "lagpm_box_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "lagpm_box_set_group_mode": (ctypes.c_int, [_P, ctypes.c_int]),
+    "lagpm_box_get_group_mode": (ctypes.c_int, [_P]),
     "lagpm_box_autotune": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "lagpm_box_timings": (ctypes.c_int, [_P, _P, ctypes.POINTER(Timings)]),
     "lagpm_evolfactor": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_double, ctypes.c_double,
@@ -149,6 +154,7 @@ SIGNATURES = {
     "lagpm_kdk_step": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double]),
     "lagpm_batch_step": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int, _D, _D]),
     "lagpm_set_step_graphs": (ctypes.c_int, [_P, ctypes.c_int]),
+    "lagpm_set_walk_census": (ctypes.c_int, [_P, ctypes.c_int]),
     "lagpm_set_batch_pipeline": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
 }
 
@@ -225,6 +231,10 @@ class Context:
     def set_step_graphs(self, enable: bool):
         self.check(self.lib.lagpm_set_step_graphs(self.h, 1 if enable else 0))
 
+    def set_walk_census(self, enable: bool):
+        """FP32 walks also count the evaluated pairs within the cutoff (TreeStats.n_pp_within)."""
+        self.check(self.lib.lagpm_set_walk_census(self.h, 1 if enable else 0))
+
     def set_batch_pipeline(self, stages: int, reserve_cu: int = 32):
         self.check(self.lib.lagpm_set_batch_pipeline(self.h, stages, reserve_cu))
 
@@ -269,6 +279,13 @@ class Box:
     @property
     def device_bytes(self):
         return self.lib.lagpm_box_device_bytes(self.h)
+
+    def set_cosmology(self, amax: float, omep: float, omeplam: float):
+        """Another cosmology for the following steps (the box's memory and state are kept)."""
+        rc = self.lib.lagpm_box_set_cosmology(self.h, amax, omep, omeplam)
+        if rc != 0:
+            raise LagpmError(rc, "bad cosmology")
+        self.params.amax, self.params.omep, self.params.omeplam = amax, omep, omeplam
 
     def upload(self, x, y, z, vx=None, vy=None, vz=None):
         a = [np.ascontiguousarray(v, dtype=np.float64) if v is not None else None for v in (x, y, z, vx, vy, vz)]
@@ -353,6 +370,15 @@ class Box:
     def treecorrection(self, ef: EvolFact):
         self.ctx.check(self.lib.lagpm_treecorrection(self.ctx.h, self.h, ctypes.byref(ef)))
 
+    def tree_accel_sample(self, ids) -> np.ndarray:
+        """Short-range acceleration (per unit kick factor) of the particles with caller indices
+        `ids`, each walked as a group of one; shape (len(ids), 3). Velocities are unchanged."""
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        acc = np.empty(3 * len(ids))
+        self.ctx.check(self.lib.lagpm_tree_accel_sample(self.ctx.h, self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                        len(ids), _dp(acc)))
+        return acc.reshape(-1, 3)
+
     def tree_stats(self) -> TreeStats:
         s = TreeStats()
         self.lib.lagpm_tree_stats_get(self.h, ctypes.byref(s))
@@ -387,21 +413,27 @@ class Box:
     def drift(self, pfact: float):
         self.ctx.check(self.lib.lagpm_drift(self.ctx.h, self.h, pfact))
 
-    def set_tuning(self, nleaf: int, group: int):
+    def set_tuning(self, nleaf: int, group: int, mode: int = None):
+        """Leaf size, group size and (when given) group mode (GROUPS_TREE or GROUPS_RUNS)."""
         rc = self.lib.lagpm_box_set_tuning(self.h, nleaf, group)
         if rc != 0:
             raise LagpmError(rc, f"bad tuning nleaf={nleaf} group={group}")
+        if mode is not None:
+            rc = self.lib.lagpm_box_set_group_mode(self.h, mode)
+            if rc != 0:
+                raise LagpmError(rc, f"bad group mode {mode}")
 
     def tuning(self):
+        """(nleaf, group, group mode)."""
         a, b = ctypes.c_int(), ctypes.c_int()
         self.lib.lagpm_box_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b))
-        return a.value, b.value
+        return a.value, b.value, self.lib.lagpm_box_get_group_mode(self.h)
 
     def autotune(self):
-        """Time tree build + walk for a set of (nleaf, group) and keep the fastest."""
+        """Time tree build + walk for a set of (nleaf, group, group mode), keep and return the fastest."""
         a, b = ctypes.c_int(), ctypes.c_int()
         self.ctx.check(self.lib.lagpm_box_autotune(self.ctx.h, self.h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+        return a.value, b.value, self.lib.lagpm_box_get_group_mode(self.h)
 
     def power_spectrum(self, boxsize: float):
         """CorrMeasure of the current particles: (pk, npk, cor, sigma8, slope), n+1 bins."""

@@ -113,6 +113,8 @@ void orc_tsc_fixed(int n, size_t np, const double* x, const double* y, const dou
                    float mass, int scale_bits, int64_t* acc) {
     size_t m = (size_t)n * n * n;
     memset(acc, 0, m * sizeof(int64_t));
+    /* integer sums: the threads' order of additions does not change the result */
+#pragma omp parallel for schedule(static)
     for (size_t p = 0; p < np; p++) {
         tsc_axis ax, ay, az;
         tsc_axis_mass(x[p], mass, &ax);
@@ -126,7 +128,9 @@ void orc_tsc_fixed(int n, size_t np, const double* x, const double* y, const dou
                     long ii = wrapl(ax.i[a], n);
                     float wxy = ax.w[a] * ay.w[b];
                     float cval = wxy * az.w[c];
-                    acc[ii + (size_t)n * (jj + (size_t)n * kk)] += llrint(ldexp((double)cval, scale_bits));
+                    const int64_t q = llrint(ldexp((double)cval, scale_bits));
+#pragma omp atomic
+                    acc[ii + (size_t)n * (jj + (size_t)n * kk)] += q;
                 }
             }
         }
@@ -433,13 +437,52 @@ void orc_fda4_kick_f64(int n, const double* phi, size_t np, const double* x, con
 #undef PHID
 
 /* Full PM chain, Cosmos/pmmain.c:390-527 (deposit, fold, -1, r2c, psolver, c2r, fda4). */
+static void pm_accel_impl(int n, size_t np, const double* x, const double* y, const double* z, double mass,
+                          double efold, int deconvolve, int prec, size_t ntarget, const int64_t* targets,
+                          double* acc, double* sec);
+
 void orc_pm_accel(int n, size_t np, const double* x, const double* y, const double* z,
                   double mass, double efold, int deconvolve, int prec, double* acc) {
+    pm_accel_impl(n, np, x, y, z, mass, efold, deconvolve, prec, np, NULL, acc, NULL);
+}
+
+/* The same chain with the wall time of each phase in sec[0..4]: TSC (deposit and fold), F-FFT,
+ * psolv, B-FFT, 4FDA (gradient, interpolation), the timers of Cosmos/pmmain.c:447-522. */
+void orc_pm_accel_timed(int n, size_t np, const double* x, const double* y, const double* z,
+                        double mass, double efold, int deconvolve, int prec, double* acc, double* sec) {
+    pm_accel_impl(n, np, x, y, z, mass, efold, deconvolve, prec, np, NULL, acc, sec);
+}
+
+/* The mesh of all np particles, the interpolation at the listed particles only:
+ * acc[3t..3t+2] for particle targets[t]. */
+void orc_pm_accel_at(int n, size_t np, const double* x, const double* y, const double* z, double mass,
+                     double efold, int deconvolve, int prec, size_t ntarget, const int64_t* targets, double* acc) {
+    pm_accel_impl(n, np, x, y, z, mass, efold, deconvolve, prec, ntarget, targets, acc, NULL);
+}
+
+static void pm_accel_impl(int n, size_t np, const double* x, const double* y, const double* z, double mass,
+                          double efold, int deconvolve, int prec, size_t ntarget, const int64_t* targets,
+                          double* acc, double* sec) {
+    double tt[6];
+    /* interpolation positions: all particles, or the listed ones */
+    const double *kx = x, *ky = y, *kz = z;
+    double* tpos = NULL;
+    if (targets) {
+        tpos = (double*)malloc(3 * ntarget * sizeof(double));
+        for (size_t t = 0; t < ntarget; t++) {
+            tpos[t] = x[targets[t]];
+            tpos[ntarget + t] = y[targets[t]];
+            tpos[2 * ntarget + t] = z[targets[t]];
+        }
+        kx = tpos; ky = tpos + ntarget; kz = tpos + 2 * ntarget;
+    }
+#define ORC_TICK(k) do { if (sec) tt[k] = omp_get_wtime(); } while (0)
     size_t m = (size_t)n * n * n;
     int h = n / 2 + 1;
     size_t K = (size_t)h * n * n;
     double* rho = (double*)malloc(m * sizeof(double));
     double* spec = (double*)malloc(2 * K * sizeof(double));
+    ORC_TICK(0);
     if (prec == 0) {
         int S = orc_fixed_scale_bits((double)np * mass);
         int64_t* a = (int64_t*)malloc(m * sizeof(int64_t));
@@ -447,19 +490,24 @@ void orc_pm_accel(int n, size_t np, const double* x, const double* y, const doub
         orc_tsc_fixed(n, np, x, y, z, (float)mass, S, a);
         orc_fixed_to_density(n, a, S, den);
         for (size_t i = 0; i < m; i++) rho[i] = den[i];
+        ORC_TICK(1);
         orc_r2c(n, rho, spec);
+        ORC_TICK(2);
         float* fs = (float*)malloc(2 * K * sizeof(float));
         for (size_t i = 0; i < 2 * K; i++) fs[i] = (float)spec[i];
         orc_psolver_f32(n, efold, deconvolve, fs);
         for (size_t i = 0; i < 2 * K; i++) spec[i] = fs[i];
+        ORC_TICK(3);
         orc_c2r(n, spec, rho);
+        ORC_TICK(4);
         for (size_t i = 0; i < m; i++) den[i] = (float)rho[i];
-        float* dv = (float*)malloc(3 * np * sizeof(float));
-        orc_fda4_kick(n, den, np, x, y, z, 1.0f, dv, dv + np, dv + 2 * np);
-        for (size_t p = 0; p < np; p++) {
-            acc[3 * p] = dv[p]; acc[3 * p + 1] = dv[np + p]; acc[3 * p + 2] = dv[2 * np + p];
+        float* dv = (float*)malloc(3 * ntarget * sizeof(float));
+        orc_fda4_kick(n, den, ntarget, kx, ky, kz, 1.0f, dv, dv + ntarget, dv + 2 * ntarget);
+        for (size_t p = 0; p < ntarget; p++) {
+            acc[3 * p] = dv[p]; acc[3 * p + 1] = dv[ntarget + p]; acc[3 * p + 2] = dv[2 * ntarget + p];
         }
         free(dv); free(fs); free(a); free(den);
+        ORC_TICK(5);
     } else {
         /* FP64: deposit with double weights, double FFT, double gradient */
         memset(rho, 0, m * sizeof(double));
@@ -474,12 +522,20 @@ void orc_pm_accel(int n, size_t np, const double* x, const double* y, const doub
                             mass * wx[a] * wy[b] * wz[c];
         }
         for (size_t i = 0; i < m; i++) rho[i] -= 1.0;
+        ORC_TICK(1);
         orc_r2c(n, rho, spec);
+        ORC_TICK(2);
         orc_psolver_f64(n, efold, deconvolve, spec);
+        ORC_TICK(3);
         orc_c2r(n, spec, rho);
-        orc_fda4_kick_f64(n, rho, np, x, y, z, 1.0, acc);
+        ORC_TICK(4);
+        orc_fda4_kick_f64(n, rho, ntarget, kx, ky, kz, 1.0, acc);
+        ORC_TICK(5);
     }
-    free(rho); free(spec);
+    free(rho); free(spec); free(tpos);
+    if (sec)
+        for (int k = 0; k < 5; k++) sec[k] = tt[k + 1] - tt[k];
+#undef ORC_TICK
 }
 
 /* ------------------------------------------------------------------------------------

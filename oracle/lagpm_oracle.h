@@ -49,6 +49,12 @@ void   orc_fda4_kick_f64(int n, const double* phi, size_t np, const double* x, c
 /* Full PM force chain. prec 0: reference storage (float mesh, float kick); prec 1: FP64. */
 void   orc_pm_accel(int n, size_t np, const double* x, const double* y, const double* z,
                     double mass, double efold, int deconvolve, int prec, double* acc /*3*np*/);
+/* the mesh of all particles, the kick interpolated at targets[0..ntarget) only (acc: 3*ntarget) */
+void   orc_pm_accel_at(int n, size_t np, const double* x, const double* y, const double* z, double mass,
+                       double efold, int deconvolve, int prec, size_t ntarget, const int64_t* targets, double* acc);
+/* the same with the wall time of each phase: sec[0..4] = TSC, F-FFT, psolv, B-FFT, 4FDA */
+void   orc_pm_accel_timed(int n, size_t np, const double* x, const double* y, const double* z,
+                          double mass, double efold, int deconvolve, int prec, double* acc, double* sec);
 
 /* Force tables, OST/Model4TreeForce.c:48-116 (law 0), erfc law (law 1), or
  * ForceLaw/treeforce.c:134-186 (law 2: eps fixed at 0.1, step 0.2 bin, entry 0 zero). */

@@ -50,6 +50,8 @@ def lib():
         "orc_fda4_kick": (None, [i, _F, sz, _D, _D, _D, f, _F, _F, _F]),
         "orc_fda4_kick_f64": (None, [i, _D, sz, _D, _D, _D, d, _D]),
         "orc_pm_accel": (None, [i, sz, _D, _D, _D, d, d, i, i, _D]),
+        "orc_pm_accel_timed": (None, [i, sz, _D, _D, _D, d, d, i, i, _D, _D]),
+        "orc_pm_accel_at": (None, [i, sz, _D, _D, _D, d, d, i, i, sz, _I64, _D]),
         "orc_force_spline": (None, [i, i, d, d, _D, _D]),
         "orc_force_spline_dr": (None, [i, i, d, d, d, _D, _D]),
         "orc_pmcf": (d, [i, d, d]), "orc_erfc_fr": (d, [i, d, d, d]),
@@ -174,11 +176,28 @@ def fda4_kick_f64(n, phi, pos, fact2=1.0):
     return dv.reshape(-1, 3)
 
 
-def pm_accel(n, pos, mass=1.0, efold=0.9, deconvolve=0, prec=0):
+def pm_accel(n, pos, mass=1.0, efold=0.9, deconvolve=0, prec=0, targets=None):
+    """PM acceleration of every particle, or (targets: indices) of those particles only, from the
+    mesh of all of them."""
     x, y, z = split(pos)
+    if targets is not None:
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        acc = np.empty(3 * len(t))
+        lib().orc_pm_accel_at(n, len(x), _dp(x), _dp(y), _dp(z), mass, efold, deconvolve, prec, len(t),
+                              t.ctypes.data_as(_I64), _dp(acc))
+        return acc.reshape(-1, 3)
     acc = np.empty(3 * len(x))
     lib().orc_pm_accel(n, len(x), _dp(x), _dp(y), _dp(z), mass, efold, deconvolve, prec, _dp(acc))
     return acc.reshape(-1, 3)
+
+
+def pm_accel_timed(n, pos, mass=1.0, efold=0.9, deconvolve=0, prec=0):
+    """pm_accel with the wall time of each phase: {TSC, F-FFT, psolv, B-FFT, 4FDA} in seconds."""
+    x, y, z = split(pos)
+    acc = np.empty(3 * len(x))
+    sec = np.zeros(5)
+    lib().orc_pm_accel_timed(n, len(x), _dp(x), _dp(y), _dp(z), mass, efold, deconvolve, prec, _dp(acc), _dp(sec))
+    return acc.reshape(-1, 3), dict(zip(("TSC", "F-FFT", "psolv", "B-FFT", "4FDA"), (float(v) for v in sec)))
 
 
 def force_spline(law, n, eps, rs, dr_bins=None):

@@ -67,3 +67,37 @@ def test_box_byte_estimate_scales():
     b2 = lagpm.estimate_box_bytes(lagpm.make_params(128), 128 ** 3)
     assert 6 < b1 / b2 < 10
     assert 2e9 < b1 < 20e9
+
+
+def test_c_translation_unit_compiles_links_and_agrees(tmp_path):
+    """tests/abi_check.c, a C program as a maintainer's EUNHA shim would be, compiles against
+    include/lagpm.h in strict C11, links against liblagpm.so with the address of every declared
+    entry point, and its host-only calls print what the Python binding computes."""
+    import shutil
+    import subprocess
+    src = os.path.join(ROOT, "tests", "abi_check.c")
+    text = open(src).read()
+    table = text[text.index("entry_points[]"):text.index("};", text.index("entry_points[]"))]
+    assert sorted(re.findall(r"\b(lagpm_[a-z0-9_]+)\b", table)) == declared_symbols()
+    cc = shutil.which("gcc") or shutil.which("cc")
+    assert cc, "no C compiler"
+    exe = str(tmp_path / "abi_check")
+    libdir = os.path.dirname(lagpm.LIB_PATH)
+    subprocess.run([cc, "-std=c11", "-Wall", "-Wextra", "-Werror", "-pedantic", "-I", os.path.join(ROOT, "include"), src,
+                    "-L", libdir, "-llagpm", f"-Wl,-rpath,{libdir}", "-o", exe], check=True)
+    out = dict(ln.split(" ", 1) for ln in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.splitlines())
+    p = lagpm.Params(64, lagpm.LAW_PMFIT, lagpm.PREC_FP32, 0, 1.0, 0.9, 0.1, 4.0, 0.5, 48.0, 0.26, 0.74)
+    diff, slope = lagpm.i_force_spline(p)
+    assert [float(v) for v in out["diff"].split()] == list(diff[100])
+    assert float(out["slope"]) == slope[2000, 0]
+    ef = lagpm.get_evol_factor(p, 1.0, 0.25)
+    # %.9g of a float names that float exactly
+    assert [np.float32(float(v)) for v in out["evolfact"].split()] == [np.float32(x) for x in (ef.fact1, ef.fact2, ef.pfact)]
+    assert int(out["bytes"]) == lagpm.estimate_box_bytes(p, 64 ** 3)
+    assert out["abi"].split()[0] == str(lagpm.load().lagpm_abi_version())
+    # the ctypes mirrors have the C structs' sizes and member offsets
+    for name, cls, last in (("params", lagpm.Params, "omeplam"), ("evolfact", lagpm.EvolFact, "fact2_pull"),
+                            ("tree_stats", lagpm.TreeStats, "n_pp_within"), ("timings", lagpm.Timings, "ms_walk"),
+                            ("snapshot_info", lagpm.SnapshotInfo, "file_bytes")):
+        size, off = (int(v) for v in out["layout_" + name].split())
+        assert (ctypes.sizeof(cls), getattr(cls, last).offset) == (size, off), name

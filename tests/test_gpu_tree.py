@@ -173,13 +173,14 @@ def test_autotune_keeps_velocities_and_a_valid_tuning(ctx):
     p = lagpm.make_params(n, theta=0.5)
     with lagpm.Box(ctx, p, len(pos)) as b:
         b.upload_pos(pos, vel)
-        nl, g = b.autotune()
-        assert (nl, g) == b.tuning() and 4 <= nl <= 256 and 8 <= g <= 64
+        nl, g, mode = b.autotune()
+        assert (nl, g, mode) == b.tuning() and 4 <= nl <= 256 and 8 <= g <= 64
+        assert mode in (lagpm.GROUPS_TREE, lagpm.GROUPS_RUNS)
         assert np.array_equal(b.velocities(), vel)
         # and the tuned tree still gives the exact sum at theta -> 0
     p0 = lagpm.make_params(n, theta=1e-6)
     with lagpm.Box(ctx, p0, len(pos)) as b:
-        b.set_tuning(nl, g)
+        b.set_tuning(nl, g, mode)
         b.upload_pos(pos)
         b.treecorrection(lagpm.evolfact(1.0))
         assert rel_rms(b.velocities(), exact_erfc(n, p0, pos)) < 3e-5
@@ -268,3 +269,127 @@ def test_node_moments_match_brute_force_and_leaves_partition_the_particles(ctx, 
     assert np.array_equal(kids, nchild)
     inner = nchild > 0
     assert np.array_equal(kid_particles[inner], count[inner])
+
+
+@pytest.mark.parametrize("prec", [lagpm.PREC_FP32, lagpm.PREC_FP64])
+def test_sampled_walk_theta_zero_is_the_direct_sum_and_the_full_walk(ctx, prec):
+    """lagpm_tree_accel_sample at theta -> 0: each sampled particle alone gets exactly its pair sum
+    within the cutoff, the same as the full walk gives it; velocities stay untouched."""
+    n = 32
+    pos = workload(n, "clustered", seed=7)
+    p = lagpm.make_params(n, theta=1e-6, precision=prec)
+    ids = np.random.default_rng(4).choice(len(pos), 300, replace=False)
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.upload_pos(pos)
+        a = b.tree_accel_sample(ids)
+        v = b.velocities()
+        b.treecorrection(lagpm.evolfact(1.0))
+        full = b.velocities()[ids]
+    assert np.all(v == 0.0)
+    ref = exact_erfc(n, p, pos)[ids]
+    e_ref, e_full = rel_rms(a, ref), rel_rms(a, full)
+    print(f"sampled theta0 prec={prec}: vs exact {e_ref:.2e}, vs full walk {e_full:.2e}", flush=True)
+    # both precisions interpolate the tabulated law (~2e-6 from the analytic one); the FP64 walk of a
+    # lone particle adds the same nonzero terms in the same order as its group's walk
+    assert e_ref < 3e-5
+    assert e_full < (3e-5 if prec == lagpm.PREC_FP32 else 1e-12)
+
+
+def test_sampled_walk_theta_half_matches_the_per_particle_reference_walk(ctx):
+    """At theta = 0.5 a sampled particle is opened with the reference's per-particle test; its
+    error against the exact sum is the same size as the reference walk's (a different octree
+    topology), and FP32 and FP64 sampled walks agree to float rounding."""
+    n = 48
+    pos = workload(n, "clustered", seed=8)
+    ids = np.sort(np.random.default_rng(5).choice(len(pos), 2000, replace=False))
+    a = {}
+    for prec in (lagpm.PREC_FP32, lagpm.PREC_FP64):
+        p = lagpm.make_params(n, theta=0.5, precision=prec)
+        with lagpm.Box(ctx, p, len(pos)) as b:
+            b.upload_pos(pos)
+            a[prec] = b.tree_accel_sample(ids)
+    exact = exact_erfc(n, p, pos)[ids]
+    diff, slope = orc.force_spline(1, n, p.epsilon, p.efold / 2)
+    ref_walk = orc.treecorrection_ref(n, pos, diff, slope, p.rsphere, 0.5, cellw=max(4.0, p.rsphere), targets=ids)
+    e32, e64, eref = rel_rms(a[lagpm.PREC_FP32], exact), rel_rms(a[lagpm.PREC_FP64], exact), rel_rms(ref_walk, exact)
+    print(f"sampled theta=0.5: fp32 {e32:.2e} fp64 {e64:.2e} reference walk {eref:.2e}", flush=True)
+    assert rel_rms(a[lagpm.PREC_FP32], a[lagpm.PREC_FP64]) < 1e-4
+    assert e64 < 2.0 * eref and e64 > 0.2 * eref
+
+
+def test_sampled_walk_rejects_bad_indices(ctx):
+    n = 32
+    pos = workload(n, "lattice", seed=9)
+    with lagpm.Box(ctx, lagpm.make_params(n), len(pos)) as b:
+        b.upload_pos(pos)
+        with pytest.raises(lagpm.LagpmError, match="repeated"):
+            b.tree_accel_sample([3, 5, 3])
+        with pytest.raises(lagpm.LagpmError, match="out of range"):
+            b.tree_accel_sample([0, len(pos)])
+        assert b.tree_accel_sample([]).shape == (0, 3)
+
+
+def test_walk_census_counts_the_pairs_within_the_cutoff(ctx):
+    """With the census on, the FP32 walk counts the evaluated pairs within the cutoff. At theta -> 0
+    every pair within the cutoff is evaluated exactly once (itself included), so the count is the
+    number of ordered periodic pairs closer than r_cut (scipy's kd-tree); the forces are the same
+    with the census on or off."""
+    from scipy.spatial import cKDTree
+    n = 32
+    pos = np.mod(workload(n, "clustered", seed=12), n)
+    pos[pos >= n] = 0.0  # np.mod can round up to n itself
+    p = lagpm.make_params(n, theta=1e-6)
+    a0, _ = tree_accel(ctx, p, pos)
+    ctx.set_walk_census(True)
+    try:
+        a1, st = tree_accel(ctx, p, pos)
+    finally:
+        ctx.set_walk_census(False)
+    exact = cKDTree(pos, boxsize=n).count_neighbors(cKDTree(pos, boxsize=n), p.rsphere)
+    print(f"census: evaluated {st.n_pp}, within cutoff {st.n_pp_within}, kd-tree {exact}", flush=True)
+    assert np.array_equal(a0, a1)
+    assert abs(int(st.n_pp_within) - int(exact)) <= 1e-5 * exact
+    assert st.n_pp_within <= st.n_pp
+    _, st_off = tree_accel(ctx, p, pos)
+    assert st_off.n_pp_within == 0
+
+
+@pytest.mark.parametrize("prec", [lagpm.PREC_FP32, lagpm.PREC_FP64])
+def test_run_groups_theta_zero_is_the_direct_sum(ctx, prec):
+    """Groups cut as runs of consecutive particles of a chaining cell give every particle its exact
+    pair sum at theta -> 0, as tree groups do."""
+    n = 32
+    pos = workload(n, "clustered", seed=14)
+    p = lagpm.make_params(n, theta=1e-6, precision=prec)
+    with lagpm.Box(ctx, p, len(pos)) as b:
+        b.set_tuning(16, 64, lagpm.GROUPS_RUNS)
+        b.upload_pos(pos)
+        b.treecorrection(lagpm.evolfact(1.0))
+        a, st = b.velocities(), b.tree_stats()
+    err = rel_rms(a, exact_erfc(n, p, pos))
+    print(f"run groups theta0 prec={prec}: rms {err:.2e}, groups {st.n_groups} ({len(pos) / st.n_groups:.1f} targets each)",
+          flush=True)
+    assert err < (3e-5 if prec == lagpm.PREC_FP32 else 1e-10)
+
+
+def test_run_groups_theta_half_fp32_vs_fp64_and_no_less_accurate(ctx):
+    """At theta = 0.5 run groups open nodes by the groups' own boxes: FP32 agrees with FP64 on the
+    same groups, and the error against the exact sum stays within the reference walk's."""
+    n = 48
+    pos = workload(n, "clustered", seed=15)
+    a = {}
+    for prec in (lagpm.PREC_FP32, lagpm.PREC_FP64):
+        with lagpm.Box(ctx, lagpm.make_params(n, theta=0.5, precision=prec), len(pos)) as b:
+            b.set_tuning(16, 64, lagpm.GROUPS_RUNS)
+            b.upload_pos(pos)
+            b.treecorrection(lagpm.evolfact(1.0))
+            a[prec] = b.velocities()
+    p = lagpm.make_params(n, theta=0.5)
+    exact = exact_erfc(n, p, pos)
+    diff, slope = orc.force_spline(1, n, p.epsilon, p.efold / 2)
+    ref_walk = orc.treecorrection_ref(n, pos, diff, slope, p.rsphere, 0.5, cellw=max(4.0, p.rsphere))
+    e_prec, e_tree, e_ref = rel_rms(a[lagpm.PREC_FP32], a[lagpm.PREC_FP64]), rel_rms(a[lagpm.PREC_FP32], exact), \
+        rel_rms(ref_walk, exact)
+    print(f"run groups theta=0.5: fp32 vs fp64 {e_prec:.2e}; vs exact {e_tree:.2e}; reference walk {e_ref:.2e}", flush=True)
+    assert e_prec < 1e-4
+    assert e_tree <= 1.02 * e_ref

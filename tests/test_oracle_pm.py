@@ -3,6 +3,7 @@ invariants and brute force. CPU only."""
 import numpy as np
 import pytest
 
+from lageunha_amd import ics
 from oracle import pyoracle as orc
 
 
@@ -201,3 +202,18 @@ def test_pmcf_fit_matches_measured_reference_pm_force():
         newton = 1.0 / r ** 2 / n ** 2
         fit_long = newton - orc.pmcf(n, 0.0, r)
         assert abs(np.mean(fx) - fit_long) < 0.06 * fit_long, (r, np.mean(fx), fit_long)
+
+
+def test_pm_kick_at_targets_equals_the_full_evaluation():
+    """pm_accel with targets interpolates the same mesh (of every particle) at the listed particles
+    only: the values are those of the full evaluation, bit for bit, in both precisions; the timed
+    variant returns the full evaluation's values and a time for each reference phase."""
+    n = 32
+    pos = ics.clustered_box(n, halo_fraction=0.3, nhalo=10, seed=2)
+    t = np.array([5, 100, 2000, 30000, 7])
+    for prec in (0, 1):
+        full = orc.pm_accel(n, pos, mass=1.0, efold=2.5, deconvolve=1, prec=prec)
+        assert np.array_equal(orc.pm_accel(n, pos, 1.0, 2.5, 1, prec, targets=t), full[t])
+        timed, ph = orc.pm_accel_timed(n, pos, 1.0, 2.5, 1, prec)
+        assert np.array_equal(timed, full)
+        assert list(ph) == ["TSC", "F-FFT", "psolv", "B-FFT", "4FDA"] and all(v >= 0 for v in ph.values())

@@ -82,7 +82,11 @@ def test_read_pm_and_tree_records(tmp_path, ptype):
     assert got.tobytes() == rec.tobytes()
 
 
-def test_write_keeps_header_lines_and_tail(tmp_path):
+def test_write_keeps_header_lines_and_drops_the_trailing_block(tmp_path):
+    """The written file is the template's header lines (time lines rewritten) and the records,
+    and ends there. jread reads a SimParameters struct back over the parsed header after the
+    particles (Cosmos/CosmosRW/kjhrw.c:118): a carried template block would make a restart resume
+    at the template's time and particle type, so none is written and the header stands."""
     src, dst = str(tmp_path / "in.00000"), str(tmp_path / "out.00000")
     tail = np.random.default_rng(5).integers(0, 256, 5000, dtype=np.uint8).tobytes()
     rec = _write_file(src, 512, 8, seed=4, tail=tail)
@@ -94,7 +98,8 @@ def test_write_keeps_header_lines_and_tail(tmp_path):
     assert info.anow == anow and info.stepcount == 7 and info.np_dm == 512
     assert got.tobytes() == new.tobytes()
     a, b = open(src, "rb").read(), open(dst, "rb").read()
-    assert b.endswith(tail)
+    assert len(b) == info.header_bytes + 512 * 32 == info.file_bytes
+    assert a.endswith(tail) and not b.endswith(tail[-64:])
     ha, hb = a[:a.index(CLOSING)].split(b"\n"), b[:b.index(CLOSING)].split(b"\n")
     assert len(ha) == len(hb)
     changed = [x for x, y in zip(ha, hb) if x != y]
