@@ -133,9 +133,9 @@ void free_box(lagpm_box* b) {
     b->spec = b->phi = nullptr;  // the same padded mesh as rho
     dfree(b->pp); dfree(b->ncount); dfree(b->noff); dfree(b->nd_info); dfree(b->nd_parent); dfree(b->nd_child);
     dfree(b->nd_com); dfree(b->nd_q0); dfree(b->nd_q1); dfree(b->nd_d); dfree(b->cellroot); dfree(b->gflag);
+    dfree(b->dep_scan_tmp);
     dfree(b->goff); dfree(b->groups); dfree(b->gbox); dfree(b->pgroup); dfree(b->counters); dfree(b->dsz);
     dfree(b->scan_tmp);
-    dfree(b->dep_scan_tmp);
 }
 
 int check_params(lagpm_ctx* ctx, const lagpm_params* p) {
@@ -495,10 +495,10 @@ int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box
     box->spec = box->phi = box->rho;
     if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->grad, m * 4 * real);
     ALLOC(box->ncount, np); ALLOC(box->noff, np);
-    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->dep_scan_tmp, lagpm_scan_tmp_bytes(np));
     ALLOC(box->cellroot, (size_t)box->mcell * box->mcell * box->mcell);
     ALLOC(box->counters, 4);
     ALLOC(box->dsz, DSZ_COUNT);
+    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->dep_scan_tmp, lagpm_scan_tmp_bytes(np));
     box->dep_cap = lagpm_deposit_chunk_cap(p->n, np) + 1;
     ALLOC(box->dep_start, box->dep_cap);
     if (f64) ALLOC(box->pgroup, np);
@@ -996,8 +996,9 @@ int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group) {
     RCHK(tree_check(ctx, box));
     if (box->np == 0) return LAGPM_OK;
     // (nleaf, group, group mode)
-    static const int cand[][3] = {{16, 64, 1}, {32, 64, 1}, {8, 64, 1}, {16, 32, 1}, {32, 32, 1}, {8, 32, 1},
-                                  {16, 64, 0}, {32, 64, 0}, {8, 64, 0}, {16, 32, 0}, {32, 32, 0}, {8, 32, 0}};
+    static const int cand[][3] = {{16, 64, 0}, {32, 64, 0}, {8, 64, 0}, {16, 32, 0}, {32, 32, 0}, {8, 32, 0},
+                                  {16, 16, 0}, {32, 16, 0}, {8, 16, 0},
+                                  {16, 64, 1}, {32, 64, 1}, {16, 32, 1}, {32, 32, 1}, {16, 16, 1}};
     double best = 1e300;
     int bl = box->tune.nleaf, bg = box->tune.group, bm = lagpm_box_get_group_mode(box);
     for (const auto& c : cand) {

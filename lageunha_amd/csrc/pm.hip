@@ -24,15 +24,21 @@
 
 namespace {
 
+#ifndef LAGPM_DEP_PPT
+#define LAGPM_DEP_PPT 4
+#endif
+#ifndef LAGPM_DEP_BLOCKS_PER_CU
+#define LAGPM_DEP_BLOCKS_PER_CU 4
+#endif
 constexpr int DEP_THREADS = 256;
-constexpr int DEP_PPT = 4;
+constexpr int DEP_PPT = LAGPM_DEP_PPT;  // particles per thread and chunk
+constexpr int DEP_BLOCKS_PER_CU = LAGPM_DEP_BLOCKS_PER_CU;  // resident deposit blocks per CU (the LDS box allows 4)
 constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;
-constexpr int DEP_LDS = 4096;
+constexpr int DEP_BOX = 16;  // the LDS box of a chunk: 16^3 cells, index x + 16 (y + 16 z)
+constexpr int DEP_LDS = DEP_BOX * DEP_BOX * DEP_BOX;
 
-__device__ inline int wrapi(int i, int n) {
-    i %= n;
-    return i < 0 ? i + n : i;
-}
+// periodic wrap of a cell index within one period of the box ([-n, 2n)): compares, no division
+__device__ inline int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // Deposit weights, x axis carries the mass (pmmain.h:11-40).
 __device__ inline void tsc_axis_mass(double xp, float pmas0, int& near, int& s, float w[3]) {
@@ -96,152 +102,176 @@ __global__ void dep_start_kernel(const uint32_t* __restrict__ flag, const uint32
     if (i == 0) start[*nchunk] = (uint32_t)N;
 }
 
-// Per-axis TSC weights in cell order (left, centre, right) = nodes near-1, near, near+1. The
-// reference's weights go to near, near+s and near-s (w1, w2, w3 with s the side of the nearest
-// node, pmmain.h:11-40); a cell's product of weights is the same, whichever order the nodes are
-// visited in.
-template <typename W>
-__device__ inline void cell_order(int s, const W w[3], W o[3]) {
-    o[0] = s > 0 ? w[2] : w[1];
-    o[1] = w[0];
-    o[2] = s > 0 ? w[1] : w[2];
+// LDS box cell of offsets (x, y, z) in [0, 16)^3 from the chunk's corner. Rows of 16 cells, with
+// bits 2-3 of x flipped by y mod 4: the 64 lanes of a wave hold 64 consecutive Morton-ordered
+// particles, on a lattice a 4x4x4 block of cells, and each atomic of the 27 then spreads them over
+// 16 bank pairs (4 lanes each, the minimum for 64 eight-byte operands) instead of 4.
+__device__ inline int dep_cell(int x, int y, int z) {
+    return (x ^ ((y & 3) << 2)) + DEP_BOX * y + DEP_BOX * DEP_BOX * z;
 }
 
-// One block per chunk. Thread t takes DEP_PPT consecutive particles of the chunk (Morton order):
-// neighbouring particles often share the nearest mesh cell and with it the same 27 cells, so a
-// thread sums the fixed-point contributions of each run of particles with one nearest cell in
-// registers and issues the run's 27 atomics once (a halo core holds thousands of particles per
-// cell, whose atomics on one LDS address would otherwise serialise across the wave). The sums
-// are integer, so the result is the same in any grouping.
-template <bool F64>
-__global__ void __launch_bounds__(DEP_THREADS) deposit_kernel(const double* __restrict__ px, const double* __restrict__ py,
+// Fixed point of a contribution 0 <= c * 2^s < 2^52: c * 2^s is exact (a power-of-two scale), and
+// adding 2^52 in the same fused operation rounds it once to an integer, half to even (llrint);
+// that integer is the low 52 bits of the sum's pattern. The host checks the bound; above it the
+// general conversion is taken.
+template <bool MAGIC>
+__device__ inline unsigned long long dep_fixed(double c, double scale) {
+    if constexpr (MAGIC) {
+        return (unsigned long long)__double_as_longlong(fma(c, scale, 0x1p52)) & 0x000fffffffffffffull;
+    } else {
+        return (unsigned long long)__double2ll_rn(c * scale);
+    }
+}
+
+// Persistent blocks over the chunk list. Thread t takes the particles t, t + DEP_THREADS, ... of a
+// chunk (coalesced loads; a wave holds 64 consecutive particles). Per chunk: the box of nearest
+// cells from the positions already in registers, the loads of the next chunk's positions issued,
+// then the 27 contributions of each particle added to the 16^3 LDS box (or straight to the mesh
+// when the box does not fit), and the box added to the mesh once per nonzero cell, which is
+// cleared in the same pass for the next chunk. Every sum is of integers, so the result is exact
+// and independent of the order of the additions.
+template <bool F64, bool MAGIC>
+__global__ void __launch_bounds__(DEP_THREADS, DEP_BLOCKS_PER_CU) deposit_kernel(const double* __restrict__ px, const double* __restrict__ py,
                                                               const double* __restrict__ pz, const uint32_t* __restrict__ start,
-                                                              const uint32_t* __restrict__ nchunk, size_t npart, int n,
-                                                              float mass, double scale, unsigned long long* __restrict__ acc) {
-    using W = typename std::conditional<F64, double, float>::type;
+                                                              const uint32_t* __restrict__ nchunk, unsigned chunk_cap,
+                                                              size_t npart, int n, float mass, double scale,
+                                                              unsigned long long* __restrict__ acc) {
     __shared__ unsigned long long box[DEP_LDS];
     __shared__ int red[6][DEP_THREADS / 64];
-    // the grid is sized for the largest possible chunk count; the actual count is on the device
-    const unsigned nch = min(*nchunk, gridDim.x);  // the grid covers every possible chunk
-    if (blockIdx.x >= nch) return;
-    const unsigned chunk = lagpm_wave::xcd_block(blockIdx.x, nch);  // neighbouring chunks on one XCD's L2
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    // chunk bounds clamped to the particle count (they come from the chunk scan of this box)
-    const size_t np = min((size_t)start[chunk + 1], npart);
-    const size_t base = min((size_t)start[chunk], np);
+    const unsigned nch = min(*nchunk, chunk_cap);
+    // XCD x (block b runs on XCD b % 8) walks the x-th eighth of the chunks, so neighbouring
+    // chunks, which add to neighbouring mesh cells, share one L2
+    const unsigned per = gridDim.x / 8u, x = blockIdx.x % 8u, j = blockIdx.x / 8u;
+    const unsigned clo = (unsigned)(((unsigned long long)nch * x) / 8u), chi = (unsigned)(((unsigned long long)nch * (x + 1)) / 8u);
+    unsigned c = clo + j;
+    if (c >= chi) return;
+    for (int q = tid; q < DEP_LDS; q += DEP_THREADS) box[q] = 0ull;
 
-    int ix[DEP_PPT] = {}, iy[DEP_PPT] = {}, iz[DEP_PPT] = {};
-    W wx[DEP_PPT][3], wy[DEP_PPT][3], wz[DEP_PPT][3];  // in cell order (left, centre, right)
-    int cnt = 0;  // this thread's particles: base + tid * DEP_PPT + [0, cnt)
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+    // this chunk's particles (clamped to the particle count) and their positions
+    auto bounds = [&](unsigned ch, size_t& b, size_t& e) {
+        e = min((size_t)start[ch + 1], npart);
+        b = min((size_t)start[ch], e);
+    };
+    size_t base, end;
+    bounds(c, base, end);
+    double X[DEP_PPT], Y[DEP_PPT], Z[DEP_PPT];
 #pragma unroll
     for (int k = 0; k < DEP_PPT; k++) {
-        const size_t i = base + (size_t)tid * DEP_PPT + k;
-        if (i >= np) break;
-        cnt = k + 1;
-        int sx, sy, sz;
-        W w[3];
-        if constexpr (F64) {
-            tsc_axis_d(px[i], ix[k], sx, w); cell_order(sx, w, wx[k]);
-            tsc_axis_d(py[i], iy[k], sy, w); cell_order(sy, w, wy[k]);
-            tsc_axis_d(pz[i], iz[k], sz, w); cell_order(sz, w, wz[k]);
-        } else {
-            tsc_axis_mass(px[i], mass, ix[k], sx, w); cell_order(sx, w, wx[k]);
-            tsc_axis_unit(py[i], iy[k], sy, w); cell_order(sy, w, wy[k]);
-            tsc_axis_unit(pz[i], iz[k], sz, w); cell_order(sz, w, wz[k]);
+        const size_t i = base + (size_t)k * DEP_THREADS + tid;
+        const bool in = i < end;
+        X[k] = in ? px[i] : 0.0; Y[k] = in ? py[i] : 0.0; Z[k] = in ? pz[i] : 0.0;
+    }
+    for (;;) {
+        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+#pragma unroll
+        for (int k = 0; k < DEP_PPT; k++) {
+            if (base + (size_t)k * DEP_THREADS + tid < end) {
+                const int nx = (int)rint(X[k]), ny = (int)rint(Y[k]), nz = (int)rint(Z[k]);
+                lo[0] = min(lo[0], nx); hi[0] = max(hi[0], nx);
+                lo[1] = min(lo[1], ny); hi[1] = max(hi[1], ny);
+                lo[2] = min(lo[2], nz); hi[2] = max(hi[2], nz);
+            }
         }
-        lo[0] = min(lo[0], ix[k]); hi[0] = max(hi[0], ix[k]);
-        lo[1] = min(lo[1], iy[k]); hi[1] = max(hi[1], iy[k]);
-        lo[2] = min(lo[2], iz[k]); hi[2] = max(hi[2], iz[k]);
-    }
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        for (int o = 32; o > 0; o >>= 1) {
-            lo[a] = min(lo[a], __shfl_xor(lo[a], o, 64));
-            hi[a] = max(hi[a], __shfl_xor(hi[a], o, 64));
+        for (int a = 0; a < 3; a++) {
+            for (int o = 32; o > 0; o >>= 1) {
+                lo[a] = min(lo[a], __shfl_xor(lo[a], o, 64));
+                hi[a] = max(hi[a], __shfl_xor(hi[a], o, 64));
+            }
+            if (lane == 0) { red[a][wid] = lo[a]; red[3 + a][wid] = hi[a]; }
         }
-        if (lane == 0) { red[a][wid] = lo[a]; red[3 + a][wid] = hi[a]; }
-    }
-    __syncthreads();
-    int blo[3], bdim[3];
+        // the next chunk of this block and the loads of its positions, in flight during this one
+        const unsigned cn = c + per;
+        const bool more = cn < chi;
+        size_t nbase = 0, nend = 0;
+        if (more) bounds(cn, nbase, nend);
+        double NX[DEP_PPT], NY[DEP_PPT], NZ[DEP_PPT];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        int l = INT_MAX, h = INT_MIN;
-        for (int w = 0; w < DEP_THREADS / 64; w++) { l = min(l, red[a][w]); h = max(h, red[3 + a][w]); }
-        blo[a] = l - 1;
-        bdim[a] = (h + 1) - (l - 1) + 1;
-    }
-    const long vol = (long)bdim[0] * bdim[1] * bdim[2];
-    const bool use_lds = vol > 0 && vol <= DEP_LDS;
-
-    if (use_lds) {
-        for (int c = tid; c < vol; c += DEP_THREADS) box[c] = 0ull;
-        __syncthreads();
-    }
-    // the 27 cells of nearest cell (cx, cy, cz) get the run's sums q[c][b][a]
-    auto flush = [&](int cx, int cy, int cz, const unsigned long long (&q)[27]) {
+        for (int k = 0; k < DEP_PPT; k++) {
+            const size_t i = nbase + (size_t)k * DEP_THREADS + tid;
+            const bool in = i < nend;
+            NX[k] = in ? px[i] : 0.0; NY[k] = in ? py[i] : 0.0; NZ[k] = in ? pz[i] : 0.0;
+        }
+        __syncthreads();  // red written; the box cleared by the previous pass
+        int blo[3], bdim[3];
 #pragma unroll
-        for (int c = 0; c < 3; c++)
+        for (int a = 0; a < 3; a++) {
+            int l = INT_MAX, h = INT_MIN;
+            for (int w = 0; w < DEP_THREADS / 64; w++) { l = min(l, red[a][w]); h = max(h, red[3 + a][w]); }
+            blo[a] = l - 1;
+            bdim[a] = (h + 1) - (l - 1) + 1;
+        }
+        // (near lies in [0, n] for wrapped positions, so every index below is in [-1, n + 1])
+        const bool use_lds = bdim[0] > 0 && bdim[0] <= DEP_BOX && bdim[1] <= DEP_BOX && bdim[2] <= DEP_BOX;
 #pragma unroll
-            for (int b = 0; b < 3; b++)
+        for (int k = 0; k < DEP_PPT; k++) {
+            if (base + (size_t)k * DEP_THREADS + tid >= end) continue;
+            int ix, iy, iz, sx, sy, sz;
+            float wx[3], wy[3], wz[3];
+            double dwx[3], dwy[3], dwz[3];
+            if constexpr (F64) {
+                tsc_axis_d(X[k], ix, sx, dwx);
+                tsc_axis_d(Y[k], iy, sy, dwy);
+                tsc_axis_d(Z[k], iz, sz, dwz);
+            } else {
+                tsc_axis_mass(X[k], mass, ix, sx, wx);
+                tsc_axis_unit(Y[k], iy, sy, wy);
+                tsc_axis_unit(Z[k], iz, sz, wz);
+            }
+            const int xi[3] = {ix, ix + sx, ix - sx};
+            const int yi[3] = {iy, iy + sy, iy - sy};
+            const int zi[3] = {iz, iz + sz, iz - sz};
+            // the fixed-point contribution of node (a, b, c): the reference's float product
+            // (wx wy) wz (pmmain.h:11-40), or the FP64 product for the FP64 path
+            auto contrib = [&](int a, int b, int cc) -> unsigned long long {
+                if constexpr (F64) {
+                    return dep_fixed<MAGIC>((double)mass * dwx[a] * dwy[b] * dwz[cc], scale);
+                } else {
+                    const float wxy = wx[a] * wy[b];
+                    return dep_fixed<MAGIC>((double)(wxy * wz[cc]), scale);
+                }
+            };
+            if (use_lds) {
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++)
+#pragma unroll
+                        for (int a = 0; a < 3; a++)
+                            atomicAdd(&box[dep_cell(xi[a] - blo[0], yi[b] - blo[1], zi[cc] - blo[2])], contrib(a, b, cc));
+            } else {
+                size_t gx[3], gy[3], gz[3];
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
-                    const unsigned long long v = q[9 * c + 3 * b + a];
-                    const int xa = cx + a - 1, yb = cy + b - 1, zc = cz + c - 1;
-                    if (use_lds) {
-                        atomicAdd(&box[(xa - blo[0]) + bdim[0] * ((yb - blo[1]) + bdim[1] * (zc - blo[2]))], v);
-                    } else {
-                        const size_t g = (size_t)wrapi(xa, n) + (size_t)n * ((size_t)wrapi(yb, n) + (size_t)n * wrapi(zc, n));
-                        atomicAdd(&acc[g], v);
-                    }
+                    gx[a] = (size_t)wrapc(xi[a], n);
+                    gy[a] = (size_t)n * (size_t)wrapc(yi[a], n);
+                    gz[a] = (size_t)n * (size_t)n * (size_t)wrapc(zi[a], n);
                 }
-    };
-    // runs of equal nearest cells among this thread's particles (the loop is unrolled, so every
-    // register index is a constant)
-    unsigned long long q[27];
 #pragma unroll
-    for (int e = 0; e < 27; e++) q[e] = 0ull;
-    int cx = ix[0], cy = iy[0], cz = iz[0];
+                for (int cc = 0; cc < 3; cc++)
 #pragma unroll
-    for (int k = 0; k < DEP_PPT; k++) {
-        if (k < cnt) {
-            if (k > 0 && (ix[k] != cx || iy[k] != cy || iz[k] != cz)) {
-                flush(cx, cy, cz, q);
+                    for (int b = 0; b < 3; b++)
 #pragma unroll
-                for (int e = 0; e < 27; e++) q[e] = 0ull;
-                cx = ix[k]; cy = iy[k]; cz = iz[k];
+                        for (int a = 0; a < 3; a++) atomicAdd(&acc[gx[a] + gy[b] + gz[cc]], contrib(a, b, cc));
             }
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-#pragma unroll
-                for (int b = 0; b < 3; b++)
-#pragma unroll
-                    for (int a = 0; a < 3; a++) {
-                        double cv;
-                        if constexpr (F64) {
-                            cv = (double)mass * wx[k][a] * wy[k][b] * wz[k][c];
-                        } else {
-                            const float wxy = wx[k][a] * wy[k][b];
-                            const float cf = wxy * wz[k][c];
-                            cv = (double)cf;
-                        }
-                        q[9 * c + 3 * b + a] += (unsigned long long)__double2ll_rn(cv * scale);
-                    }
         }
-    }
-    if (cnt > 0) flush(cx, cy, cz, q);
-    if (use_lds) {
-        __syncthreads();
-        for (int c = tid; c < vol; c += DEP_THREADS) {
-            unsigned long long v = box[c];
-            if (v == 0ull) continue;
-            int a = c % bdim[0];
-            int r = c / bdim[0];
-            int b = r % bdim[1];
-            int d = r / bdim[1];
-            size_t g = (size_t)wrapi(blo[0] + a, n) + (size_t)n * ((size_t)wrapi(blo[1] + b, n) + (size_t)n * wrapi(blo[2] + d, n));
-            atomicAdd(&acc[g], v);
+        __syncthreads();  // the box is complete
+        if (use_lds) {
+            for (int q = tid; q < DEP_LDS; q += DEP_THREADS) {
+                const unsigned long long v = box[q];
+                if (v == 0ull) continue;  // cells outside the chunk's box are never written
+                box[q] = 0ull;
+                const int d = q / (DEP_BOX * DEP_BOX), b = (q / DEP_BOX) & (DEP_BOX - 1);
+                const int a = (q & (DEP_BOX - 1)) ^ ((b & 3) << 2);
+                const size_t g = (size_t)wrapc(blo[0] + a, n) + (size_t)n * ((size_t)wrapc(blo[1] + b, n) + (size_t)n * wrapc(blo[2] + d, n));
+                atomicAdd(&acc[g], v);
+            }
         }
+        if (!more) break;
+        c = cn; base = nbase; end = nend;
+#pragma unroll
+        for (int k = 0; k < DEP_PPT; k++) { X[k] = NX[k]; Y[k] = NY[k]; Z[k] = NZ[k]; }
     }
 }
 
@@ -466,10 +496,13 @@ __global__ void __launch_bounds__(INTERP_THREADS) interp_kick_f32(const double* 
     const long long ex = (long long)box[3] - bx + 1, ey = (long long)box[4] - by + 1, ez = (long long)box[5] - bz + 1;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     if (ex * ey * ez <= INTERP_LDS) {  // block-uniform
-        const int vol = (int)(ex * ey * ez);
-        for (int t = threadIdx.x; t < vol; t += INTERP_THREADS) {
-            const int a = t % (int)ex, b = (t / (int)ex) % (int)ey, c = t / (int)(ex * ey);
-            sg[t] = grad[(size_t)n * ((size_t)wrapi(by + b, n) + (size_t)n * wrapi(bz + c, n)) + wrapi(bx + a, n)];
+        // rows (b, c) of the box to groups of 16 threads, cells of a row to the 16 lanes of the group:
+        // one division per row, not per cell, and the periodic wrap by compares
+        const int nrow = (int)(ey * ez), exi = (int)ex, eyi = (int)ey;
+        for (int r = threadIdx.x >> 4; r < nrow; r += INTERP_THREADS >> 4) {
+            const int b = r % eyi, c = r / eyi;
+            const size_t rowg = (size_t)n * ((size_t)wrapc(by + b, n) + (size_t)n * wrapc(bz + c, n));
+            for (int a = threadIdx.x & 15; a < exi; a += 16) sg[a + exi * r] = grad[rowg + wrapc(bx + a, n)];
         }
         __syncthreads();
         if (!valid) return;
@@ -479,9 +512,9 @@ __global__ void __launch_bounds__(INTERP_THREADS) interp_kick_f32(const double* 
         interp_sum(wx, wy, wz, xi, yi, zi, (size_t)ex, (size_t)(ex * ey), sg, fx, fy, fz);
     } else {
         if (!valid) return;
-        const int xi[3] = {wrapi(near[0], n), wrapi(near[0] + sgn[0], n), wrapi(near[0] - sgn[0], n)};
-        const int yi[3] = {wrapi(near[1], n), wrapi(near[1] + sgn[1], n), wrapi(near[1] - sgn[1], n)};
-        const int zi[3] = {wrapi(near[2], n), wrapi(near[2] + sgn[2], n), wrapi(near[2] - sgn[2], n)};
+        const int xi[3] = {wrapc(near[0], n), wrapc(near[0] + sgn[0], n), wrapc(near[0] - sgn[0], n)};
+        const int yi[3] = {wrapc(near[1], n), wrapc(near[1] + sgn[1], n), wrapc(near[1] - sgn[1], n)};
+        const int zi[3] = {wrapc(near[2], n), wrapc(near[2] + sgn[2], n), wrapc(near[2] - sgn[2], n)};
         interp_sum(wx, wy, wz, xi, yi, zi, (size_t)n, (size_t)n * n, grad, fx, fy, fz);
     }
     vx[p] += (double)(fact2 * fx);
@@ -501,9 +534,9 @@ __global__ void interp_kick_f64(const double* __restrict__ px, const double* __r
     tsc_axis_d(px[p], ix, sx, wx);
     tsc_axis_d(py[p], iy, sy, wy);
     tsc_axis_d(pz[p], iz, sz, wz);
-    int xi[3] = {wrapi(ix, n), wrapi(ix + sx, n), wrapi(ix - sx, n)};
-    int yi[3] = {wrapi(iy, n), wrapi(iy + sy, n), wrapi(iy - sy, n)};
-    int zi[3] = {wrapi(iz, n), wrapi(iz + sz, n), wrapi(iz - sz, n)};
+    int xi[3] = {wrapc(ix, n), wrapc(ix + sx, n), wrapc(ix - sx, n)};
+    int yi[3] = {wrapc(iy, n), wrapc(iy + sy, n), wrapc(iy - sy, n)};
+    int zi[3] = {wrapc(iz, n), wrapc(iz + sz, n), wrapc(iz - sz, n)};
     double fx = 0, fy = 0, fz = 0;
     for (int c = 0; c < 3; c++)
         for (int b = 0; b < 3; b++) {
@@ -913,13 +946,24 @@ int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
         hipLaunchKernelGGL(dep_start_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, st, flag, off, np,
                            box->dep_start, nchunk);
         HIPCHK(ctx, hipGetLastError());
-        const unsigned grid = (unsigned)(box->dep_cap - 1);
-        if (f64)
-            hipLaunchKernelGGL(deposit_kernel<true>, dim3(grid), dim3(DEP_THREADS), 0, st, box->x, box->y, box->z,
-                               box->dep_start, nchunk, np, n, (float)box->p.mass, scale, box->acc);
-        else
-            hipLaunchKernelGGL(deposit_kernel<false>, dim3(grid), dim3(DEP_THREADS), 0, st, box->x, box->y, box->z,
-                               box->dep_start, nchunk, np, n, (float)box->p.mass, scale, box->acc);
+        // persistent blocks, a multiple of the 8 XCDs, as many as are resident at once
+        const unsigned cap = (unsigned)(box->dep_cap - 1);
+        const size_t resident = (size_t)(std::max(ctx->num_cu, 8) * DEP_BLOCKS_PER_CU + 7) / 8 * 8;
+        const unsigned grid = (unsigned)std::min(resident, ((size_t)cap + 7) / 8 * 8);
+        // contributions are at most mass (a product of TSC weights is below 1): the 2^52 rounding
+        // applies when mass 2^s stays below 2^51
+        const bool magic = std::ldexp((double)box->p.mass, box->scale_bits) < 0x1p51;
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(DEP_THREADS), 0, st, box->x, box->y, box->z, box->dep_start, nchunk,
+                               cap, np, n, (float)box->p.mass, scale, box->acc);
+        };
+        if (f64) {
+            if (magic) launch(deposit_kernel<true, true>);
+            else launch(deposit_kernel<true, false>);
+        } else {
+            if (magic) launch(deposit_kernel<false, true>);
+            else launch(deposit_kernel<false, false>);
+        }
         HIPCHK(ctx, hipGetLastError());
     }
     const double inv_scale = std::ldexp(1.0, -box->scale_bits);

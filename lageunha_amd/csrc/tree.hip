@@ -953,7 +953,6 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                         dx[u] = xt.x - c[u].x; dy[u] = xt.y - c[u].y; dz[u] = xt.z - c[u].z;
                         r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
                         if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
-                        if constexpr (CENSUS) nin += (valid && r2[u] <= A.rc2) ? 1u : 0u;
                     }
                     if constexpr (!LOGT)
                         if (!__any(any)) continue;  // wave-uniform skip of two far nodes

@@ -250,4 +250,10 @@ class Server:
                         "iteration_ns_per_particle_p50": float(np.percentile(it_ms * 1e6 / parts, 50)),
                         "iteration_ns_per_particle_p99": float(np.percentile(it_ms * 1e6 / parts, 99)),
                         "iterations_with_new_boxes": int(sum(1 for it in self.iteration_log if it[4]))})
+            for tag, sel in (("with_new_boxes", [it for it in self.iteration_log if it[4]]),
+                             ("without_new_boxes", [it for it in self.iteration_log if not it[4]])):
+                if sel:
+                    ns = np.array([it[1] * 1e9 / it[3] for it in sel])
+                    out[f"iteration_ns_per_particle_{tag}_p50"] = float(np.percentile(ns, 50))
+                    out[f"iteration_ns_per_particle_{tag}_p99"] = float(np.percentile(ns, 99))
         return out
