@@ -210,6 +210,9 @@ def cpu_baseline(args, n_cfg, pos_cfg, gpu_force):
 SMOOTH_SIZES = (32, 36, 48, 54, 64, 72, 96, 108, 128, 144, 162, 192)
 
 
+MODE_NAMES = {0: "tree", 1: "runs", 2: "packed"}  # lagpm.GROUPS_TREE, GROUPS_RUNS, GROUPS_PACKED
+
+
 def smooth_round(n):
     """Nearest mesh size with prime factors 2 and 3 only (rocFFT single-kernel lengths)."""
     return min(SMOOTH_SIZES, key=lambda s: (abs(np.log(s / n)), s))
@@ -299,7 +302,7 @@ def ensemble(args, ctx, lagpm, ics):
             bx.close()
     out = {"boxes": len(specs), "particles": total, "steps": steps,
            "sizes": sorted(n for n, _ in specs), "identical_plain_vs_batched": bool(same),
-           "tree_tuning": {str(n): {"nleaf": t[0], "group": t[1], "group_mode": "runs" if t[2] else "tree"}
+           "tree_tuning": {str(n): {"nleaf": t[0], "group": t[1], "group_mode": MODE_NAMES[t[2]]}
                            for n, t in sorted(tuned.items())},
            "batch_pipeline": {"stages": 3, "reserve_cu": 32, "note": "library default (lagpm_set_batch_pipeline)"}}
     for k in ("plain", "batched"):
@@ -331,8 +334,8 @@ def ensemble(args, ctx, lagpm, ics):
 def serving(args, ctx, lagpm, ics):
     """Poisson stream of box requests (rate 20/s over a window), n log-uniform in [32, 192]
     rounded to 2,3-smooth meshes, 5..50 steps each, Omega_m ~ U[0.25, 0.35]; admission under a
-    200 GB HBM budget; continuous batching (one lagpm_batch_step over all active boxes per
-    iteration). Initial conditions: one perturbed lattice per mesh size, prepared before the
+    200 GB HBM budget and a cap of 32M active particles per iteration (the per-step latency
+    bound); continuous batching (one lagpm_batch_step over all active boxes per iteration). Initial conditions: one perturbed lattice per mesh size, prepared before the
     clock starts, shifted by a random whole-cell offset per request."""
     from lageunha_amd.serve import Request, Server
     g = np.random.default_rng(args.seed + 7)
@@ -416,7 +419,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--nleaf", type=int, default=0)
     ap.add_argument("--group", type=int, default=0)
-    ap.add_argument("--group-mode", choices=["tree", "runs"], default=None)
+    ap.add_argument("--group-mode", choices=["tree", "runs", "packed"], default=None)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-boxes", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=10)
@@ -424,7 +427,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-window", type=float, default=8.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
-    ap.add_argument("--serve-max-particles", type=int, default=0)
+    # an iteration over at most 32M particles bounds the per-step latency (about 35 ms); 0: no cap
+    ap.add_argument("--serve-max-particles", type=int, default=32_000_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -456,7 +460,8 @@ def main():
     box.upload_pos(pos)
     if args.nleaf or args.group or args.group_mode:
         box.set_tuning(args.nleaf or 16, args.group or 64,
-                       lagpm.GROUPS_RUNS if args.group_mode == "runs" else lagpm.GROUPS_TREE)
+                       {"tree": lagpm.GROUPS_TREE, "runs": lagpm.GROUPS_RUNS, "packed": lagpm.GROUPS_PACKED}[args.group_mode]
+                       if args.group_mode else None)
     elif not args.no_autotune:
         t_at = time.perf_counter()
         nl, gs, gm = box.autotune()
@@ -581,7 +586,7 @@ def main():
                                f"(drag + PM kick + tree kick + drift) per step",
                    "n": n, "particles": N, "parallelism": "single GPU",
                    "tree_tuning": {"nleaf": tuning[0], "group": tuning[1],
-                                   "group_mode": "runs" if tuning[2] == lagpm.GROUPS_RUNS else "tree"}},
+                                   "group_mode": MODE_NAMES[tuning[2]]}},
         "roofline": roof,
         "phases": phase_report,
     }

@@ -287,10 +287,14 @@ int lagpm_box_get_tuning(const lagpm_box* box, int* nleaf, int* group);
 /* How target groups are formed. TREE (default): a tree node of at most `group` particles whose
  * parent holds more, consecutive small siblings merged up to `group`. RUNS: each chaining cell's
  * particles, in Morton order, cut into runs of `group` (the last shorter), so all groups but one
- * per cell fill a wavefront. Either way a group stays inside one chaining cell and the opening
- * test is taken from the group's bounding box; at theta > 0 the forces differ slightly. */
+ * per cell fill a wavefront. PACKED: tree groups, but the small children of a parent are packed
+ * into runs of exactly `group` consecutive particles (a run may end inside a child), so all of a
+ * parent's runs but the last fill a wavefront and each stays inside its parent. Either way a group
+ * stays inside one chaining cell and the opening test is taken from the group's bounding box; at
+ * theta > 0 the forces differ slightly. */
 #define LAGPM_GROUPS_TREE 0
 #define LAGPM_GROUPS_RUNS 1
+#define LAGPM_GROUPS_PACKED 2
 int lagpm_box_set_group_mode(lagpm_box* box, int mode);
 int lagpm_box_get_group_mode(const lagpm_box* box);
 /* Times the tree build + walk of the box's current particles for a set of (nleaf, group, group

@@ -980,15 +980,17 @@ int lagpm_box_get_tuning(const lagpm_box* box, int* nleaf, int* group) {
 }
 
 int lagpm_box_set_group_mode(lagpm_box* box, int mode) {
-    if (!box || (mode != LAGPM_GROUPS_TREE && mode != LAGPM_GROUPS_RUNS)) return LAGPM_E_INVAL;
-    const int runs = mode == LAGPM_GROUPS_RUNS ? 1 : 0;
+    if (!box || (mode != LAGPM_GROUPS_TREE && mode != LAGPM_GROUPS_RUNS && mode != LAGPM_GROUPS_PACKED))
+        return LAGPM_E_INVAL;
+    const int runs = mode == LAGPM_GROUPS_RUNS ? 1 : (mode == LAGPM_GROUPS_PACKED ? 2 : 0);
     if (box->tune.runs != runs) lagpm_box_graphs_release(box);  // captured steps hold the mode
     box->tune.runs = runs;
     return LAGPM_OK;
 }
 
 int lagpm_box_get_group_mode(const lagpm_box* box) {
-    return box ? (box->tune.runs ? LAGPM_GROUPS_RUNS : LAGPM_GROUPS_TREE) : LAGPM_E_INVAL;
+    if (!box) return LAGPM_E_INVAL;
+    return box->tune.runs == 1 ? LAGPM_GROUPS_RUNS : (box->tune.runs == 2 ? LAGPM_GROUPS_PACKED : LAGPM_GROUPS_TREE);
 }
 
 int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group) {
@@ -998,7 +1000,8 @@ int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group) {
     // (nleaf, group, group mode)
     static const int cand[][3] = {{16, 64, 0}, {32, 64, 0}, {8, 64, 0}, {16, 32, 0}, {32, 32, 0}, {8, 32, 0},
                                   {16, 16, 0}, {32, 16, 0}, {8, 16, 0},
-                                  {16, 64, 1}, {32, 64, 1}, {16, 32, 1}, {32, 32, 1}, {16, 16, 1}};
+                                  {16, 64, 1}, {32, 64, 1}, {16, 32, 1}, {32, 32, 1}, {16, 16, 1},
+                                  {16, 64, 2}, {32, 64, 2}};
     double best = 1e300;
     int bl = box->tune.nleaf, bg = box->tune.group, bm = lagpm_box_get_group_mode(box);
     for (const auto& c : cand) {

@@ -79,7 +79,7 @@ struct lagpm_ctx {
 struct tree_tune {
     int nleaf = 16;        // a node with more particles than this is split
     int group = 64;        // maximal target group (one wavefront)
-    int runs = 0;          // 1: groups are runs of `group` particles of a chaining cell; 0: tree groups
+    int runs = 0;          // 0: tree groups; 1: runs of `group` particles of a chaining cell; 2: packed tree groups
     int walk_waves = 0;    // persistent walk waves (0: auto)
 };
 

@@ -522,14 +522,18 @@ __global__ void moments_f64_kernel(const double* __restrict__ x, const double* _
 // lattice or a halo are uneven (many of a few particles each), so merging cuts the number of
 // groups, and with it of tree walks, to about half while keeping each group inside its parent.
 // The parent (or the big leaf, or the small cell root) owns its groups; group_pack visits them.
-// Run groups (runs != 0): a chaining cell's particles, in their Morton order, are cut into
+// Run groups (runs == 1): a chaining cell's particles, in their Morton order, are cut into
 // consecutive runs of G (the last one shorter), whatever the tree below the cell: every group but
 // the last of a cell fills a wavefront.
+// Packed groups (runs == 2): tree groups, except that the small children of a parent are packed
+// into runs of exactly G consecutive particles (a run may end inside a child), so the runs of a
+// parent fill their wavefronts but the last, and each still lies inside its parent.
 template <typename F>
 __device__ inline uint32_t group_pack(const int4& nd, int L, int G, int runs, const int4* __restrict__ info,
                                       const int* __restrict__ child, const int* __restrict__ parent, size_t v,
                                       F&& emit) {
-    if (runs) {
+    const bool packed = runs == 2;
+    if (runs == 1) {
         if (nd.w != L) return 0u;
         const uint32_t c = (uint32_t)((nd.y + G - 1) / G);
         for (uint32_t t = 0; t < c; t++) {
@@ -559,6 +563,12 @@ __device__ inline uint32_t group_pack(const int4& nd, int L, int G, int runs, co
         const int4 ci = info[child[8 * v + k]];
         if (ci.y > G) {  // a big child owns its own groups and ends the run
             if (rn) { emit(rb, rn); c++; rn = 0; }
+            continue;
+        }
+        if (packed) {  // the run takes the child's particles and is cut at exactly G
+            if (!rn) rb = ci.x;
+            rn += ci.y;
+            while (rn >= G) { emit(rb, G); c++; rb += G; rn -= G; }
             continue;
         }
         if (rn && rn + ci.y > G) { emit(rb, rn); c++; rn = 0; }

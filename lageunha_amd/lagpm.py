@@ -24,7 +24,7 @@ import numpy as np
 PUSH, PULL, KICK = 0, 1, 2  # eunha.h:69-77
 LAW_PMFIT, LAW_ERFC, LAW_TREEFORCE = 0, 1, 2
 PREC_FP32, PREC_FP64 = 0, 1
-GROUPS_TREE, GROUPS_RUNS = 0, 1
+GROUPS_TREE, GROUPS_RUNS, GROUPS_PACKED = 0, 1, 2
 NSPLINE = 4096
 
 _STATUS = {
@@ -414,7 +414,7 @@ class Box:
         self.ctx.check(self.lib.lagpm_drift(self.ctx.h, self.h, pfact))
 
     def set_tuning(self, nleaf: int, group: int, mode: int = None):
-        """Leaf size, group size and (when given) group mode (GROUPS_TREE or GROUPS_RUNS)."""
+        """Leaf size, group size and (when given) group mode (GROUPS_TREE, GROUPS_RUNS or GROUPS_PACKED)."""
         rc = self.lib.lagpm_box_set_tuning(self.h, nleaf, group)
         if rc != 0:
             raise LagpmError(rc, f"bad tuning nleaf={nleaf} group={group}")

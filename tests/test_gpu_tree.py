@@ -175,7 +175,7 @@ def test_autotune_keeps_velocities_and_a_valid_tuning(ctx):
         b.upload_pos(pos, vel)
         nl, g, mode = b.autotune()
         assert (nl, g, mode) == b.tuning() and 4 <= nl <= 256 and 8 <= g <= 64
-        assert mode in (lagpm.GROUPS_TREE, lagpm.GROUPS_RUNS)
+        assert mode in (lagpm.GROUPS_TREE, lagpm.GROUPS_RUNS, lagpm.GROUPS_PACKED)
         assert np.array_equal(b.velocities(), vel)
         # and the tuned tree still gives the exact sum at theta -> 0
     p0 = lagpm.make_params(n, theta=1e-6)
@@ -354,33 +354,56 @@ def test_walk_census_counts_the_pairs_within_the_cutoff(ctx):
     assert st_off.n_pp_within == 0
 
 
+@pytest.mark.parametrize("mode", [lagpm.GROUPS_RUNS, lagpm.GROUPS_PACKED])
 @pytest.mark.parametrize("prec", [lagpm.PREC_FP32, lagpm.PREC_FP64])
-def test_run_groups_theta_zero_is_the_direct_sum(ctx, prec):
-    """Groups cut as runs of consecutive particles of a chaining cell give every particle its exact
-    pair sum at theta -> 0, as tree groups do."""
+def test_run_groups_theta_zero_is_the_direct_sum(ctx, prec, mode):
+    """Groups cut as runs of consecutive particles (of a chaining cell, or packed from the small
+    children of a node) give every particle its exact pair sum at theta -> 0, as tree groups do."""
     n = 32
     pos = workload(n, "clustered", seed=14)
     p = lagpm.make_params(n, theta=1e-6, precision=prec)
     with lagpm.Box(ctx, p, len(pos)) as b:
-        b.set_tuning(16, 64, lagpm.GROUPS_RUNS)
+        b.set_tuning(16, 64, mode)
+        assert b.tuning() == (16, 64, mode)
         b.upload_pos(pos)
         b.treecorrection(lagpm.evolfact(1.0))
         a, st = b.velocities(), b.tree_stats()
     err = rel_rms(a, exact_erfc(n, p, pos))
-    print(f"run groups theta0 prec={prec}: rms {err:.2e}, groups {st.n_groups} ({len(pos) / st.n_groups:.1f} targets each)",
-          flush=True)
+    print(f"groups mode {mode} theta0 prec={prec}: rms {err:.2e}, groups {st.n_groups} "
+          f"({len(pos) / st.n_groups:.1f} targets each)", flush=True)
     assert err < 3e-5  # both precisions interpolate the tabulated law (~2e-6 from the analytic one)
 
 
-def test_run_groups_theta_half_fp32_vs_fp64_and_no_less_accurate(ctx):
-    """At theta = 0.5 run groups open nodes by the groups' own boxes: FP32 agrees with FP64 on the
-    same groups, and the error against the exact sum stays within the reference walk's."""
+def test_packed_groups_fill_more_lanes_than_tree_groups(ctx):
+    """Packing the small children of a node into runs of exactly `group` particles leaves fewer,
+    fuller groups than merging whole children; every particle is in exactly one group either way
+    (theta -> 0 gives the same exact sums)."""
+    n = 48
+    pos = workload(n, "clustered", seed=16)
+    p = lagpm.make_params(n, theta=1e-6)
+    res = {}
+    for mode in (lagpm.GROUPS_TREE, lagpm.GROUPS_PACKED):
+        with lagpm.Box(ctx, p, len(pos)) as b:
+            b.set_tuning(16, 64, mode)
+            b.upload_pos(pos)
+            b.treecorrection(lagpm.evolfact(1.0))
+            res[mode] = (b.velocities(), b.tree_stats())
+    (a_t, st_t), (a_p, st_p) = res[lagpm.GROUPS_TREE], res[lagpm.GROUPS_PACKED]
+    print(f"targets per group: tree {len(pos) / st_t.n_groups:.1f}, packed {len(pos) / st_p.n_groups:.1f}", flush=True)
+    assert st_p.n_groups < st_t.n_groups
+    assert rel_rms(a_p, a_t) < 1e-5
+
+
+@pytest.mark.parametrize("mode", [lagpm.GROUPS_RUNS, lagpm.GROUPS_PACKED])
+def test_run_groups_theta_half_fp32_vs_fp64_and_no_less_accurate(ctx, mode):
+    """At theta = 0.5 run or packed groups open nodes by the groups' own boxes: FP32 agrees with
+    FP64 on the same groups, and the error against the exact sum stays within the reference walk's."""
     n = 48
     pos = workload(n, "clustered", seed=15)
     a = {}
     for prec in (lagpm.PREC_FP32, lagpm.PREC_FP64):
         with lagpm.Box(ctx, lagpm.make_params(n, theta=0.5, precision=prec), len(pos)) as b:
-            b.set_tuning(16, 64, lagpm.GROUPS_RUNS)
+            b.set_tuning(16, 64, mode)
             b.upload_pos(pos)
             b.treecorrection(lagpm.evolfact(1.0))
             a[prec] = b.velocities()
@@ -390,6 +413,7 @@ def test_run_groups_theta_half_fp32_vs_fp64_and_no_less_accurate(ctx):
     ref_walk = orc.treecorrection_ref(n, pos, diff, slope, p.rsphere, 0.5, cellw=max(4.0, p.rsphere))
     e_prec, e_tree, e_ref = rel_rms(a[lagpm.PREC_FP32], a[lagpm.PREC_FP64]), rel_rms(a[lagpm.PREC_FP32], exact), \
         rel_rms(ref_walk, exact)
-    print(f"run groups theta=0.5: fp32 vs fp64 {e_prec:.2e}; vs exact {e_tree:.2e}; reference walk {e_ref:.2e}", flush=True)
+    print(f"groups mode {mode} theta=0.5: fp32 vs fp64 {e_prec:.2e}; vs exact {e_tree:.2e}; reference walk {e_ref:.2e}",
+          flush=True)
     assert e_prec < 1e-4
     assert e_tree <= 1.02 * e_ref
