@@ -27,12 +27,15 @@ namespace {
 #ifndef LAGPM_DEP_PPT
 #define LAGPM_DEP_PPT 4
 #endif
+// Resident deposit blocks per CU (the LDS box allows 4). Two leave room for the tree build's
+// kernels, which run beside the deposit in a step: 4 per CU deposit faster alone but hold the CUs'
+// LDS while the build waits, and the whole step is slower.
 #ifndef LAGPM_DEP_BLOCKS_PER_CU
-#define LAGPM_DEP_BLOCKS_PER_CU 4
+#define LAGPM_DEP_BLOCKS_PER_CU 2
 #endif
 constexpr int DEP_THREADS = 256;
 constexpr int DEP_PPT = LAGPM_DEP_PPT;  // particles per thread and chunk
-constexpr int DEP_BLOCKS_PER_CU = LAGPM_DEP_BLOCKS_PER_CU;  // resident deposit blocks per CU (the LDS box allows 4)
+constexpr int DEP_BLOCKS_PER_CU = LAGPM_DEP_BLOCKS_PER_CU;
 constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;
 constexpr int DEP_BOX = 16;  // the LDS box of a chunk: 16^3 cells, index x + 16 (y + 16 z)
 constexpr int DEP_LDS = DEP_BOX * DEP_BOX * DEP_BOX;
