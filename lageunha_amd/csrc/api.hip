@@ -133,7 +133,7 @@ void free_box(lagpm_box* b) {
     b->spec = b->phi = nullptr;  // the same padded mesh as rho
     dfree(b->pp); dfree(b->ncount); dfree(b->noff); dfree(b->nd_info); dfree(b->nd_parent); dfree(b->nd_child);
     dfree(b->nd_com); dfree(b->nd_q0); dfree(b->nd_q1); dfree(b->nd_d); dfree(b->cellroot); dfree(b->gflag);
-    dfree(b->dep_scan_tmp);
+    dfree(b->dep_scan_tmp); dfree(b->dep_blk);
     dfree(b->goff); dfree(b->groups); dfree(b->gbox); dfree(b->pgroup); dfree(b->counters); dfree(b->dsz);
     dfree(b->scan_tmp);
 }
@@ -168,7 +168,7 @@ double apf(const cosmo& c, double a) {
     return std::sqrt(8.0 * M_PI / 3.0 / (c.amax * c.om) / (zp1 * zp1) * h * h);
 }
 double integ(const cosmo& c, double a1, double a2) {
-    const int nsub = 32;
+    const int nsub = 4;  // 8-point Gauss-Legendre on 4 pieces: the integrand is smooth over a step
     double s = 0, hstep = (a2 - a1) / nsub;
     for (int k = 0; k < nsub; k++) {
         double lo = a1 + k * hstep, mid = lo + 0.5 * hstep, half = 0.5 * hstep;
@@ -457,10 +457,10 @@ size_t lagpm_estimate_box_bytes(const lagpm_params* p, size_t np) {
     b += m * 8 + K * 2 * real + 4 * m * real;              // acc, padded FFT mesh, grad
     b += 4 * mcell * mcell * mcell + 4 * 8 + DSZ_COUNT * 4;  // cellroot, counters, sizes
     b += 4 * (lagpm_deposit_chunk_cap(p->n, np) + 1);     // deposit chunk starts
+    b += 12 * (lagpm_deposit_blocks(p->n) + 1) + lagpm_scan_tmp_bytes(lagpm_deposit_blocks(p->n));  // chunk list
     const size_t cap = lagpm_tree_node_cap_guess(np, (int)mcell), gcap = cap + np / 64 + 1;
     b += cap * (16 + 4 + 32 + 4 + 4 + (f64 ? 96 : 48)) + gcap * (8 + (f64 ? 48 : 0));
     b += std::max(lagpm_scan_tmp_bytes(cap), std::max(lagpm_scan_tmp_bytes(np), lagpm_sort_tmp_bytes(np)));
-    b += lagpm_scan_tmp_bytes(np);  // the deposit's own scan scratch
     return b;
 }
 
@@ -498,7 +498,9 @@ int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box
     ALLOC(box->cellroot, (size_t)box->mcell * box->mcell * box->mcell);
     ALLOC(box->counters, 4);
     ALLOC(box->dsz, DSZ_COUNT);
-    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->dep_scan_tmp, lagpm_scan_tmp_bytes(np));
+    const size_t nblk = lagpm_deposit_blocks(p->n);
+    if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->dep_scan_tmp, lagpm_scan_tmp_bytes(nblk));
+    ALLOC(box->dep_blk, 3 * (nblk + 1));
     box->dep_cap = lagpm_deposit_chunk_cap(p->n, np) + 1;
     ALLOC(box->dep_start, box->dep_cap);
     if (f64) ALLOC(box->pgroup, np);

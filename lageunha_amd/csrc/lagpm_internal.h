@@ -151,6 +151,7 @@ struct lagpm_box {
     void* phi = nullptr;
     void* grad = nullptr;                // n^3 x 4 real (fx, fy, fz, phi)
     uint32_t* dep_start = nullptr;       // deposit chunk starts (+ sentinel)
+    uint32_t* dep_blk = nullptr;         // per Morton block: first particle, chunk count, chunk offset
     size_t dep_cap = 0;
 
     // tree
@@ -224,6 +225,7 @@ int lagpm_green_device(lagpm_ctx* ctx, hipStream_t s, int n, int prec, double ef
 // the context's Green factor table for (n, efold, deconvolve); allocates (call before capture)
 int lagpm_green_prepare(lagpm_ctx* ctx, hipStream_t s, int n, double efold, int deconvolve);
 size_t lagpm_deposit_chunk_cap(int n, size_t np);
+size_t lagpm_deposit_blocks(int n);  // Morton blocks of the deposit's chunking
 int lagpm_fft_release(lagpm_ctx* ctx);
 int lagpm_box_fft_release(lagpm_box* box);
 int lagpm_pm_prepare(lagpm_ctx* ctx, lagpm_box* box);

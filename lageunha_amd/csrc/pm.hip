@@ -86,23 +86,47 @@ __global__ void zero_u64_kernel(unsigned long long* __restrict__ a, size_t m) {
 }
 
 // Deposit chunks: runs of at most DEP_CHUNK consecutive (Morton-sorted) particles that stay
-// inside one Morton block of <= 13 mesh cells per side, so every chunk's TSC footprint fits
-// the 4096-cell LDS box. A chunk starts where the block prefix changes or at a multiple of
-// DEP_CHUNK.
-// chunk starts (flag) of the N particles, and the accumulator cleared (m cells), in one pass
-__global__ void dep_flag_kernel(const uint64_t* __restrict__ K, size_t N, int sh, uint32_t* __restrict__ flag,
-                                unsigned long long* __restrict__ acc, size_t m) {
+// inside one Morton block of <= 13 mesh cells per side, so every chunk's TSC footprint fits the
+// 16^3 LDS box. Block b's particles are the sorted keys with block prefix b; the block is cut into
+// ceil(count / DEP_CHUNK) chunks. The list is built per block (a binary search for each block's
+// first particle, a scan over the blocks' chunk counts, the starts written per block), so no pass
+// touches every particle.
+//
+// bstart[b] = first particle of block b (b <= nblk; bstart[nblk] = N), bcnt[b] = its chunk count;
+// the accumulator is cleared (m cells) in the same launch.
+__global__ void dep_blocks_kernel(const uint64_t* __restrict__ K, size_t N, int sh, uint32_t nblk,
+                                  uint32_t* __restrict__ bstart, uint32_t* __restrict__ bcnt,
+                                  unsigned long long* __restrict__ acc, size_t m) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m || i < N; i += stride) {
-        if (i < m) acc[i] = 0ull;
-        if (i < N) flag[i] = (i == 0 || (i % DEP_CHUNK) == 0 || (K[i] >> sh) != (K[i - 1] >> sh)) ? 1u : 0u;
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t i = t0; i < m; i += stride) acc[i] = 0ull;
+    auto first_at_least = [&](uint64_t pfx) -> size_t {  // first index whose prefix is >= pfx
+        size_t lo = 0, hi = N;
+        while (lo < hi) {
+            const size_t mid = lo + ((hi - lo) >> 1);
+            if ((K[mid] >> sh) >= pfx) hi = mid;
+            else lo = mid + 1;
+        }
+        return lo;
+    };
+    for (size_t b = t0; b <= nblk; b += stride) {
+        const size_t s = b == nblk ? N : first_at_least(b);
+        bstart[b] = (uint32_t)s;
+        if (b < nblk) {
+            const size_t e = b + 1 == nblk ? N : first_at_least(b + 1);
+            bcnt[b] = (uint32_t)((e - s + DEP_CHUNK - 1) / DEP_CHUNK);
+        }
     }
 }
-__global__ void dep_start_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off, size_t N,
-                                 uint32_t* __restrict__ start, const uint32_t* __restrict__ nchunk) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N && flag[i]) start[off[i]] = (uint32_t)i;
-    if (i == 0) start[*nchunk] = (uint32_t)N;
+// chunk starts of block b from its first particle and chunk offset; the sentinel start[nchunk] = N
+__global__ void dep_chunks_kernel(const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bcnt,
+                                  const uint32_t* __restrict__ boff, uint32_t nblk, size_t N,
+                                  uint32_t* __restrict__ start, const uint32_t* __restrict__ nchunk) {
+    const size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) start[*nchunk] = (uint32_t)N;
+    if (b >= nblk) return;
+    const uint32_t c = bcnt[b], o = boff[b], s = bstart[b];
+    for (uint32_t j = 0; j < c; j++) start[o + j] = s + j * (uint32_t)DEP_CHUNK;
 }
 
 // LDS box cell of offsets (x, y, z) in [0, 16)^3 from the chunk's corner. Rows of 16 cells, with
@@ -917,10 +941,10 @@ static int deposit_block_level(int n) {
     return lvl;
 }
 
+size_t lagpm_deposit_blocks(int n) { return (size_t)1 << (3 * deposit_block_level(n)); }
+
 size_t lagpm_deposit_chunk_cap(int n, size_t np) {
-    const int lvl = deposit_block_level(n);
-    const size_t blocks = lvl >= 10 ? np : std::min(np, (size_t)1 << (3 * lvl));
-    return np / DEP_CHUNK + blocks + 2;
+    return np / DEP_CHUNK + std::min(np, lagpm_deposit_blocks(n)) + 2;
 }
 
 int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
@@ -935,19 +959,22 @@ int lagpm_pm_deposit(lagpm_ctx* ctx, lagpm_box* box) {
     } else {
         // chunk list from the sorted keys (the caller sorted the particles); the chunk count
         // stays on the device and the deposit grid covers its upper bound. Its scratch is its own
-        // (perm2 and key2 hold nothing between sorts, dep_scan_tmp is the deposit's): a step runs
-        // the deposit beside the tree build, which uses ncount, noff and scan_tmp.
-        const int sh = 3 * (LAGPM_KEY_BITS - deposit_block_level(n));
+        // (dep_blk, dep_scan_tmp): a step runs the deposit beside the tree build, which uses
+        // ncount, noff and scan_tmp.
+        const int lvl = deposit_block_level(n);
+        const int sh = 3 * (LAGPM_KEY_BITS - lvl);
+        const uint32_t nblk = (uint32_t)lagpm_deposit_blocks(n);
         const size_t np = box->np;
-        uint32_t* flag = box->perm2;
-        uint32_t* off = reinterpret_cast<uint32_t*>(box->key2);
-        hipLaunchKernelGGL(dep_flag_kernel, dim3(grid_for(std::max(np, m), 256)), dim3(256), 0, st, box->key, np, sh, flag,
-                           box->acc, m);
+        uint32_t* bstart = box->dep_blk;
+        uint32_t* bcnt = bstart + (nblk + 1);
+        uint32_t* boff = bcnt + (nblk + 1);
+        hipLaunchKernelGGL(dep_blocks_kernel, dim3(grid_for(std::max((size_t)nblk + 1, m), 256)), dim3(256), 0, st, box->key,
+                           np, sh, nblk, bstart, bcnt, box->acc, m);
         HIPCHK(ctx, hipGetLastError());
         uint32_t* nchunk = box->dsz + DSZ_CHUNKS;
-        RCHK(lagpm_exclusive_scan_u32(ctx, st, box->dep_scan_tmp, flag, off, np, nchunk));
-        hipLaunchKernelGGL(dep_start_kernel, dim3(grid_for(np, 256, 1u << 30)), dim3(256), 0, st, flag, off, np,
-                           box->dep_start, nchunk);
+        RCHK(lagpm_exclusive_scan_u32(ctx, st, box->dep_scan_tmp, bcnt, boff, nblk, nchunk));
+        hipLaunchKernelGGL(dep_chunks_kernel, dim3(grid_for((size_t)nblk, 256)), dim3(256), 0, st, bstart, bcnt, boff, nblk,
+                           np, box->dep_start, nchunk);
         HIPCHK(ctx, hipGetLastError());
         // persistent blocks, a multiple of the 8 XCDs, as many as are resident at once
         const unsigned cap = (unsigned)(box->dep_cap - 1);

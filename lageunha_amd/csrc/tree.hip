@@ -377,34 +377,47 @@ __device__ inline void kahan(float& s, float& c, float v) {
     s = t;
 }
 
-// FP32 moments with Kahan-compensated sums, one level per launch (deepest first).
-// Node moments, one node per thread: the com (Kahan sums relative to the first member), the
-// second moments about it (Kahan), and rmax. Leaves read their particles, inner nodes their
-// children, whose moments the previous level's launch wrote (and small inner nodes their
-// particles again for the exact rmax).
-__device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp, const int4* __restrict__ info,
+// FP32 moments, one level per launch (deepest first), 8 lanes per node: the lanes take
+// the node's particles (or children) in turn, so a row's loads are consecutive particles, and the
+// row's partial sums meet in a butterfly. Each lane sums with Kahan compensation, relative to the
+// node's first member (or first child's com); the compensated lane sums are then added in a fixed
+// order, so the moments are deterministic. Leaves read their particles; inner nodes read their
+// children's moments, which the previous level's launch wrote (and inner nodes of at most
+// RMAX_EXACT_MAX particles read their particles again for the exact rmax).
+constexpr int MOM_ROW = 8;  // lanes per node: a leaf's <= 32 particles are 4 per lane, an inner node's <= 8 children 1
+__device__ inline float row_sum(float v) {
+#pragma unroll
+    for (int o = MOM_ROW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, MOM_ROW);
+    return v;
+}
+__device__ inline float row_max(float v) {
+#pragma unroll
+    for (int o = MOM_ROW / 2; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, MOM_ROW));
+    return v;
+}
+// every lane of the node's row calls this with its index `sub` in the row
+__device__ inline void moments_f32_node(size_t v, int sub, const float4* __restrict__ pp, const int4* __restrict__ info,
                                         const int* __restrict__ child, float4* __restrict__ com, float4* __restrict__ q0,
                                         float4* __restrict__ q1) {
     const int4 nd = info[v];
     float M = 0, cM = 0, Sx = 0, cx = 0, Sy = 0, cy = 0, Sz = 0, cz = 0;
     float Q[6] = {0, 0, 0, 0, 0, 0}, cQ[6] = {0, 0, 0, 0, 0, 0};
-    float3 cm;
     float r2max = 0.f, rbound = 0.f;
+    float3 cm;
     if (nd.z == 0) {
         const float4 r = pp[nd.x];
-#pragma unroll 4
-        for (int p = nd.x; p < nd.x + nd.y; p++) {
-            float4 a = pp[p];
+        for (int p = nd.x + sub; p < nd.x + nd.y; p += MOM_ROW) {
+            const float4 a = pp[p];
             kahan(M, cM, a.w);
             kahan(Sx, cx, a.w * (a.x - r.x));
             kahan(Sy, cy, a.w * (a.y - r.y));
             kahan(Sz, cz, a.w * (a.z - r.z));
         }
-        cm = make_float3(r.x + Sx / M, r.y + Sy / M, r.z + Sz / M);
-#pragma unroll 4
-        for (int p = nd.x; p < nd.x + nd.y; p++) {
-            float4 a = pp[p];
-            float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
+        M = row_sum(M - cM);
+        cm = make_float3(r.x + row_sum(Sx - cx) / M, r.y + row_sum(Sy - cy) / M, r.z + row_sum(Sz - cz) / M);
+        for (int p = nd.x + sub; p < nd.x + nd.y; p += MOM_ROW) {
+            const float4 a = pp[p];
+            const float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
             kahan(Q[0], cQ[0], a.w * dx * dx);
             kahan(Q[1], cQ[1], a.w * dy * dy);
             kahan(Q[2], cQ[2], a.w * dz * dz);
@@ -414,41 +427,42 @@ __device__ inline void moments_f32_node(size_t v, const float4* __restrict__ pp,
             r2max = fmaxf(r2max, dx * dx + dy * dy + dz * dz);
         }
     } else {
+        // lane c < nd.z takes child c
+        const bool has = sub < nd.z;
+        const int cv = has ? child[8 * v + sub] : 0;
         const float4 r = com[child[8 * v]];
-        for (int c = 0; c < nd.z; c++) {
-            float4 a = com[child[8 * v + c]];
-            kahan(M, cM, a.w);
-            kahan(Sx, cx, a.w * (a.x - r.x));
-            kahan(Sy, cy, a.w * (a.y - r.y));
-            kahan(Sz, cz, a.w * (a.z - r.z));
+        const float4 a = has ? com[cv] : make_float4(0.f, 0.f, 0.f, 0.f);
+        M = row_sum(a.w);
+        cm = make_float3(r.x + row_sum(a.w * (a.x - r.x)) / M, r.y + row_sum(a.w * (a.y - r.y)) / M,
+                         r.z + row_sum(a.w * (a.z - r.z)) / M);
+        if (has) {
+            const float4 b0 = q0[cv], b1 = q1[cv];
+            const float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
+            Q[0] = b0.x + a.w * dx * dx;
+            Q[1] = b0.y + a.w * dy * dy;
+            Q[2] = b0.z + a.w * dz * dz;
+            Q[3] = b0.w + a.w * dx * dy;
+            Q[4] = b1.x + a.w * dx * dz;
+            Q[5] = b1.y + a.w * dy * dz;
+            rbound = sqrtf(dx * dx + dy * dy + dz * dz) + b1.w;
         }
-        cm = make_float3(r.x + Sx / M, r.y + Sy / M, r.z + Sz / M);
-        for (int c = 0; c < nd.z; c++) {
-            int cv = child[8 * v + c];
-            float4 a = com[cv], b0 = q0[cv], b1 = q1[cv];
-            float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
-            kahan(Q[0], cQ[0], b0.x + a.w * dx * dx);
-            kahan(Q[1], cQ[1], b0.y + a.w * dy * dy);
-            kahan(Q[2], cQ[2], b0.z + a.w * dz * dz);
-            kahan(Q[3], cQ[3], b0.w + a.w * dx * dy);
-            kahan(Q[4], cQ[4], b1.x + a.w * dx * dz);
-            kahan(Q[5], cQ[5], b1.y + a.w * dy * dz);
-            rbound = fmaxf(rbound, sqrtf(dx * dx + dy * dy + dz * dz) + b1.w);
-        }
+        rbound = row_max(rbound);
         if (nd.y <= RMAX_EXACT_MAX) {
-#pragma unroll 4
-            for (int p = nd.x; p < nd.x + nd.y; p++) {
-                const float4 a = pp[p];
-                const float dx = a.x - cm.x, dy = a.y - cm.y, dz = a.z - cm.z;
+            for (int p = nd.x + sub; p < nd.x + nd.y; p += MOM_ROW) {
+                const float4 s = pp[p];
+                const float dx = s.x - cm.x, dy = s.y - cm.y, dz = s.z - cm.z;
                 r2max = fmaxf(r2max, dx * dx + dy * dy + dz * dz);
             }
-        } else {
-            r2max = rbound * rbound;
         }
     }
-    com[v] = make_float4(cm.x, cm.y, cm.z, M);
-    q0[v] = make_float4(Q[0], Q[1], Q[2], Q[3]);
-    q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], nd.z == 0 || nd.y <= RMAX_EXACT_MAX ? sqrtf(r2max) : rbound);
+#pragma unroll
+    for (int k = 0; k < 6; k++) Q[k] = row_sum(Q[k] - cQ[k]);
+    r2max = row_max(r2max);
+    if (sub == 0) {
+        com[v] = make_float4(cm.x, cm.y, cm.z, M);
+        q0[v] = make_float4(Q[0], Q[1], Q[2], Q[3]);
+        q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], nd.z == 0 || nd.y <= RMAX_EXACT_MAX ? sqrtf(r2max) : rbound);
+    }
 }
 
 constexpr int LEVEL_BLOCKS = 2048;  // grid of the per-level launches (grid-stride over the level)
@@ -458,8 +472,11 @@ __global__ void moments_f32_kernel(const float4* __restrict__ pp, const uint32_t
                                    float4* __restrict__ q1) {
     if (dsz[DSZ_OVERFLOW]) return;
     const uint32_t n = dsz[DSZ_LVL + level], o = dsz[DSZ_LOFF + level];
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        moments_f32_node(order[o + i], pp, info, child, com, q0, q1);
+    const int sub = (int)(threadIdx.x & (MOM_ROW - 1));
+    // a node's lanes share i, so they are active or done together and their shuffles stay among them
+    for (size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / MOM_ROW; i < n;
+         i += (size_t)gridDim.x * blockDim.x / MOM_ROW)
+        moments_f32_node(order[o + i], sub, pp, info, child, com, q0, q1);
 }
 
 // FP64 moments for the plain FP64 path (12 doubles per node).
@@ -1458,6 +1475,7 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     hipLaunchKernelGGL(level_mark_kernel, dim3(1), dim3(64), 0, st, box->dsz, cap, L, L);
     HIPCHK(ctx, hipGetLastError());
     const unsigned lblocks = (unsigned)std::min<size_t>(LEVEL_BLOCKS, blocks_for(cap, 256));
+    const unsigned mblocks = (unsigned)std::min<size_t>(2 * LEVEL_BLOCKS, blocks_for(cap * MOM_ROW, 256));
     const int D = std::min(LAGPM_KEY_BITS, std::max(L, box->depth));
     for (int l = L; l < D; l++) {
         hipLaunchKernelGGL(split_kernel, dim3(lblocks), dim3(SPLIT_THREADS), 0, st, box->key, l, D, nleaf, cap, box->dsz,
@@ -1478,8 +1496,8 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
         if (f64)
             hipLaunchKernelGGL(moments_f64_kernel, dim3(lblocks), dim3(256), 0, st, box->x, box->y, box->z, 1.0, box->dsz, l,
                                order, box->nd_info, box->nd_child, box->nd_d);
-        else
-            hipLaunchKernelGGL(moments_f32_kernel, dim3(lblocks), dim3(256), 0, st, box->pp, box->dsz, l, order,
+        else  // (MOM_ROW lanes per node: a larger grid keeps as many nodes in flight)
+            hipLaunchKernelGGL(moments_f32_kernel, dim3(mblocks), dim3(256), 0, st, box->pp, box->dsz, l, order,
                                box->nd_info, box->nd_child, box->nd_com, box->nd_q0, box->nd_q1);
         HIPCHK(ctx, hipGetLastError());
     }
