@@ -68,6 +68,25 @@ def test_tsc_deposit_repeated_in_one_box(ctx, n, npart):
             assert np.array_equal(acc, oacc), f"deposit {rep}"
 
 
+@pytest.mark.parametrize("n", [48, 96, 100])
+def test_tsc_deposit_chunking_by_morton_block(ctx, n):
+    """The chunk list is made per Morton block (side n / 2^l <= 13 cells: 12 at n = 48 and 96,
+    12.5 at n = 100): blocks holding many chunks (a dense clump), empty blocks and a non-integer
+    block side all give the oracle's sum bit for bit."""
+    g = np.random.default_rng(n + 5)
+    bg = g.random((3 * n * n, 3)) * n                                   # a sparse background: many empty blocks
+    clump = (n / 3 + g.standard_normal((40000, 3)) * 0.6) % n           # thousands of particles in a few blocks
+    edge = np.stack([np.full(2000, n - 1e-9), g.random(2000) * n, g.random(2000) * n], axis=1)  # at the face
+    pos = np.concatenate([bg, clump, edge])
+    print(f"deposit chunking n={n} N={len(pos)}", flush=True)
+    with lagpm.Box(ctx, lagpm.make_params(n), len(pos)) as b:
+        b.upload_pos(pos)
+        acc, bits = b.tsc_fixed()
+    oacc, obits = orc.tsc_fixed(n, pos, mass=1.0)
+    assert bits == obits
+    assert np.array_equal(acc, oacc)
+
+
 def test_tsc_deposit_dense_halo_falls_back_and_stays_exact(ctx):
     n = 32
     g = np.random.default_rng(9)
