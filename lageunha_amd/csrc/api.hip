@@ -525,6 +525,8 @@ int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box
     box->st = ctx->stream;
     for (int i = 0; i < 15; i++) (void)hipEventCreate(&box->ev[i]);
     (void)hipEventCreateWithFlags(&box->evfork, hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&box->evsort, hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&box->evvel, hipEventDisableTiming);
     (void)hipMemsetAsync(box->vx, 0, np * sizeof(double), ctx->stream);
     (void)hipMemsetAsync(box->vy, 0, np * sizeof(double), ctx->stream);
     (void)hipMemsetAsync(box->vz, 0, np * sizeof(double), ctx->stream);
@@ -540,6 +542,8 @@ int lagpm_box_destroy(lagpm_ctx* ctx, lagpm_box* box) {
     for (int i = 0; i < 15; i++)
         if (box->ev[i]) (void)hipEventDestroy(box->ev[i]);
     if (box->evfork) (void)hipEventDestroy(box->evfork);
+    if (box->evsort) (void)hipEventDestroy(box->evsort);
+    if (box->evvel) (void)hipEventDestroy(box->evvel);
     free_box(box);
     delete box;
     return LAGPM_OK;
@@ -1128,8 +1132,10 @@ int ensure_pool(lagpm_ctx* ctx, size_t want) {
 int front_body(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     hipStream_t st = box->st, st2 = box->st2;
     if (ev) HIPCHK(ctx, hipEventRecord(ev[0], st));
-    // the Hubble drag (v *= dfac[0]) rides in the sort's key pass
-    RCHK(lagpm_sort_particles(ctx, box, (const double*)box->dfac));
+    // the Hubble drag (v *= dfac[0]) rides in the sort's gather; a lone box gathers its
+    // velocities on st3, beside the deposit and the build, and the PM kick waits for them
+    const bool vsplit = box->st3 && box->st3 != st;
+    RCHK(lagpm_sort_particles(ctx, box, (const double*)box->dfac, vsplit ? box->st3 : nullptr));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[1], st));
     HIPCHK(ctx, hipEventRecord(box->evfork, st));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[12], st));
@@ -1138,6 +1144,7 @@ int front_body(lagpm_ctx* ctx, lagpm_box* box, hipEvent_t* ev) {
     RCHK(lagpm_pm_solve(ctx, box, ev ? ev + 3 : nullptr));
     RCHK(lagpm_pm_gradient(ctx, box));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[6], st));
+    if (vsplit) HIPCHK(ctx, hipStreamWaitEvent(st, box->evvel, 0));
     RCHK(lagpm_pm_interp(ctx, box));
     if (ev) HIPCHK(ctx, hipEventRecord(ev[7], st));
     const bool fork = st2 != st;  // several boxes in a batch build on their own streams
@@ -1264,7 +1271,7 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
     }
     if (live.empty()) return LAGPM_OK;
     const size_t S = std::min<size_t>(live.size(), 8);
-    RCHK(ensure_pool(ctx, S));
+    RCHK(ensure_pool(ctx, std::max<size_t>(S, 2)));  // a lone box uses a second pool stream
     // the pool streams start after everything queued on the context stream so far
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     for (size_t k = 0; k < S; k++) HIPCHK(ctx, hipStreamWaitEvent(ctx->pool[k], ctx->ev[0], 0));
@@ -1293,6 +1300,7 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
             if (!b->np) continue;
             b->st = ctx->pool[k++ % S];
             b->st2 = b->st;
+            b->st3 = b->st;
             grp.push_back(b);
             gf.push_back(fact[(size_t)i]);
             gef.push_back(&ef[(size_t)i]);
@@ -1334,6 +1342,7 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
             // own stream, as one shared build stream would serialise the builds of all boxes.
             b->st = ctx->pool[k++ % S];
             b->st2 = live.size() == 1 ? ctx->stream : b->st;
+            b->st3 = live.size() == 1 ? ctx->pool[1] : b->st;  // a lone box's velocity gather
             fb.push_back(b);
             fe.push_back(&ef[(size_t)i]);
         }

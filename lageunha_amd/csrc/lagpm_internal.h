@@ -136,6 +136,9 @@ struct lagpm_box {
     // captured front half of a step (drag .. tree build), one per particle-set parity
     hipGraphExec_t gfront[2] = {nullptr, nullptr};  // captured step front per particle-set parity
     hipEvent_t evfork = nullptr;  // fork of the step front: the tree build branches off after the sort
+    hipEvent_t evsort = nullptr;  // keys sorted: the velocity gather branches off
+    hipEvent_t evvel = nullptr;   // velocities and ids gathered
+    hipStream_t st3 = nullptr;    // the velocity gather's stream in a lone box's step front (else st)
     bool graph_failed = false;
     uint32_t* scan_tmp = nullptr;  // scan / radix-sort scratch of this box (sort and tree build)
     size_t scan_tmp_bytes = 0;
@@ -238,7 +241,9 @@ int lagpm_pk_scratch_init(lagpm_ctx* ctx);
 void lagpm_box_graphs_release(lagpm_box* box);
 
 // ---- tree (tree.hip) ----
-int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box, const double* drag = nullptr);
+// With vst (a stream other than box->st): the velocities and ids are gathered on vst, and
+// box->evvel marks their completion (the caller makes its stream wait for it).
+int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box, const double* drag = nullptr, hipStream_t vst = nullptr);
 // Queues the octree build on box->st; sizes stay on the device (dsz).
 int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box);
 // One walk launch on ctx->stream over all groups of the given boxes (FP32 boxes sharing one

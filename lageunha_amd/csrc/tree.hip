@@ -142,22 +142,14 @@ __device__ inline size_t lower_bound_pfx(const uint64_t* __restrict__ K, size_t 
     return lo;
 }
 
-// Morton keys of the positions and the identity permutation; with drag, also the Hubble drag of
-// the step's front (v *= drag[0], Cosmos/globalT.c:102-110) in the same pass over the particles.
+// Morton keys of the positions and the identity permutation. (The Hubble drag of the step's front,
+// v *= drag[0], Cosmos/globalT.c:102-110, rides in the gather after the sort.)
 __global__ void keys_kernel(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-                            size_t np, double s, uint64_t* __restrict__ key, uint32_t* __restrict__ perm,
-                            double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
-                            const double* __restrict__ drag) {
+                            size_t np, double s, uint64_t* __restrict__ key, uint32_t* __restrict__ perm) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     key[i] = spread3(quant(x[i], s)) | (spread3(quant(y[i], s)) << 1) | (spread3(quant(z[i], s)) << 2);
     perm[i] = (uint32_t)i;
-    if (drag) {
-        const double f = drag[0];
-        vx[i] *= f;
-        vy[i] *= f;
-        vz[i] *= f;
-    }
 }
 
 // all particle arrays into the new order in one pass (one permutation load, seven gathers in flight)
@@ -169,8 +161,11 @@ struct gather_set {
 };
 // The sources of neighbouring outputs share cache lines (the order changes little between steps),
 // so the gather gives each XCD one contiguous range of outputs (lagpm_wave::xcd_block): the lines
-// those outputs read are fetched into one L2 instead of into several.
-__global__ void gather_all_kernel(gather_set g, const uint32_t* __restrict__ perm, size_t np) {
+// those outputs read are fetched into one L2 instead of into several. With drag, the velocities
+// are multiplied by drag[0] on the way (the Hubble drag of the step's front), a pass over them
+// that the keys would otherwise make.
+__global__ void gather_all_kernel(gather_set g, const uint32_t* __restrict__ perm, size_t np,
+                                  const double* __restrict__ drag) {
     const size_t i = (size_t)lagpm_wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const uint32_t p = perm[i];
@@ -178,8 +173,46 @@ __global__ void gather_all_kernel(gather_set g, const uint32_t* __restrict__ per
 #pragma unroll
     for (int a = 0; a < 6; a++) v[a] = g.src[a][p];
     const uint32_t id = g.id[p];
+    if (drag) {
+        const double f = drag[0];
+        v[3] *= f;
+        v[4] *= f;
+        v[5] *= f;
+    }
 #pragma unroll
     for (int a = 0; a < 6; a++) g.dst[a][i] = v[a];
+    g.id_dst[i] = id;
+}
+
+// The same gather in two halves, for a step front that runs them on two streams: the positions
+// (which the deposit and the tree build wait for) and the velocities with the ids and the drag
+// (which only the PM kick and the drift need, so they load beside the deposit and the build).
+__global__ void gather_pos_kernel(gather_set g, const uint32_t* __restrict__ perm, size_t np) {
+    const size_t i = (size_t)lagpm_wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const uint32_t p = perm[i];
+    double v[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) v[a] = g.src[a][p];
+#pragma unroll
+    for (int a = 0; a < 3; a++) g.dst[a][i] = v[a];
+}
+__global__ void gather_vel_kernel(gather_set g, const uint32_t* __restrict__ perm, size_t np,
+                                  const double* __restrict__ drag) {
+    const size_t i = (size_t)lagpm_wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const uint32_t p = perm[i];
+    double v[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) v[a] = g.src[3 + a][p];
+    const uint32_t id = g.id[p];
+    if (drag) {
+        const double f = drag[0];
+#pragma unroll
+        for (int a = 0; a < 3; a++) v[a] *= f;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) g.dst[3 + a][i] = v[a];
     g.id_dst[i] = id;
 }
 
@@ -1399,13 +1432,13 @@ int lagpm_tree_alloc(lagpm_ctx* ctx, lagpm_box* box, size_t cap) {
     return LAGPM_OK;
 }
 
-int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box, const double* drag) {
+int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box, const double* drag, hipStream_t vst) {
     const size_t np = box->np;
     if (np == 0) return LAGPM_OK;
     hipStream_t st = box->st;
     const double s = 65536.0 / (double)box->p.n;
     hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, box->x, box->y, box->z, np, s,
-                       box->key, box->perm, box->vx, box->vy, box->vz, drag);
+                       box->key, box->perm);
     HIPCHK(ctx, hipGetLastError());
     // The tree splits nodes down to box->depth at most (deeper trees are detected and redone after
     // a full sort), so only the top 3 * depth key bits must be in order: a box whose tree is
@@ -1426,7 +1459,16 @@ int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box, const double* drag) {
     }
     g.id = box->id;
     g.id_dst = box->id_alt;
-    hipLaunchKernelGGL(gather_all_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, g, box->perm, np);
+    if (vst && vst != st) {  // velocities and ids on vst; the caller joins box->evvel before using them
+        HIPCHK(ctx, hipEventRecord(box->evsort, st));
+        HIPCHK(ctx, hipStreamWaitEvent(vst, box->evsort, 0));
+        hipLaunchKernelGGL(gather_vel_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, vst, g, box->perm, np, drag);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipEventRecord(box->evvel, vst));
+        hipLaunchKernelGGL(gather_pos_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, g, box->perm, np);
+    } else {
+        hipLaunchKernelGGL(gather_all_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, g, box->perm, np, drag);
+    }
     HIPCHK(ctx, hipGetLastError());
     for (int a = 0; a < 6; a++) std::swap(*arrs[a], box->alt[a]);
     std::swap(box->id, box->id_alt);
