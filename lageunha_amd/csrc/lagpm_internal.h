@@ -136,7 +136,7 @@ struct lagpm_box {
     // captured front half of a step (drag .. tree build), one per particle-set parity
     hipGraphExec_t gfront[2] = {nullptr, nullptr};  // captured step front per particle-set parity
     hipEvent_t evfork = nullptr;  // fork of the step front: the tree build branches off after the sort
-    hipEvent_t evsort = nullptr;  // keys sorted: the velocity gather branches off
+    hipEvent_t evsort = nullptr;  // positions gathered: the velocity gather branches off
     hipEvent_t evvel = nullptr;   // velocities and ids gathered
     hipStream_t st3 = nullptr;    // the velocity gather's stream in a lone box's step front (else st)
     bool graph_failed = false;

@@ -1459,13 +1459,16 @@ int lagpm_sort_particles(lagpm_ctx* ctx, lagpm_box* box, const double* drag, hip
     }
     g.id = box->id;
     g.id_dst = box->id_alt;
-    if (vst && vst != st) {  // velocities and ids on vst; the caller joins box->evvel before using them
+    if (vst && vst != st) {
+        // the positions first, alone (everything after the sort waits for them), then the
+        // velocities and ids on vst beside what follows; the caller joins box->evvel before using them
+        hipLaunchKernelGGL(gather_pos_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, g, box->perm, np);
+        HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(box->evsort, st));
         HIPCHK(ctx, hipStreamWaitEvent(vst, box->evsort, 0));
         hipLaunchKernelGGL(gather_vel_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, vst, g, box->perm, np, drag);
         HIPCHK(ctx, hipGetLastError());
         HIPCHK(ctx, hipEventRecord(box->evvel, vst));
-        hipLaunchKernelGGL(gather_pos_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, g, box->perm, np);
     } else {
         hipLaunchKernelGGL(gather_all_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, st, g, box->perm, np, drag);
     }
