@@ -429,6 +429,8 @@ int lagpm_destroy(lagpm_ctx* ctx) {
     dfree(ctx->walk_ticket);
     dfree(ctx->walk_boxes);
     dfree(ctx->walk_prefix);
+    if (ctx->readback) (void)hipHostFree(ctx->readback);
+    if (ctx->readback1) (void)hipHostFree(ctx->readback1);
     for (int i = 0; i < 4; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     for (hipStream_t s : ctx->pool) (void)hipStreamDestroy(s);
@@ -1387,12 +1389,30 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
     // completion: sizes and counters of every box; a box whose node arrays overflowed got
     // neither tree kick nor drift, so it is rebuilt with larger arrays and finished here
     int first_err = rc;
+    // every box's sizes and counters in one wait: the copies follow each box's work on its stream
+    tree_readback* rb = nullptr;
+    if (first_err == LAGPM_OK) {
+        if (ctx->readback_count < (size_t)nbox) {
+            if (ctx->readback) (void)hipHostFree(ctx->readback);
+            ctx->readback = nullptr;
+            ctx->readback_count = 0;
+            const hipError_t e = hipHostMalloc(&ctx->readback, sizeof(tree_readback) * (size_t)nbox, hipHostMallocDefault);
+            if (e != hipSuccess) first_err = lagpm_fail(ctx, LAGPM_E_HIP, std::string("pinned readback: ") + hipGetErrorString(e));
+            else ctx->readback_count = (size_t)nbox;
+        }
+        rb = static_cast<tree_readback*>(ctx->readback);
+        for (int i = 0; i < nbox && first_err == LAGPM_OK; i++)
+            if (boxes[i]->np) first_err = lagpm_tree_readback_enqueue(ctx, boxes[i], rb + i);
+        for (size_t q = 0; q < ctx->pool.size() && first_err == LAGPM_OK; q++)
+            if (hipStreamSynchronize(ctx->pool[q]) != hipSuccess) first_err = LAGPM_E_HIP;
+        if (first_err == LAGPM_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) first_err = LAGPM_E_HIP;
+    }
     for (int i = 0; i < nbox; i++) {
         lagpm_box* b = boxes[i];
         if (!b->np) continue;
         if (first_err == LAGPM_OK) {
             bool retry = false;
-            int r = lagpm_tree_finish(ctx, b, &retry);
+            int r = lagpm_tree_finish_with(ctx, b, rb[i], &retry);
             // redo (at most twice: a depth redo may still need larger node arrays)
             for (int attempt = 0; retry && attempt < 2; attempt++) {
                 retry = false;

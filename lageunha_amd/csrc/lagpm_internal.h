@@ -73,6 +73,11 @@ struct lagpm_ctx {
     double* pk_scratch = nullptr;  // this device's static power-spectrum scratch (device address)
     // Green factor tables, (n, efold, deconvolve) -> (n/2+1)^3 doubles (pm.hip)
     std::map<std::tuple<int, double, int>, double*> green_tabs;
+    // pinned host buffers for the end-of-step readbacks: one record per box of a batch, and one
+    // for a lone box's finish (so a redo inside a batch does not overwrite the batch's records)
+    void* readback = nullptr;
+    size_t readback_count = 0;
+    void* readback1 = nullptr;
 };
 
 // Tunables of the tree correction (chosen per box; the autotuner may override).
@@ -99,6 +104,12 @@ enum {
     DSZ_LOFF = DSZ_LVL + LAGPM_KEY_BITS + 1,
     DSZ_LCUR = DSZ_LOFF + LAGPM_KEY_BITS + 1,
     DSZ_COUNT = DSZ_LCUR + LAGPM_KEY_BITS + 1
+};
+
+// A box's tree sizes and interaction counters, read back at the end of a step.
+struct tree_readback {
+    uint32_t d[DSZ_COUNT];
+    unsigned long long c[4];
 };
 
 struct lagpm_box {
@@ -267,6 +278,10 @@ int lagpm_tree_walk_sample(lagpm_ctx* ctx, lagpm_box* box, const int64_t* ids, s
 // Reads the device sizes and counters after the walk (synchronises box->st); grows the node
 // arrays and returns LAGPM_E_OVERFLOW with *retry set when they were too small.
 int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry);
+// the two halves of lagpm_tree_finish: queue the copy (box->st, no wait), then, after the caller
+// has waited for the stream, act on it
+int lagpm_tree_readback_enqueue(lagpm_ctx* ctx, lagpm_box* box, tree_readback* dst);
+int lagpm_tree_finish_with(lagpm_ctx* ctx, lagpm_box* box, const tree_readback& r, bool* retry);
 int lagpm_tree_alloc(lagpm_ctx* ctx, lagpm_box* box, size_t node_cap);
 double lagpm_tree_scale(const lagpm_box* box);  // FP32 tree coordinate scale (sqrt(lam) or 1)
 size_t lagpm_tree_node_cap_guess(size_t np, int mcell);

@@ -33,7 +33,10 @@ namespace {
 #ifndef LAGPM_DEP_BLOCKS_PER_CU
 #define LAGPM_DEP_BLOCKS_PER_CU 2
 #endif
-constexpr int DEP_THREADS = 256;
+#ifndef LAGPM_DEP_THREADS
+#define LAGPM_DEP_THREADS 256
+#endif
+constexpr int DEP_THREADS = LAGPM_DEP_THREADS;
 constexpr int DEP_PPT = LAGPM_DEP_PPT;  // particles per thread and chunk
 constexpr int DEP_BLOCKS_PER_CU = LAGPM_DEP_BLOCKS_PER_CU;
 constexpr int DEP_CHUNK = DEP_THREADS * DEP_PPT;

@@ -1647,12 +1647,25 @@ int lagpm_tree_walk_f64(lagpm_ctx* ctx, lagpm_box* box, double fact, const walk_
     return LAGPM_OK;
 }
 
+int lagpm_tree_readback_enqueue(lagpm_ctx* ctx, lagpm_box* box, tree_readback* dst) {
+    HIPCHK(ctx, hipMemcpyAsync(dst->d, box->dsz, sizeof(dst->d), hipMemcpyDeviceToHost, box->st));
+    HIPCHK(ctx, hipMemcpyAsync(dst->c, box->counters, sizeof(dst->c), hipMemcpyDeviceToHost, box->st));
+    return LAGPM_OK;
+}
+
 int lagpm_tree_finish(lagpm_ctx* ctx, lagpm_box* box, bool* retry) {
     *retry = false;
-    uint32_t d[DSZ_COUNT];
-    unsigned long long c[4];
-    RCHK(lagpm_d2h(ctx, d, box->dsz, sizeof(d), box->st));
-    RCHK(lagpm_d2h(ctx, c, box->counters, sizeof(c), box->st));
+    if (!ctx->readback1) HIPCHK(ctx, hipHostMalloc(&ctx->readback1, sizeof(tree_readback), hipHostMallocDefault));
+    tree_readback* r = static_cast<tree_readback*>(ctx->readback1);
+    RCHK(lagpm_tree_readback_enqueue(ctx, box, r));
+    HIPCHK(ctx, hipStreamSynchronize(box->st));
+    return lagpm_tree_finish_with(ctx, box, *r, retry);
+}
+
+int lagpm_tree_finish_with(lagpm_ctx* ctx, lagpm_box* box, const tree_readback& r, bool* retry) {
+    *retry = false;
+    const uint32_t* d = r.d;
+    const unsigned long long* c = r.c;
     box->stats.n_pp = c[0];
     box->stats.n_pc = c[1];
     box->stats.n_pp_within = c[3];
