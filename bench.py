@@ -304,7 +304,8 @@ def ensemble(args, ctx, lagpm, ics):
            "sizes": sorted(n for n, _ in specs), "identical_plain_vs_batched": bool(same),
            "tree_tuning": {str(n): {"nleaf": t[0], "group": t[1], "group_mode": MODE_NAMES[t[2]]}
                            for n, t in sorted(tuned.items())},
-           "batch_pipeline": {"stages": 3, "reserve_cu": 32, "note": "library default (lagpm_set_batch_pipeline)"}}
+           "batch_pipeline": {"stages": 5, "reserve_cu": 32, "share_growth": 1.5,
+                              "note": "library default (lagpm_set_batch_pipeline)"}}
     for k in ("plain", "batched"):
         L = np.array(lat[k]) * 1e3
         out[k] = {"particle_steps_per_s": total * steps / tt[k], "seconds": tt[k],

@@ -336,8 +336,9 @@ int lagpm_set_step_graphs(lagpm_ctx* ctx, int enable);
  * (lagpm_tree_stats.n_pp_within): the useful fraction of the interaction lists. Costs a compare
  * and an add per pair; off by default. No counterpart in the reference. */
 int lagpm_set_walk_census(lagpm_ctx* ctx, int enable);
-/* Batch steps of FP32 boxes sharing one force law run as `stages` groups of about equal particle
- * count (default 3, reserve_cu 32; used when the batch has at least 2 * stages boxes): a group's
+/* Batch steps of FP32 boxes sharing one force law run as `stages` groups of consecutive boxes
+ * (default 5, reserve_cu 32; at most one group per two boxes), whose particle shares grow by 1.5
+ * from one group to the next: the first group's front, which nothing overlaps, is short. A group's
  * fused walk leaves reserve_cu CUs free and runs beside the next group's front halves (PM and
  * tree build). stages = 1 queues every front, then one walk. Results are identical either way.
  * No counterpart in the reference. */

@@ -1115,6 +1115,10 @@ int lagpm_drift(lagpm_ctx* ctx, lagpm_box* box, double pfact) {
 
 namespace {
 
+#ifndef LAGPM_BATCH_GROWTH
+#define LAGPM_BATCH_GROWTH 1.5
+#endif
+
 int ensure_pool(lagpm_ctx* ctx, size_t want) {
     while (ctx->pool.size() < want) {
         hipStream_t s = nullptr;
@@ -1280,14 +1284,15 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
     std::vector<double> fact((size_t)nbox);
     for (int i = 0; i < nbox; i++) fact[(size_t)i] = ef[(size_t)i].fact2;
     // Pipelined batch: all boxes FP32 with one force table, enough of them for the stages. The
-    // boxes are cut into `stages` consecutive groups of about equal particle count; group g's walk
+    // boxes are cut into `stages` consecutive groups of growing particle shares; group g's walk
     // (one fused launch, leaving ctx->stage_reserve_cu CUs free) runs while the fronts of group
     // g+1 are on the pool streams, whose small kernels are latency bound and fill those CUs. Every
     // drift is queued after all fronts, so no front waits behind a walk on its pool stream.
     int stages = std::max(1, ctx->batch_stages);
     for (lagpm_box* b : live)
         if (b->p.precision != LAGPM_PREC_FP32 || b->tab != live[0]->tab) stages = 1;
-    if (live.size() < (size_t)(2 * stages)) stages = 1;
+    stages = std::min<int>(stages, (int)(live.size() / 2));  // at least two boxes per group
+    if (stages < 2) stages = 1;
     int rc = LAGPM_OK;
     if (stages > 1) {
         size_t ptot = 0;
@@ -1309,8 +1314,13 @@ int batch_step(lagpm_ctx* ctx, lagpm_box* const* boxes, int nbox, const double* 
             acc += b->np;
             li++;
             const bool last = li == live.size();
-            // close the group at its share of the particles (the last group takes the rest)
-            if (last || (g < stages - 1 && acc * (size_t)stages >= ptot * (size_t)(g + 1))) {
+            // close the group at its share of the particles (the last group takes the rest):
+            // shares grow by LAGPM_BATCH_GROWTH per group, so the first group's front, which
+            // nothing overlaps, is short, and each later group's front fits beside the previous
+            // group's (longer) walk
+            const double gr = LAGPM_BATCH_GROWTH;
+            const double upto = gr == 1.0 ? (double)(g + 1) / stages : (std::pow(gr, g + 1) - 1.0) / (std::pow(gr, stages) - 1.0);
+            if (last || (g < stages - 1 && (double)acc >= upto * (double)ptot)) {
                 rc = queue_fronts(ctx, grp, gef);
                 for (lagpm_box* c : grp)
                     if (hipStreamWaitEvent(ctx->stream, c->ev[13], 0) != hipSuccess) rc = LAGPM_E_HIP;

@@ -67,7 +67,7 @@ struct lagpm_ctx {
     size_t walk_box_cap = 0;
     bool use_graphs = true;                  // capture box step fronts in hipGraphs
     bool walk_census = false;                // FP32 walks count the pairs within the cutoff
-    int batch_stages = 3;                    // groups of a pipelined batch step (1: no pipeline)
+    int batch_stages = 5;                    // groups of a pipelined batch step (1: no pipeline)
     int stage_reserve_cu = 32;               // CUs a non-final group's walk leaves to the next fronts
     hipEvent_t ev[4] = {};
     double* pk_scratch = nullptr;  // this device's static power-spectrum scratch (device address)
