@@ -363,6 +363,13 @@ def serving(args, ctx, lagpm, ics):
         off = np.random.default_rng(r.seed).integers(0, r.n, 3)
         return np.mod(base[r.n] + off, r.n)
 
+    # one tree tuning for every request's box, autotuned on a 128^3 box (the batched walk is one
+    # fused launch over all active boxes; see the ensemble)
+    tuning = None
+    if not args.no_autotune:
+        with lagpm.Box(ctx, params_for(Request(-1, 128, 1, 0.0)), 128 ** 3) as tb:
+            tb.upload_pos(ics.perturbed_lattice(128, rms=0.3, seed=128))
+            tuning = tb.autotune()
     # warm the FFT plans of every size (not timed)
     for n in base:
         with lagpm.Box(ctx, params_for(Request(-1, n, 1, 0.0)), n ** 3) as b:
@@ -371,7 +378,7 @@ def serving(args, ctx, lagpm, ics):
     start = time.perf_counter()
     clock = lambda: time.perf_counter() - start  # noqa: E731
     srv = Server(ctx, params_for, positions_for, hbm_budget=args.serve_budget_gb * 1e9,
-                 max_active_particles=args.serve_max_particles or None, clock=clock)
+                 max_active_particles=args.serve_max_particles or None, clock=clock, tuning=tuning)
     for r in reqs:
         srv.submit(r)
     last = 0.0
@@ -388,6 +395,7 @@ def serving(args, ctx, lagpm, ics):
     st.update({"wall_seconds": wall, "particle_steps_per_s": st["particle_steps"] / wall,
                "arrival_rate_per_s": args.serve_rate, "window_s": args.serve_window,
                "hbm_budget_gb": args.serve_budget_gb, "max_active_particles": args.serve_max_particles or None,
+               "tree_tuning": list(tuning) if tuning else None,
                "latency_unit": "one box-step = the lagpm_batch_step iteration that carried it"})
     return st
 

@@ -96,7 +96,8 @@ class Server:
     def __init__(self, ctx: lagpm.Context, params_for: Callable[[Request], lagpm.Params],
                  positions_for: Callable[[Request], np.ndarray], hbm_budget: float,
                  max_active_particles: Optional[int] = None, keep_results: bool = False,
-                 clock: Callable[[], float] = time.perf_counter, pool: bool = True):
+                 clock: Callable[[], float] = time.perf_counter, pool: bool = True,
+                 tuning: Optional[tuple] = None):
         self.ctx = ctx
         self.params_for = params_for
         self.positions_for = positions_for
@@ -105,6 +106,7 @@ class Server:
         self.keep_results = keep_results
         self.clock = clock
         self.pool = BoxPool() if pool else None
+        self.tuning = tuning  # (nleaf, group[, group mode]) for every new box; None: the library's
         self.queue: Deque[Request] = deque()
         self.active: Dict[int, tuple] = {}   # rid -> (request, box, class)
         self.done: List[Request] = []
@@ -150,6 +152,8 @@ class Server:
                 if self.bytes_held + need > self.hbm_budget:
                     return
                 box = lagpm.Box(self.ctx, p, npart)
+                if self.tuning is not None:
+                    box.set_tuning(*self.tuning)
                 self.boxes_created += 1
             else:
                 box.set_cosmology(p.amax, p.omep, p.omeplam)
