@@ -245,22 +245,19 @@ def ensemble(args, ctx, lagpm, ics):
             bx.upload_pos(pos)
             sets[k].append(bx)
         total += len(pos)
-    # One tree tuning for all boxes, autotuned on the largest box (on a scratch box with its
-    # particles) and applied to every box of both sets. The batched walk is one fused launch over all
-    # boxes, which the largest boxes dominate: tuning each size alone picks small groups for small
-    # boxes (good when a small box has the GPU to itself), and the fused walk then took 2-3 % longer
-    # (40.0-40.2 against 38.9-39.0 ms per batch step); the plain per-box rounds did not change.
+    # autotuned tree tuning per box size (on a scratch box with the first box of that size's
+    # particles), applied to the boxes of that size in both sets
     tuned = {}
     if not args.no_autotune:
-        big = max(range(len(specs)), key=lambda b: specs[b][0])
-        nb = specs[big][0]
-        with lagpm.Box(ctx, sets["plain"][big].params, sets["plain"][big].np) as tb:
-            tb.upload_pos(ics.perturbed_lattice(nb, rms=0.3, seed=100 + big))
-            t = tb.autotune()
-        tuned = {n: t for n, _ in specs}
+        for b, (n, om) in enumerate(specs):
+            if n in tuned:
+                continue
+            with lagpm.Box(ctx, sets["plain"][b].params, sets["plain"][b].np) as tb:
+                tb.upload_pos(ics.perturbed_lattice(n, rms=0.3, seed=100 + b))
+                tuned[n] = tb.autotune()
         for k in sets:
-            for bx in sets[k]:
-                bx.set_tuning(*t)
+            for bx, (n, _) in zip(sets[k], specs):
+                bx.set_tuning(*tuned[n])
     log(f"[bench] ensemble: {len(specs)} boxes, {total} particles, sizes {sorted(set(n for n, _ in specs))} "
         f"(setup {time.perf_counter() - t0:.1f} s; tuning {tuned})")
     a0, da, steps = 1.0, 0.1, args.ens_steps
