@@ -757,6 +757,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
     __shared__ int s_pcidx[WALK_WAVES][PC_CAP];
     __shared__ float4 s_pp[WALK_WAVES][64];
     __shared__ float4 s_pc[WALK_WAVES][3][64];
+    __shared__ float4 s_shift[WALK_WAVES][27];  // the group's image shift of each neighbour cell
     constexpr int NBIN = LOGT ? LAGPM_LOG_BINS : 4096;
     for (int i = threadIdx.x; i < NBIN; i += WALK_THREADS) s_tab0[i] = tab0[i];
     __syncthreads();
@@ -846,10 +847,20 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         }
         int pp_n = 0, pc_n = 0;
 
+        // The periodic image shift of each of the 27 neighbour cells, made once per group (lane nb
+        // makes entry nb) and then read from LDS: a lookup, where computing it (three compares and
+        // selects per axis and the base-3 digits of nb) cost some 25 operations per use, and every
+        // staged particle, staged node and popped node uses one.
+        if (lane < 27)
+            s_shift[wid][lane] = make_float4(image_shift(cx, lane % 3 - 1, A.mcell, A.box),
+                                             image_shift(cy, (lane / 3) % 3 - 1, A.mcell, A.box),
+                                             image_shift(cz, lane / 9 - 1, A.mcell, A.box), 0.f);
+        wave_sync();
         auto shift_of = [&](int nb, float& sx, float& sy, float& sz) {
-            sx = image_shift(cx, nb % 3 - 1, A.mcell, A.box);
-            sy = image_shift(cy, (nb / 3) % 3 - 1, A.mcell, A.box);
-            sz = image_shift(cz, nb / 9 - 1, A.mcell, A.box);
+            const float4 sh = s_shift[wid][nb];
+            sx = sh.x;
+            sy = sh.y;
+            sz = sh.z;
         };
         // Source chunks are staged in LDS as float4 (x, y, z, m); entries past m are far-away
         // dummies of zero mass, so every lane runs the same unrolled trip count and the cutoff
