@@ -814,20 +814,52 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         const int2 gr = A.groups[g - __builtin_amdgcn_readfirstlane((int)gprefix[blo])];
         const int cnt = __builtin_amdgcn_readfirstlane(gr.y);
         const int gbeg = __builtin_amdgcn_readfirstlane(gr.x);
-        // Lane replication: a group of cnt <= 32 targets gets R = 64/Tw lanes per target; the
-        // R lanes of a target split each source chunk and are summed at the end.
+        // Lane layout. Lane (tgt, rep) holds M targets, tgt + i Tw for i < M, and takes every R-th
+        // source of a chunk from rep on; the R partial sums of a target are added at the end.
+        // M = 1: R = 64/Tw lanes per target, Tw the smallest power of two >= cnt. The lanes past
+        // cnt idle, which for 33-48 targets is a third of the wave or more; M = 3 there puts
+        // three targets in each of ceil(cnt/3) <= 16 slots with R = 4, so that 48 targets fill the
+        // wave and each staged source or node is read once for three targets. Three targets per
+        // lane are taken when they give at least 4/3 of the single-target lane use
+        // (cnt in 33-48, 17-24, 9-12, 5-6).
         int R = 1;
         while (cnt * 2 * R <= 64) R <<= 1;
+        int M = 1;
+        {
+            const int s3 = (cnt + 2) / 3;
+            int R3 = 1;
+            while (s3 * 2 * R3 <= 64) R3 <<= 1;
+            if (R3 >= 4 * R) {
+                M = 3;
+                R = R3;
+            }
+        }
         const int Tw = 64 / R;
         const int tgt = lane & (Tw - 1), rep = lane / Tw;
-        const bool valid = tgt < cnt;
-        const int me = gbeg + (valid ? tgt : 0);
-        const float4 xt = A.pp[me];
-        const float lox = wave_minf(xt.x), loy = wave_minf(xt.y), loz = wave_minf(xt.z);
-        const float hix = wave_maxf(xt.x), hiy = wave_maxf(xt.y), hiz = wave_maxf(xt.z);
         const uint64_t cellkey = A.key[gbeg] >> A.cell_shift;
         const int cx = (int)compact3(cellkey), cy = (int)compact3(cellkey >> 1), cz = (int)compact3(cellkey >> 2);
-        float ax = 0.f, ay = 0.f, az = 0.f;
+        // the rest of the group, compiled once per targets-per-lane count so that the single-target
+        // path carries no registers for the other two
+        auto group_body = [&](auto mc) {
+        constexpr int M = decltype(mc)::value;
+        bool valid[M];
+        int me[M];
+        float4 xt[M];
+        float ax[M], ay[M], az[M];
+        float lx = 0.f, ly = 0.f, lz = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;
+#pragma unroll
+        for (int i = 0; i < M; i++) {
+            const int t = tgt + i * Tw;
+            valid[i] = t < cnt;
+            me[i] = gbeg + (valid[i] ? t : 0);  // an idle slot repeats target 0: inside the group box
+            xt[i] = A.pp[me[i]];
+            ax[i] = ay[i] = az[i] = 0.f;
+            lx = i ? fminf(lx, xt[i].x) : xt[i].x; hx = i ? fmaxf(hx, xt[i].x) : xt[i].x;
+            ly = i ? fminf(ly, xt[i].y) : xt[i].y; hy = i ? fmaxf(hy, xt[i].y) : xt[i].y;
+            lz = i ? fminf(lz, xt[i].z) : xt[i].z; hz = i ? fmaxf(hz, xt[i].z) : xt[i].z;
+        }
+        const float lox = wave_minf(lx), loy = wave_minf(ly), loz = wave_minf(lz);
+        const float hix = wave_maxf(hx), hiy = wave_maxf(hy), hiz = wave_maxf(hz);
         unsigned long long npp = 0, npc = 0;
         unsigned nin = 0;  // CENSUS: this lane's pairs within the cutoff
         int sp = 0;
@@ -884,58 +916,70 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         // are taken four at a time: their LDS loads, then their table loads, are issued together
         // so one wave's LDS latency overlaps its own arithmetic.
         auto pp_one = [&](float4 s) {
-            float dx = xt.x - s.x, dy = xt.y - s.y, dz = xt.z - s.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            if constexpr (CENSUS) nin += (valid && r2 <= A.rc2) ? 1u : 0u;
-            float t;
-            const int nt = table_bin<LOGT>(r2, t, lam);
-            float2 tb = s_tab0[nt];
-            float f = tb.x + tb.y * t;  // particle masses are 1 here: the box mass is in fact
-            if constexpr (!LOGT) f = (r2 <= A.rc2) ? f : 0.f;
-            ax += dx * f; ay += dy * f; az += dz * f;
+#pragma unroll
+            for (int i = 0; i < M; i++) {
+                float dx = xt[i].x - s.x, dy = xt[i].y - s.y, dz = xt[i].z - s.z;
+                float r2 = dx * dx + dy * dy + dz * dz;
+                if constexpr (CENSUS) nin += (valid[i] && r2 <= A.rc2) ? 1u : 0u;
+                float t;
+                const int nt = table_bin<LOGT>(r2, t, lam);
+                float2 tb = s_tab0[nt];
+                float f = tb.x + tb.y * t;  // particle masses are 1 here: the box mass is in fact
+                if constexpr (!LOGT) f = (r2 <= A.rc2) ? f : 0.f;
+                ax[i] += dx * f; ay[i] += dy * f; az[i] += dz * f;
+            }
         };
         auto pp_chunk_r = [&](auto rc) {
             constexpr int RR = decltype(rc)::value;
             constexpr int iters = 64 / RR;
+            constexpr int U = M == 1 ? 4 : 2;  // sources per round: U M pairs in flight
             const float4* src = cpp + rep;
-            if constexpr (iters >= 4) {
+            if constexpr (iters >= U) {
 #pragma unroll 2
-                for (int k = 0; k < iters; k += 4) {
-                    float4 s[4];
-                    float dx[4], dy[4], dz[4], r2[4];
+                for (int k = 0; k < iters; k += U) {
+                    float4 s[U];
+                    float dx[U][M], dy[U][M], dz[U][M], r2[U][M];
                     bool any = false;
 #pragma unroll
-                    for (int u = 0; u < 4; u++) s[u] = src[(k + u) * RR];
+                    for (int u = 0; u < U; u++) s[u] = src[(k + u) * RR];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        dx[u] = xt.x - s[u].x; dy[u] = xt.y - s[u].y; dz[u] = xt.z - s[u].z;
-                        r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-                        if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
-                        if constexpr (CENSUS) nin += (valid && r2[u] <= A.rc2) ? 1u : 0u;
-                    }
+                    for (int u = 0; u < U; u++)
+#pragma unroll
+                        for (int i = 0; i < M; i++) {
+                            dx[u][i] = xt[i].x - s[u].x; dy[u][i] = xt[i].y - s[u].y; dz[u][i] = xt[i].z - s[u].z;
+                            r2[u][i] = dx[u][i] * dx[u][i] + dy[u][i] * dy[u][i] + dz[u][i] * dz[u][i];
+                            if constexpr (!LOGT) any = any || (r2[u][i] <= A.rc2);
+                            if constexpr (CENSUS) nin += (valid[i] && r2[u][i] <= A.rc2) ? 1u : 0u;
+                        }
                     if constexpr (!LOGT)
-                        if (!__any(any)) continue;  // wave-uniform skip of four far sources
-                    float2 tb[4];
-                    float t[4];
+                        if (!__any(any)) continue;  // wave-uniform skip of U far sources
+                    float2 tb[U][M];
+                    float t[U][M];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) tb[u] = s_tab0[table_bin<LOGT>(r2[u], t[u], lam)];
+                    for (int u = 0; u < U; u++)
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        float f = tb[u].x + tb[u].y * t[u];
-                        if constexpr (!LOGT) f = (r2[u] <= A.rc2) ? f : 0.f;
-                        ax += dx[u] * f; ay += dy[u] * f; az += dz[u] * f;
-                    }
+                        for (int i = 0; i < M; i++) tb[u][i] = s_tab0[table_bin<LOGT>(r2[u][i], t[u][i], lam)];
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+#pragma unroll
+                        for (int i = 0; i < M; i++) {
+                            float f = tb[u][i].x + tb[u][i].y * t[u][i];
+                            if constexpr (!LOGT) f = (r2[u][i] <= A.rc2) ? f : 0.f;
+                            ax[i] += dx[u][i] * f; ay[i] += dy[u][i] * f; az[i] += dz[u][i] * f;
+                        }
                 }
             } else {
 #pragma unroll
                 for (int k = 0; k < iters; k++) pp_one(src[k * RR]);
             }
         };
-        // groups of more than 8 targets (R <= 4) take the unrolled loops; smaller ones a plain loop
+        // the common replications take the unrolled loops (one target per lane: groups of more
+        // than 8 targets, R <= 4; three: R = 4 or 8); the rest a plain loop
         auto pp_chunk = [&]() {
-            if (R == 1) pp_chunk_r(ic<1>{});
-            else if (R == 2) pp_chunk_r(ic<2>{});
+            if (R == 1 && M == 1) pp_chunk_r(ic<1>{});
+            else if (R == 2 && M == 1) pp_chunk_r(ic<2>{});
             else if (R == 4) pp_chunk_r(ic<4>{});
+            else if (R == 8 && M == 3) pp_chunk_r(ic<8>{});
             else
                 for (int k = 0; k < 64 / R; k++) pp_one(cpp[rep + k * R]);
         };
@@ -985,7 +1029,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         // One node interaction: monopole plus quadrupole, Q d computed once and reused for
         // d.Q.d (the quadratic form) and for the 2 f2 Q d term.
         auto pc_eval = [&](float4 c, float4 qa, float4 qb, float2 t0, float4 t12, float t, float dx, float dy, float dz,
-                           bool in) {
+                           bool in, float& ax, float& ay, float& az) {
             float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
             float qx1 = qa.x * dx + qa.w * dy + qb.x * dz;
             float qx2 = qa.w * dx + qa.y * dy + qb.y * dz;
@@ -1001,53 +1045,66 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             ay = fmaf(qx2, two, fmaf(dy, tmp, ay));
             az = fmaf(qx3, two, fmaf(dz, tmp, az));
         };
+        // one node against this lane's M targets
+        auto pc_node = [&](const float4& c, const float4& qa, const float4& qb) {
+#pragma unroll
+            for (int i = 0; i < M; i++) {
+                const float dx = xt[i].x - c.x, dy = xt[i].y - c.y, dz = xt[i].z - c.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                float t;
+                const int nt = table_bin<LOGT>(r2, t, lam);
+                pc_eval(c, qa, qb, s_tab0[nt], tab12[nt], t, dx, dy, dz, r2 <= A.rc2, ax[i], ay[i], az[i]);
+            }
+        };
         auto pc_chunk_r = [&](auto rc) {
             constexpr int RR = decltype(rc)::value;
             constexpr int iters = 64 / RR;
+            constexpr int U = M == 1 ? 2 : 1;  // nodes per round: U M interactions in flight
             const float4* n0 = cpc0 + rep;
             const float4* n1 = cpc1 + rep;
             const float4* n2 = cpc2 + rep;
             if constexpr (iters >= 2) {
-                // nodes two at a time, loads first, as for the particles
+                // the nodes' loads first, then their table loads, as for the particles
 #pragma unroll 2
-                for (int k = 0; k < iters; k += 2) {
-                    float4 c[2], qa[2], qb[2];
-                    float dx[2], dy[2], dz[2], r2[2];
+                for (int k = 0; k < iters; k += U) {
+                    float4 c[U], qa[U], qb[U];
+                    float dx[U][M], dy[U][M], dz[U][M], r2[U][M];
                     bool any = false;
 #pragma unroll
-                    for (int u = 0; u < 2; u++) {
+                    for (int u = 0; u < U; u++) {
                         const int j = (k + u) * RR;
                         c[u] = n0[j]; qa[u] = n1[j]; qb[u] = n2[j];
                     }
 #pragma unroll
-                    for (int u = 0; u < 2; u++) {
-                        dx[u] = xt.x - c[u].x; dy[u] = xt.y - c[u].y; dz[u] = xt.z - c[u].z;
-                        r2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-                        if constexpr (!LOGT) any = any || (r2[u] <= A.rc2);
-                    }
+                    for (int u = 0; u < U; u++)
+#pragma unroll
+                        for (int i = 0; i < M; i++) {
+                            dx[u][i] = xt[i].x - c[u].x; dy[u][i] = xt[i].y - c[u].y; dz[u][i] = xt[i].z - c[u].z;
+                            r2[u][i] = dx[u][i] * dx[u][i] + dy[u][i] * dy[u][i] + dz[u][i] * dz[u][i];
+                            if constexpr (!LOGT) any = any || (r2[u][i] <= A.rc2);
+                        }
                     if constexpr (!LOGT)
-                        if (!__any(any)) continue;  // wave-uniform skip of two far nodes
-                    float t[2];
-                    int nt[2];
-                    float2 t0[2];
-                    float4 t12[2];
+                        if (!__any(any)) continue;  // wave-uniform skip of far nodes
+                    float t[U][M];
+                    float2 t0[U][M];
+                    float4 t12[U][M];
 #pragma unroll
-                    for (int u = 0; u < 2; u++) {
-                        nt[u] = table_bin<LOGT>(r2[u], t[u], lam);
-                        t0[u] = s_tab0[nt[u]];
-                        t12[u] = tab12[nt[u]];
-                    }
+                    for (int u = 0; u < U; u++)
 #pragma unroll
-                    for (int u = 0; u < 2; u++)
-                        pc_eval(c[u], qa[u], qb[u], t0[u], t12[u], t[u], dx[u], dy[u], dz[u], r2[u] <= A.rc2);
+                        for (int i = 0; i < M; i++) {
+                            const int nt = table_bin<LOGT>(r2[u][i], t[u][i], lam);
+                            t0[u][i] = s_tab0[nt];
+                            t12[u][i] = tab12[nt];
+                        }
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+#pragma unroll
+                        for (int i = 0; i < M; i++)
+                            pc_eval(c[u], qa[u], qb[u], t0[u][i], t12[u][i], t[u][i], dx[u][i], dy[u][i], dz[u][i],
+                                    r2[u][i] <= A.rc2, ax[i], ay[i], az[i]);
                 }
             } else {
-                const float4 c = n0[0], qa = n1[0], qb = n2[0];
-                float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
-                float r2 = dx * dx + dy * dy + dz * dz;
-                float t;
-                const int nt = table_bin<LOGT>(r2, t, lam);
-                pc_eval(c, qa, qb, s_tab0[nt], tab12[nt], t, dx, dy, dz, r2 <= A.rc2);
+                pc_node(n0[0], n1[0], n2[0]);
             }
         };
         auto flush_pc = [&](int start, int m) {
@@ -1066,18 +1123,14 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 cpc2[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
             wave_sync();
-            if (R == 1) pc_chunk_r(ic<1>{});
-            else if (R == 2) pc_chunk_r(ic<2>{});
+            if (R == 1 && M == 1) pc_chunk_r(ic<1>{});
+            else if (R == 2 && M == 1) pc_chunk_r(ic<2>{});
             else if (R == 4) pc_chunk_r(ic<4>{});
+            else if (R == 8 && M == 3) pc_chunk_r(ic<8>{});
             else
                 for (int k = 0; k < 64 / R; k++) {
                     const int j = rep + k * R;
-                    const float4 c = cpc0[j], qa = cpc1[j], qb = cpc2[j];
-                    const float dx = xt.x - c.x, dy = xt.y - c.y, dz = xt.z - c.z;
-                    const float r2 = dx * dx + dy * dy + dz * dz;
-                    float t;
-                    const int nt = table_bin<LOGT>(r2, t, lam);
-                    pc_eval(c, qa, qb, s_tab0[nt], tab12[nt], t, dx, dy, dz, r2 <= A.rc2);
+                    pc_node(cpc0[j], cpc1[j], cpc2[j]);
                 }
             npc += (unsigned long long)m;
             wave_sync();
@@ -1199,9 +1252,12 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         }
         // sum the R partial accumulators of each target (lanes tgt, tgt + Tw, ...)
         for (int o = Tw; o < 64; o <<= 1) {
-            ax += __shfl_xor(ax, o, 64);
-            ay += __shfl_xor(ay, o, 64);
-            az += __shfl_xor(az, o, 64);
+#pragma unroll
+            for (int i = 0; i < M; i++) {
+                ax[i] += __shfl_xor(ax[i], o, 64);
+                ay[i] += __shfl_xor(ay[i], o, 64);
+                az[i] += __shfl_xor(az[i], o, 64);
+            }
         }
         // The kick is written to its own arrays rather than added to the velocities, so the walk
         // can run while the PM kick of the same step updates them; the drift (or, for a tree
@@ -1210,10 +1266,13 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             atomicAdd(&A.counters[2], 1ull);
             atomicOr(&A.dsz[DSZ_WALKOVF], 1u);  // the step's drift reads it and leaves the box unmoved
         }
-        if (valid && rep == 0) {
-            A.tvx[me] = overflow ? 0.0 : (double)(A.fact * ax);
-            A.tvy[me] = overflow ? 0.0 : (double)(A.fact * ay);
-            A.tvz[me] = overflow ? 0.0 : (double)(A.fact * az);
+#pragma unroll
+        for (int i = 0; i < M; i++) {
+            if (valid[i] && rep == 0) {
+                A.tvx[me[i]] = overflow ? 0.0 : (double)(A.fact * ax[i]);
+                A.tvy[me[i]] = overflow ? 0.0 : (double)(A.fact * ay[i]);
+                A.tvz[me[i]] = overflow ? 0.0 : (double)(A.fact * az[i]);
+            }
         }
         if (lane == 0) {
             atomicAdd(&A.counters[0], npp * (unsigned long long)cnt);
@@ -1223,6 +1282,9 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             for (int o = 32; o > 0; o >>= 1) nin += (unsigned)__shfl_xor((int)nin, o, 64);
             if (lane == 0) atomicAdd(&A.counters[3], (unsigned long long)nin);
         }
+        };
+        if (M == 3) group_body(ic<3>{});
+        else group_body(ic<1>{});
     }
 }
 
