@@ -281,7 +281,8 @@ int lagpm_tree_accel_sample(lagpm_ctx* ctx, lagpm_box* box, const int64_t* ids, 
 int lagpm_tree_nodes_get(lagpm_ctx* ctx, lagpm_box* box, size_t cap, size_t* nnodes, int32_t* info, double* com,
                          double* quad, uint32_t* order);
 /* Tree tuning of a box: nleaf (a node with more particles is split, 4..256) and the target
- * group size (8..64, one wavefront walks one group). Defaults 16 and 64. */
+ * group size (8..128; one wavefront walks one group, with 1 or 3 targets per lane). Defaults
+ * 16 and 64. */
 int lagpm_box_set_tuning(lagpm_box* box, int nleaf, int group);
 int lagpm_box_get_tuning(const lagpm_box* box, int* nleaf, int* group);
 /* How target groups are formed. TREE (default): a tree node of at most `group` particles whose

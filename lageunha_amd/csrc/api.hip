@@ -970,7 +970,7 @@ int lagpm_tree_accel_sample(lagpm_ctx* ctx, lagpm_box* box, const int64_t* ids, 
 }
 
 int lagpm_box_set_tuning(lagpm_box* box, int nleaf, int group) {
-    if (!box || nleaf < 4 || nleaf > 256 || group < 8 || group > 64) return LAGPM_E_INVAL;
+    if (!box || nleaf < 4 || nleaf > 256 || group < 8 || group > 128) return LAGPM_E_INVAL;
     if (box->tune.nleaf != nleaf || box->tune.group != group) {
         lagpm_box_graphs_release(box);
         box->depth = LAGPM_KEY_BITS;  // another leaf size: find the tree depth again
@@ -1007,9 +1007,9 @@ int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group) {
     if (box->np == 0) return LAGPM_OK;
     // (nleaf, group, group mode)
     static const int cand[][3] = {{16, 64, 0}, {32, 64, 0}, {8, 64, 0}, {16, 32, 0}, {32, 32, 0}, {8, 32, 0},
-                                  {16, 16, 0}, {32, 16, 0}, {8, 16, 0},
+                                  {16, 16, 0}, {32, 16, 0}, {8, 16, 0}, {32, 48, 0}, {16, 96, 0}, {32, 96, 0},
                                   {16, 64, 1}, {32, 64, 1}, {16, 32, 1}, {32, 32, 1}, {16, 16, 1},
-                                  {16, 64, 2}, {32, 64, 2}};
+                                  {16, 64, 2}, {32, 64, 2}, {32, 96, 2}};
     double best = 1e300;
     int bl = box->tune.nleaf, bg = box->tune.group, bm = lagpm_box_get_group_mode(box);
     for (const auto& c : cand) {

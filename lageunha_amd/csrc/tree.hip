@@ -14,7 +14,7 @@
 // octree nodes of level L (2^L per axis, width n/2^L >= cutoff). A node is the run of
 // particles sharing a key prefix; nodes are numbered in pre-order by (first particle, level),
 // which a per-particle count + exclusive scan produces without any level-by-level pass.
-// Target groups are subtrees of <= 64 particles; one wavefront walks the 27 neighbouring
+// Target groups are subtrees of <= 128 particles; one wavefront walks the 27 neighbouring
 // cell trees for its group with a group-conservative opening test, stages accepted nodes
 // and opened leaves in LDS, and evaluates the interactions lane-per-target.
 #include "lagpm_internal.h"
@@ -816,12 +816,13 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         const int gbeg = __builtin_amdgcn_readfirstlane(gr.x);
         // Lane layout. Lane (tgt, rep) holds M targets, tgt + i Tw for i < M, and takes every R-th
         // source of a chunk from rep on; the R partial sums of a target are added at the end.
-        // M = 1: R = 64/Tw lanes per target, Tw the smallest power of two >= cnt. The lanes past
-        // cnt idle, which for 33-48 targets is a third of the wave or more; M = 3 there puts
-        // three targets in each of ceil(cnt/3) <= 16 slots with R = 4, so that 48 targets fill the
-        // wave and each staged source or node is read once for three targets. Three targets per
-        // lane are taken when they give at least 4/3 of the single-target lane use
-        // (cnt in 33-48, 17-24, 9-12, 5-6).
+        // With M = 1, R = 64/Tw lanes per target, Tw the smallest power of two >= cnt, the lanes
+        // past cnt idle: for 33-48 targets a third of the wave or more. M = 3 targets per lane
+        // fill ceil(cnt/3) slots instead, and each staged source or node is read once for three
+        // targets. A group takes M = 3 where it uses at least 4/3 of the lanes M = 1 would
+        // (cnt in 33-48, 17-24, 9-12, 5-6), and for 65-192 targets, which one target per lane
+        // cannot hold. (Two and five targets per lane as well, for the remaining gaps, made the
+        // kernel's code four times larger and spilled registers: the walk got slower.)
         int R = 1;
         while (cnt * 2 * R <= 64) R <<= 1;
         int M = 1;
@@ -829,7 +830,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             const int s3 = (cnt + 2) / 3;
             int R3 = 1;
             while (s3 * 2 * R3 <= 64) R3 <<= 1;
-            if (R3 >= 4 * R) {
+            if (cnt > 64 || R3 >= 4 * R) {
                 M = 3;
                 R = R3;
             }
@@ -974,12 +975,12 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             }
         };
         // the common replications take the unrolled loops (one target per lane: groups of more
-        // than 8 targets, R <= 4; three: R = 4 or 8); the rest a plain loop
+        // than 8 targets, R <= 4; three: more than 12, R <= 8); the rest a plain loop
         auto pp_chunk = [&]() {
-            if (R == 1 && M == 1) pp_chunk_r(ic<1>{});
-            else if (R == 2 && M == 1) pp_chunk_r(ic<2>{});
+            if (R == 1) pp_chunk_r(ic<1>{});
+            else if (R == 2) pp_chunk_r(ic<2>{});
             else if (R == 4) pp_chunk_r(ic<4>{});
-            else if (R == 8 && M == 3) pp_chunk_r(ic<8>{});
+            else if (M == 3 && R == 8) pp_chunk_r(ic<8>{});
             else
                 for (int k = 0; k < 64 / R; k++) pp_one(cpp[rep + k * R]);
         };
@@ -1123,10 +1124,10 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 cpc2[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
             wave_sync();
-            if (R == 1 && M == 1) pc_chunk_r(ic<1>{});
-            else if (R == 2 && M == 1) pc_chunk_r(ic<2>{});
+            if (R == 1) pc_chunk_r(ic<1>{});
+            else if (R == 2) pc_chunk_r(ic<2>{});
             else if (R == 4) pc_chunk_r(ic<4>{});
-            else if (R == 8 && M == 3) pc_chunk_r(ic<8>{});
+            else if (M == 3 && R == 8) pc_chunk_r(ic<8>{});
             else
                 for (int k = 0; k < 64 / R; k++) {
                     const int j = rep + k * R;

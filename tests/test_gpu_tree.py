@@ -174,7 +174,7 @@ def test_autotune_keeps_velocities_and_a_valid_tuning(ctx):
     with lagpm.Box(ctx, p, len(pos)) as b:
         b.upload_pos(pos, vel)
         nl, g, mode = b.autotune()
-        assert (nl, g, mode) == b.tuning() and 4 <= nl <= 256 and 8 <= g <= 64
+        assert (nl, g, mode) == b.tuning() and 4 <= nl <= 256 and 8 <= g <= 128
         assert mode in (lagpm.GROUPS_TREE, lagpm.GROUPS_RUNS, lagpm.GROUPS_PACKED)
         assert np.array_equal(b.velocities(), vel)
         # and the tuned tree still gives the exact sum at theta -> 0
@@ -417,3 +417,31 @@ def test_run_groups_theta_half_fp32_vs_fp64_and_no_less_accurate(ctx, mode):
           flush=True)
     assert e_prec < 1e-4
     assert e_tree <= 1.02 * e_ref
+
+
+@pytest.mark.parametrize("group", [24, 48, 96, 128])
+def test_groups_walked_three_targets_per_lane_are_exact(ctx, group):
+    """A wave walks a group of 33-48 targets (17-24, 9-12, 5-6) with three targets per lane, and
+    of 65-192 targets only that way: at theta -> 0 every particle still gets its exact pair sum,
+    and at theta = 0.5 the FP32 walk agrees with the plain FP64 walk of the same groups."""
+    n = 40
+    pos = workload(n, "clustered", seed=21)
+    p0 = lagpm.make_params(n, theta=1e-6)
+    with lagpm.Box(ctx, p0, len(pos)) as b:
+        b.set_tuning(32, group, lagpm.GROUPS_TREE)
+        b.upload_pos(pos)
+        b.treecorrection(lagpm.evolfact(1.0))
+        a0, st = b.velocities(), b.tree_stats()
+    err0 = rel_rms(a0, exact_erfc(n, p0, pos))
+    a = {}
+    for prec in (lagpm.PREC_FP32, lagpm.PREC_FP64):
+        with lagpm.Box(ctx, lagpm.make_params(n, theta=0.5, precision=prec), len(pos)) as b:
+            b.set_tuning(32, group, lagpm.GROUPS_TREE)
+            b.upload_pos(pos)
+            b.treecorrection(lagpm.evolfact(1.0))
+            a[prec] = b.velocities()
+    e_prec = rel_rms(a[lagpm.PREC_FP32], a[lagpm.PREC_FP64])
+    print(f"group {group}: {len(pos) / st.n_groups:.1f} targets per group; theta0 rms {err0:.2e}; "
+          f"theta 0.5 fp32 vs fp64 {e_prec:.2e}", flush=True)
+    assert err0 < 3e-5
+    assert e_prec < 1e-4
