@@ -68,7 +68,13 @@ __device__ inline int table_bin(float r2, float& t, float lam) {
         constexpr uint32_t HI = ((uint32_t)(LAGPM_LOG_BASE + LAGPM_LOG_BINS) << LAGPM_LOG_SHIFT) - 1u;
         const uint32_t b = min(max(__float_as_uint(lam * r2), LO), HI);
         t = r2;
-        return (int)(b >> LAGPM_LOG_SHIFT) - (int)LAGPM_LOG_BASE;
+        // The bin, bits 15..30 of b, as one bit-field extract: the table address is then one
+        // shift-and-add from it. Written as (b >> 15) the compiler turns the shift and the
+        // address's scaling by 8 into a shift, a mask and an add: one more operation per pair.
+        static_assert(LAGPM_LOG_SHIFT == 15, "the extract below takes bits 15..30");
+        uint32_t f;
+        asm("v_bfe_u32 %0, %1, 15, 16" : "=v"(f) : "v"(b));
+        return (int)f - (int)LAGPM_LOG_BASE;
     } else {
         const float r = __builtin_amdgcn_sqrtf(r2);
         const int nt = min((int)(r * (1.0f / RAN_F)), 4095);
