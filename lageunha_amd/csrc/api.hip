@@ -133,6 +133,7 @@ void free_box(lagpm_box* b) {
     b->spec = b->phi = nullptr;  // the same padded mesh as rho
     dfree(b->pp); dfree(b->ncount); dfree(b->noff); dfree(b->nd_info); dfree(b->nd_parent); dfree(b->nd_child);
     dfree(b->nd_com); dfree(b->nd_q0); dfree(b->nd_q1); dfree(b->nd_d); dfree(b->cellroot); dfree(b->gflag);
+    dfree(b->w_link); dfree(b->w_geo); dfree(b->w_q0); dfree(b->w_q1); dfree(b->w_rfc); dfree(b->w_cellroot);
     dfree(b->dep_scan_tmp); dfree(b->dep_blk);
     dfree(b->goff); dfree(b->groups); dfree(b->gbox); dfree(b->pgroup); dfree(b->counters); dfree(b->dsz);
     dfree(b->scan_tmp);
@@ -457,11 +458,11 @@ size_t lagpm_estimate_box_bytes(const lagpm_params* p, size_t np) {
     size_t b = np * (12 * 8 + 4 * 4 + 2 * 8 + 2 * 4);  // two sets of x..vz, id/id_alt/perm/perm2, keys, node counts
     b += np * (f64 ? 4 : 16);                             // pgroup / pp
     b += m * 8 + K * 2 * real + 4 * m * real;              // acc, padded FFT mesh, grad
-    b += 4 * mcell * mcell * mcell + 4 * 8 + DSZ_COUNT * 4;  // cellroot, counters, sizes
+    b += (f64 ? 4 : 8) * mcell * mcell * mcell + 4 * 8 + DSZ_COUNT * 4;  // cell roots, counters, sizes
     b += 4 * (lagpm_deposit_chunk_cap(p->n, np) + 1);     // deposit chunk starts
     b += 12 * (lagpm_deposit_blocks(p->n) + 1) + lagpm_scan_tmp_bytes(lagpm_deposit_blocks(p->n));  // chunk list
     const size_t cap = lagpm_tree_node_cap_guess(np, (int)mcell), gcap = cap + np / 64 + 1;
-    b += cap * (16 + 4 + 32 + 4 + 4 + (f64 ? 96 : 48)) + gcap * (8 + (f64 ? 48 : 0));
+    b += cap * (16 + 4 + 32 + 4 + 4 + (f64 ? 96 : 48 + 68)) + gcap * (8 + (f64 ? 48 : 0));  // (+ the FP32 walk's copy)
     b += std::max(lagpm_scan_tmp_bytes(cap), std::max(lagpm_scan_tmp_bytes(np), lagpm_sort_tmp_bytes(np)));
     return b;
 }
@@ -498,6 +499,7 @@ int lagpm_box_create(lagpm_ctx* ctx, const lagpm_params* p, size_t np, lagpm_box
     if (rc == LAGPM_OK) rc = dalloc(ctx, box, (char**)&box->grad, m * 4 * real);
     ALLOC(box->ncount, np); ALLOC(box->noff, np);
     ALLOC(box->cellroot, (size_t)box->mcell * box->mcell * box->mcell);
+    if (!f64) ALLOC(box->w_cellroot, (size_t)box->mcell * box->mcell * box->mcell);
     ALLOC(box->counters, 4);
     ALLOC(box->dsz, DSZ_COUNT);
     const size_t nblk = lagpm_deposit_blocks(p->n);

@@ -183,6 +183,16 @@ struct lagpm_box {
     float4* nd_q1 = nullptr;      // Qxz Qyz trQ rmax
     double* nd_d = nullptr;       // FP64 moments: 12 per node (com3, mass, Q6, trQ, rmax)
     int* cellroot = nullptr;      // mcell^3
+    // The FP32 walk's copy of the tree by record (the build's order: level by level, the children of
+    // a node consecutive): links (first particle, count, first child record, children), the opening
+    // test's geometry (com, rmax), the quadrupole (q0; q1 with the mass in place of rmax), the split's
+    // first child record of each record, and the cell roots as records (mcell^3).
+    int4* w_link = nullptr;
+    float4* w_geo = nullptr;
+    float4* w_q0 = nullptr;
+    float4* w_q1 = nullptr;
+    int* w_rfc = nullptr;
+    int* w_cellroot = nullptr;
     uint32_t* gflag = nullptr;    // per-node group count, then offsets
     uint32_t* goff = nullptr;
     size_t group_cap = 0;

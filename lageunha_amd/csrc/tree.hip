@@ -237,15 +237,18 @@ __device__ inline void tree_reset(int t, uint32_t* __restrict__ dsz, unsigned lo
 // from pp (centres, second moments, node sizes, group boxes) is in the same units.
 __global__ void tree_prep_kernel(const double* __restrict__ x, const double* __restrict__ y,
                                  const double* __restrict__ z, size_t np, double sc, float4* __restrict__ pp,
-                                 uint32_t* __restrict__ ncount, int* __restrict__ cellroot, size_t ncell,
-                                 uint32_t* __restrict__ dsz, unsigned long long* __restrict__ counters) {
+                                 uint32_t* __restrict__ ncount, int* __restrict__ cellroot, int* __restrict__ wcellroot,
+                                 size_t ncell, uint32_t* __restrict__ dsz, unsigned long long* __restrict__ counters) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x < 64) tree_reset(threadIdx.x, dsz, counters);
     if (i < np) {
         if (pp) pp[i] = make_float4((float)(x[i] * sc), (float)(y[i] * sc), (float)(z[i] * sc), 1.0f);
         ncount[i] = 0u;
     }
-    if (i < ncell) cellroot[i] = -1;
+    if (i < ncell) {
+        cellroot[i] = -1;
+        if (wcellroot) wcellroot[i] = -1;
+    }
 }
 
 // node count on the device, clamped to the array capacity
@@ -315,7 +318,7 @@ constexpr int SPLIT_THREADS = 256;
 __global__ void __launch_bounds__(SPLIT_THREADS) split_kernel(const uint64_t* __restrict__ K, int l, int lmax, int nleaf,
                                                               size_t cap, uint32_t* __restrict__ dsz, int4* __restrict__ rec,
                                                               int* __restrict__ rch0, int* __restrict__ rch1,
-                                                              uint32_t* __restrict__ ncount) {
+                                                              uint32_t* __restrict__ ncount, int* __restrict__ rfc) {
     __shared__ uint32_t wcnt[SPLIT_THREADS / 64];
     __shared__ uint32_t bbase;
     const uint32_t n = dsz[DSZ_LVL + l], o = dsz[DSZ_LOFF + l];
@@ -371,6 +374,9 @@ __global__ void __launch_bounds__(SPLIT_THREADS) split_kernel(const uint64_t* __
         if (make) {
             if (ci < 4) rch0[4 * R + ci] = (int)b;
             else rch1[4 * R + ci - 4] = (int)b;
+            // the record's children are consecutive records (a record's lanes append in octant
+            // order, records in lane order): the first one's index is all the walk needs
+            if (ci == 0 && rfc) rfc[R] = (int)rr;
         }
         if (valid && sub == 0) rec[R].w = __popc(grp);
     }
@@ -383,7 +389,8 @@ __device__ inline int chain_first_level(const uint64_t* __restrict__ K, uint32_t
 __global__ void number_kernel(const uint64_t* __restrict__ K, int L, size_t cap, uint32_t* __restrict__ dsz,
                               const int4* __restrict__ rec, const int* __restrict__ rch0, const int* __restrict__ rch1,
                               const uint32_t* __restrict__ noff, int4* __restrict__ info, int* __restrict__ child,
-                              int* __restrict__ parent, int* __restrict__ cellroot, int mcell, uint32_t* __restrict__ order) {
+                              int* __restrict__ parent, int* __restrict__ cellroot, int mcell, uint32_t* __restrict__ order,
+                              const int* __restrict__ rfc, int4* __restrict__ wlink, int* __restrict__ wcellroot) {
     const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (dsz[DSZ_OVERFLOW] || r >= dev_count(dsz, DSZ_NODES, cap)) return;  // an overflowed build is redone
     const int4 nd = rec[r];  // (first, count, level, children)
@@ -391,10 +398,14 @@ __global__ void number_kernel(const uint64_t* __restrict__ K, int L, size_t cap,
     if ((size_t)v >= cap) return;  // cannot happen for sorted keys (the chain levels are consecutive)
     order[r] = v;
     info[v] = make_int4(nd.x, nd.y, nd.w, nd.z);
+    // the FP32 walk's links, by record: first particle, count, first child record, children
+    if (wlink) wlink[r] = make_int4(nd.x, nd.y, nd.w > 0 ? rfc[r] : 0, nd.w);
     if (nd.z == L) {
         parent[v] = -1;
         const uint64_t cell = K[nd.x] >> key_shift(L);
-        cellroot[compact3(cell) + (size_t)mcell * (compact3(cell >> 1) + (size_t)mcell * compact3(cell >> 2))] = (int)v;
+        const size_t ci = compact3(cell) + (size_t)mcell * (compact3(cell >> 1) + (size_t)mcell * compact3(cell >> 2));
+        cellroot[ci] = (int)v;
+        if (wcellroot) wcellroot[ci] = (int)r;
     }
     const uint32_t nodes = dsz[DSZ_NODES];
     for (int c = 0; c < nd.w; c++) {
@@ -435,9 +446,10 @@ __device__ inline float row_max(float v) {
     return v;
 }
 // every lane of the node's row calls this with its index `sub` in the row
-__device__ inline void moments_f32_node(size_t v, int sub, const float4* __restrict__ pp, const int4* __restrict__ info,
+__device__ inline void moments_f32_node(size_t v, size_t rec, int sub, const float4* __restrict__ pp, const int4* __restrict__ info,
                                         const int* __restrict__ child, float4* __restrict__ com, float4* __restrict__ q0,
-                                        float4* __restrict__ q1) {
+                                        float4* __restrict__ q1, float4* __restrict__ wgeo, float4* __restrict__ wq0,
+                                        float4* __restrict__ wq1) {
     const int4 nd = info[v];
     float M = 0, cM = 0, Sx = 0, cx = 0, Sy = 0, cy = 0, Sz = 0, cz = 0;
     float Q[6] = {0, 0, 0, 0, 0, 0}, cQ[6] = {0, 0, 0, 0, 0, 0};
@@ -498,9 +510,15 @@ __device__ inline void moments_f32_node(size_t v, int sub, const float4* __restr
     for (int k = 0; k < 6; k++) Q[k] = row_sum(Q[k] - cQ[k]);
     r2max = row_max(r2max);
     if (sub == 0) {
+        const float rmax = nd.z == 0 || nd.y <= RMAX_EXACT_MAX ? sqrtf(r2max) : rbound;
         com[v] = make_float4(cm.x, cm.y, cm.z, M);
         q0[v] = make_float4(Q[0], Q[1], Q[2], Q[3]);
-        q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], nd.z == 0 || nd.y <= RMAX_EXACT_MAX ? sqrtf(r2max) : rbound);
+        q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], rmax);
+        // the walk's copy at the node's record: what the opening test reads in one load, and the
+        // quadrupole with the mass in place of rmax
+        wgeo[rec] = make_float4(cm.x, cm.y, cm.z, rmax);
+        wq0[rec] = make_float4(Q[0], Q[1], Q[2], Q[3]);
+        wq1[rec] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], M);
     }
 }
 
@@ -508,14 +526,15 @@ constexpr int LEVEL_BLOCKS = 2048;  // grid of the per-level launches (grid-stri
 __global__ void moments_f32_kernel(const float4* __restrict__ pp, const uint32_t* __restrict__ dsz, int level,
                                    const uint32_t* __restrict__ order, const int4* __restrict__ info,
                                    const int* __restrict__ child, float4* __restrict__ com, float4* __restrict__ q0,
-                                   float4* __restrict__ q1) {
+                                   float4* __restrict__ q1, float4* __restrict__ wgeo, float4* __restrict__ wq0,
+                                   float4* __restrict__ wq1) {
     if (dsz[DSZ_OVERFLOW]) return;
     const uint32_t n = dsz[DSZ_LVL + level], o = dsz[DSZ_LOFF + level];
     const int sub = (int)(threadIdx.x & (MOM_ROW - 1));
     // a node's lanes share i, so they are active or done together and their shuffles stay among them
     for (size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / MOM_ROW; i < n;
          i += (size_t)gridDim.x * blockDim.x / MOM_ROW)
-        moments_f32_node(order[o + i], sub, pp, info, child, com, q0, q1);
+        moments_f32_node(order[o + i], o + i, sub, pp, info, child, com, q0, q1, wgeo, wq0, wq1);
 }
 
 // FP64 moments for the plain FP64 path (12 doubles per node).
@@ -708,12 +727,12 @@ __device__ inline float image_shift(int c, int d, int mcell, float box) {
 // equal law, softening and split), so `fact` carries the box's kick factor times 1/n^2.
 struct walk_box {
     const float4* pp;
-    const int4* info;
-    const int* child;
-    const float4* com;
-    const float4* q0;
-    const float4* q1;
-    const int* cellroot;
+    // the tree by record (the build's order: the children of a node are consecutive records)
+    const int4* link;      // first particle, count, first child record, children
+    const float4* geo;     // com, rmax
+    const float4* q0;      // Qxx Qyy Qzz Qxy
+    const float4* q1;      // Qxz Qyz trQ, mass
+    const int* cellroot;   // root record of each chaining cell, or -1
     const int2* groups;
     const uint64_t* key;
     double *tvx, *tvy, *tvz;       // the tree kick of each particle (written, not added: see below)
@@ -1120,10 +1139,10 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 float sx, sy, sz;
                 shift_of(e & 31, sx, sy, sz);
                 int v = e >> 5;
-                float4 c = A.com[v];
-                cpc0[lane] = make_float4(c.x + sx, c.y + sy, c.z + sz, c.w);
+                const float4 c = A.geo[v], q1 = A.q1[v];
+                cpc0[lane] = make_float4(c.x + sx, c.y + sy, c.z + sz, q1.w);  // (centre, mass)
                 cpc1[lane] = A.q0[v];
-                cpc2[lane] = A.q1[v];
+                cpc2[lane] = q1;
             } else {
                 cpc0[lane] = dummy;
                 cpc1[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1153,21 +1172,19 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             sp -= take;
             int e = lane < take ? sget(sp + lane) : -1;
             int kind = 0;  // 0 none, 1 accept, 2 leaf, 3 open
-            int4 nd = make_int4(0, 0, 0, 0);
-            int4 ch0 = make_int4(0, 0, 0, 0), ch1 = ch0;
+            int4 nd = make_int4(0, 0, 0, 0);  // (first particle, count, first child record, children)
             int v = 0, nb = 0;
             if (e >= 0) {
                 v = e >> 5;
                 nb = e & 31;
                 float sx, sy, sz;
                 shift_of(nb, sx, sy, sz);
-                float4 c = A.com[v];
-                float rmax = A.q1[v].w;
-                nd = A.info[v];
-                // the child slots (8 ints, 32-byte aligned) load with the node itself, so opening
-                // it needs no second dependent round trip to memory
-                ch0 = reinterpret_cast<const int4*>(A.child)[2 * v];
-                ch1 = reinterpret_cast<const int4*>(A.child)[2 * v + 1];
+                // two 16-byte loads per node: the children are consecutive records, so opening a
+                // node needs only the first one's index, and the popped nodes of a round (mostly
+                // the children of the previous round's opened nodes) share cache lines
+                const float4 c = A.geo[v];
+                const float rmax = c.w;
+                nd = A.link[v];
                 float px = c.x + sx, py = c.y + sy, pz = c.z + sz;
                 float ddx = fmaxf(fmaxf(lox - px, px - hix), 0.f);
                 float ddy = fmaxf(fmaxf(loy - py, py - hiy), 0.f);
@@ -1176,22 +1193,21 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 float rr = A.rcut + rmax;
                 if (dmin2 > rr * rr) kind = 0;
                 else if (nd.y >= A.naccept && dmin2 >= rmax * rmax * A.inv_theta2) kind = 1;
-                else if (nd.z == 0) kind = 2;
+                else if (nd.w == 0) kind = 2;
                 else kind = 3;
             }
             // children onto the stack
             {
-                int nch = kind == 3 ? nd.z : 0;
+                int nch = kind == 3 ? nd.w : 0;
                 int tot;
                 int off = wave_excl_scan(nch, lane, tot);
                 if (sp + tot > STACK_MAX) {
                     overflow = true;
                     break;
                 }
-                const int chv[8] = {ch0.x, ch0.y, ch0.z, ch0.w, ch1.x, ch1.y, ch1.z, ch1.w};
 #pragma unroll
                 for (int c = 0; c < 8; c++)
-                    if (c < nch) sset(sp + off + c, (chv[c] << 5) | nb);
+                    if (c < nch) sset(sp + off + c, ((nd.z + c) << 5) | nb);
                 sp += tot;
                 wave_sync();
             }
@@ -1501,6 +1517,11 @@ int lagpm_tree_alloc(lagpm_ctx* ctx, lagpm_box* box, size_t cap) {
         RCHK(regrow(ctx, &box->nd_com, cap));
         RCHK(regrow(ctx, &box->nd_q0, cap));
         RCHK(regrow(ctx, &box->nd_q1, cap));
+        RCHK(regrow(ctx, &box->w_link, cap));
+        RCHK(regrow(ctx, &box->w_geo, cap));
+        RCHK(regrow(ctx, &box->w_q0, cap));
+        RCHK(regrow(ctx, &box->w_q1, cap));
+        RCHK(regrow(ctx, &box->w_rfc, cap));
     }
     const size_t need = std::max(lagpm_scan_tmp_bytes(cap), std::max(lagpm_scan_tmp_bytes(box->np), lagpm_sort_tmp_bytes(box->np)));
     if (need > box->scan_tmp_bytes) {
@@ -1579,7 +1600,7 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     const size_t ncell = (size_t)box->mcell * box->mcell * box->mcell;
     hipLaunchKernelGGL(tree_prep_kernel, dim3(blocks_for(std::max(np, ncell), 256)), dim3(256), 0, st, box->x, box->y,
                        box->z, np, f64 ? 1.0 : lagpm_tree_scale(box), f64 ? (float4*)nullptr : box->pp, box->ncount,
-                       box->cellroot, ncell, box->dsz, box->counters);
+                       box->cellroot, f64 ? (int*)nullptr : box->w_cellroot, ncell, box->dsz, box->counters);
     HIPCHK(ctx, hipGetLastError());
     // top-down build: records of level-L cells, then one split per level, then the numbering.
     // The records and child lists live in the moment arrays until the moments overwrite them.
@@ -1602,20 +1623,22 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
     const unsigned lblocks = (unsigned)std::min<size_t>(LEVEL_BLOCKS, blocks_for(cap, 256));
     const unsigned mblocks = (unsigned)std::min<size_t>(2 * LEVEL_BLOCKS, blocks_for(cap * MOM_ROW, 256));
     const int D = std::min(LAGPM_KEY_BITS, std::max(L, box->depth));
+    int* rfc = f64 ? (int*)nullptr : box->w_rfc;
     for (int l = L; l < D; l++) {
         hipLaunchKernelGGL(split_kernel, dim3(lblocks), dim3(SPLIT_THREADS), 0, st, box->key, l, D, nleaf, cap, box->dsz,
-                           rec, rch0, rch1, box->ncount);
+                           rec, rch0, rch1, box->ncount, rfc);
         hipLaunchKernelGGL(level_mark_kernel, dim3(1), dim3(64), 0, st, box->dsz, cap, l + 1, L);
         HIPCHK(ctx, hipGetLastError());
     }
     if (D < LAGPM_KEY_BITS) {  // the limit level: flag a record that still needs splitting
         hipLaunchKernelGGL(split_kernel, dim3(lblocks), dim3(SPLIT_THREADS), 0, st, box->key, D, D, nleaf, cap, box->dsz,
-                           rec, rch0, rch1, box->ncount);
+                           rec, rch0, rch1, box->ncount, rfc);
         HIPCHK(ctx, hipGetLastError());
     }
     RCHK(lagpm_exclusive_scan_u32(ctx, st, box->scan_tmp, box->ncount, box->noff, np, box->dsz + DSZ_SCRATCH));
     hipLaunchKernelGGL(number_kernel, dim3(blocks_for(cap, 256)), dim3(256), 0, st, box->key, L, cap, box->dsz, rec, rch0,
-                       rch1, box->noff, box->nd_info, box->nd_child, box->nd_parent, box->cellroot, box->mcell, order);
+                       rch1, box->noff, box->nd_info, box->nd_child, box->nd_parent, box->cellroot, box->mcell, order, rfc,
+                       f64 ? (int4*)nullptr : box->w_link, f64 ? (int*)nullptr : box->w_cellroot);
     HIPCHK(ctx, hipGetLastError());
     for (int l = D; l >= L; l--) {  // levels below D hold no records
         if (f64)
@@ -1623,7 +1646,8 @@ int lagpm_tree_build(lagpm_ctx* ctx, lagpm_box* box) {
                                order, box->nd_info, box->nd_child, box->nd_d);
         else  // (MOM_ROW lanes per node: a larger grid keeps as many nodes in flight)
             hipLaunchKernelGGL(moments_f32_kernel, dim3(mblocks), dim3(256), 0, st, box->pp, box->dsz, l, order,
-                               box->nd_info, box->nd_child, box->nd_com, box->nd_q0, box->nd_q1);
+                               box->nd_info, box->nd_child, box->nd_com, box->nd_q0, box->nd_q1, box->w_geo, box->w_q0,
+                               box->w_q1);
         HIPCHK(ctx, hipGetLastError());
     }
 
@@ -1662,8 +1686,8 @@ int lagpm_tree_walk_f32(lagpm_ctx* ctx, lagpm_box* const* boxes, const double* f
             return lagpm_fail(ctx, LAGPM_E_INVAL, "boxes of one walk launch must share the FP32 force table");
         const double th = box->p.theta;
         walk_box& A = hb[(size_t)b];
-        A.pp = box->pp; A.info = box->nd_info; A.child = box->nd_child; A.com = box->nd_com; A.q0 = box->nd_q0;
-        A.q1 = box->nd_q1; A.cellroot = box->cellroot; A.groups = box->groups; A.key = box->key;
+        A.pp = box->pp; A.link = box->w_link; A.geo = box->w_geo; A.q0 = box->w_q0;
+        A.q1 = box->w_q1; A.cellroot = box->w_cellroot; A.groups = box->groups; A.key = box->key;
         A.tvx = box->alt[0]; A.tvy = box->alt[1]; A.tvz = box->alt[2];  // free between sorts
         A.counters = box->counters; A.dsz = box->dsz;
         if (sample) {  // the sampled particles as groups of one (sizes: the sample's own words)
