@@ -344,12 +344,14 @@ int lagpm_get_tables(lagpm_ctx* ctx, const lagpm_params* p, const law_tables** o
     // copy), where its variable u = lam r^2 is the plain squared distance. With d' = sqrt(lam) d and
     // Q' = lam Q the kick times sqrt(lam) is d' (M f0 + tr(Q') f1 / lam + d'Q'd' f2 / lam^2)
     // + 2 (f1 / lam) Q'd', so the FP32 lines are f0, f1 / lam and f2 / lam^2 as lines in u.
+    // The f1 line is stored doubled (exact: a power of two), so the walk's 2 f1 Q'd' needs no
+    // multiply; the walk's node copy holds tr(Q')/2 to match, and tr(Q') f1 is unchanged bit for bit.
     // The FP64 walk keeps the plain lines in r^2 (tabd).
     const double L = lt.logr2 ? (double)lt.lam : 1.0;
     for (int i = 0; i < NB; i++) {
         t0[i] = make_float2((float)t[6 * i], (float)(t[6 * i + 1] / L));
-        t12[i] = make_float4((float)(t[6 * i + 2] / L), (float)(t[6 * i + 3] / (L * L)), (float)(t[6 * i + 4] / (L * L)),
-                             (float)(t[6 * i + 5] / (L * L * L)));
+        t12[i] = make_float4(2.0f * (float)(t[6 * i + 2] / L), 2.0f * (float)(t[6 * i + 3] / (L * L)),
+                             (float)(t[6 * i + 4] / (L * L)), (float)(t[6 * i + 5] / (L * L * L)));
     }
     RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tab0, NB * sizeof(float2)));
     RCHK(lagpm_dalloc_bytes(ctx, (void**)&lt.tab12, NB * sizeof(float4)));

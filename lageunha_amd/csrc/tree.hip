@@ -515,10 +515,11 @@ __device__ inline void moments_f32_node(size_t v, size_t rec, int sub, const flo
         q0[v] = make_float4(Q[0], Q[1], Q[2], Q[3]);
         q1[v] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], rmax);
         // the walk's copy at the node's record: what the opening test reads in one load, and the
-        // quadrupole with the mass in place of rmax
+        // quadrupole with the mass in place of rmax and half the trace (the walk's table holds the
+        // trace term's line doubled)
         wgeo[rec] = make_float4(cm.x, cm.y, cm.z, rmax);
         wq0[rec] = make_float4(Q[0], Q[1], Q[2], Q[3]);
-        wq1[rec] = make_float4(Q[4], Q[5], Q[0] + Q[1] + Q[2], M);
+        wq1[rec] = make_float4(Q[4], Q[5], 0.5f * (Q[0] + Q[1] + Q[2]), M);
     }
 }
 
@@ -1053,7 +1054,8 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             }
         };
         // One node interaction: monopole plus quadrupole, Q d computed once and reused for
-        // d.Q.d (the quadratic form) and for the 2 f2 Q d term.
+        // d.Q.d (the quadratic form) and for the 2 f2 Q d term. The table's f2 line is stored
+        // doubled and the node's trace halved (both exact), so 2 f2 is the table value itself.
         auto pc_eval = [&](float4 c, float4 qa, float4 qb, float2 t0, float4 t12, float t, float dx, float dy, float dz,
                            bool in, float& ax, float& ay, float& az) {
             float f1 = t0.x + t0.y * t, f2 = t12.x + t12.y * t, f3 = t12.z + t12.w * t;
@@ -1062,7 +1064,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             float qx3 = qb.x * dx + qb.y * dy + qa.z * dz;
             float qxy = dx * qx1 + dy * qx2 + dz * qx3;
             float tmp = c.w * f1 + qb.z * f2 + qxy * f3;
-            float two = 2.f * f2;
+            float two = f2;
             if constexpr (!LOGT) {
                 tmp = in ? tmp : 0.f;
                 two = in ? two : 0.f;
