@@ -768,6 +768,34 @@ __global__ void walk_prefix_kernel(const walk_box* __restrict__ B, int nbox, uin
 
 // CENSUS: also count the evaluated particle pairs that lie within the cutoff (counters[3]), a
 // measurement of the lists' useful fraction; it adds a compare and an add per pair.
+// Pointers read from a descriptor in memory are generic to the compiler, which then emits flat
+// loads and stores. A flat access also counts against the LDS wait counter, so every LDS wait that
+// follows one waits for the memory access too; these casts make them global ones.
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T* gld(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T* gst(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+// (the HIP vector types do not copy from a global reference: load as clang vectors)
+typedef float gv4f __attribute__((ext_vector_type(4)));
+typedef int gv4i __attribute__((ext_vector_type(4)));
+typedef int gv2i __attribute__((ext_vector_type(2)));
+__device__ inline float4 gld4(const float4* p, size_t i) {
+    const gv4f v = ((const __attribute__((address_space(1))) gv4f*)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline int4 gld4(const int4* p, size_t i) {
+    const gv4i v = ((const __attribute__((address_space(1))) gv4i*)p)[i];
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ inline int2 gld2(const int2* p, size_t i) {
+    const gv2i v = ((const __attribute__((address_space(1))) gv2i*)p)[i];
+    return make_int2(v.x, v.y);
+}
+
 template <bool LOGT, bool CENSUS>
 __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* __restrict__ B, const uint32_t* __restrict__ gprefix,
                                                                 int nbox, const float2* __restrict__ tab0,
@@ -797,7 +825,15 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
     int* gstack = stacks + (size_t)(blockIdx.x * WALK_WAVES + wid) * STACK_MAX;
     int* lstack = s_stack[wid];
     // stack entry i: LDS below LSTACK (96% of pop rounds stay there), global memory above
-    auto sget = [&](int i) -> int { return i < LSTACK ? lstack[i] : gstack[i]; };
+    // (the LDS read is unconditional, from a clamped index, so that the compiler does not merge
+    // the two reads into one flat load through a selected pointer: a flat load also holds up every
+    // LDS wait behind it)
+    auto sget = [&](int i) -> int {
+        const int l = lstack[i < LSTACK ? i : LSTACK - 1];
+        int g = 0;
+        if (i >= LSTACK) g = gstack[i];
+        return i < LSTACK ? l : g;
+    };
     auto sset = [&](int i, int v) {
         if (i < LSTACK) lstack[i] = v;
         else gstack[i] = v;
@@ -837,7 +873,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             else bhi = mid;
         }
         const walk_box A = B[blo];
-        const int2 gr = A.groups[g - __builtin_amdgcn_readfirstlane((int)gprefix[blo])];
+        const int2 gr = gld2(A.groups, g - __builtin_amdgcn_readfirstlane((int)gprefix[blo]));
         const int cnt = __builtin_amdgcn_readfirstlane(gr.y);
         const int gbeg = __builtin_amdgcn_readfirstlane(gr.x);
         // Lane layout. Lane (tgt, rep) holds M targets, tgt + i Tw for i < M, and takes every R-th
@@ -863,7 +899,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
         }
         const int Tw = 64 / R;
         const int tgt = lane & (Tw - 1), rep = lane / Tw;
-        const uint64_t cellkey = A.key[gbeg] >> A.cell_shift;
+        const uint64_t cellkey = gld(A.key)[gbeg] >> A.cell_shift;
         const int cx = (int)compact3(cellkey), cy = (int)compact3(cellkey >> 1), cz = (int)compact3(cellkey >> 2);
         // the rest of the group, compiled once per targets-per-lane count so that the single-target
         // path carries no registers for the other two
@@ -879,7 +915,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             const int t = tgt + i * Tw;
             valid[i] = t < cnt;
             me[i] = gbeg + (valid[i] ? t : 0);  // an idle slot repeats target 0: inside the group box
-            xt[i] = A.pp[me[i]];
+            xt[i] = gld4(A.pp, me[i]);
             ax[i] = ay[i] = az[i] = 0.f;
             lx = i ? fminf(lx, xt[i].x) : xt[i].x; hx = i ? fmaxf(hx, xt[i].x) : xt[i].x;
             ly = i ? fminf(ly, xt[i].y) : xt[i].y; hy = i ? fmaxf(hy, xt[i].y) : xt[i].y;
@@ -896,7 +932,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             if (lane < 27) {
                 int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
                 int nx = (cx + dx + A.mcell) % A.mcell, ny = (cy + dy + A.mcell) % A.mcell, nz = (cz + dz + A.mcell) % A.mcell;
-                int r = A.cellroot[nx + A.mcell * (ny + A.mcell * nz)];
+                int r = gld(A.cellroot)[nx + A.mcell * (ny + A.mcell * nz)];
                 if (r >= 0) e = (r << 5) | lane;
             }
             int tot;
@@ -932,7 +968,7 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 uint32_t e = ppidx[start + lane];
                 float sx, sy, sz;
                 shift_of((int)(e & 31u), sx, sy, sz);
-                s = A.pp[e >> 5];
+                s = gld4(A.pp, e >> 5);
                 s.x += sx; s.y += sy; s.z += sz;
             }
             return s;
@@ -1141,9 +1177,9 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 float sx, sy, sz;
                 shift_of(e & 31, sx, sy, sz);
                 int v = e >> 5;
-                const float4 c = A.geo[v], q1 = A.q1[v];
+                const float4 c = gld4(A.geo, v), q1 = gld4(A.q1, v);
                 cpc0[lane] = make_float4(c.x + sx, c.y + sy, c.z + sz, q1.w);  // (centre, mass)
-                cpc1[lane] = A.q0[v];
+                cpc1[lane] = gld4(A.q0, v);
                 cpc2[lane] = q1;
             } else {
                 cpc0[lane] = dummy;
@@ -1184,9 +1220,9 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
                 // two 16-byte loads per node: the children are consecutive records, so opening a
                 // node needs only the first one's index, and the popped nodes of a round (mostly
                 // the children of the previous round's opened nodes) share cache lines
-                const float4 c = A.geo[v];
+                const float4 c = gld4(A.geo, v);
                 const float rmax = c.w;
-                nd = A.link[v];
+                nd = gld4(A.link, v);
                 float px = c.x + sx, py = c.y + sy, pz = c.z + sz;
                 float ddx = fmaxf(fmaxf(lox - px, px - hix), 0.f);
                 float ddy = fmaxf(fmaxf(loy - py, py - hiy), 0.f);
@@ -1294,9 +1330,9 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
 #pragma unroll
         for (int i = 0; i < M; i++) {
             if (valid[i] && rep == 0) {
-                A.tvx[me[i]] = overflow ? 0.0 : (double)(A.fact * ax[i]);
-                A.tvy[me[i]] = overflow ? 0.0 : (double)(A.fact * ay[i]);
-                A.tvz[me[i]] = overflow ? 0.0 : (double)(A.fact * az[i]);
+                gst(A.tvx)[me[i]] = overflow ? 0.0 : (double)(A.fact * ax[i]);
+                gst(A.tvy)[me[i]] = overflow ? 0.0 : (double)(A.fact * ay[i]);
+                gst(A.tvz)[me[i]] = overflow ? 0.0 : (double)(A.fact * az[i]);
             }
         }
         if (lane == 0) {
