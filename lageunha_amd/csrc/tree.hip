@@ -1336,12 +1336,13 @@ __global__ void __launch_bounds__(WALK_THREADS) walk_f32_kernel(const walk_box* 
             }
         }
         if (lane == 0) {
-            atomicAdd(&A.counters[0], npp * (unsigned long long)cnt);
-            atomicAdd(&A.counters[1], npc * (unsigned long long)cnt);
+            // (global atomics: a flat one would hold up the next group's first LDS wait)
+            __hip_atomic_fetch_add(gst(A.counters), npp * (unsigned long long)cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(gst(A.counters) + 1, npc * (unsigned long long)cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if constexpr (CENSUS) {
             for (int o = 32; o > 0; o >>= 1) nin += (unsigned)__shfl_xor((int)nin, o, 64);
-            if (lane == 0) atomicAdd(&A.counters[3], (unsigned long long)nin);
+            if (lane == 0) __hip_atomic_fetch_add(gst(A.counters) + 3, (unsigned long long)nin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         };
         if (M == 3) group_body(ic<3>{});
