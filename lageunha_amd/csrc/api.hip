@@ -1011,7 +1011,7 @@ int lagpm_box_autotune(lagpm_ctx* ctx, lagpm_box* box, int* nleaf, int* group) {
     if (box->np == 0) return LAGPM_OK;
     // (nleaf, group, group mode)
     static const int cand[][3] = {{16, 64, 0}, {32, 64, 0}, {8, 64, 0}, {16, 32, 0}, {32, 32, 0}, {8, 32, 0},
-                                  {16, 16, 0}, {32, 16, 0}, {8, 16, 0}, {32, 48, 0}, {16, 96, 0}, {32, 96, 0},
+                                  {16, 16, 0}, {32, 16, 0}, {8, 16, 0}, {16, 96, 0}, {32, 96, 0},
                                   {16, 64, 1}, {32, 64, 1}, {16, 32, 1}, {32, 32, 1}, {16, 16, 1},
                                   {16, 64, 2}, {32, 64, 2}, {32, 96, 2}};
     double best = 1e300;
