@@ -419,16 +419,18 @@ def test_run_groups_theta_half_fp32_vs_fp64_and_no_less_accurate(ctx, mode):
     assert e_tree <= 1.02 * e_ref
 
 
-@pytest.mark.parametrize("group", [24, 48, 96, 128])
-def test_groups_walked_three_targets_per_lane_are_exact(ctx, group):
+@pytest.mark.parametrize("group,mode", [(24, lagpm.GROUPS_TREE), (48, lagpm.GROUPS_TREE), (96, lagpm.GROUPS_TREE),
+                                        (128, lagpm.GROUPS_TREE), (96, lagpm.GROUPS_PACKED), (128, lagpm.GROUPS_RUNS)])
+def test_groups_walked_three_targets_per_lane_are_exact(ctx, group, mode):
     """A wave walks a group of 33-48 targets (17-24, 9-12, 5-6) with three targets per lane, and
-    of 65-192 targets only that way: at theta -> 0 every particle still gets its exact pair sum,
-    and at theta = 0.5 the FP32 walk agrees with the plain FP64 walk of the same groups."""
+    of 65-192 targets only that way (packed and run groups fill 96 and 128): at theta -> 0 every
+    particle still gets its exact pair sum, and at theta = 0.5 the FP32 walk agrees with the plain
+    FP64 walk of the same groups."""
     n = 40
     pos = workload(n, "clustered", seed=21)
     p0 = lagpm.make_params(n, theta=1e-6)
     with lagpm.Box(ctx, p0, len(pos)) as b:
-        b.set_tuning(32, group, lagpm.GROUPS_TREE)
+        b.set_tuning(32, group, mode)
         b.upload_pos(pos)
         b.treecorrection(lagpm.evolfact(1.0))
         a0, st = b.velocities(), b.tree_stats()
@@ -436,12 +438,12 @@ def test_groups_walked_three_targets_per_lane_are_exact(ctx, group):
     a = {}
     for prec in (lagpm.PREC_FP32, lagpm.PREC_FP64):
         with lagpm.Box(ctx, lagpm.make_params(n, theta=0.5, precision=prec), len(pos)) as b:
-            b.set_tuning(32, group, lagpm.GROUPS_TREE)
+            b.set_tuning(32, group, mode)
             b.upload_pos(pos)
             b.treecorrection(lagpm.evolfact(1.0))
             a[prec] = b.velocities()
     e_prec = rel_rms(a[lagpm.PREC_FP32], a[lagpm.PREC_FP64])
-    print(f"group {group}: {len(pos) / st.n_groups:.1f} targets per group; theta0 rms {err0:.2e}; "
+    print(f"group {group} mode {mode}: {len(pos) / st.n_groups:.1f} targets per group; theta0 rms {err0:.2e}; "
           f"theta 0.5 fp32 vs fp64 {e_prec:.2e}", flush=True)
     assert err0 < 3e-5
     assert e_prec < 1e-4
